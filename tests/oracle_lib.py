@@ -1,0 +1,298 @@
+"""ctypes bindings for the oracle (test infrastructure only).
+
+`Oracle` wraps oracle/liboracle.so, the C++ restatement of the reference
+kernels (float and double instantiations). `RefHarness` wraps
+oracle/_ref/libarborref.so, the reference CPU harness compiled from its own
+sources; it exists only where it was built (it is never required on a GPU box).
+Both follow the launch emulation of tests/native_harness.cpp:35-39.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(REPO, "oracle", "liboracle.so")
+REF_SO = os.path.join(REPO, "oracle", "_ref", "libarborref.so")
+
+CAPACITY = 32768
+NODE_COUNT = 65536
+PS = 24
+CS = 20
+CURVES = 2109
+OBJECT_COUNT = 29412
+FAR = np.float32(100000.0)
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i = ct.c_int
+_f = ct.c_float
+
+
+class Scene:
+    """Host copies of the build outputs B, P, Keys, Order, Nodes."""
+
+    def __init__(self):
+        self.B = np.zeros(CURVES * CS, np.float32)
+        self.P = np.zeros(CAPACITY * PS, np.float32)
+        self.Keys = np.zeros(CAPACITY, np.uint32)
+        self.Order = np.zeros(CAPACITY, np.uint32)
+        self.Nodes = np.zeros(NODE_COUNT * 8, np.float32)
+
+
+def _bind(lib, name, res, args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = args
+    return fn
+
+
+class _Common:
+    """Entry points both libraries share (prefix orc_ or ref_)."""
+
+    def _setup_common(self, p):
+        L = self.lib
+        self._generate = _bind(L, p + "generate", _i, [_i, _f32p, _f32p, _u32p, _u32p, _f32p, _i])
+        self._init_camera = _bind(L, p + "init_camera", _i, [_f32p, _i])
+        self._update_camera = _bind(L, p + "update_camera", _i, [_f32p, _f32p, _f, _i, _i])
+        self._trace = _bind(L, p + "trace_pixels" + self._suf, _i, [_f32p, _f32p, _u32p, _f32p, _f32p, _i, _i, _i])
+        self._trace_tree = _bind(L, p + "trace_tree_pixels" + self._suf, _i,
+                                 [_f32p, _f32p, _u32p, _f32p, _f32p, _i, _i, _i])
+        self._shade = _bind(L, p + "shade_pixels" + self._suf, _i,
+                            [_f32p, _f32p, _u32p, _f32p, _f32p, _f32p, _f32p, _i, _i, _i])
+        self._filter = _bind(L, p + "filter_lighting" + self._suf, _i,
+                             [_f32p, _f32p, _f32p, _f32p, _f32p, _i, _i, _i])
+        self._resolve = _bind(L, p + "resolve_pixels" + self._suf, _i, [_f32p, _f32p, _u32p, _f32p, _i, _i, _i])
+        self._tree_rays = _bind(L, p + "trace_tree_rays", _i,
+                                [_f32p, _f32p, _u32p, _f32p, _f32p, _i, _f, _f, _f, _f, _i, _f32p])
+        self._bark_cut = _bind(L, p + "bark_cut", _f, [_f32p, _i, _f, _f, _f, _f])
+        self._leaf_t = _bind(L, p + "leaf_t", _f, [_f32p, _i, _f32p, _f32p, _f, _f, _f])
+        self._grow = _bind(L, p + "grow_tree", _i, [_f32p, _i, _i])
+        self._hash_u = _bind(L, p + "hash_u", ct.c_uint, [ct.c_uint])
+        self._bark_sweep = _bind(L, p + "bark_sweep", _f, [_f32p])
+        self._leaf_probes = _bind(L, p + "leaf_probes", _i, [_f32p, _f32p])
+
+    def generate(self, seed, bitonic=False):
+        s = Scene()
+        rc = self._generate(seed, s.B, s.P, s.Keys, s.Order, s.Nodes, int(bitonic))
+        assert rc == 0
+        return s
+
+    def init_camera(self, seed):
+        C = np.zeros(64, np.float32)
+        self._init_camera(C, seed)
+        return C
+
+    def update_camera(self, C, I, delta, w, h):
+        self._update_camera(C, np.ascontiguousarray(I, np.float32), delta, w, h)
+
+    def trace(self, s, C, w, h, threads=0):
+        Hit = np.zeros(w * h * 4, np.float32)
+        self._trace(s.P, s.Nodes, s.Order, C, Hit, w, h, threads)
+        return Hit
+
+    def trace_tree(self, s, C, w, h, threads=0):
+        Hit = np.zeros(w * h * 4, np.float32)
+        self._trace_tree(s.P, s.Nodes, s.Order, C, Hit, w, h, threads)
+        return Hit
+
+    def shade(self, s, C, Hit, w, h, threads=0):
+        Linear = np.zeros(w * h * 4, np.float32)
+        Guide = np.zeros(w * h * 4, np.float32)
+        self._shade(s.P, s.Nodes, s.Order, C, Hit, Linear, Guide, w, h, threads)
+        return Linear, Guide
+
+    def filter(self, Linear, Guide, Hit, C, w, h, threads=0):
+        out = np.zeros(w * h * 4, np.float32)
+        self._filter(Linear, Guide, Hit, out, C, w, h, threads)
+        return out
+
+    def resolve(self, Filtered, History, C, w, h, threads=0):
+        Pixels = np.zeros(w * h, np.uint32)
+        self._resolve(Filtered, History, Pixels, C, w, h, threads)
+        return Pixels
+
+    def trace_tree_rays(self, s, ro, rd, cone=0.0005, time=0.0, wind=0.0, best=float(FAR), shadows=False):
+        ro = np.ascontiguousarray(ro, np.float32).reshape(-1)
+        rd = np.ascontiguousarray(rd, np.float32).reshape(-1)
+        n = ro.size // 3
+        out = np.zeros(2 * n, np.float32)
+        self._tree_rays(s.P, s.Nodes, s.Order, ro, rd, n, cone, time, wind, best, int(shadows), out)
+        return out.reshape(n, 2)
+
+    def bark_cut(self, P, oid, p, footprint):
+        return self._bark_cut(P, oid, float(p[0]), float(p[1]), float(p[2]), footprint)
+
+    def leaf_t(self, P, oid, ro, rd, footprint, time=0.0, wind=0.0):
+        return self._leaf_t(P, oid, np.asarray(ro, np.float32), np.asarray(rd, np.float32), footprint, time, wind)
+
+    def bark_sweep(self, P):
+        return self._bark_sweep(P)
+
+    def leaf_probes(self, P):
+        out = np.zeros(100, np.float32)
+        self._leaf_probes(P, out)
+        return out.reshape(50, 2)
+
+    def grow_tree(self, B, level, seed):
+        return self._grow(B, level, seed)
+
+    def hash_u(self, x):
+        return self._hash_u(x)
+
+
+class Oracle(_Common):
+    """The C++ restatement (oracle/liboracle.so)."""
+
+    _suf = "_f32"
+
+    def __init__(self, path=ORACLE_SO):
+        self.lib = ct.CDLL(path)
+        self._setup_common("orc_")
+        L = self.lib
+        self._trace64 = _bind(L, "orc_trace_pixels_f64", _i, [_f64p, _f64p, _u32p, _f64p, _f64p, _i, _i, _i])
+        self._trace_tree64 = _bind(L, "orc_trace_tree_pixels_f64", _i, [_f64p, _f64p, _u32p, _f64p, _f64p, _i, _i, _i])
+        self._shade64 = _bind(L, "orc_shade_pixels_f64", _i,
+                              [_f64p, _f64p, _u32p, _f64p, _f64p, _f64p, _f64p, _i, _i, _i])
+        self._filter64 = _bind(L, "orc_filter_lighting_f64", _i, [_f64p, _f64p, _f64p, _f64p, _f64p, _i, _i, _i])
+        self._resolve64 = _bind(L, "orc_resolve_pixels_f64", _i, [_f64p, _f64p, _u32p, _f64p, _i, _i, _i])
+        self._make_keys = _bind(L, "orc_make_keys", _i, [_f32p, _u32p, _u32p])
+        self._sort_pass = _bind(L, "orc_sort_pass", _i, [_u32p, _u32p, _i, _i])
+        self._sort_pairs = _bind(L, "orc_sort_pairs", _i, [_u32p, _u32p, _i])
+        self._make_bounds = _bind(L, "orc_make_bounds", _i, [_f32p, _u32p, _f32p])
+        self._reduce_bounds = _bind(L, "orc_reduce_bounds", _i, [_f32p, _i])
+        self._make_prims = _bind(L, "orc_make_primitives", _i, [_f32p, _f32p, _i])
+        self._exhaustive = _bind(L, "orc_exhaustive_rays", _i, [_f32p, _f32p, _f32p, _i, _f, _f32p])
+        self._spread = _bind(L, "orc_spread", ct.c_uint, [ct.c_uint])
+        self._band = _bind(L, "orc_band_weight", _f, [_f, _f])
+        self._obj_t = _bind(L, "orc_object_t", _f, [_f32p, _i, _f32p, _f32p, _f, _f, _f, _i])
+        self._sphere_t = _bind(L, "orc_sphere_t", _f, [_f32p, _f32p, _f32p, _f])
+        self._tube_t = _bind(L, "orc_tube_t", _f, [_f32p, _f32p, _f32p, _f32p, _f, _f])
+        self._aces = _bind(L, "orc_aces", _f, [_f])
+        self._forest = _bind(L, "orc_forest_table", _i, [_i, _f32p])
+
+    # -- FP64 run: float inputs promoted to double, arithmetic in double --
+    def trace64(self, s, C, w, h, threads=0):
+        Hit = np.zeros(w * h * 4, np.float64)
+        self._trace64(s.P.astype(np.float64), s.Nodes.astype(np.float64), s.Order, C.astype(np.float64), Hit, w, h,
+                      threads)
+        return Hit
+
+    def trace_tree64(self, s, C, w, h, threads=0):
+        Hit = np.zeros(w * h * 4, np.float64)
+        self._trace_tree64(s.P.astype(np.float64), s.Nodes.astype(np.float64), s.Order, C.astype(np.float64), Hit,
+                           w, h, threads)
+        return Hit
+
+    def shade64(self, s, C, Hit, w, h, threads=0):
+        Linear = np.zeros(w * h * 4, np.float64)
+        Guide = np.zeros(w * h * 4, np.float64)
+        self._shade64(s.P.astype(np.float64), s.Nodes.astype(np.float64), s.Order, C.astype(np.float64),
+                      np.ascontiguousarray(Hit, np.float64), Linear, Guide, w, h, threads)
+        return Linear, Guide
+
+    def filter64(self, Linear, Guide, Hit, C, w, h, threads=0):
+        out = np.zeros(w * h * 4, np.float64)
+        self._filter64(np.ascontiguousarray(Linear, np.float64), np.ascontiguousarray(Guide, np.float64),
+                       np.ascontiguousarray(Hit, np.float64), out, C.astype(np.float64), w, h, threads)
+        return out
+
+    def make_keys(self, P):
+        k = np.zeros(CAPACITY, np.uint32)
+        o = np.zeros(CAPACITY, np.uint32)
+        self._make_keys(P, k, o)
+        return k, o
+
+    def sort_pass(self, k, o, distance, span):
+        self._sort_pass(k, o, distance, span)
+
+    def sort_pairs(self, k, o):
+        self._sort_pairs(k, o, k.size)
+
+    def make_bounds(self, P, Order, Nodes):
+        self._make_bounds(P, Order, Nodes)
+
+    def reduce_bounds(self, Nodes, first):
+        self._reduce_bounds(Nodes, first)
+
+    def make_primitives(self, B, seed):
+        P = np.zeros(CAPACITY * PS, np.float32)
+        self._make_prims(B, P, seed)
+        return P
+
+    def exhaustive_rays(self, P, ro, rd, cone=0.0005):
+        ro = np.ascontiguousarray(ro, np.float32).reshape(-1)
+        rd = np.ascontiguousarray(rd, np.float32).reshape(-1)
+        n = ro.size // 3
+        out = np.zeros(2 * n, np.float32)
+        self._exhaustive(P, ro, rd, n, cone, out)
+        return out.reshape(n, 2)
+
+    def spread(self, x):
+        return self._spread(x)
+
+    def band_weight(self, period, fp):
+        return self._band(period, fp)
+
+    def object_t(self, P, oid, ro, rd, fp, time=0.0, wind=0.0, refine=True):
+        return self._obj_t(P, oid, np.asarray(ro, np.float32), np.asarray(rd, np.float32), fp, time, wind, int(refine))
+
+    def sphere_t(self, ro, rd, c, r):
+        return self._sphere_t(np.asarray(ro, np.float32), np.asarray(rd, np.float32), np.asarray(c, np.float32), r)
+
+    def tube_t(self, ro, rd, a, b, ra, rb):
+        return self._tube_t(*(np.asarray(v, np.float32) for v in (ro, rd, a, b)), ra, rb)
+
+    def aces(self, x):
+        return self._aces(x)
+
+    def forest_table(self, world_seed):
+        out = np.zeros(324 * 7, np.float32)
+        self._forest(world_seed, out)
+        return out.reshape(324, 7)
+
+
+class RefHarness(_Common):
+    """The reference harness compiled from its sources (oracle/_ref/libarborref.so)."""
+
+    _suf = ""
+
+    def __init__(self, path=REF_SO):
+        self.lib = ct.CDLL(path)
+        self._setup_common("ref_")
+        self._rnd = _bind(self.lib, "ref_rnd", _f, [_i, _i])
+        self._rays = _bind(self.lib, "ref_harness_rays", _i, [_f32p, _f32p])
+
+    def rnd(self, n, seed):
+        return self._rnd(n, seed)
+
+    def harness_rays(self):
+        ro = np.zeros(300, np.float32)
+        rd = np.zeros(300, np.float32)
+        self._rays(ro, rd)
+        return ro.reshape(100, 3), rd.reshape(100, 3)
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def fnv1a64(arr):
+    """FNV-1a 64-bit over the raw little-endian bytes of an array."""
+    data = np.ascontiguousarray(arr).view(np.uint8)
+    h = np.uint64(0xCBF29CE484222325)
+    prime = np.uint64(0x100000001B3)
+    # vectorised in chunks is not possible for FNV; do it in Python over bytes
+    hv = int(h)
+    for b in data.tobytes():
+        hv ^= b
+        hv = (hv * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return "%016x" % hv
+
+
+def digest(arr):
+    """Fast content digest (sha256 of raw bytes), used by the golden fixtures."""
+    import hashlib
+
+    return hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest()
