@@ -1,0 +1,109 @@
+// Procedural materials and lighting of the ARBOR render path (Arbor.cu:256-289),
+// evaluated per pixel from a loaded primitive record.
+#pragma once
+#include "arbor_trace.hpp"
+
+namespace arbor {
+
+AD v3 sky(v3 rd, v3 sun) {
+  float h = sat(rd.y);
+  v3 c = mix(mk(0.75f, 0.79f, 0.77f), mk(0.19f, 0.35f, 0.54f), powf(h, 0.48f));
+  float warm = powf(sat(dot(rd, sun)), 32.0f);
+  c = c + mk(0.48f, 0.26f, 0.075f) * warm;
+  float disc = smooth(0.99996f, 0.999984f, dot(rd, sun));
+  c = c + mk(8.0f, 6.7f, 4.9f) * disc;
+  float clouds = noise(rd.x * 5.0f / (0.2f + h), rd.z * 5.0f / (0.2f + h), 231);
+  float veil = smooth(0.62f, 0.81f, clouds) * smooth(0.02f, 0.18f, h) * 0.38f;
+  return mix(c, mk(1.1f, 1.10f, 1.05f), veil);
+}
+__device__ inline v3 barkColor(const Prim& pr, v3 p, float footprint) {
+  int seed = (int)pr.seed();
+  v2 uv = woodUV(pr, p);
+  v3 coord = mk(cosf(uv.x) * 9.0f, sinf(uv.x) * 9.0f, uv.y * 7.0f);
+  float mottle = noise3(p * 3.0f, 714);
+  float rough = noise3(coord, seed);
+  v3 d = pr.b() - pr.a();
+  float t = sat(dot(p - pr.a(), d) / fmaxf(0.000001f, dot(d, d)));
+  float amp = fminf(0.021f, lerp(pr.ra(), pr.rb(), t) * 0.055f);
+  float cut = barkCut(pr, p, footprint) / fmaxf(0.00001f, amp);
+  v3 c = mix(mk(0.10f, 0.075f, 0.050f), mk(0.25f, 0.21f, 0.16f), 0.25f + rough * 0.62f);
+  c = c * (0.75f + 0.35f * mottle);
+  c = c * (1.0f - 0.62f * smooth(0.28f, 0.85f, cut));
+  float moss = smooth(0.52f, 0.72f, noise3(p * 5.0f, 189)) * (1.0f - smooth(0.15f, 1.4f, p.y));
+  c = mix(c, mk(0.045f, 0.064f, 0.018f), moss * 0.62f);
+  float lichen = smooth(0.65f, 0.77f, noise3(p * 23.0f, 27)) * smooth(0.25f, 0.65f, noise3(p * 90.0f, 712)) *
+                 bandWeight(0.018f, footprint);
+  c = mix(c, mk(0.31f, 0.33f, 0.23f), lichen * 0.5f);
+  float grain = (noise3(coord * 15.0f, seed) - 0.5f) * bandWeight(0.0025f, footprint);
+  return c * (1.0f + grain * 0.38f);
+}
+AD float leafVein(const Prim& pr, v3 u, v3 p, float footprint) {
+  v3 q = p - pr.a();
+  float x = dot(q, u) - pr.f16();
+  float y = dot(q, pr.c());
+  float len = pr.ra();
+  float mid = expf(-fabsf(y) / fmaxf(0.00065f, footprint * 0.55f));
+  float fy = frac((x - fabsf(y) * 0.70f) / len * 7.0f);
+  float dd = fminf(fy, 1.0f - fy) * len / 7.0f;
+  float lateral = expf(-dd / fmaxf(0.00033f, footprint * 0.5f));
+  return sat(mid * 0.82f + lateral * 0.48f) * bandWeight(0.003f, footprint);
+}
+AD v3 leafColor(const Prim& pr, v3 u, v3 p, float fp) {
+  float h = rnd((int)pr.seed(), 887);
+  v3 q = p - pr.a();
+  float x = dot(q, u) - pr.f16();
+  if (x < 0.0f) return mk(0.17f, 0.15f, 0.048f);
+  float y = dot(q, pr.c());
+  float v = leafVein(pr, u, p, fp);
+  v3 c = mix(mk(0.035f, 0.11f, 0.008f), mk(0.15f, 0.255f, 0.025f), h);
+  c = mix(c, mk(0.22f, 0.26f, 0.055f), v * 0.57f);
+  float speck = (noise(x * 700.0f, y * 700.0f, (int)pr.seed()) - 0.5f) * bandWeight(0.0017f, fp);
+  float edge = smooth(0.78f, 1.0f, fabsf(y) / fmaxf(0.001f, leafWidth(x, pr.ra(), pr.rb(), fp)));
+  c = c * (1.0f + speck * 0.27f);
+  c = mix(c, mk(0.23f, 0.20f, 0.05f), edge * 0.22f);
+  return c;
+}
+AD v3 groundColor(v3 p, float fp) {
+  float a = noise(p.x * 3.0f, p.z * 3.0f, 819);
+  float b = noise(p.x * 15.0f, p.z * 15.0f, 28);
+  float r = sqrtf(p.x * p.x + p.z * p.z);
+  float soil = 1.0f - smooth(4.1f, 6.8f, r + noise(p.x, p.z, 13));
+  v3 grass = mix(mk(0.085f, 0.115f, 0.035f), mk(0.23f, 0.24f, 0.092f), a);
+  v3 dirt = mix(mk(0.075f, 0.055f, 0.035f), mk(0.16f, 0.13f, 0.080f), b);
+  float moss = smooth(0.53f, 0.73f, a);
+  dirt = mix(dirt, mk(0.049f, 0.079f, 0.022f), moss * 0.72f);
+  float grit = (noise(p.x * 200.0f, p.z * 200.0f, 127) - 0.5f) * bandWeight(0.007f, fp);
+  return mix(grass, dirt, soil) * (1.0f + grit * 0.40f);
+}
+AD float specGGX(v3 n, v3 v, v3 l, float rough) {
+  v3 h = unit(v + l);
+  float nh = sat(dot(n, h));
+  float nv = fmaxf(0.002f, sat(dot(n, v)));
+  float nl = sat(dot(n, l));
+  float vh = sat(dot(v, h));
+  float a = rough * rough, aa = a * a;
+  float den = nh * nh * (aa - 1.0f) + 1.0f;
+  float D = aa / (kPi * den * den);
+  float k = (rough + 1.0f) * (rough + 1.0f) * 0.125f;
+  float G = nv / (nv * (1.0f - k) + k) * nl / (nl * (1.0f - k) + k);
+  float F = 0.035f + 0.965f * powf(1.0f - vh, 5.0f);
+  return D * G * F / (4.0f * nv + 0.0001f);
+}
+// Up to four shadow traversals through leaves (Arbor.cu:271-274).
+__device__ inline v3 lightVisibility(const float* P, const float4* Nodes4, const uint32_t* Order, const TreeInst* trees,
+                                     v3 p, v3 direction, float time, float wind) {
+  v3 transmission = mk(1.0f, 1.0f, 1.0f);
+  v3 ro = p;
+  for (int j = 0; j < 4; j++) {
+    int tree = -1;
+    v2 h = traceForest(P, Nodes4, Order, trees, ro, direction, 0.0001f, time, wind, 36.0f, &tree, true);
+    int id = (int)h.y;
+    if (id < 0) break;
+    if (P[id * kPS + 3] != 1.0f) return mk(0.0f, 0.0f, 0.0f);
+    transmission = transmission * mk(0.18f, 0.29f, 0.065f);
+    ro = ro + direction * (h.x + 0.0018f);
+  }
+  return transmission;
+}
+
+}  // namespace arbor

@@ -1,0 +1,361 @@
+// Ray queries of the ARBOR render path on gfx950: primitive records, surface
+// fields, exact intersections, the stackless heap-BVH walk and the forest DDA.
+//
+// Arithmetic follows Arbor.cu expression by expression (cited per function).
+// Structure differs: a primitive's 20 used floats are loaded once as five
+// float4 (the reference re-reads P inside every helper), the ray reciprocal is
+// hoisted out of the box tests, and the traversal stack is a 16-bit mask over
+// heap depths instead of stack[32] (see traceTree below).
+#pragma once
+#include "arbor_device.hpp"
+
+namespace arbor {
+
+// One primitive record, floats 0..19 of its 24 (ARCHITECTURE.md:13).
+struct Prim {
+  float4 q0, q1, q2, q3, q4;
+  AD v3 a() const { return mk(q0.x, q0.y, q0.z); }   // wood start / leaf anchor / stone centre
+  AD float type() const { return q0.w; }
+  AD v3 b() const { return mk(q1.x, q1.y, q1.z); }   // wood end / leaf u / stone half-axes
+  AD float ra() const { return q1.w; }                // wood radius at a / leaf length
+  AD v3 c() const { return mk(q2.x, q2.y, q2.z); }   // wood transported axis / leaf v
+  AD float rb() const { return q2.w; }                // wood radius at b / leaf half-width
+  AD v3 n() const { return mk(q3.x, q3.y, q3.z); }   // leaf normal
+  AD float seed() const { return q3.w; }
+  AD float f16() const { return q4.x; }               // wood arc offset / leaf petiole
+  AD float f17() const { return q4.y; }               // wood curve id / leaf camber
+  AD float f18() const { return q4.z; }               // wood curve radius / leaf twist
+};
+AD Prim loadPrim(const float* P, int id) {
+  const float4* r = reinterpret_cast<const float4*>(P + id * kPS);
+  Prim p;
+  p.q0 = r[0]; p.q1 = r[1]; p.q2 = r[2]; p.q3 = r[3]; p.q4 = r[4];
+  return p;
+}
+// boundLow/boundHigh (Arbor.cu:132-133) from a loaded record.
+AD void primBounds(const Prim& p, v3& lo, v3& hi) {
+  int type = (int)p.type();
+  if (type < 0) { lo = mk(kFar, kFar, kFar); hi = mk(-kFar, -kFar, -kFar); return; }
+  v3 a = p.a();
+  if (type == 0) {
+    float r = fmaxf(p.ra(), p.rb()) + 0.002f;
+    lo = vmin(a, p.b()) - mk(r, r, r);
+    hi = vmax(a, p.b()) + mk(r, r, r);
+    return;
+  }
+  if (type == 2) { lo = a - p.b(); hi = a + p.b(); return; }
+  float radius = p.ra() + p.f16() + 0.025f;
+  lo = a - mk(radius, radius, radius);
+  hi = a + mk(radius, radius, radius);
+}
+
+// ---- wood surface (Arbor.cu:148-170) ----
+AD v2 woodUV(const Prim& pr, v3 p) {
+  v3 a = pr.a();
+  v3 d = unit(pr.b() - a);
+  v3 initial = pr.c();
+  v3 u = unit(initial - d * dot(initial, d));
+  v3 v = cross(d, u);
+  v3 q = p - a;
+  float z = dot(q, d);
+  float theta = atan2f(dot(q, v), dot(q, u));
+  return {theta, pr.f16() + z};
+}
+__device__ inline float barkCut(const Prim& pr, v3 p, float footprint) {
+  v3 a = pr.a();
+  v3 d = pr.b() - a;
+  float along = sat(dot(p - a, d) / fmaxf(0.000001f, dot(d, d)));
+  float radius = lerp(pr.ra(), pr.rb(), along);
+  float amp = fminf(0.021f, radius * 0.055f);
+  int columns = (int)fmaxf(7.0f, floorf(pr.f18() * 2.0f * kPi / 0.065f + 0.5f));
+  float period = radius * 2.0f * kPi / (float)columns;
+  float coarse = bandWeight(period, footprint);
+  float fine = bandWeight(0.010f, footprint);
+  float grain = bandWeight(0.0025f, footprint);
+  if (coarse < 0.000001f && fine < 0.000001f) return amp * 0.32f;
+  v2 uv = woodUV(pr, p);
+  float theta = uv.x, arc = uv.y;
+  int seed = (int)pr.seed();
+  float st = sinf(theta), ct = cosf(theta);
+  float x = (theta / (2.0f * kPi) + 0.5f) * (float)columns + 0.32f * sinf(arc * 6.7f + ct * 3.0f) +
+            0.15f * sinf(arc * 19.0f + st * 2.0f);
+  float y = arc / 0.19f + 0.25f * sinf(theta * 7.0f) + 0.1f * sinf(theta * 19.0f);
+  int ix = (int)floorf(x), iy = (int)floorf(y);
+  float first = 100.0f, second = 100.0f;
+  for (int j = -1; j <= 1; j++)
+    for (int k = -1; k <= 1; k++) {
+      int cell = ix + k;
+      int wrapped = ((cell % columns) + columns) % columns;
+      float xx = (float)cell + 0.18f + 0.64f * hash2(wrapped, iy + j, seed) - x;
+      float yy = (float)(iy + j) + 0.18f + 0.64f * hash2(wrapped + columns, iy + j, seed + 31) - y;
+      float dd = xx * xx + yy * yy;
+      if (dd < first) { second = first; first = dd; }
+      else if (dd < second) second = dd;
+    }
+  float gap = sqrtf(second) - sqrtf(first);
+  float fissure = 1.0f - smooth(0.015f, 0.17f, gap);
+  float plates = fissure * fissure;
+  v3 coord = mk(ct * 9.0f, st * 9.0f, arc * 23.0f);
+  float splinter = noise3(coord, seed);
+  float pores = noise3(coord * 4.0f, seed + 731);
+  return amp * clampf(0.32f + coarse * (plates - 0.37f) * 0.69f + fine * (splinter - 0.5f) * 0.24f +
+                          grain * (pores - 0.5f) * 0.08f,
+                      0.005f, 1.0f);
+}
+__device__ inline float woodField(const Prim& pr, v3 p, float footprint) {
+  v3 a = pr.a();
+  v3 ab = pr.b() - a;
+  float t = sat(dot(p - a, ab) / fmaxf(0.0000001f, dot(ab, ab)));
+  float radius = lerp(pr.ra(), pr.rb(), t);
+  return length(p - a - ab * t) - radius + barkCut(pr, p, footprint);
+}
+
+// ---- leaf frame (Arbor.cu:171-172) ----
+struct LeafFrame { v3 u, n; };
+AD LeafFrame leafFrame(const Prim& pr, float time, float wind) {
+  LeafFrame f;
+  // Breeze off: the rotation angle is wind*(...) = 0, cos = 1 and sin = 0, so
+  // the rotated basis equals the stored one.
+  if (wind == 0.0f) { f.u = pr.b(); f.n = pr.n(); return f; }
+  float a = windAngle(pr.seed(), time, wind);
+  float ca = cosf(a), sa = sinf(a);
+  f.u = pr.b() * ca + pr.n() * sa;
+  f.n = pr.n() * ca - pr.b() * sa;
+  return f;
+}
+
+// ---- exact intersections (Arbor.cu:188-219) ----
+__device__ inline float tubeT(v3 ro, v3 rd, v3 a, v3 b, float ra, float rb) {
+  v3 axis = b - a;
+  float len = length(axis);
+  axis = axis / fmaxf(len, 0.000001f);
+  v3 q = ro - a;
+  float z = dot(q, axis), dz = dot(rd, axis);
+  float k = (rb - ra) / fmaxf(0.000001f, len);
+  v3 v = q - axis * z, d = rd - axis * dz;
+  float radius = ra + k * z;
+  float aa = dot(d, d) - k * k * dz * dz;
+  float bb = dot(v, d) - radius * k * dz;
+  float cc = dot(v, v) - radius * radius;
+  float best = fminf(sphereT(ro, rd, a, ra), sphereT(ro, rd, b, rb));
+  float disc = bb * bb - aa * cc;
+  if (disc >= 0.0f && fabsf(aa) > 0.0000001f) {
+    float sq = sqrtf(disc);
+    for (int j = 0; j < 2; j++) {
+      float t = (-bb + (j == 0 ? -1.0f : 1.0f) * sq) / aa;
+      float h = z + t * dz;
+      if (t > 0.0001f && h >= 0.0f && h <= len) best = fminf(best, t);
+    }
+  }
+  return best;
+}
+__device__ inline float woodT(const Prim& pr, v3 ro, v3 rd, float footprint, bool refine) {
+  v3 a = pr.a(), end = pr.b();
+  if (!refine) {
+    float ra = pr.ra() - fminf(0.021f, pr.ra() * 0.055f) * 0.32f;
+    float rb = pr.rb() - fminf(0.021f, pr.rb() * 0.055f) * 0.32f;
+    return tubeT(ro, rd, a, end, ra, rb);
+  }
+  float t = tubeT(ro, rd, a, end, pr.ra(), pr.rb());
+  if (t >= kFar) return kFar;
+  v3 lo, hi;
+  primBounds(pr, lo, hi);
+  v2 span = boxSpan(ro, rd, lo, hi);
+  float eps = fmaxf(0.000025f, fminf(0.00025f, footprint * 0.13f));
+  for (int k = 0; k < 64; k++) {
+    float f = woodField(pr, ro + rd * t, footprint);
+    if (f < eps) return t;
+    t += fmaxf(eps * 0.35f, f * 0.27f);
+    if (t > span.y) return kFar;
+  }
+  return kFar;
+}
+__device__ inline float leafT(const Prim& pr, v3 ro, v3 rd, float footprint, float time, float wind) {
+  v3 origin = pr.a();
+  LeafFrame fr = leafFrame(pr, time, wind);
+  v3 u = fr.u, v = pr.c(), n = fr.n;
+  float stem = pr.f16(), len = pr.ra(), width = pr.rb();
+  float curl = pr.f17(), twist = pr.f18();
+  v3 q = ro - origin;
+  float x = dot(q, u) - stem, y = dot(q, v), z = dot(q, n);
+  float dx = dot(rd, u), dy = dot(rd, v), dz = dot(rd, n);
+  float best = tubeT(ro, rd, origin, origin + u * stem, 0.0011f, 0.00065f);
+  v2 span = boxSpan(mk(x, y, z), mk(dx, dy, dz), mk(-0.001f, -width, -0.035f), mk(len + 0.001f, width, 0.065f));
+  if (span.y < fmaxf(0.0001f, span.x)) return best;
+  float t = fabsf(dz) > 0.000001f ? -z / dz : (span.x + span.y) * 0.5f;
+  t = clampf(t, fmaxf(0.0001f, span.x), span.y);
+  // leafHeight(u, v) = curl*s(u) + tw(u)*v + w2(v)*s(u) + m(v)*s(u)*band with
+  // s, tw functions of u only and w2, m of v only; the five evaluations of a
+  // Newton step share them (3 sines and 3 exponentials instead of 5 + 5),
+  // each term still computed exactly as leafHeight computes it.
+  const float band = bandWeight(0.007f, footprint);
+  const float hw2 = width * width;
+  for (int k = 0; k < 7; k++) {
+    float xx = x + dx * t, yy = y + dy * t;
+    const float e = 0.00015f;
+    LeafU u0 = leafUTerms(xx, len, twist), up = leafUTerms(xx + e, len, twist), um = leafUTerms(xx - e, len, twist);
+    LeafV v0 = leafVTerms(yy, hw2), vp = leafVTerms(yy + e, hw2), vm = leafVTerms(yy - e, hw2);
+    float height = leafHeightOf(u0, v0, curl, band);
+    float f = z + dz * t - height;
+    float hx = (leafHeightOf(up, v0, curl, band) - leafHeightOf(um, v0, curl, band)) / (2.0f * e);
+    float hy = (leafHeightOf(u0, vp, curl, band) - leafHeightOf(u0, vm, curl, band)) / (2.0f * e);
+    float deriv = dz - hx * dx - hy * dy;
+    if (fabsf(deriv) < 0.00001f) break;
+    t -= clampf(f / deriv, -0.13f, 0.13f);
+    t = clampf(t, span.x, span.y);
+  }
+  float xx = x + dx * t, yy = y + dy * t;
+  float err = fabsf(z + dz * t - leafHeightOf(leafUTerms(xx, len, twist), leafVTerms(yy, hw2), curl, band));
+  // same predicate as the reference, cheapest terms first
+  if (t > 0.0001f && xx > 0.0f && xx < len && err < 0.0003f && fabsf(yy) < leafWidth(xx, len, width, footprint))
+    best = fminf(best, t);
+  return best;
+}
+__device__ inline float objectT(const Prim& pr, v3 ro, v3 rd, float footprint, float time, float wind, bool refine) {
+  int type = (int)pr.type();
+  if (type < 0) return kFar;
+  if (type == 0) return woodT(pr, ro, rd, footprint, refine);
+  if (type == 1) return leafT(pr, ro, rd, footprint, time, wind);
+  v3 h = pr.b();
+  v3 q = ro - pr.a();
+  v3 p = mk(q.x / h.x, q.y / h.y, q.z / h.z);
+  v3 d = mk(rd.x / h.x, rd.y / h.y, rd.z / h.z);
+  float aa = dot(d, d), bb = dot(p, d), cc = dot(p, p) - 1.0f;
+  float disc = bb * bb - aa * cc;
+  if (disc < 0.0f) return kFar;
+  float t = (-bb - sqrtf(disc)) / aa;
+  return t > 0.0001f ? t : kFar;
+}
+
+// ---- heap-BVH walk (Arbor.cu:220-230) ----
+// Node n's record is two float4 {lo, count} {hi, id}; children 2n, 2n+1 are
+// adjacent, so an interior visit reads one 64-byte pair.
+struct NodeBox { float4 lo, hi; };
+AD bool spanHit(v2 s, float count, float best) { return count > 0.0f && s.y >= fmaxf(0.0001f, s.x) && s.x < best; }
+AD int heapDepth(uint32_t node) { return 31 - __builtin_clz(node); }
+
+// Same visiting order and the same (best, id) as the reference's stack[32]
+// loop. The reference pushes the far child only when both children are hit
+// and always takes the nearer one (ties -> right); the stack is therefore
+// always "the far siblings along the current root path", deepest on top. A bit
+// per heap depth records them: popping takes the deepest set bit d and goes
+// to the sibling of the current path's depth-d ancestor. A popped node is
+// re-tested against the current best exactly as the reference does; a node
+// reached directly reuses the span its parent computed (identical values).
+__device__ inline v2 traceTree(const float* P, const float4* Nodes4, const uint32_t* Order, v3 ro, v3 rd, float cone,
+                               float time, float wind, float best, int ignore, bool shadows) {
+  const v3 inv = invDir(rd);
+  int found = -1;
+  uint32_t node = 1, pending = 0;
+  float4 rl = Nodes4[2], rh = Nodes4[3];
+  v2 span = boxSpanInv(ro, inv, mk(rl.x, rl.y, rl.z), mk(rh.x, rh.y, rh.z));
+  bool live = spanHit(span, rl.w, best);
+  while (true) {
+    if (live) {
+      if (node < (uint32_t)kCapacity) {
+        const float4* c = Nodes4 + node * 4;  // children 2n, 2n+1
+        float4 l0 = c[0], l1 = c[1], r0 = c[2], r1 = c[3];
+        v2 ls = boxSpanInv(ro, inv, mk(l0.x, l0.y, l0.z), mk(l1.x, l1.y, l1.z));
+        v2 rs = boxSpanInv(ro, inv, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z));
+        bool lh = spanHit(ls, l0.w, best), rhit = spanHit(rs, r0.w, best);
+        if (lh || rhit) {
+          bool takeLeft = lh && (!rhit || ls.x < rs.x);
+          if (lh && rhit) pending |= 1u << (heapDepth(node) + 1);
+          node = node * 2 + (takeLeft ? 0u : 1u);
+          span = takeLeft ? ls : rs;
+          continue;
+        }
+      } else {
+        int id = (int)Order[node - kCapacity];
+        if (id != ignore) {
+          float footprint = fmaxf(0.000005f, cone * fmaxf(0.05f, span.x));
+          Prim pr = loadPrim(P, id);
+          float t = objectT(pr, ro, rd, footprint, time, wind, !shadows);
+          if (t < best) { best = t; found = id; }
+        }
+      }
+    }
+    if (pending == 0) break;
+    int d = 31 - __builtin_clz(pending);
+    pending &= ~(1u << d);
+    node = (node >> (heapDepth(node) - d)) ^ 1u;
+    float4 b0 = Nodes4[node * 2], b1 = Nodes4[node * 2 + 1];
+    span = boxSpanInv(ro, inv, mk(b0.x, b0.y, b0.z), mk(b1.x, b1.y, b1.z));
+    live = spanHit(span, b0.w, best);
+  }
+  return {best, (float)found};
+}
+
+// ---- forest DDA (Arbor.cu:235-251) ----
+// `trees` holds the 324 instancing records (LDS); every value is what the
+// reference recomputes inline per cell.
+__device__ inline v2 traceForest(const float* P, const float4* Nodes4, const uint32_t* Order, const TreeInst* trees,
+                                 v3 ro, v3 rd, float cone, float time, float wind, float best, int* treeOut,
+                                 bool shadows) {
+  int foundTree = -1;
+  float found = -1.0f;
+  const float half = (float)kSide * kSpacing * 0.5f;
+  v2 worldSpan = boxSpan(ro, rd, mk(-half, -1.2f, -half), mk(half, 13.5f, half));
+  float enter = fmaxf(0.0f, worldSpan.x), exit = fminf(best, worldSpan.y);
+  if (exit < enter) { *treeOut = -1; return {best, -1.0f}; }
+  v3 pos = ro + rd * (enter + 0.0002f);
+  int ix = (int)floorf(pos.x / kSpacing + (float)kSide * 0.5f + 0.5f);
+  int iz = (int)floorf(pos.z / kSpacing + (float)kSide * 0.5f + 0.5f);
+  int sx = rd.x >= 0.0f ? 1 : -1, sz = rd.z >= 0.0f ? 1 : -1;
+  float invx = invSafe(rd.x), invz = invSafe(rd.z);
+  float bx = ((float)(ix - kSide / 2) + (sx > 0 ? 0.5f : -0.5f)) * kSpacing;
+  float bz = ((float)(iz - kSide / 2) + (sz > 0 ? 0.5f : -0.5f)) * kSpacing;
+  float tx = (bx - ro.x) * invx, tz = (bz - ro.z) * invz;
+  float dx = kSpacing * fabsf(invx), dz = kSpacing * fabsf(invz);
+  for (int step = 0; step < 48; step++) {
+    if (ix < 0 || iz < 0 || ix >= kSide || iz >= kSide) break;
+    int tree = iz * kSide + ix;
+    const TreeInst& ti = trees[tree];
+    float sc = ti.sc;
+    v3 centre = mk(ti.px, 0.0f, ti.pz) + mk(0.0f, 4.4f * sc, 0.0f);
+    float broad = sphereT(ro, rd, centre, 6.6f * sc);
+    if (broad < best) {
+      v3 lr = toLocalPoint(ro, ti), ld = toLocalDir(rd, ti);
+      v2 h = traceTree(P, Nodes4, Order, lr, ld, cone / sc, time + ti.phase, wind, best / sc, -1, shadows);
+      if (h.y >= 0.0f && h.x * sc < best) { best = h.x * sc; found = h.y; foundTree = tree; }
+    }
+    float next = tx < tz ? tx : tz;
+    if (next > exit || next > best) break;
+    if (tx < tz) { ix += sx; tx += dx; } else { iz += sz; tz += dz; }
+  }
+  *treeOut = foundTree;
+  return {best, found};
+}
+
+// ---- surface normal (Arbor.cu:231-234) ----
+__device__ inline v3 objectNormal(const Prim& pr, v3 p, float footprint, float time, float wind) {
+  int type = (int)pr.type();
+  if (type == 0) {
+    float e = fmaxf(0.000035f, fminf(0.0005f, footprint * 0.18f));
+    v3 dx = mk(e, 0.0f, 0.0f), dy = mk(0.0f, e, 0.0f), dz = mk(0.0f, 0.0f, e);
+    return unit(mk(woodField(pr, p + dx, footprint) - woodField(pr, p - dx, footprint),
+                   woodField(pr, p + dy, footprint) - woodField(pr, p - dy, footprint),
+                   woodField(pr, p + dz, footprint) - woodField(pr, p - dz, footprint)));
+  }
+  if (type == 1) {
+    LeafFrame fr = leafFrame(pr, time, wind);
+    v3 u = fr.u, v = pr.c(), n = fr.n;
+    v3 q = p - pr.a();
+    float x = dot(q, u) - pr.f16(), y = dot(q, v);
+    if (x < 0.0f) {
+      v3 centre = pr.a() + u * (x + pr.f16());
+      return unit(p - centre);
+    }
+    const float e = 0.0002f;
+    float len = pr.ra(), hw = pr.rb(), curl = pr.f17(), twist = pr.f18();
+    float a = (leafHeight(x + e, y, len, hw, curl, twist, footprint) - leafHeight(x - e, y, len, hw, curl, twist, footprint)) /
+              (2.0f * e);
+    float c = (leafHeight(x, y + e, len, hw, curl, twist, footprint) - leafHeight(x, y - e, len, hw, curl, twist, footprint)) /
+              (2.0f * e);
+    return unit(n - u * a - v * c);
+  }
+  v3 q = p - pr.a(), h = pr.b();
+  return unit(mk(q.x / (h.x * h.x), q.y / (h.y * h.y), q.z / (h.z * h.z)));
+}
+
+}  // namespace arbor

@@ -1,0 +1,283 @@
+// C++ host engine: the generate / resize / frame contract of the reference's
+// host (src/engine.js:27-53) over the C-ABI kernels, with owned device buffers,
+// per-stage HIP-event timing and a seeded walk-path input generator.
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+
+#include "arbor_engine.h"
+#include "launch.hpp"
+
+namespace {
+
+struct Buffers {
+  float *B = nullptr, *P = nullptr, *Nodes = nullptr, *C = nullptr, *I = nullptr;
+  unsigned *Keys = nullptr, *Order = nullptr;
+  float *Hit = nullptr, *Linear = nullptr, *Guide = nullptr, *Filtered = nullptr, *History = nullptr;
+  unsigned* Pixels = nullptr;
+};
+
+template <class T>
+int alloc(T** p, size_t bytes) {
+  return hipMalloc(reinterpret_cast<void**>(p), bytes) == hipSuccess ? ARBOR_OK : ARBOR_ERR_DEVICE;
+}
+template <class T>
+void release(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct arbor_engine {
+  Buffers b;
+  hipStream_t stream = nullptr;
+  bool owns_stream = false;
+  int width = 0, height = 0, seed = -1;
+  long frames = 0;
+  bool generated = false;
+  hipEvent_t ev[6] = {};
+  bool timing = false;
+  float stage_ms[6] = {0, 0, 0, 0, 0, 0};  // update, trace, shade, filter, resolve, whole frame
+  float input[ARBOR_I_FLOATS] = {};
+};
+
+namespace {
+
+size_t pixel_bytes(int w, int h) { return (size_t)w * h * 16; }
+
+void free_pixels(arbor_engine* e) {
+  release(e->b.Hit); release(e->b.Linear); release(e->b.Guide); release(e->b.Filtered);
+  release(e->b.History); release(e->b.Pixels);
+}
+
+}  // namespace
+
+extern "C" {
+
+void arbor_engine_round_size(int* width, int* height) {
+  // engine.js:36: width up to a multiple of 64 (min 320), height to 8 (min 192).
+  int w = ((*width + 63) / 64) * 64, h = ((*height + 7) / 8) * 8;
+  *width = w < 320 ? 320 : w;
+  *height = h < 192 ? 192 : h;
+}
+
+int arbor_engine_create(arbor_engine** out, int width, int height, int seed, int round_size, hipStream_t stream) {
+  if (!out) return ARBOR_ERR_NULL;
+  *out = nullptr;
+  if (seed < 0 || seed > 1000000) return ARBOR_ERR_ARG;
+  arbor_engine* e = new arbor_engine();
+  if (stream) e->stream = stream;
+  else {
+    if (hipStreamCreate(&e->stream) != hipSuccess) { delete e; return ARBOR_ERR_DEVICE; }
+    e->owns_stream = true;
+  }
+  int rc = ARBOR_OK;
+  rc |= alloc(&e->b.B, (size_t)ARBOR_CURVES * ARBOR_CS * 4);
+  rc |= alloc(&e->b.P, (size_t)ARBOR_CAPACITY * ARBOR_PS * 4);
+  rc |= alloc(&e->b.Keys, (size_t)ARBOR_CAPACITY * 4);
+  rc |= alloc(&e->b.Order, (size_t)ARBOR_CAPACITY * 4);
+  rc |= alloc(&e->b.Nodes, (size_t)ARBOR_NODE_COUNT * 8 * 4);
+  rc |= alloc(&e->b.C, ARBOR_C_FLOATS * 4);
+  rc |= alloc(&e->b.I, ARBOR_I_FLOATS * 4);
+  for (int i = 0; i < 6 && rc == ARBOR_OK; i++)
+    if (hipEventCreate(&e->ev[i]) != hipSuccess) rc = ARBOR_ERR_DEVICE;
+  if (rc == ARBOR_OK) {
+    // B, Nodes and I start zeroed as WebGPU buffers do (engine.js:22).
+    if (hipMemsetAsync(e->b.B, 0, (size_t)ARBOR_CURVES * ARBOR_CS * 4, e->stream) != hipSuccess ||
+        hipMemsetAsync(e->b.Nodes, 0, (size_t)ARBOR_NODE_COUNT * 32, e->stream) != hipSuccess ||
+        hipMemsetAsync(e->b.I, 0, ARBOR_I_FLOATS * 4, e->stream) != hipSuccess)
+      rc = ARBOR_ERR_DEVICE;
+  }
+  if (rc == ARBOR_OK) rc = arbor_engine_resize(e, width, height, round_size);
+  if (rc == ARBOR_OK) rc = arbor_engine_generate(e, seed);
+  if (rc != ARBOR_OK) {
+    arbor_engine_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return ARBOR_OK;
+}
+
+int arbor_engine_destroy(arbor_engine* e) {
+  if (!e) return ARBOR_ERR_NULL;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  free_pixels(e);
+  release(e->b.B); release(e->b.P); release(e->b.Keys); release(e->b.Order); release(e->b.Nodes);
+  release(e->b.C); release(e->b.I);
+  for (auto& v : e->ev)
+    if (v) (void)hipEventDestroy(v);
+  if (e->owns_stream && e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return ARBOR_OK;
+}
+
+int arbor_engine_generate(arbor_engine* e, int seed) {
+  if (!e) return ARBOR_ERR_NULL;
+  if (seed < 0 || seed > 1000000) return ARBOR_ERR_ARG;  // engine.js:28
+  int rc = arbor_generate(e->b.B, e->b.P, e->b.Keys, e->b.Order, e->b.Nodes, e->b.C, seed, e->b.History, e->width,
+                          e->height, e->stream);
+  if (rc == ARBOR_OK) {
+    e->seed = seed;
+    e->generated = true;
+  }
+  return rc;
+}
+
+int arbor_engine_resize(arbor_engine* e, int width, int height, int round_size) {
+  if (!e) return ARBOR_ERR_NULL;
+  if (round_size) arbor_engine_round_size(&width, &height);
+  if (!arbor::dims_ok(width, height)) return ARBOR_ERR_ARG;
+  if (width == e->width && height == e->height && e->b.Hit) return ARBOR_OK;
+  (void)hipStreamSynchronize(e->stream);
+  free_pixels(e);
+  size_t n = pixel_bytes(width, height);
+  int rc = ARBOR_OK;
+  rc |= alloc(&e->b.Hit, n);
+  rc |= alloc(&e->b.Linear, n);
+  rc |= alloc(&e->b.Guide, n);
+  rc |= alloc(&e->b.Filtered, n);
+  rc |= alloc(&e->b.History, n);
+  rc |= alloc(&e->b.Pixels, (size_t)width * height * 4);
+  if (rc != ARBOR_OK) {
+    free_pixels(e);
+    e->width = e->height = 0;
+    return ARBOR_ERR_DEVICE;
+  }
+  if (hipMemsetAsync(e->b.History, 0, n, e->stream) != hipSuccess) return ARBOR_ERR_DEVICE;
+  e->width = width;
+  e->height = height;
+  return ARBOR_OK;
+}
+
+int arbor_engine_reset_camera(arbor_engine* e) {
+  if (!e) return ARBOR_ERR_NULL;
+  if (!e->generated) return ARBOR_ERR_STATE;
+  return arbor_initCamera(e->b.C, e->b.P, e->seed, e->stream);
+}
+
+int arbor_engine_set_input(arbor_engine* e, const float* input32) {
+  if (!e || !input32) return ARBOR_ERR_NULL;
+  memcpy(e->input, input32, sizeof(e->input));
+  return ARBOR_OK;
+}
+
+int arbor_engine_set_timing(arbor_engine* e, int on) {
+  if (!e) return ARBOR_ERR_NULL;
+  e->timing = on != 0;
+  return ARBOR_OK;
+}
+
+// One frame: updateCamera -> tracePixels -> shadePixels -> filterLighting ->
+// resolvePixels, five dispatches (engine.js:45-53). With device_input NULL
+// the host input set by arbor_engine_set_input is uploaded first.
+int arbor_engine_frame(arbor_engine* e, float dt, const float* device_input) {
+  if (!e) return ARBOR_ERR_NULL;
+  if (!e->generated || !e->b.Hit) return ARBOR_ERR_STATE;
+  const float* I = device_input;
+  if (!I) {
+    if (hipMemcpyAsync(e->b.I, e->input, sizeof(e->input), hipMemcpyHostToDevice, e->stream) != hipSuccess)
+      return ARBOR_ERR_DEVICE;
+    I = e->b.I;
+  }
+  float delta = dt < 0.0f ? 0.0f : (dt > 0.1f ? 0.1f : dt);
+  const int w = e->width, h = e->height;
+  int rc;
+  if (e->timing) (void)hipEventRecord(e->ev[0], e->stream);
+  if ((rc = arbor_updateCamera(e->b.C, I, e->b.P, delta, w, h, e->stream)) != ARBOR_OK) return rc;
+  if (e->timing) (void)hipEventRecord(e->ev[1], e->stream);
+  if ((rc = arbor_tracePixels(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, w, h, e->stream)) != ARBOR_OK) return rc;
+  if (e->timing) (void)hipEventRecord(e->ev[2], e->stream);
+  if ((rc = arbor_shadePixels(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Linear, e->b.Guide, w, h,
+                              e->stream)) != ARBOR_OK)
+    return rc;
+  if (e->timing) (void)hipEventRecord(e->ev[3], e->stream);
+  if ((rc = arbor_filterLighting(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.C, w, h, e->stream)) !=
+      ARBOR_OK)
+    return rc;
+  if (e->timing) (void)hipEventRecord(e->ev[4], e->stream);
+  if ((rc = arbor_resolvePixels(e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w, h, e->stream)) != ARBOR_OK)
+    return rc;
+  if (e->timing) {
+    (void)hipEventRecord(e->ev[5], e->stream);
+    if (hipEventSynchronize(e->ev[5]) != hipSuccess) return ARBOR_ERR_DEVICE;
+    for (int s = 0; s < 5; s++) (void)hipEventElapsedTime(&e->stage_ms[s], e->ev[s], e->ev[s + 1]);
+    (void)hipEventElapsedTime(&e->stage_ms[5], e->ev[0], e->ev[5]);
+  }
+  e->frames++;
+  return ARBOR_OK;
+}
+
+int arbor_engine_stage_ms(const arbor_engine* e, float* out6) {
+  if (!e || !out6) return ARBOR_ERR_NULL;
+  for (int i = 0; i < 6; i++) out6[i] = e->stage_ms[i];
+  return ARBOR_OK;
+}
+
+int arbor_engine_sync(arbor_engine* e) {
+  if (!e) return ARBOR_ERR_NULL;
+  return hipStreamSynchronize(e->stream) == hipSuccess ? ARBOR_OK : ARBOR_ERR_DEVICE;
+}
+
+void* arbor_engine_buffer(arbor_engine* e, const char* name) {
+  if (!e || !name) return nullptr;
+  const Buffers& b = e->b;
+  struct { const char* n; void* p; } table[] = {
+      {"B", b.B}, {"P", b.P}, {"Keys", b.Keys}, {"Order", b.Order}, {"Nodes", b.Nodes}, {"C", b.C}, {"I", b.I},
+      {"Hit", b.Hit}, {"Linear", b.Linear}, {"Guide", b.Guide}, {"Filtered", b.Filtered}, {"History", b.History},
+      {"Pixels", b.Pixels}};
+  for (auto& t : table)
+    if (strcmp(t.n, name) == 0) return t.p;
+  return nullptr;
+}
+
+hipStream_t arbor_engine_stream(arbor_engine* e) { return e ? e->stream : nullptr; }
+
+int arbor_engine_size(const arbor_engine* e, int* width, int* height) {
+  if (!e || !width || !height) return ARBOR_ERR_NULL;
+  *width = e->width;
+  *height = e->height;
+  return ARBOR_OK;
+}
+
+// inspect() (engine.js:55): JSON with sizes, counters and timings.
+int arbor_engine_inspect(const arbor_engine* e, char* out, int cap) {
+  if (!e || !out || cap <= 0) return ARBOR_ERR_NULL;
+  size_t scene = (size_t)ARBOR_CURVES * ARBOR_CS * 4 + (size_t)ARBOR_CAPACITY * ARBOR_PS * 4 +
+                 (size_t)ARBOR_CAPACITY * 8 + (size_t)ARBOR_NODE_COUNT * 32;
+  size_t allocated = scene + ARBOR_C_FLOATS * 4 + ARBOR_I_FLOATS * 4 + 5 * pixel_bytes(e->width, e->height) +
+                     (size_t)e->width * e->height * 4;
+  int n = snprintf(out, (size_t)cap,
+                   "{\"width\":%d,\"height\":%d,\"seed\":%d,\"frames\":%ld,\"sceneBytes\":%zu,"
+                   "\"allocatedBytes\":%zu,\"timings_ms\":{\"camera\":%.4f,\"visibility\":%.4f,\"lighting\":%.4f,"
+                   "\"filter\":%.4f,\"resolve\":%.4f,\"frame\":%.4f},\"abi\":{\"CAPACITY\":%d,\"NODE_COUNT\":%d,"
+                   "\"PS\":%d,\"CS\":%d,\"CURVES\":%d,\"WOOD_COUNT\":%d,\"LEAF_COUNT\":%d,\"ROCK_COUNT\":%d,"
+                   "\"OBJECT_COUNT\":%d}}",
+                   e->width, e->height, e->seed, e->frames, scene, allocated, e->stage_ms[0], e->stage_ms[1],
+                   e->stage_ms[2], e->stage_ms[3], e->stage_ms[4], e->stage_ms[5], ARBOR_CAPACITY, ARBOR_NODE_COUNT, ARBOR_PS,
+                   ARBOR_CS, ARBOR_CURVES, ARBOR_WOOD_COUNT, ARBOR_LEAF_COUNT, ARBOR_ROCK_COUNT, ARBOR_OBJECT_COUNT);
+  return (n >= 0 && n < cap) ? ARBOR_OK : ARBOR_ERR_ARG;
+}
+
+// Seeded first-person walk: per frame, forward/back and strafe keys in
+// {-1,0,1} and a mouse-x delta in [-20,20) from the integer hash of
+// (frame, seed). Input slots follow src/app.js:10 as updateCamera reads them.
+static unsigned walk_hash(unsigned x) {
+  x ^= x >> 16; x *= 2146121005u; x ^= x >> 15; x *= 2221713035u; x ^= x >> 16;
+  return x;
+}
+int arbor_walk_input(int seed, int frame, float* input32) {
+  if (!input32) return ARBOR_ERR_NULL;
+  if (frame < 0) return ARBOR_ERR_ARG;
+  for (int i = 0; i < ARBOR_I_FLOATS; i++) input32[i] = 0.0f;
+  unsigned h = walk_hash((unsigned)frame * 2654435761u ^ walk_hash((unsigned)seed + 40503u));
+  static const float fwd[4] = {1.0f, 1.0f, 0.0f, -1.0f};
+  static const float strafe[4] = {0.0f, 1.0f, 0.0f, -1.0f};
+  input32[15] = fwd[h & 3u];
+  input32[13] = strafe[(h >> 2) & 3u];
+  input32[0] = (float)((h >> 8) & 0xFFFFu) / 65536.0f * 40.0f - 20.0f;
+  return ARBOR_OK;
+}
+
+}  // extern "C"

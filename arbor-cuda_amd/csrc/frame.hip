@@ -1,0 +1,293 @@
+// Per-frame kernels of the ARBOR render path on gfx950: primary visibility,
+// shading, the same-surface lighting filter and the progressive resolve.
+//
+// Visibility and shading run one wave64 per 8x8 pixel tile; a block of four
+// waves builds the 324-tree instancing table in LDS once and then walks tiles
+// in a grid-stride loop, so the table and camera frame are amortised over many
+// tiles. Filter and resolve are HBM-bound per-pixel kernels over float4 rows.
+#include "arbor_shade.hpp"
+#include "launch.hpp"
+
+namespace arbor {
+
+constexpr int kTileBlock = 256;  // 4 waves, one 8x8 tile each
+constexpr int kWavesPerBlock = kTileBlock / 64;
+
+__device__ __forceinline__ void fillTrees(TreeInst* trees, int ws) {
+  for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, ws);
+  __syncthreads();
+}
+
+// tracePixels (Arbor.cu:252) for every pixel of the view.
+__global__ __launch_bounds__(kTileBlock) void k_tracePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
+                                                            const float* C, float4* Hit, int width, int height,
+                                                            int tilesX, int tiles) {
+  __shared__ TreeInst trees[kTrees];
+  fillTrees(trees, (int)C[31]);
+  const View view = makeView(C);
+  const float cone = C[14] / (float)height, time = C[8], wind = C[11];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x * kWavesPerBlock + wave; t < tiles; t += gridDim.x * kWavesPerBlock) {
+    int x = (t % tilesX) * 8 + (lane & 7), y = (t / tilesX) * 8 + (lane >> 3);
+    if (x >= width || y >= height) continue;
+    v3 ro = view.eye, rd = viewRay(view, x, y, width, height);
+    float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
+    int tree = -1;
+    v2 h = traceForest(P, Nodes4, Order, trees, ro, rd, cone, time, wind, ground > 0.0f ? ground : kFar, &tree, false);
+    Hit[y * width + x] = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)height);
+  }
+}
+
+// Single-specimen primary rays: traceTree in tree-local space (no instancing).
+__global__ __launch_bounds__(kTileBlock) void k_traceTreePixels(const float* P, const float4* Nodes4,
+                                                                const uint32_t* Order, const float* C, float4* Hit,
+                                                                int width, int height, int tilesX, int tiles) {
+  const View view = makeView(C);
+  const float cone = C[14] / (float)height, time = C[8], wind = C[11];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x * kWavesPerBlock + wave; t < tiles; t += gridDim.x * kWavesPerBlock) {
+    int x = (t % tilesX) * 8 + (lane & 7), y = (t / tilesX) * 8 + (lane >> 3);
+    if (x >= width || y >= height) continue;
+    v3 ro = view.eye, rd = viewRay(view, x, y, width, height);
+    float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
+    v2 h = traceTree(P, Nodes4, Order, ro, rd, cone, time, wind, ground > 0.0f ? ground : kFar, -1, false);
+    Hit[y * width + x] = make_float4(h.x, h.y, -1.0f, C[14] * h.x / (float)height);
+  }
+}
+
+// shadePixels (Arbor.cu:277-289).
+__global__ __launch_bounds__(kTileBlock) void k_shadePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
+                                                            const float* C, const float4* Hit, float4* Linear,
+                                                            float4* Guide, int width, int height, int tilesX,
+                                                            int tiles) {
+  __shared__ TreeInst trees[kTrees];
+  fillTrees(trees, (int)C[31]);
+  const View view = makeView(C);
+  const v3 sun = unit(mk(cos_cr(C[12]) * cos_cr(C[13]), sin_cr(C[13]), sin_cr(C[12]) * cos_cr(C[13])));
+  const v3 su = basisU(sun), sv = cross(sun, su);
+  const float time = C[8], wind = C[11], mode = C[15];
+  const bool shadows = C[22] > 0.5f;
+  const int sample = (int)C[9];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x * kWavesPerBlock + wave; t < tiles; t += gridDim.x * kWavesPerBlock) {
+    int x = (t % tilesX) * 8 + (lane & 7), y = (t / tilesX) * 8 + (lane >> 3);
+    if (x >= width || y >= height) continue;
+    int pixel = y * width + x;
+    float4 hit = Hit[pixel];
+    float tt = hit.x;
+    int id = (int)hit.y, tree = (int)hit.z;
+    v3 ro = view.eye, rd = viewRay(view, x, y, width, height);
+    v3 col = sky(rd, sun);
+    v3 guide = mk(1.0f, 1.0f, 1.0f);
+    if (tt < kFar) {
+      v3 p = ro + rd * tt;
+      float fp = fmaxf(0.000005f, hit.w);
+      v3 localP = p, n = mk(0.0f, 1.0f, 0.0f);
+      float localFp = fp, phase = 0.0f;
+      Prim pr;
+      bool leaf = false;
+      if (id >= 0) {
+        pr = loadPrim(P, id);
+        leaf = pr.type() == 1.0f;
+      }
+      v3 leafUdir = mk(0.0f, 0.0f, 0.0f);
+      if (id >= 0 && tree >= 0) {
+        const TreeInst& ti = trees[tree];
+        localP = toLocalPoint(p, ti);
+        localFp = fp / ti.sc;
+        phase = ti.phase;
+        n = toWorldNormal(objectNormal(pr, localP, localFp, time + phase, wind), ti);
+      }
+      if (leaf) leafUdir = leafFrame(pr, time + phase, wind).u;
+      if (dot(n, rd) > 0.0f) n = n * -1.0f;
+      if (id < 0) {
+        float e = fmaxf(0.001f, fp);
+        float nx = (noise((p.x + e) * 18.0f, p.z * 18.0f, 133) - noise((p.x - e) * 18.0f, p.z * 18.0f, 133)) * 0.18f;
+        float nz = (noise(p.x * 18.0f, (p.z + e) * 18.0f, 133) - noise(p.x * 18.0f, (p.z - e) * 18.0f, 133)) * 0.18f;
+        n = unit(mk(nx, 1.0f, nz));
+      }
+      v3 albedo;
+      if (id < 0) albedo = groundColor(p, localFp);
+      else {
+        int type = (int)pr.type();
+        if (type == 0) albedo = barkColor(pr, localP, localFp);
+        else if (type == 1) albedo = leafColor(pr, leafUdir, localP, localFp);
+        else albedo = mix(mk(0.09f, 0.10f, 0.086f), mk(0.31f, 0.30f, 0.24f), noise3(localP * 42.0f, (int)pr.seed()));
+      }
+      if (tree >= 0) {
+        int ts = trees[tree].seed;
+        float hue = rnd(8, ts), age = rnd(9, ts), season = rnd(10, ts);
+        float dark = 0.76f + age * 0.38f;
+        v3 tint = mk(0.88f + 0.22f * hue, 0.84f + 0.28f * (1.0f - fabsf(hue - 0.55f)), 0.72f + 0.25f * (1.0f - hue));
+        if (leaf) tint = mix(tint, mk(1.12f, 0.88f, 0.48f), smooth(0.78f, 0.98f, season) * 0.38f);
+        else tint = mix(tint, mk(0.78f, 0.66f, 0.53f), season * 0.18f);
+        albedo = albedo * tint * dark;
+      }
+      guide = albedo;
+      float r1 = rnd(pixel + sample * 1823, 991), r2 = rnd(pixel + sample * 971, 731);
+      float angle = r1 * 6.28318f, disk = sqrtf(r2) * 0.012f;
+      v3 light = unit(sun + su * (cosf(angle) * disk) + sv * (sinf(angle) * disk));
+      float offset = id >= 0 ? 0.0014f : 0.004f;
+      v3 visibility = mk(1.0f, 1.0f, 1.0f);
+      if (shadows) visibility = lightVisibility(P, Nodes4, Order, trees, p + light * offset, light, time, wind);
+      float nl = sat(dot(n, light)), back = sat(-dot(n, light));
+      v3 direct = albedo * (nl / kPi);
+      if (leaf) {
+        float vein = leafVein(pr, leafUdir, localP, localFp);
+        direct = direct + albedo * (back * (0.75f - 0.25f * vein) + powf(sat(dot(rd, light)), 9.0f) * 0.12f);
+      }
+      float spec = specGGX(n, rd * -1.0f, light, leaf ? 0.39f : 0.82f);
+      direct = direct + mk(spec, spec, spec);
+      const float ao = 1.0f;
+      v3 ambient = mk(0.30f, 0.37f, 0.45f) * (0.45f + 0.55f * sat(n.y)) * ao + mk(0.08f, 0.068f, 0.035f) * sat(-n.y);
+      if (leaf) ambient = ambient + mk(0.04f, 0.055f, 0.022f) * ao;
+      col = direct * visibility * mk(4.8f, 4.2f, 3.1f) + albedo * ambient;
+      float haze = 1.0f - expf(-tt * 0.0018f);
+      col = mix(col, sky(rd, sun), haze);
+      if (mode == 1.0f) {
+        float w0 = bandWeight(0.065f, localFp), w1 = bandWeight(0.005f, localFp), w2 = bandWeight(0.001f, localFp);
+        col = mk(w0 * 0.80f, w1 * 0.85f, w2 * 0.9f);
+      }
+      if (mode == 2.0f) {
+        float hh = rnd(id + 177 + (tree + 1) * 71, 32);
+        col = id < 0 ? mk(0.10f, 0.10f, 0.10f)
+                     : (leaf ? mk(0.1f, 0.3f + hh * 0.5f, 0.17f) : mk(0.47f + hh * 0.3f, 0.25f, 0.10f));
+      }
+      if (mode == 3.0f) col = n * 0.5f + mk(0.5f, 0.5f, 0.5f);
+    }
+    Linear[pixel] = make_float4(col.x, col.y, col.z, 1.0f);
+    Guide[pixel] = make_float4(guide.x, guide.y, guide.z, 1.0f);
+  }
+}
+
+// filterLighting (Arbor.cu:292-302): 5x5 cross-bilateral, same id/tree/depth.
+__global__ __launch_bounds__(256) void k_filterLighting(const float4* Linear, const float4* Guide, const float4* Hit,
+                                                        float4* Filtered, const float* C, int width, int height) {
+  int x = (int)(blockIdx.x * blockDim.x + threadIdx.x), y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (x >= width || y >= height) return;
+  int b = y * width + x;
+  float4 hb = Hit[b], lb = Linear[b], gb = Guide[b];
+  v3 col = mk(lb.x, lb.y, lb.z);
+  v3 base = mk(gb.x, gb.y, gb.z) + mk(0.02f, 0.02f, 0.02f);
+  v3 sum = mk(0.0f, 0.0f, 0.0f);
+  float weight = 0.0f;
+  int id = (int)hb.y, tree = (int)hb.z;
+  float depth = hb.x;
+  if (depth < kFar && C[15] < 0.5f) {
+    float tol = fmaxf(0.005f, hb.w * 6.0f);
+    for (int j = -2; j <= 2; j++)
+      for (int k = -2; k <= 2; k++) {
+        int xx = x + k, yy = y + j;
+        if (xx < 0 || yy < 0 || xx >= width || yy >= height) continue;
+        int q = yy * width + xx;
+        float4 hq = Hit[q];
+        if ((int)hq.y != id || (int)hq.z != tree || fabsf(hq.x - depth) > tol) continue;
+        float w = expf(-(float)(j * j + k * k) * 0.30f);
+        float4 lq = Linear[q], gq = Guide[q];
+        v3 g = mk(gq.x, gq.y, gq.z) + mk(0.02f, 0.02f, 0.02f);
+        sum = sum + mk(lq.x / g.x, lq.y / g.y, lq.z / g.z) * w;
+        weight += w;
+      }
+    if (weight > 0.0f) col = mix(col, (sum / weight) * base, 0.86f);
+  }
+  Filtered[b] = make_float4(col.x, col.y, col.z, 1.0f);
+}
+
+AD float aces(float x) { return sat((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f)); }
+
+// resolvePixels (Arbor.cu:304).
+__global__ __launch_bounds__(256) void k_resolvePixels(const float4* Filtered, float4* History, uint32_t* Pixels,
+                                                       const float* C, int width, int height) {
+  int x = (int)(blockIdx.x * blockDim.x + threadIdx.x), y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (x >= width || y >= height) return;
+  int i = y * width + x;
+  float4 f = Filtered[i];
+  v3 current = mk(f.x, f.y, f.z);
+  float count = C[10] > 0.5f ? 0.0f : C[9];
+  v3 mean = current;
+  if (!(count < 0.5f)) {
+    float4 h = History[i];
+    mean = mix(mk(h.x, h.y, h.z), current, 1.0f / (count + 1.0f));
+  }
+  History[i] = make_float4(mean.x, mean.y, mean.z, count + 1.0f);
+  v3 col = mean * C[20];
+  col = mk(aces(col.x), aces(col.y), aces(col.z));
+  const float g = 1.0f / 2.2f;
+  col = mk(powf(col.x, g), powf(col.y, g), powf(col.z, g));
+  float u = ((float)x / (float)width - 0.5f) * 2.0f, v = ((float)y / (float)height - 0.5f) * 2.0f;
+  col = col * (1.0f - 0.09f * (u * u + v * v));
+  Pixels[i] = packRGB(col);
+}
+
+}  // namespace arbor
+
+using namespace arbor;
+
+namespace {
+// Grid for the tile kernels: enough blocks to fill 256 CUs several times over,
+// never more than there are tiles.
+inline int tileGrid(int tiles) {
+  int blocks = (tiles + kWavesPerBlock - 1) / kWavesPerBlock;
+  return blocks < 2048 ? blocks : 2048;
+}
+}  // namespace
+
+extern "C" {
+
+int arbor_tracePixels(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
+                      int width, int height, hipStream_t stream) {
+  ARBOR_REQUIRE(P && Nodes && Order && C && Hit);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  hipLaunchKernelGGL(k_tracePixels, dim3(tileGrid(tiles)), dim3(kTileBlock), 0, stream, P,
+                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
+                     tx, tiles);
+  return launch_status();
+}
+
+int arbor_traceTreePixels(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
+                          int width, int height, hipStream_t stream) {
+  ARBOR_REQUIRE(P && Nodes && Order && C && Hit);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  hipLaunchKernelGGL(k_traceTreePixels, dim3(tileGrid(tiles)), dim3(kTileBlock), 0, stream, P,
+                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
+                     tx, tiles);
+  return launch_status();
+}
+
+int arbor_shadePixels(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                      float* Linear, float* Guide, int width, int height, hipStream_t stream) {
+  ARBOR_REQUIRE(P && Nodes && Order && C && Hit && Linear && Guide);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  hipLaunchKernelGGL(k_shadePixels, dim3(tileGrid(tiles)), dim3(kTileBlock), 0, stream, P,
+                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
+                     reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles);
+  return launch_status();
+}
+
+int arbor_filterLighting(const float* Linear, const float* Guide, const float* Hit, float* Filtered, const float* C,
+                         int width, int height, hipStream_t stream) {
+  ARBOR_REQUIRE(Linear && Guide && Hit && Filtered && C);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  dim3 block(64, 4), grid((width + 63) / 64, (height + 3) / 4);
+  hipLaunchKernelGGL(k_filterLighting, grid, block, 0, stream, reinterpret_cast<const float4*>(Linear),
+                     reinterpret_cast<const float4*>(Guide), reinterpret_cast<const float4*>(Hit),
+                     reinterpret_cast<float4*>(Filtered), C, width, height);
+  return launch_status();
+}
+
+int arbor_resolvePixels(const float* Filtered, float* History, unsigned* Pixels, const float* C, int width, int height,
+                        hipStream_t stream) {
+  ARBOR_REQUIRE(Filtered && History && Pixels && C);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  dim3 block(64, 4), grid((width + 63) / 64, (height + 3) / 4);
+  hipLaunchKernelGGL(k_resolvePixels, grid, block, 0, stream, reinterpret_cast<const float4*>(Filtered),
+                     reinterpret_cast<float4*>(History), Pixels, C, width, height);
+  return launch_status();
+}
+
+const char* arbor_version(void) { return "arbor-mi355x 0.1 gfx950"; }
+
+}  // extern "C"

@@ -1,0 +1,286 @@
+"""Python binding of libarbor.so (the HIP render path) over ctypes.
+
+Device memory and streams come from PyTorch (ROCm build); every compute call
+goes through the C-ABI declared in include/arbor.h. There is no fallback: if
+the library or a HIP device is missing, calls raise.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libarbor.so")
+
+CAPACITY = 32768
+NODE_COUNT = 65536
+PS = 24
+CS = 20
+CURVES = 2109
+OBJECT_COUNT = 29412
+
+STATUS = {0: "ARBOR_OK", -1: "ARBOR_ERR_NULL", -2: "ARBOR_ERR_ARG", -3: "ARBOR_ERR_LAUNCH",
+          -4: "ARBOR_ERR_DEVICE", -5: "ARBOR_ERR_STATE"}
+
+_vp = ct.c_void_p
+_i = ct.c_int
+_f = ct.c_float
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+SIGNATURES = {
+    "arbor_growTree": [_vp, _i, _i, _vp],
+    "arbor_makePrimitives": [_vp, _vp, _i, _vp],
+    "arbor_makeKeys": [_vp, _vp, _vp, _vp],
+    "arbor_sortPass": [_vp, _vp, _i, _i, _vp],
+    "arbor_makeBounds": [_vp, _vp, _vp, _vp],
+    "arbor_reduceBounds": [_vp, _i, _vp],
+    "arbor_initCamera": [_vp, _vp, _i, _vp],
+    "arbor_updateCamera": [_vp, _vp, _vp, _f, _i, _i, _vp],
+    "arbor_tracePixels": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "arbor_shadePixels": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "arbor_filterLighting": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "arbor_resolvePixels": [_vp, _vp, _vp, _vp, _i, _i, _vp],
+    "arbor_sortKeys": [_vp, _vp, _vp],
+    "arbor_buildBVH": [_vp, _vp, _vp, _vp, _vp],
+    "arbor_generate": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
+    "arbor_traceTreePixels": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "arbor_version": [],
+    "arbor_engine_round_size": [ct.POINTER(_i), ct.POINTER(_i)],
+    "arbor_engine_create": [ct.POINTER(_vp), _i, _i, _i, _i, _vp],
+    "arbor_engine_destroy": [_vp],
+    "arbor_engine_generate": [_vp, _i],
+    "arbor_engine_resize": [_vp, _i, _i, _i],
+    "arbor_engine_reset_camera": [_vp],
+    "arbor_engine_set_input": [_vp, ct.POINTER(_f)],
+    "arbor_engine_set_timing": [_vp, _i],
+    "arbor_engine_frame": [_vp, _f, _vp],
+    "arbor_engine_stage_ms": [_vp, ct.POINTER(_f)],
+    "arbor_engine_sync": [_vp],
+    "arbor_engine_buffer": [_vp, ct.c_char_p],
+    "arbor_engine_stream": [_vp],
+    "arbor_engine_size": [_vp, ct.POINTER(_i), ct.POINTER(_i)],
+    "arbor_engine_inspect": [_vp, ct.c_char_p, _i],
+    "arbor_walk_input": [_i, _i, ct.POINTER(_f)],
+}
+_RESTYPES = {"arbor_version": ct.c_char_p, "arbor_engine_round_size": None, "arbor_engine_buffer": _vp,
+             "arbor_engine_stream": _vp}
+
+
+class ArborError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """Loads libarbor.so and binds every C-ABI symbol. Importing torch first
+    lets the library share torch's HIP runtime (same soname)."""
+    if not os.path.exists(path):
+        raise ArborError("libarbor.so is not built (%s): run __graft_entry__.build()" % path)
+    lib = ct.CDLL(path)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, _i)
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        import torch  # noqa: F401  (shares the HIP runtime with the library)
+
+        _LIB = load_library()
+    return _LIB
+
+
+def check(rc, what=""):
+    if rc != 0:
+        raise ArborError("%s failed: %s" % (what, STATUS.get(rc, rc)))
+
+
+def _ptr(t):
+    return None if t is None else ct.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ct.c_void_p(s.cuda_stream)
+
+
+def call(name, *args, stream=None):
+    """Calls C-ABI entry `name` with torch tensors / scalars; raises on a
+    non-zero status. The HIP stream is appended (torch's current stream)."""
+    conv = []
+    for a in args:
+        if hasattr(a, "data_ptr"):
+            conv.append(_ptr(a))
+        else:
+            conv.append(a)
+    rc = getattr(lib(), name)(*conv, _stream(stream))
+    check(rc, name)
+
+
+class DeviceScene:
+    """Device buffers B, P, Keys, Order, Nodes, C of one scene."""
+
+    def __init__(self, device="cuda"):
+        import torch
+
+        z = dict(device=device)
+        self.B = torch.zeros(CURVES * CS, dtype=torch.float32, **z)
+        self.P = torch.zeros(CAPACITY * PS, dtype=torch.float32, **z)
+        self.Keys = torch.zeros(CAPACITY, dtype=torch.int32, **z)
+        self.Order = torch.zeros(CAPACITY, dtype=torch.int32, **z)
+        self.Nodes = torch.zeros(NODE_COUNT * 8, dtype=torch.float32, **z)
+        self.C = torch.zeros(64, dtype=torch.float32, **z)
+
+    def generate(self, seed, history=None, width=0, height=0):
+        call("arbor_generate", self.B, self.P, self.Keys, self.Order, self.Nodes, self.C, seed, history, width, height)
+        return self
+
+    def host(self, name):
+        import torch
+
+        t = getattr(self, name)
+        torch.cuda.synchronize()
+        a = t.cpu().numpy()
+        return a.view(np.uint32) if t.dtype == torch.int32 else a
+
+
+class FrameBuffers:
+    """Per-view device buffers Hit, Linear, Guide, Filtered, History, Pixels."""
+
+    def __init__(self, width, height, device="cuda"):
+        import torch
+
+        n = width * height
+        self.width, self.height = width, height
+        self.Hit = torch.zeros(n * 4, dtype=torch.float32, device=device)
+        self.Linear = torch.zeros(n * 4, dtype=torch.float32, device=device)
+        self.Guide = torch.zeros(n * 4, dtype=torch.float32, device=device)
+        self.Filtered = torch.zeros(n * 4, dtype=torch.float32, device=device)
+        self.History = torch.zeros(n * 4, dtype=torch.float32, device=device)
+        self.Pixels = torch.zeros(n, dtype=torch.int32, device=device)
+
+
+def trace(scene, C, fb):
+    call("arbor_tracePixels", scene.P, scene.Nodes, scene.Order, C, fb.Hit, fb.width, fb.height)
+
+
+def trace_tree(scene, C, fb):
+    call("arbor_traceTreePixels", scene.P, scene.Nodes, scene.Order, C, fb.Hit, fb.width, fb.height)
+
+
+def shade(scene, C, fb):
+    call("arbor_shadePixels", scene.P, scene.Nodes, scene.Order, C, fb.Hit, fb.Linear, fb.Guide, fb.width, fb.height)
+
+
+def filter_lighting(C, fb):
+    call("arbor_filterLighting", fb.Linear, fb.Guide, fb.Hit, fb.Filtered, C, fb.width, fb.height)
+
+
+def resolve(C, fb):
+    call("arbor_resolvePixels", fb.Filtered, fb.History, fb.Pixels, C, fb.width, fb.height)
+
+
+def walk_input(seed, frame):
+    out = (ct.c_float * 32)()
+    check(lib().arbor_walk_input(seed, frame, out), "arbor_walk_input")
+    return np.frombuffer(out, dtype=np.float32).copy()
+
+
+class Engine:
+    """The C++ engine (include/arbor_engine.h): generate / resize / frame."""
+
+    def __init__(self, width, height, seed, round_size=False, stream=None):
+        self._lib = lib()
+        h = ct.c_void_p()
+        s = _stream(stream) if stream is not None else _stream()
+        check(self._lib.arbor_engine_create(ct.byref(h), width, height, seed, int(round_size), s),
+              "arbor_engine_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            check(self._lib.arbor_engine_destroy(self.h), "arbor_engine_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def generate(self, seed):
+        check(self._lib.arbor_engine_generate(self.h, seed), "arbor_engine_generate")
+
+    def resize(self, w, h, round_size=False):
+        check(self._lib.arbor_engine_resize(self.h, w, h, int(round_size)), "arbor_engine_resize")
+
+    def size(self):
+        w, h = ct.c_int(), ct.c_int()
+        check(self._lib.arbor_engine_size(self.h, ct.byref(w), ct.byref(h)), "arbor_engine_size")
+        return w.value, h.value
+
+    def reset_camera(self):
+        check(self._lib.arbor_engine_reset_camera(self.h), "arbor_engine_reset_camera")
+
+    def set_input(self, inp):
+        arr = (ct.c_float * 32)(*[float(v) for v in inp])
+        check(self._lib.arbor_engine_set_input(self.h, arr), "arbor_engine_set_input")
+
+    def set_timing(self, on):
+        check(self._lib.arbor_engine_set_timing(self.h, int(on)), "arbor_engine_set_timing")
+
+    def frame(self, dt=1.0 / 60.0, device_input=None):
+        dev = None if device_input is None else ct.c_void_p(device_input.data_ptr())
+        check(self._lib.arbor_engine_frame(self.h, dt, dev), "arbor_engine_frame")
+
+    def stage_ms(self):
+        out = (ct.c_float * 6)()
+        check(self._lib.arbor_engine_stage_ms(self.h, out), "arbor_engine_stage_ms")
+        return list(out)
+
+    def sync(self):
+        check(self._lib.arbor_engine_sync(self.h), "arbor_engine_sync")
+
+    def buffer_ptr(self, name):
+        return self._lib.arbor_engine_buffer(self.h, name.encode())
+
+    def inspect(self):
+        import json
+
+        buf = ct.create_string_buffer(4096)
+        check(self._lib.arbor_engine_inspect(self.h, buf, 4096), "arbor_engine_inspect")
+        return json.loads(buf.value.decode())
+
+    def download(self, name, count, dtype=np.float32):
+        """Copies `count` elements of a named engine buffer to the host."""
+        import torch
+
+        self.sync()
+        ptr = self.buffer_ptr(name)
+        if not ptr:
+            raise ArborError("unknown buffer " + name)
+        nbytes = count * np.dtype(dtype).itemsize
+        host = torch.empty(nbytes, dtype=torch.uint8)
+        # device -> host through a torch tensor view of the engine allocation
+        dev = _wrap_device(ptr, nbytes)
+        host.copy_(dev)
+        return host.numpy().view(dtype).copy()
+
+
+def _wrap_device(ptr, nbytes):
+    """A uint8 torch tensor aliasing `nbytes` of device memory at `ptr`."""
+    import torch
+
+    class _Iface:
+        def __init__(self):
+            self.__cuda_array_interface__ = {"shape": (nbytes,), "typestr": "|u1", "data": (ptr, False),
+                                             "version": 3, "strides": None}
+
+    return torch.as_tensor(_Iface(), device="cuda")

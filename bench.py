@@ -1,0 +1,183 @@
+"""Throughput of the ARBOR render path on one MI355X.
+
+Workload (the full-forest configuration): seed 7, 18x18 = 324 instanced trees
+sharing one resident BVH, 1920x1080, 1 spp primary rays, shade, filterLighting
+and resolve, tree-to-tree sun visibility off, breeze off, along a seeded
+120-frame WASD walk (dt = 1/60 s). A step is one frame of the C++ engine:
+updateCamera -> tracePixels -> shadePixels -> filterLighting -> resolvePixels,
+inputs resident in HBM (the walk's 32-float input rows are uploaded before
+the timed region).
+
+Prints one JSON line. `value` is end-to-end primary Mrays/s (W*H per frame /
+frame time); the trace kernel's own rate, per-stage ms, its roofline fraction
+and the reference CPU harness timed on this host are reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "arbor-cuda_amd"))
+
+# Algorithmic FP32 operations per primary ray of this workload (walk frame 0,
+# 1920x1080, seed 7, shadows off), counted by the instrumented oracle
+# (oracle/flops.cpp: every +,-,*,/,sqrt,min,max,abs,floor and every libm call
+# counts 1). Trace only.
+TRACE_FLOP_PER_RAY = None
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def walk_inputs(px, seed, n):
+    import numpy as np
+
+    rows = [px.walk_input(seed, f) for f in range(n)]
+    rows[0][10] = 1.0  # frame 0 toggles tree-to-tree sun visibility off (updateCamera I[10])
+    return np.stack(rows).astype(np.float32)
+
+
+def cpu_baseline(seed, width, height, threads):
+    """The reference CPU harness (compiled from its sources into
+    oracle/_ref/libarborref.so) on this host's cores, one frame (walk frame 0)
+    of the same workload: trace + shade + filter + resolve."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import oracle_lib
+
+    kind = "reference" if oracle_lib.ref_available() else "port"
+    cpu = oracle_lib.RefHarness() if kind == "reference" else oracle_lib.Oracle()
+    if threads <= 0:
+        threads = min(16, len(os.sched_getaffinity(0)))
+    scene = cpu.generate(seed)
+    C = cpu.init_camera(seed)
+    import pyarbor
+
+    I = walk_inputs(pyarbor, seed, 1)[0]
+    cpu.update_camera(C, I, 1.0 / 60.0, width, height)
+    t0 = time.perf_counter()
+    H = cpu.trace(scene, C, width, height, threads)
+    t1 = time.perf_counter()
+    L, G = cpu.shade(scene, C, H, width, height, threads)
+    F = cpu.filter(L, G, H, C, width, height, threads)
+    hist = np.zeros_like(F)
+    cpu.resolve(F, hist, C, width, height, threads)
+    t2 = time.perf_counter()
+    rays = width * height
+    return {"value": round(rays / (t2 - t0) / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": kind,
+            "sample": "walk frame 0 of the same workload (%dx%d, seed %d, shadows off), trace+shade+filter+resolve, "
+                      "OpenMP schedule(dynamic,2) rows" % (width, height, seed),
+            "ms_per_frame": round((t2 - t0) * 1e3, 1), "trace_ms": round((t1 - t0) * 1e3, 1)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import pyarbor as px
+
+    W, H, K, Wm = args.width, args.height, args.steps, args.warmup
+    eng = px.Engine(W, H, args.seed)
+    inputs = torch.from_numpy(walk_inputs(px, args.seed, max(K, Wm, 1))).cuda()
+    dt = 1.0 / 60.0
+
+    # warmup: the first Wm walk frames, then back to the initial camera
+    for f in range(Wm):
+        eng.frame(dt, inputs[f])
+    eng.sync()
+    eng.reset_camera()
+    eng.sync()
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(K):
+        eng.frame(dt, inputs[f])
+    eng.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / K * 1e3
+
+    # per-stage kernel times: replay the same walk with HIP events around each
+    # stage on the engine's stream (synchronises per frame, outside the timed loop)
+    eng.reset_camera()
+    eng.set_timing(True)
+    stages = []
+    for f in range(K):
+        eng.frame(dt, inputs[f])
+        stages.append(eng.stage_ms())
+    eng.set_timing(False)
+    st = np.array(stages)
+    mean = st.mean(axis=0)
+    trace_ms = float(mean[1])
+
+    roofline = {"bound": "mfma", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS, "kernel": "tracePixels",
+                "compute": "FP32 vector ALU (no MFMA shape in ray traversal); peak is MI355X's FP32 157.3 TFLOP/s",
+                "traffic": None}
+    if TRACE_FLOP_PER_RAY:
+        achieved = TRACE_FLOP_PER_RAY * W * H / (trace_ms * 1e-3) / 1e12
+        roofline.update({"achieved": round(achieved, 3), "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                         "flop_per_ray": TRACE_FLOP_PER_RAY})
+    else:
+        roofline.update({"achieved": None, "frac": None})
+
+    rays = W * H
+    result = {
+        "metric": "Mrays/s, ms/frame @1080p 324 trees",
+        "value": round(rays * K / elapsed / 1e6, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": Wm,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (procedural forest from seed; seeded walk-path inputs)",
+        "config": {"workload": "full forest plain method: 324 trees, 1920x1080, 1 spp primary+shade+filter+resolve, "
+                               "shadows off, seeded 120-frame walk", "width": W, "height": H, "trees": 324,
+                   "seed": args.seed, "spp": 1, "frames": K},
+        "ms_per_frame": round(ms_per_step, 4),
+        "trace_mrays_per_s": round(rays / (trace_ms * 1e-3) / 1e6, 2),
+        "stage_ms": {"camera": round(float(mean[0]), 4), "trace": round(trace_ms, 4),
+                     "shade": round(float(mean[2]), 4), "filter": round(float(mean[3]), 4),
+                     "resolve": round(float(mean[4]), 4), "frame": round(float(mean[5]), 4)},
+        "roofline": roofline,
+    }
+    # HBM figures of the two bandwidth-bound kernels (64 and 52 algorithmic B/px)
+    result["hbm_kernels"] = {
+        "filterLighting": {"GB/s": round(64 * rays / (mean[3] * 1e-3) / 1e9, 1),
+                           "frac": round(64 * rays / (mean[3] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "resolvePixels": {"GB/s": round(52 * rays / (mean[4] * 1e-3) / 1e9, 1),
+                          "frac": round(52 * rays / (mean[4] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+    eng.close()
+    if not args.no_cpu:
+        cb = cpu_baseline(args.seed, W, H, args.cpu_threads)
+        result["cpu_baseline"] = cb
+        result["gpu_vs_cpu"] = round(result["value"] / cb["value"], 1)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,136 @@
+/* ARBOR render path on MI355X (gfx950): the C-ABI drop-in boundary.
+ *
+ * The twelve entry points keep the parameter order, types and buffer layouts
+ * of the reference CUDA kernels (Arbor.cu, line cited per entry) and add the
+ * HIP stream they are ordered on. Launch geometry is internal. Every pointer
+ * is a device pointer the caller allocated with the sizes below; no entry
+ * point allocates. Every call returns ARBOR_OK or a negative status and never
+ * aborts. One buffer set must not be used from two streams at once.
+ *
+ * Buffer layouts (flat little-endian f32/u32, docs/ARCHITECTURE.md:9-15):
+ *   B      CURVES x CS f32     Bezier curves: ctrl0..3, ra, rb, arc0, seed, chord
+ *   P      CAPACITY x PS f32   primitives; P[3] = -1 unused, 0 wood, 1 leaf, 2 stone
+ *   Keys   CAPACITY u32        30-bit Morton keys (0xFFFFFFFF for unused)
+ *   Order  CAPACITY u32        Morton-sorted slot -> primitive id
+ *   Nodes  NODE_COUNT x 8 f32  heap BVH {lo.xyz, count, hi.xyz, id|-1}, root 1,
+ *                              leaves CAPACITY+i, node 0 unused
+ *   C      64 f32              camera/state; I 32 f32 input
+ *   Hit, Linear, Guide, Filtered, History   W*H x 4 f32
+ *   Pixels W*H u32             RGBA8
+ */
+#ifndef ARBOR_H
+#define ARBOR_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ABI constants (Arbor.cu:5-15,54-56; src/engine.js:2). */
+#define ARBOR_CAPACITY 32768
+#define ARBOR_NODE_COUNT 65536
+#define ARBOR_PS 24
+#define ARBOR_CS 20
+#define ARBOR_CURVES 2109
+#define ARBOR_WOOD_COUNT 5124
+#define ARBOR_LEAF_COUNT 24192
+#define ARBOR_ROCK_COUNT 96
+#define ARBOR_OBJECT_COUNT 29412
+#define ARBOR_FOREST_SIDE 18
+#define ARBOR_FOREST_COUNT 324
+#define ARBOR_C_FLOATS 64
+#define ARBOR_I_FLOATS 32
+/* Resolution limits: the reference harness accepts 32..4096 (native_harness.cpp:61);
+ * this library accepts 1..8192 per side. */
+#define ARBOR_MAX_DIM 8192
+
+/* Status codes. */
+#define ARBOR_OK 0
+#define ARBOR_ERR_NULL (-1)     /* a required pointer is NULL */
+#define ARBOR_ERR_ARG (-2)      /* a scalar argument is out of its documented range */
+#define ARBOR_ERR_LAUNCH (-3)   /* the HIP launch failed (hipGetLastError) */
+#define ARBOR_ERR_DEVICE (-4)   /* no usable HIP device / runtime call failed */
+#define ARBOR_ERR_STATE (-5)    /* engine used after destroy or before generate */
+
+/* ---- the twelve reference entry points ---------------------------------- */
+
+/* Arbor.cu:76  growTree(float* B, int level, int seed): level 0..4 in order;
+ * level L reads only curves of level L-1. seed in [0, 1000000] (engine.js:28). */
+int arbor_growTree(float* B, int level, int seed, hipStream_t stream);
+
+/* Arbor.cu:103  makePrimitives(const float* B, float* P, int seed). */
+int arbor_makePrimitives(const float* B, float* P, int seed, hipStream_t stream);
+
+/* Arbor.cu:139  makeKeys(const float* P, unsigned* Keys, unsigned* Order). */
+int arbor_makeKeys(const float* P, unsigned* Keys, unsigned* Order, hipStream_t stream);
+
+/* Arbor.cu:140  sortPass(unsigned* Keys, unsigned* Order, int distance, int span):
+ * one bitonic compare-exchange stage; distance and span powers of two with
+ * 1 <= distance < span <= CAPACITY. */
+int arbor_sortPass(unsigned* Keys, unsigned* Order, int distance, int span, hipStream_t stream);
+
+/* Arbor.cu:141  makeBounds(const float* P, const unsigned* Order, float* Nodes). */
+int arbor_makeBounds(const float* P, const unsigned* Order, float* Nodes, hipStream_t stream);
+
+/* Arbor.cu:142  reduceBounds(float* Nodes, int first): first a power of two
+ * in [1, CAPACITY/2]. */
+int arbor_reduceBounds(float* Nodes, int first, hipStream_t stream);
+
+/* Arbor.cu:181  initCamera(float* C, const float* P, int seed). P is unused by
+ * the reference and may be NULL. */
+int arbor_initCamera(float* C, const float* P, int seed, hipStream_t stream);
+
+/* Arbor.cu:182  updateCamera(float* C, const float* I, const float* P,
+ * float delta, int width, int height). P is unused and may be NULL. */
+int arbor_updateCamera(float* C, const float* I, const float* P, float delta, int width, int height,
+                       hipStream_t stream);
+
+/* Arbor.cu:252  tracePixels(P, Nodes, Order, C, Hit, width, height):
+ * Hit[4p..] = {t, primitive id | -1, tree | -1, C[14]*t/height}. */
+int arbor_tracePixels(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
+                      int width, int height, hipStream_t stream);
+
+/* Arbor.cu:277  shadePixels(P, Nodes, Order, C, Hit, Linear, Guide, width, height). */
+int arbor_shadePixels(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                      float* Linear, float* Guide, int width, int height, hipStream_t stream);
+
+/* Arbor.cu:292  filterLighting(Linear, Guide, Hit, Filtered, C, width, height). */
+int arbor_filterLighting(const float* Linear, const float* Guide, const float* Hit, float* Filtered, const float* C,
+                         int width, int height, hipStream_t stream);
+
+/* Arbor.cu:304  resolvePixels(Filtered, History, Pixels, C, width, height). */
+int arbor_resolvePixels(const float* Filtered, float* History, unsigned* Pixels, const float* C, int width,
+                        int height, hipStream_t stream);
+
+/* ---- additions beside the reference interface --------------------------- */
+
+/* Stable radix sort of the CAPACITY (Keys, Order) pairs into ascending
+ * (key, order) order: the same result as the full 120-stage sortPass network
+ * (engine.js:31) and as std::sort of pairs (native_harness.cpp:47). One
+ * workgroup, LDS-staged; no workspace. */
+int arbor_sortKeys(unsigned* Keys, unsigned* Order, hipStream_t stream);
+
+/* makeKeys -> sortKeys -> makeBounds -> reduceBounds(16384..1) in one launch:
+ * Keys, Order and Nodes bit-identical to the reference build sequence
+ * (engine.js:30-32) for the same P. */
+int arbor_buildBVH(const float* P, unsigned* Keys, unsigned* Order, float* Nodes, hipStream_t stream);
+
+/* Whole one-time build, engine.js:27-34: growTree 0..4, makePrimitives,
+ * buildBVH, initCamera. History (W*H*4 f32, may be NULL) is cleared. */
+int arbor_generate(float* B, float* P, unsigned* Keys, unsigned* Order, float* Nodes, float* C, int seed,
+                   float* History, int width, int height, hipStream_t stream);
+
+/* Single-specimen primary rays: traceTree in tree-local space with the ground
+ * plane as initial best and no forest instancing (the single-tree workload of
+ * native_harness.cpp:84 per pixel). Hit layout as tracePixels with tree = -1. */
+int arbor_traceTreePixels(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
+                          int width, int height, hipStream_t stream);
+
+/* Library build identification (git-independent). */
+const char* arbor_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARBOR_H */

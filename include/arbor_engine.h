@@ -1,0 +1,55 @@
+/* ARBOR host engine over the C-ABI kernels: the generate / resize / frame /
+ * inspect contract of the reference host (src/engine.js:6-57) in C++, with
+ * device buffers owned by the engine. All calls return ARBOR_OK or a negative
+ * status (arbor.h). */
+#ifndef ARBOR_ENGINE_H
+#define ARBOR_ENGINE_H
+
+#include "arbor.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct arbor_engine arbor_engine;
+
+/* engine.js:36 rounding: width up to a multiple of 64 (min 320), height up to
+ * a multiple of 8 (min 192). */
+void arbor_engine_round_size(int* width, int* height);
+
+/* Engine.init + generate (engine.js:6-24,27-34). round_size != 0 applies the
+ * engine.js resize rounding; stream NULL makes the engine create its own. */
+int arbor_engine_create(arbor_engine** out, int width, int height, int seed, int round_size, hipStream_t stream);
+int arbor_engine_destroy(arbor_engine* e);
+/* engine.js:27: seed must be an integer in [0, 1000000]. */
+int arbor_engine_generate(arbor_engine* e, int seed);
+/* engine.js:35-43. */
+int arbor_engine_resize(arbor_engine* e, int width, int height, int round_size);
+/* initCamera on the engine's C buffer (the state generate() leaves). */
+int arbor_engine_reset_camera(arbor_engine* e);
+/* Host copy of the 32-float input slot array used by the next frame. */
+int arbor_engine_set_input(arbor_engine* e, const float* input32);
+/* Per-stage HIP-event timing of every frame (synchronises each frame). */
+int arbor_engine_set_timing(arbor_engine* e, int on);
+/* engine.js:45-53: updateCamera, tracePixels, shadePixels, filterLighting,
+ * resolvePixels. device_input, if not NULL, is a device array of 32 floats
+ * used instead of the host input. */
+int arbor_engine_frame(arbor_engine* e, float dt, const float* device_input);
+/* Last timed frame: {camera, visibility, lighting, filter, resolve, frame} in
+ * ms, measured with HIP events on the engine's stream. */
+int arbor_engine_stage_ms(const arbor_engine* e, float* out6);
+int arbor_engine_sync(arbor_engine* e);
+/* Device pointer of a named buffer: B P Keys Order Nodes C I Hit Linear Guide
+ * Filtered History Pixels; NULL for an unknown name. */
+void* arbor_engine_buffer(arbor_engine* e, const char* name);
+hipStream_t arbor_engine_stream(arbor_engine* e);
+int arbor_engine_size(const arbor_engine* e, int* width, int* height);
+/* engine.js:55 inspect(): JSON into out (cap bytes). */
+int arbor_engine_inspect(const arbor_engine* e, char* out, int cap);
+/* Seeded walk-path input for frame `frame` (32 floats). */
+int arbor_walk_input(int seed, int frame, float* input32);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARBOR_ENGINE_H */

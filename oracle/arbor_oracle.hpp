@@ -684,9 +684,70 @@ template <class R> struct Arbor {
     return t > K(0.0001) ? t : FAR();
   }
 
+  // When set, traceTree runs the lean equivalent walk below instead of the
+  // reference's stack loop (used to cross-check the GPU's traversal order and
+  // to count its operations; results are bit-identical).
+  static inline bool leanWalk = false;
+
+  // Heap BVH walk with the traversal stack as a bit per heap depth: the
+  // reference pushes only the far sibling of the child it descends into, so
+  // the stack always holds far siblings along the current root path, deepest
+  // on top. Popping takes the deepest set bit d and visits the sibling of the
+  // path's depth-d node; a popped node is re-tested with the current best as
+  // the reference does. A child reached directly keeps the span its parent
+  // computed, and the ray reciprocal is computed once (identical values).
+  static F2 traceTreeLean(const R* P, const R* Nodes, const unsigned* Order, F3 ro, F3 rd, R cone, R time, R wind,
+                          R best, int ignore, bool shadows) {
+    const F3 inv = c3(invSafe(rd.x), invSafe(rd.y), invSafe(rd.z));
+    auto span_of = [&](unsigned n) {
+      F3 a = (ld3(Nodes, n * 8) - ro) * inv, b = (ld3(Nodes, n * 8 + 4) - ro) * inv;
+      F3 mn = vmin(a, b), mx = vmax(a, b);
+      return F2{m_max(mn.x, m_max(mn.y, mn.z)), m_min(mx.x, m_min(mx.y, mx.z))};
+    };
+    auto live = [&](F2 s, unsigned n) {
+      return Nodes[n * 8 + 3] > K(0.0) && s.y >= m_max(K(0.0001), s.x) && s.x < best;
+    };
+    auto depth = [](unsigned n) { return 31 - __builtin_clz(n); };
+    int found = -1;
+    unsigned node = 1, pending = 0;
+    F2 span = span_of(1);
+    bool hit = live(span, 1);
+    while (true) {
+      if (hit) {
+        if (node < (unsigned)CAPACITY) {
+          unsigned l = node * 2, r = l + 1;
+          F2 ls = span_of(l), rs = span_of(r);
+          bool lh = live(ls, l), rh = live(rs, r);
+          if (lh || rh) {
+            bool takeLeft = lh && (!rh || ls.x < rs.x);
+            if (lh && rh) pending |= 1u << (depth(node) + 1);
+            node = takeLeft ? l : r;
+            span = takeLeft ? ls : rs;
+            continue;
+          }
+        } else {
+          int id = (int)Order[node - CAPACITY];
+          if (id != ignore) {
+            R footprint = m_max(K(0.000005), cone * m_max(K(0.05), span.x));
+            R t = objectT(P, id, ro, rd, footprint, time, wind, !shadows);
+            if (t < best) { best = t; found = id; }
+          }
+        }
+      }
+      if (pending == 0) break;
+      int d = 31 - __builtin_clz(pending);
+      pending &= ~(1u << d);
+      node = (node >> (depth(node) - d)) ^ 1u;
+      span = span_of(node);
+      hit = live(span, node);
+    }
+    return {best, R(found)};
+  }
+
   // Heap BVH walk, explicit stack (Arbor.cu:220-230).
   static F2 traceTree(const R* P, const R* Nodes, const unsigned* Order, F3 ro, F3 rd, R cone, R time, R wind,
                       R best, int ignore, bool shadows) {
+    if (leanWalk) return traceTreeLean(P, Nodes, Order, ro, rd, cone, time, wind, best, ignore, shadows);
     int stack[32], top = 0, node = 1, found = -1;
     for (int visit = 0; visit < NODE_COUNT; visit++) {
       if (node <= 0) break;

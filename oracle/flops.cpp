@@ -1,0 +1,120 @@
+// Operation counter for the oracle (test / measurement infrastructure only).
+//
+// Instantiates the restatement with a float wrapper that counts every
+// arithmetic operation it performs, so the algorithmic work per primary ray
+// of a real workload is measured, not estimated. Counted as one operation
+// each: + - * / (and their compound forms), sqrt, min, max, abs, floor, and
+// every libm call (sin, cos, exp, pow, atan2); comparisons, negation and
+// conversions are free. The traversal counted is the lean walk the GPU runs
+// (same visiting order and results as the reference's stack loop, without
+// its re-tests of directly taken nodes).
+#include <cstdint>
+
+#include "arbor_oracle.hpp"
+
+namespace orc {
+
+struct OpCount {
+  uint64_t add = 0, mul = 0, div = 0, sqrt = 0, minmax = 0, misc = 0, trans = 0;
+};
+static OpCount g_ops;  // single-threaded use only
+
+struct CF {
+  float v;
+  CF() = default;
+  constexpr CF(float x) : v(x) {}
+  constexpr CF(double x) : v((float)x) {}
+  constexpr CF(int x) : v((float)x) {}
+  constexpr CF(unsigned x) : v((float)x) {}
+  explicit operator int() const { return (int)v; }
+  explicit operator unsigned() const { return (unsigned)v; }
+  CF& operator+=(CF o) { g_ops.add++; v += o.v; return *this; }
+  CF& operator-=(CF o) { g_ops.add++; v -= o.v; return *this; }
+  CF& operator*=(CF o) { g_ops.mul++; v *= o.v; return *this; }
+  CF& operator/=(CF o) { g_ops.div++; v /= o.v; return *this; }
+};
+inline CF operator+(CF a, CF b) { g_ops.add++; return CF(a.v + b.v); }
+inline CF operator-(CF a, CF b) { g_ops.add++; return CF(a.v - b.v); }
+inline CF operator*(CF a, CF b) { g_ops.mul++; return CF(a.v * b.v); }
+inline CF operator/(CF a, CF b) { g_ops.div++; return CF(a.v / b.v); }
+inline CF operator-(CF a) { return CF(-a.v); }
+inline bool operator<(CF a, CF b) { return a.v < b.v; }
+inline bool operator>(CF a, CF b) { return a.v > b.v; }
+inline bool operator<=(CF a, CF b) { return a.v <= b.v; }
+inline bool operator>=(CF a, CF b) { return a.v >= b.v; }
+inline bool operator==(CF a, CF b) { return a.v == b.v; }
+inline bool operator!=(CF a, CF b) { return a.v != b.v; }
+inline CF m_sin(CF x) { g_ops.trans++; return CF(::sinf(x.v)); }
+inline CF m_cos(CF x) { g_ops.trans++; return CF(::cosf(x.v)); }
+inline CF m_exp(CF x) { g_ops.trans++; return CF(::expf(x.v)); }
+inline CF m_pow(CF x, CF y) { g_ops.trans++; return CF(::powf(x.v, y.v)); }
+inline CF m_atan2(CF y, CF x) { g_ops.trans++; return CF(::atan2f(y.v, x.v)); }
+inline CF m_sqrt(CF x) { g_ops.sqrt++; return CF(::sqrtf(x.v)); }
+inline CF m_floor(CF x) { g_ops.misc++; return CF(::floorf(x.v)); }
+inline CF m_abs(CF x) { g_ops.misc++; return CF(::fabsf(x.v)); }
+inline CF m_min(CF a, CF b) { g_ops.minmax++; return CF(::fminf(a.v, b.v)); }
+inline CF m_max(CF a, CF b) { g_ops.minmax++; return CF(::fmaxf(a.v, b.v)); }
+
+}  // namespace orc
+
+using namespace orc;
+using ACF = Arbor<CF>;
+
+namespace {
+std::vector<CF> wrap(const float* a, size_t n) {
+  std::vector<CF> v(n);
+  for (size_t i = 0; i < n; i++) v[i] = CF(a[i]);
+  return v;
+}
+}  // namespace
+
+extern "C" {
+// Counts the operations of tracePixels (forest, lean walk) over every
+// stride-th pixel in x and y of a width x height view. out[0] rays, out[1]
+// total operations, out[2..8] add, mul, div, sqrt, min/max, abs/floor, libm.
+int orc_count_trace_ops(const float* P, const float* Nodes, const unsigned* Order, const float* C, int width,
+                        int height, int stride, int single_tree, double* out) {
+  if (stride < 1) return -1;
+  auto p = wrap(P, (size_t)CAPACITY * PS), n = wrap(Nodes, (size_t)NODE_COUNT * 8), c = wrap(C, 64);
+  std::vector<CF> hit((size_t)width * height * 4);
+  ACF::leanWalk = true;
+  g_ops = OpCount();
+  double rays = 0;
+  for (int y = stride / 2; y < height; y += stride)
+    for (int x = stride / 2; x < width; x += stride) {
+      CF* h = hit.data();
+      if (single_tree) ACF::traceTreePixel(p.data(), n.data(), Order, c.data(), h, width, height, x, y);
+      else ACF::tracePixel(p.data(), n.data(), Order, c.data(), h, width, height, x, y);
+      rays += 1;
+    }
+  ACF::leanWalk = false;
+  const OpCount& o = g_ops;
+  out[0] = rays;
+  out[1] = (double)(o.add + o.mul + o.div + o.sqrt + o.minmax + o.misc + o.trans);
+  out[2] = (double)o.add; out[3] = (double)o.mul; out[4] = (double)o.div; out[5] = (double)o.sqrt;
+  out[6] = (double)o.minmax; out[7] = (double)o.misc; out[8] = (double)o.trans;
+  return 0;
+}
+
+// Reference walk vs lean walk over whole frames: returns the number of
+// pixels whose Hit differs in any bit (0 expected).
+int orc_lean_walk_mismatches(const float* P, const float* Nodes, const unsigned* Order, const float* C, int width,
+                             int height, int single_tree) {
+  using A = Arbor<float>;
+  std::vector<float> a((size_t)width * height * 4), b(a.size());
+  for (int lean = 0; lean < 2; lean++) {
+    A::leanWalk = lean != 0;
+    float* H = lean ? b.data() : a.data();
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = 0; y < height; y++)
+      for (int x = 0; x < width; x++)
+        if (single_tree) A::traceTreePixel(P, Nodes, Order, C, H, width, height, x, y);
+        else A::tracePixel(P, Nodes, Order, C, H, width, height, x, y);
+  }
+  A::leanWalk = false;
+  int bad = 0;
+  for (size_t i = 0; i < a.size(); i += 4)
+    if (std::memcmp(&a[i], &b[i], 16) != 0) bad++;
+  return bad;
+}
+}  // extern "C"

@@ -171,6 +171,26 @@ class Oracle(_Common):
         self._tube_t = _bind(L, "orc_tube_t", _f, [_f32p, _f32p, _f32p, _f32p, _f, _f])
         self._aces = _bind(L, "orc_aces", _f, [_f])
         self._forest = _bind(L, "orc_forest_table", _i, [_i, _f32p])
+        self._count_ops = _bind(L, "orc_count_trace_ops", _i,
+                                [_f32p, _f32p, _u32p, _f32p, _i, _i, _i, _i,
+                                 np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")])
+        self._lean_mismatch = _bind(L, "orc_lean_walk_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i, _i])
+
+    def count_trace_ops(self, s, C, w, h, stride, single_tree=False):
+        """Operations per primary ray (lean walk), sampled every `stride` pixels.
+        Returns dict with rays, ops_per_ray and the per-kind breakdown per ray."""
+        out = np.zeros(9, np.float64)
+        rc = self._count_ops(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h, stride,
+                             int(single_tree), out)
+        assert rc == 0
+        rays = out[0]
+        kinds = ("add", "mul", "div", "sqrt", "minmax", "absfloor", "libm")
+        return {"rays": int(rays), "ops_per_ray": out[1] / rays,
+                **{k: out[2 + i] / rays for i, k in enumerate(kinds)}}
+
+    def lean_walk_mismatches(self, s, C, w, h, single_tree=False):
+        return self._lean_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h,
+                                   int(single_tree))
 
     # -- FP64 run: float inputs promoted to double, arithmetic in double --
     def trace64(self, s, C, w, h, threads=0):

@@ -1,0 +1,117 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every symbol
+the public headers declare, and rejects bad arguments with status codes
+before touching a device (the reference's host-forcible failures:
+engine.js:28 seed range, engine.js:36-37 dimensions, runtime.js:352 limits)."""
+import ctypes as ct
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyarbor
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NULL = None
+DUMMY = ct.c_void_p(4096)  # never dereferenced: every case below fails validation first
+
+
+def declared_symbols():
+    names = set()
+    for h in ("arbor.h", "arbor_engine.h"):
+        with open(os.path.join(REPO, "include", h)) as f:
+            text = f.read()
+        names |= set(re.findall(r"\b(arbor_\w+)\s*\(", text))
+    return sorted(names)
+
+
+@pytest.fixture(scope="module")
+def L():
+    return pyarbor.lib()
+
+
+def test_every_declared_symbol_is_exported(L):
+    names = declared_symbols()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in pyarbor.SIGNATURES, n
+
+
+def test_twelve_reference_entry_points_present(L):
+    for n in ("growTree", "makePrimitives", "makeKeys", "sortPass", "makeBounds", "reduceBounds", "initCamera",
+              "updateCamera", "tracePixels", "shadePixels", "filterLighting", "resolvePixels"):
+        assert hasattr(L, "arbor_" + n)
+
+
+def test_null_pointers_rejected(L):
+    E = -1
+    assert L.arbor_growTree(NULL, 0, 1, NULL) == E
+    assert L.arbor_makePrimitives(DUMMY, NULL, 1, NULL) == E
+    assert L.arbor_makeKeys(DUMMY, DUMMY, NULL, NULL) == E
+    assert L.arbor_sortPass(NULL, DUMMY, 1, 2, NULL) == E
+    assert L.arbor_makeBounds(DUMMY, NULL, DUMMY, NULL) == E
+    assert L.arbor_reduceBounds(NULL, 1, NULL) == E
+    assert L.arbor_initCamera(NULL, NULL, 1, NULL) == E
+    assert L.arbor_updateCamera(DUMMY, NULL, NULL, 0.01, 64, 64, NULL) == E
+    assert L.arbor_tracePixels(DUMMY, DUMMY, DUMMY, DUMMY, NULL, 64, 64, NULL) == E
+    assert L.arbor_shadePixels(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, NULL, 64, 64, NULL) == E
+    assert L.arbor_filterLighting(DUMMY, DUMMY, DUMMY, NULL, DUMMY, 64, 64, NULL) == E
+    assert L.arbor_resolvePixels(DUMMY, DUMMY, NULL, DUMMY, 64, 64, NULL) == E
+    assert L.arbor_sortKeys(NULL, DUMMY, NULL) == E
+    assert L.arbor_buildBVH(DUMMY, DUMMY, DUMMY, NULL, NULL) == E
+    assert L.arbor_traceTreePixels(NULL, DUMMY, DUMMY, DUMMY, DUMMY, 8, 8, NULL) == E
+
+
+def test_scalar_ranges_rejected(L):
+    A = -2
+    assert L.arbor_growTree(DUMMY, 5, 1, NULL) == A
+    assert L.arbor_growTree(DUMMY, -1, 1, NULL) == A
+    assert L.arbor_growTree(DUMMY, 0, -1, NULL) == A  # engine.js:28 seed range
+    assert L.arbor_growTree(DUMMY, 0, 1000001, NULL) == A
+    assert L.arbor_makePrimitives(DUMMY, DUMMY, -5, NULL) == A
+    for d, s in ((3, 8), (2, 2), (4, 2), (1, 65536), (0, 2)):
+        assert L.arbor_sortPass(DUMMY, DUMMY, d, s, NULL) == A, (d, s)
+    for first in (0, 3, 32768, -4):
+        assert L.arbor_reduceBounds(DUMMY, first, NULL) == A, first
+    for w, h in ((0, 10), (10, 0), (8193, 10), (-1, -1)):
+        assert L.arbor_tracePixels(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, w, h, NULL) == A
+        assert L.arbor_resolvePixels(DUMMY, DUMMY, DUMMY, DUMMY, w, h, NULL) == A
+        assert L.arbor_updateCamera(DUMMY, DUMMY, NULL, 0.0, w, h, NULL) == A
+    assert L.arbor_initCamera(DUMMY, NULL, 1000001, NULL) == A
+    assert L.arbor_generate(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, -1, NULL, 0, 0, NULL) == A
+
+
+def test_engine_argument_checks(L):
+    h = ct.c_void_p()
+    assert L.arbor_engine_create(None, 64, 64, 1, 0, NULL) == -1
+    assert L.arbor_engine_create(ct.byref(h), 64, 64, -1, 0, NULL) == -2  # engine.js:28
+    assert not h.value
+    assert L.arbor_engine_frame(NULL, 0.016, NULL) == -1
+    assert L.arbor_engine_destroy(NULL) == -1
+    assert L.arbor_engine_buffer(NULL, b"P") is None
+
+
+def test_engine_resize_rounding(L):
+    # engine.js:36: width -> multiple of 64 (min 320), height -> multiple of 8 (min 192)
+    for (w, h), want in (((1920, 1080), (1920, 1080)), ((1000, 701), (1024, 704)), ((100, 100), (320, 192)),
+                         ((3840, 2160), (3840, 2160)), ((256, 256), (320, 256))):
+        cw, ch = ct.c_int(w), ct.c_int(h)
+        L.arbor_engine_round_size(ct.byref(cw), ct.byref(ch))
+        assert (cw.value, ch.value) == want
+
+
+def test_walk_input_deterministic(L):
+    a = pyarbor.walk_input(7, 3)
+    b = pyarbor.walk_input(7, 3)
+    assert np.array_equal(a, b)
+    seen = set()
+    for f in range(200):
+        I = pyarbor.walk_input(7, f)
+        assert I[15] in (-1.0, 0.0, 1.0) and I[13] in (-1.0, 0.0, 1.0)
+        assert -20.0 <= I[0] < 20.0
+        assert (np.delete(I, [0, 13, 15]) == 0).all()
+        seen.add((I[15], I[13]))
+    assert len(seen) >= 6
+    assert not np.array_equal(pyarbor.walk_input(7, 3), pyarbor.walk_input(8, 3))
+    assert L.arbor_walk_input(7, -1, (ct.c_float * 32)()) == -2

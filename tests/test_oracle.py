@@ -239,6 +239,30 @@ def test_aces_is_narkowicz_fit(oracle):
         assert oracle.aces(float(x)) == pytest.approx(want, abs=2e-6)
 
 
+@pytest.mark.parametrize("view,single", [(1, False), (3, False), (5, False), (2, True)])
+def test_lean_walk_is_the_reference_walk(oracle, scenes, view, single):
+    """The GPU's traversal (stack as one bit per heap depth, no re-test of
+    directly taken children, hoisted reciprocal) gives bit-identical Hit
+    buffers to the reference's stack[32] loop (Arbor.cu:220-230)."""
+    C = oracle.init_camera(7)
+    I = np.zeros(32, np.float32)
+    I[5] = view
+    oracle.update_camera(C, I, 1.0 / 60, 200, 120)
+    C[22] = 0
+    assert oracle.lean_walk_mismatches(scenes["7"], C, 200, 120, single) == 0
+
+
+def test_operation_count_is_stable(oracle, scenes):
+    """The op counter is deterministic and reports the workload's mix."""
+    C = oracle.init_camera(7)
+    a = oracle.count_trace_ops(scenes["7"], C, 160, 90, 4)
+    b = oracle.count_trace_ops(scenes["7"], C, 160, 90, 4)
+    assert a == b and a["rays"] == 40 * 22
+    parts = sum(a[k] for k in ("add", "mul", "div", "sqrt", "minmax", "absfloor", "libm"))
+    assert parts == pytest.approx(a["ops_per_ray"])
+    assert a["ops_per_ray"] > 1000
+
+
 def test_fp64_run_agrees_with_fp32(oracle, scenes):
     """The double instantiation is the same algorithm.
 
