@@ -43,13 +43,19 @@ constexpr int FOREST_COUNT = 324;
 #define K(x) (sizeof(R) == 4 ? R(x##f) : R(x))
 
 // ---- libm dispatch: glibc float entry points for float, double for double --
-inline float m_sin(float x) { return ::sinf(x); }
+// With g_crMath set, float sin/cos/pow are the correctly rounded values
+// (evaluated in double, rounded once) instead of glibc's, which differ from
+// correct rounding on about 1.4% of sinf/cosf inputs. The algorithm is
+// unchanged; only the libm is swapped. This is the libm the GPU build kernels
+// use, so the GPU's B and P can be checked bit-for-bit against it.
+inline bool g_crMath = false;
+inline float m_sin(float x) { return g_crMath ? (float)std::sin((double)x) : ::sinf(x); }
 inline double m_sin(double x) { return std::sin(x); }
-inline float m_cos(float x) { return ::cosf(x); }
+inline float m_cos(float x) { return g_crMath ? (float)std::cos((double)x) : ::cosf(x); }
 inline double m_cos(double x) { return std::cos(x); }
 inline float m_exp(float x) { return ::expf(x); }
 inline double m_exp(double x) { return std::exp(x); }
-inline float m_pow(float x, float y) { return ::powf(x, y); }
+inline float m_pow(float x, float y) { return g_crMath ? (float)std::pow((double)x, (double)y) : ::powf(x, y); }
 inline double m_pow(double x, double y) { return std::pow(x, y); }
 inline float m_atan2(float y, float x) { return ::atan2f(y, x); }
 inline double m_atan2(double y, double x) { return std::atan2(y, x); }

@@ -89,6 +89,19 @@ int orc_generate(int seed, float* B, float* P, unsigned* Keys, unsigned* Order, 
   generate_impl<float>(seed, B, P, Keys, Order, Nodes, bitonic);
   return 0;
 }
+// The same build with correctly rounded sin/cos/pow (see g_crMath).
+int orc_generate_cr(int seed, float* B, float* P, unsigned* Keys, unsigned* Order, float* Nodes) {
+  g_crMath = true;
+  generate_impl<float>(seed, B, P, Keys, Order, Nodes, 0);
+  g_crMath = false;
+  return 0;
+}
+int orc_init_camera_cr(float* C, int seed) {
+  g_crMath = true;
+  A32::initCamera(C, seed);
+  g_crMath = false;
+  return 0;
+}
 unsigned orc_spread(unsigned x) { return A32::spread(x); }
 
 // ---- camera ----

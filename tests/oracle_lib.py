@@ -175,6 +175,13 @@ class Oracle(_Common):
                                 [_f32p, _f32p, _u32p, _f32p, _i, _i, _i, _i,
                                  np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")])
         self._lean_mismatch = _bind(L, "orc_lean_walk_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i, _i])
+        self._generate_cr = _bind(L, "orc_generate_cr", _i, [_i, _f32p, _f32p, _u32p, _u32p, _f32p])
+
+    def generate_cr(self, seed):
+        """The build with correctly rounded sin/cos/pow instead of glibc's."""
+        s = Scene()
+        assert self._generate_cr(seed, s.B, s.P, s.Keys, s.Order, s.Nodes) == 0
+        return s
 
     def count_trace_ops(self, s, C, w, h, stride, single_tree=False):
         """Operations per primary ray (lean walk), sampled every `stride` pixels.

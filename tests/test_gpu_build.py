@@ -1,9 +1,13 @@
 """GPU parity of the one-time build (growTree ... reduceBounds, sortKeys,
 buildBVH, initCamera/updateCamera) through the C-ABI against the oracle.
 
-Bar: Keys, Order and Nodes bit-exact (keys, permutation and node topology);
-Nodes boxes bit-exact when built from identical P; B and P within 2 ulp
-(their libm calls are evaluated in double and rounded once on the GPU).
+Bar: Keys, Order and the node topology (counts, primitive ids) bit-exact
+with the reference-harness restatement (glibc libm); Nodes bit-exact when
+built from identical P. The GPU evaluates the build's sin/cos/pow correctly
+rounded (in double, rounded once); glibc's sinf/cosf are not correctly
+rounded on ~1.4% of inputs, so B and P differ from the glibc restatement by
+at most 1e-5 m and are bit-exact against the restatement with correctly
+rounded sin/cos/pow (oracle generate_cr).
 """
 import numpy as np
 import pytest
@@ -33,16 +37,18 @@ def test_generate_matches_oracle(px, oracle, seed):
     print("case: arbor_generate seed", seed, flush=True)
     dev = px.DeviceScene().generate(seed)
     ref = oracle.generate(seed)
-    dB = _report("B", dev.host("B"), ref.B)
-    dP = _report("P", dev.host("P"), ref.P)
-    assert dB.max() <= 2 and dP.max() <= 2
+    B, P = dev.host("B"), dev.host("P")
+    _report("B vs glibc restatement", B, ref.B)
+    _report("P vs glibc restatement", P, ref.P)
+    assert np.abs(B - ref.B).max() <= 1e-5 and np.abs(P - ref.P).max() <= 1e-5
     assert np.array_equal(dev.host("Keys"), ref.Keys)
     assert np.array_equal(dev.host("Order"), ref.Order)
     N, R = dev.host("Nodes").reshape(-1, 8), ref.Nodes.reshape(-1, 8)
     assert np.array_equal(N[:, 3], R[:, 3]) and np.array_equal(N[:, 7], R[:, 7])  # topology, counts, ids
-    if dP.max() == 0:
-        assert np.array_equal(N.view(np.uint32), R.view(np.uint32))
     assert N[1, 3] == OBJECT_COUNT
+    cr = oracle.generate_cr(seed)
+    for name, got, want in (("B", B, cr.B), ("P", P, cr.P), ("Nodes", N.reshape(-1), cr.Nodes)):
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), name + " vs correctly rounded libm"
     C = dev.host("C")
     assert np.array_equal(C.view(np.uint32), oracle.init_camera(seed).view(np.uint32))
 
@@ -69,16 +75,17 @@ def test_reference_stage_sequence(px, oracle):
 
     seed = 1788
     ref = oracle.generate(seed)
+    cr = oracle.generate_cr(seed)
     s = px.DeviceScene()
     for level in range(5):
         px.call("arbor_growTree", s.B, level, seed)
     B = s.host("B")
-    assert ulp_diff(B, ref.B).max() <= 2
-    # makePrimitives from the oracle's B isolates the stage
-    Bd = to_dev(ref.B)
+    assert np.array_equal(B.view(np.uint32), cr.B.view(np.uint32))
+    # makePrimitives from the (correctly rounded) oracle's B isolates the stage
+    Bd = to_dev(cr.B)
     Pd = torch.zeros(CAPACITY * PS, dtype=torch.float32, device="cuda")
     px.call("arbor_makePrimitives", Bd, Pd, seed)
-    assert ulp_diff(to_host(Pd), ref.P).max() <= 2
+    assert np.array_equal(to_host(Pd).view(np.uint32), cr.P.view(np.uint32))
     # keys -> bitonic -> bounds -> reduce from the oracle's P
     P = to_dev(ref.P)
     px.call("arbor_makeKeys", P, s.Keys, s.Order)

@@ -5,9 +5,9 @@ oracle's FP32 restatement and its FP64 run.
 The reference algorithm is ill-conditioned in FP32 (its own FP32 and FP64
 runs disagree on a few percent of pixels: the 7-step leaf Newton solve and
 the 3e-4 acceptance threshold). The bar, per view:
-  * hit ids agree with the FP32 oracle except on near-tie rays; a near-tie
-    ray is one where the FP32 and FP64 oracle runs themselves disagree, or
-    whose FP32 t is within 1e-3 relative of the FP64 t of a different id;
+  * hit ids agree with the FP32 oracle except on near-tie rays (rays where
+    the FP32 and FP64 oracle runs themselves disagree), with at most 2e-4 of
+    the view's pixels disagreeing outside them (measured: <= 3 of 57,600);
   * on rays where the GPU and FP64 ids agree, t relative error <= 1e-5 for
     at least the fraction the FP32 oracle itself reaches, minus 0.5%;
   * radiance (Linear) within 1e-3 absolute of the FP64 run on the same
@@ -64,7 +64,7 @@ def _check_hits(tag, gpu, h32, h64):
     tie = _near_tie(h32, h64)
     bad = ~s32["same_id_mask"] & ~tie
     print(tag, "id mismatches outside near-ties:", int(bad.sum()), "of", bad.size)
-    assert bad.mean() <= 0.01
+    assert bad.mean() <= 2e-4  # measured <= 5.2e-5 (3 of 57,600 pixels)
     assert s64["t_rel_le_1e5"] >= ref["t_rel_le_1e5"] - 0.005
     assert s64["id_match"] >= ref["id_match"] - 0.01
     return s32, s64, ref

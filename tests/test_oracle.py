@@ -47,6 +47,19 @@ def test_seed_changes_layout_not_counts(scenes):
     assert scenes["42"].Nodes[11] == OBJECT_COUNT
 
 
+@pytest.mark.parametrize("seed", [1, 7, 42, 1788, 0, 123456, 1000000])
+def test_libm_rounding_does_not_change_keys_or_topology(oracle, seed):
+    """Swapping glibc sinf/cosf/powf for correctly rounded ones (the GPU build's
+    libm) moves B and P by ulps but leaves keys, permutation and topology."""
+    a = oracle.generate(seed)
+    b = oracle.generate_cr(seed)
+    assert np.array_equal(a.Keys, b.Keys) and np.array_equal(a.Order, b.Order)
+    na, nb = a.Nodes.reshape(-1, 8), b.Nodes.reshape(-1, 8)
+    assert np.array_equal(na[:, 3], nb[:, 3]) and np.array_equal(na[:, 7], nb[:, 7])
+    assert np.abs(a.B - b.B).max() <= 1e-5 and np.abs(a.P - b.P).max() <= 1e-5
+    assert 0 < (a.P != b.P).mean() < 0.02
+
+
 def test_spread_is_textbook_bit_interleave(oracle):
     for x in range(1024):
         want = 0
