@@ -5,8 +5,12 @@
 // waves builds the 324-tree instancing table in LDS once and then walks tiles
 // in a grid-stride loop, so the table and camera frame are amortised over many
 // tiles. Filter and resolve are HBM-bound per-pixel kernels over float4 rows.
+#include <string.h>
+
+#include "arbor_batched.hpp"
 #include "arbor_shade.hpp"
 #include "launch.hpp"
+#include "tuning.hpp"
 
 namespace arbor {
 
@@ -35,6 +39,28 @@ __global__ __launch_bounds__(kTileBlock) void k_tracePixels(const float* P, cons
     int tree = -1;
     v2 h = traceForest(P, Nodes4, Order, trees, ro, rd, cone, time, wind, ground > 0.0f ? ground : kFar, &tree, false);
     Hit[y * width + x] = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)height);
+  }
+}
+
+// tracePixels with the wave-cooperative walk (leaf tests batched per wave).
+__global__ __launch_bounds__(kTileBlock) void k_tracePixelsBatched(const float* P, const float4* Nodes4,
+                                                                   const uint32_t* Order, const float* C, float4* Hit,
+                                                                   int width, int height, int tilesX, int tiles) {
+  __shared__ TreeInst trees[kTrees];
+  fillTrees(trees, (int)C[31]);
+  const View view = makeView(C);
+  const float cone = C[14] / (float)height, time = C[8], wind = C[11];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x * kWavesPerBlock + wave; t < tiles; t += gridDim.x * kWavesPerBlock) {
+    int x = (t % tilesX) * 8 + (lane & 7), y = (t / tilesX) * 8 + (lane >> 3);
+    bool inside = x < width && y < height;
+    v3 ro = view.eye, rd = viewRay(view, inside ? x : 0, inside ? y : 0, width, height);
+    float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
+    ForestRay s;
+    forestBegin(s, ro, rd, ground > 0.0f ? ground : kFar);
+    if (!inside) s.done = true;
+    forestTraceBatched(s, P, Nodes4, Order, trees, cone, time, wind, false);
+    if (inside) Hit[y * width + x] = make_float4(s.best, s.found, (float)s.foundTree, C[14] * s.best / (float)height);
   }
 }
 
@@ -224,11 +250,12 @@ __global__ __launch_bounds__(256) void k_resolvePixels(const float4* Filtered, f
 using namespace arbor;
 
 namespace {
-// Grid for the tile kernels: enough blocks to fill 256 CUs several times over,
-// never more than there are tiles.
+// Grid for the tile kernels: enough blocks to fill 256 CUs several times over
+// (the cap is a tuning knob), never more than there are tiles.
 inline int tileGrid(int tiles) {
   int blocks = (tiles + kWavesPerBlock - 1) / kWavesPerBlock;
-  return blocks < 2048 ? blocks : 2048;
+  int cap = tuning().grid_cap;
+  return blocks < cap ? blocks : cap;
 }
 }  // namespace
 
@@ -239,7 +266,8 @@ int arbor_tracePixels(const float* P, const float* Nodes, const unsigned* Order,
   ARBOR_REQUIRE(P && Nodes && Order && C && Hit);
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
-  hipLaunchKernelGGL(k_tracePixels, dim3(tileGrid(tiles)), dim3(kTileBlock), 0, stream, P,
+  auto kernel = tuning().trace_variant == 1 ? k_tracePixelsBatched : k_tracePixels;
+  hipLaunchKernelGGL(kernel, dim3(tileGrid(tiles)), dim3(kTileBlock), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
                      tx, tiles);
   return launch_status();
@@ -289,5 +317,29 @@ int arbor_resolvePixels(const float* Filtered, float* History, unsigned* Pixels,
 }
 
 const char* arbor_version(void) { return "arbor-mi355x 0.1 gfx950"; }
+
+int arbor_tune_set(const char* key, int value) {
+  if (!key) return ARBOR_ERR_NULL;
+  Tuning& t = tuning();
+  if (!strcmp(key, "trace_variant")) {
+    if (value < 0 || value > 1) return ARBOR_ERR_ARG;
+    t.trace_variant = value;
+    return ARBOR_OK;
+  }
+  if (!strcmp(key, "grid_cap")) {
+    if (value < 1 || value > 65535) return ARBOR_ERR_ARG;
+    t.grid_cap = value;
+    return ARBOR_OK;
+  }
+  return ARBOR_ERR_ARG;
+}
+
+int arbor_tune_get(const char* key) {
+  if (!key) return ARBOR_ERR_NULL;
+  const Tuning& t = tuning();
+  if (!strcmp(key, "trace_variant")) return t.trace_variant;
+  if (!strcmp(key, "grid_cap")) return t.grid_cap;
+  return ARBOR_ERR_ARG;
+}
 
 }  // extern "C"

@@ -45,6 +45,8 @@ SIGNATURES = {
     "arbor_generate": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
     "arbor_traceTreePixels": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_version": [],
+    "arbor_tune_set": [ct.c_char_p, _i],
+    "arbor_tune_get": [ct.c_char_p],
     "arbor_engine_round_size": [ct.POINTER(_i), ct.POINTER(_i)],
     "arbor_engine_create": [ct.POINTER(_vp), _i, _i, _i, _i, _vp],
     "arbor_engine_destroy": [_vp],
@@ -185,6 +187,17 @@ def filter_lighting(C, fb):
 
 def resolve(C, fb):
     call("arbor_resolvePixels", fb.Filtered, fb.History, fb.Pixels, C, fb.width, fb.height)
+
+
+def tune_set(key, value):
+    check(lib().arbor_tune_set(key.encode(), int(value)), "arbor_tune_set(%s)" % key)
+
+
+def tune_get(key):
+    v = lib().arbor_tune_get(key.encode())
+    if v < 0:
+        raise ArborError("unknown tuning key " + key)
+    return v
 
 
 def walk_input(seed, frame):

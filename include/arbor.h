@@ -130,6 +130,14 @@ int arbor_traceTreePixels(const float* P, const float* Nodes, const unsigned* Or
 /* Library build identification (git-independent). */
 const char* arbor_version(void);
 
+/* Launch tuning (no counterpart in the reference, whose launch shapes are
+ * fixed by its WebGPU host). Keys: "trace_variant" (0 per-lane walk, 1
+ * wave-batched leaf tests), "grid_cap" (max blocks of a tile kernel, 1..65535).
+ * Every setting gives identical results. Process-wide; set before launching. */
+int arbor_tune_set(const char* key, int value);
+/* Current value of a tuning key, or a negative status for an unknown key. */
+int arbor_tune_get(const char* key);
+
 #ifdef __cplusplus
 }
 #endif

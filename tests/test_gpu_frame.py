@@ -104,6 +104,29 @@ def test_forest_trace(px, oracle, scene7, view):
     _check_hits("forest v%d" % view, gpu, h32, h64)
 
 
+def test_trace_variants_identical(px, oracle, scene7):
+    """Every trace_variant / grid_cap setting gives bit-identical Hit (they
+    change only how lanes are scheduled)."""
+    print("case: tracePixels variants seed 7 view 2 240x136", flush=True)
+    dev, _ = scene7
+    w, h = 240, 136
+    Cd = to_dev(_camera(oracle, 7, 2, w, h))
+    old = (px.tune_get("trace_variant"), px.tune_get("grid_cap"))
+    outs = []
+    try:
+        for variant, cap in ((0, 2048), (1, 2048), (1, 7), (0, 5)):
+            px.tune_set("trace_variant", variant)
+            px.tune_set("grid_cap", cap)
+            fb = px.FrameBuffers(w, h)
+            px.trace(dev, Cd, fb)
+            outs.append(to_host(fb.Hit))
+    finally:
+        px.tune_set("trace_variant", old[0])
+        px.tune_set("grid_cap", old[1])
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32))
+
+
 @pytest.mark.parametrize("view,shadows", [(1, 0), (3, 1), (4, 0)])
 def test_shade_filter_resolve(px, oracle, scene7, view, shadows):
     """Shading on identical hits (the oracle's FP32 Hit uploaded) isolates the
