@@ -90,13 +90,14 @@ AD float specGGX(v3 n, v3 v, v3 l, float rough) {
   return D * G * F / (4.0f * nv + 0.0001f);
 }
 // Up to four shadow traversals through leaves (Arbor.cu:271-274).
-__device__ inline v3 lightVisibility(const float* P, const float4* Nodes4, const uint32_t* Order, const TreeInst* trees,
-                                     v3 p, v3 direction, float time, float wind) {
+template <class W>
+__device__ inline v3 lightVisibility(const float* P, const float4* Nodes4, const float4* top, const uint32_t* Order,
+                                     const TreeInst* trees, v3 p, v3 direction, float time, float wind) {
   v3 transmission = mk(1.0f, 1.0f, 1.0f);
   v3 ro = p;
   for (int j = 0; j < 4; j++) {
     int tree = -1;
-    v2 h = traceForest(P, Nodes4, Order, trees, ro, direction, 0.0001f, time, wind, 36.0f, &tree, true);
+    v2 h = traceForest<W>(P, Nodes4, top, Order, trees, ro, direction, 0.0001f, time, wind, 36.0f, &tree, true);
     int id = (int)h.y;
     if (id < 0) break;
     if (P[id * kPS + 3] != 1.0f) return mk(0.0f, 0.0f, 0.0f);

@@ -124,6 +124,26 @@ AD LeafFrame leafFrame(const Prim& pr, float time, float wind) {
   return f;
 }
 
+// Conservative rejection: true when the ray's line passes farther than the
+// padded radius from `centre`, or the whole padded sphere lies behind the
+// origin. The perpendicular offset is formed as q - rd*(q.rd) (no
+// cancellation of two large squares). The pad is 1 mm plus padPerMetre*|q|:
+// sphereT and tubeT decide hits from discriminants that cancel
+// catastrophically far from a small object, so the reference reports hits up
+// to ~8e-4*|q| outside a thin tube or sphere at distance |q|; tests ending in
+// such a discriminant use 4e-3 per metre. The leaf slab (boxSpan) and the wood
+// SDF march (which must get within 2.5e-4 m of the surface) are well
+// conditioned and use 1 mm. A rejected object is one the exact tests miss.
+AD bool sphereMiss(v3 ro, v3 rd, v3 centre, float radius, float padPerMetre) {
+  v3 q = ro - centre;
+  float b = dot(q, rd);
+  v3 p = q - rd * b;
+  float r = radius + 0.001f + padPerMetre * length(q);
+  return dot(p, p) > r * r || b > r;
+}
+constexpr float kDiscriminantPad = 0.004f;
+constexpr float kSlabPad = 0.00001f;
+
 // ---- exact intersections (Arbor.cu:188-219) ----
 __device__ inline float tubeT(v3 ro, v3 rd, v3 a, v3 b, float ra, float rb) {
   v3 axis = b - a;
@@ -149,8 +169,15 @@ __device__ inline float tubeT(v3 ro, v3 rd, v3 a, v3 b, float ra, float rb) {
   }
   return best;
 }
-__device__ inline float woodT(const Prim& pr, v3 ro, v3 rd, float footprint, bool refine) {
+__device__ inline float woodT(const Prim& pr, v3 ro, v3 rd, float footprint, bool refine, bool cull) {
   v3 a = pr.a(), end = pr.b();
+  // Every surface woodT can return lies inside the radius-max(ra,rb) capsule
+  // (bark only cuts inward); a ray missing its bounding sphere returns FAR.
+  if (cull) {
+    v3 half = (end - a) * 0.5f;
+    float r = length(half) + fmaxf(pr.ra(), pr.rb());
+    if (sphereMiss(ro, rd, a + half, r, refine ? 0.0f : kDiscriminantPad)) return kFar;
+  }
   if (!refine) {
     float ra = pr.ra() - fminf(0.021f, pr.ra() * 0.055f) * 0.32f;
     float rb = pr.rb() - fminf(0.021f, pr.rb() * 0.055f) * 0.32f;
@@ -170,16 +197,28 @@ __device__ inline float woodT(const Prim& pr, v3 ro, v3 rd, float footprint, boo
   }
   return kFar;
 }
-__device__ inline float leafT(const Prim& pr, v3 ro, v3 rd, float footprint, float time, float wind) {
+__device__ inline float leafT(const Prim& pr, v3 ro, v3 rd, float footprint, float time, float wind, bool cull) {
   v3 origin = pr.a();
   LeafFrame fr = leafFrame(pr, time, wind);
   v3 u = fr.u, v = pr.c(), n = fr.n;
   float stem = pr.f16(), len = pr.ra(), width = pr.rb();
   float curl = pr.f17(), twist = pr.f18();
+  // Separate rejections for the petiole capsule (radius <= 1.1 mm along u
+  // over [0, stem]) and the Newton slab (u in [stem-0.001, stem+len+0.001],
+  // |v| <= width, n in [-0.035, 0.065]); a ray that misses both returns FAR.
+  bool petiole = true, slab = true;
+  if (cull) {
+    petiole = !sphereMiss(ro, rd, origin + u * (stem * 0.5f), stem * 0.5f + 0.0011f, kDiscriminantPad);
+    float hx = len * 0.5f + 0.001f;
+    slab = !sphereMiss(ro, rd, origin + u * (stem + len * 0.5f) + n * 0.015f,
+                       sqrtf(hx * hx + width * width + 0.05f * 0.05f), kSlabPad);
+    if (!petiole && !slab) return kFar;
+  }
   v3 q = ro - origin;
   float x = dot(q, u) - stem, y = dot(q, v), z = dot(q, n);
   float dx = dot(rd, u), dy = dot(rd, v), dz = dot(rd, n);
-  float best = tubeT(ro, rd, origin, origin + u * stem, 0.0011f, 0.00065f);
+  float best = petiole ? tubeT(ro, rd, origin, origin + u * stem, 0.0011f, 0.00065f) : kFar;
+  if (!slab) return best;
   v2 span = boxSpan(mk(x, y, z), mk(dx, dy, dz), mk(-0.001f, -width, -0.035f), mk(len + 0.001f, width, 0.065f));
   if (span.y < fmaxf(0.0001f, span.x)) return best;
   float t = fabsf(dz) > 0.000001f ? -z / dz : (span.x + span.y) * 0.5f;
@@ -211,11 +250,12 @@ __device__ inline float leafT(const Prim& pr, v3 ro, v3 rd, float footprint, flo
     best = fminf(best, t);
   return best;
 }
-__device__ inline float objectT(const Prim& pr, v3 ro, v3 rd, float footprint, float time, float wind, bool refine) {
+__device__ inline float objectT(const Prim& pr, v3 ro, v3 rd, float footprint, float time, float wind, bool refine,
+                               bool cull) {
   int type = (int)pr.type();
   if (type < 0) return kFar;
-  if (type == 0) return woodT(pr, ro, rd, footprint, refine);
-  if (type == 1) return leafT(pr, ro, rd, footprint, time, wind);
+  if (type == 0) return woodT(pr, ro, rd, footprint, refine, cull);
+  if (type == 1) return leafT(pr, ro, rd, footprint, time, wind, cull);
   v3 h = pr.b();
   v3 q = ro - pr.a();
   v3 p = mk(q.x / h.x, q.y / h.y, q.z / h.z);
@@ -234,6 +274,27 @@ struct NodeBox { float4 lo, hi; };
 AD bool spanHit(v2 s, float count, float best) { return count > 0.0f && s.y >= fmaxf(0.0001f, s.x) && s.x < best; }
 AD int heapDepth(uint32_t node) { return 31 - __builtin_clz(node); }
 
+// Compile-time configuration of the walk. Every configuration visits the same
+// nodes in the same order and returns the same (best, id); they differ in how
+// the data reaches the lanes.
+//   kCull:     bounding-sphere rejections inside woodT / leafT;
+//   kPrefetch: while testing a node's children, load its four grandchildren
+//              (the next step's children whichever child is taken), so each
+//              dependent fetch overlaps one step of box tests;
+//   kLdsTop:   heap levels 0-7 (nodes 0..255, 8 KiB) are copied into LDS once
+//              per block and read from there.
+template <bool kCull, bool kPrefetch, bool kLdsTop>
+struct Walk {
+  static constexpr bool cull = kCull, prefetch = kPrefetch, ldsTop = kLdsTop;
+};
+constexpr int kTopNodes = 256;
+
+template <class W>
+AD float4 nodeQ(const float4* Nodes4, const float4* top, uint32_t q) {
+  if (W::ldsTop && q < (uint32_t)(2 * kTopNodes)) return top[q];
+  return Nodes4[q];
+}
+
 // Same visiting order and the same (best, id) as the reference's stack[32]
 // loop. The reference pushes the far child only when both children are hit
 // and always takes the nearer one (ties -> right); the stack is therefore
@@ -242,27 +303,48 @@ AD int heapDepth(uint32_t node) { return 31 - __builtin_clz(node); }
 // to the sibling of the current path's depth-d ancestor. A popped node is
 // re-tested against the current best exactly as the reference does; a node
 // reached directly reuses the span its parent computed (identical values).
-__device__ inline v2 traceTree(const float* P, const float4* Nodes4, const uint32_t* Order, v3 ro, v3 rd, float cone,
-                               float time, float wind, float best, int ignore, bool shadows) {
+template <class W>
+__device__ inline v2 traceTree(const float* P, const float4* Nodes4, const float4* top, const uint32_t* Order, v3 ro,
+                               v3 rd, float cone, float time, float wind, float best, int ignore, bool shadows) {
   const v3 inv = invDir(rd);
   int found = -1;
   uint32_t node = 1, pending = 0;
-  float4 rl = Nodes4[2], rh = Nodes4[3];
+  float4 rl = nodeQ<W>(Nodes4, top, 2), rh = nodeQ<W>(Nodes4, top, 3);
   v2 span = boxSpanInv(ro, inv, mk(rl.x, rl.y, rl.z), mk(rh.x, rh.y, rh.z));
   bool live = spanHit(span, rl.w, best);
+  float4 k0, k1, k2, k3;  // children boxes of `node` when haveKids
+  bool haveKids = false;
   while (true) {
     if (live) {
       if (node < (uint32_t)kCapacity) {
-        const float4* c = Nodes4 + node * 4;  // children 2n, 2n+1
-        float4 l0 = c[0], l1 = c[1], r0 = c[2], r1 = c[3];
-        v2 ls = boxSpanInv(ro, inv, mk(l0.x, l0.y, l0.z), mk(l1.x, l1.y, l1.z));
-        v2 rs = boxSpanInv(ro, inv, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z));
-        bool lh = spanHit(ls, l0.w, best), rhit = spanHit(rs, r0.w, best);
+        if (!haveKids) {
+          k0 = nodeQ<W>(Nodes4, top, node * 4);
+          k1 = nodeQ<W>(Nodes4, top, node * 4 + 1);
+          k2 = nodeQ<W>(Nodes4, top, node * 4 + 2);
+          k3 = nodeQ<W>(Nodes4, top, node * 4 + 3);
+        }
+        // grandchildren exist below depth 14 (node < CAPACITY/2)
+        const bool pf = W::prefetch && node < (uint32_t)(kCapacity / 2);
+        float4 g[8];
+        if (pf) {
+#pragma unroll
+          for (int j = 0; j < 8; j++) g[j] = nodeQ<W>(Nodes4, top, node * 8 + j);
+        }
+        v2 ls = boxSpanInv(ro, inv, mk(k0.x, k0.y, k0.z), mk(k1.x, k1.y, k1.z));
+        v2 rs = boxSpanInv(ro, inv, mk(k2.x, k2.y, k2.z), mk(k3.x, k3.y, k3.z));
+        bool lh = spanHit(ls, k0.w, best), rhit = spanHit(rs, k2.w, best);
         if (lh || rhit) {
           bool takeLeft = lh && (!rhit || ls.x < rs.x);
           if (lh && rhit) pending |= 1u << (heapDepth(node) + 1);
           node = node * 2 + (takeLeft ? 0u : 1u);
           span = takeLeft ? ls : rs;
+          if (pf) {
+            k0 = takeLeft ? g[0] : g[4];
+            k1 = takeLeft ? g[1] : g[5];
+            k2 = takeLeft ? g[2] : g[6];
+            k3 = takeLeft ? g[3] : g[7];
+          }
+          haveKids = pf;
           continue;
         }
       } else {
@@ -270,16 +352,17 @@ __device__ inline v2 traceTree(const float* P, const float4* Nodes4, const uint3
         if (id != ignore) {
           float footprint = fmaxf(0.000005f, cone * fmaxf(0.05f, span.x));
           Prim pr = loadPrim(P, id);
-          float t = objectT(pr, ro, rd, footprint, time, wind, !shadows);
+          float t = objectT(pr, ro, rd, footprint, time, wind, !shadows, W::cull);
           if (t < best) { best = t; found = id; }
         }
       }
     }
+    haveKids = false;
     if (pending == 0) break;
     int d = 31 - __builtin_clz(pending);
     pending &= ~(1u << d);
     node = (node >> (heapDepth(node) - d)) ^ 1u;
-    float4 b0 = Nodes4[node * 2], b1 = Nodes4[node * 2 + 1];
+    float4 b0 = nodeQ<W>(Nodes4, top, node * 2), b1 = nodeQ<W>(Nodes4, top, node * 2 + 1);
     span = boxSpanInv(ro, inv, mk(b0.x, b0.y, b0.z), mk(b1.x, b1.y, b1.z));
     live = spanHit(span, b0.w, best);
   }
@@ -289,9 +372,10 @@ __device__ inline v2 traceTree(const float* P, const float4* Nodes4, const uint3
 // ---- forest DDA (Arbor.cu:235-251) ----
 // `trees` holds the 324 instancing records (LDS); every value is what the
 // reference recomputes inline per cell.
-__device__ inline v2 traceForest(const float* P, const float4* Nodes4, const uint32_t* Order, const TreeInst* trees,
-                                 v3 ro, v3 rd, float cone, float time, float wind, float best, int* treeOut,
-                                 bool shadows) {
+template <class W>
+__device__ inline v2 traceForest(const float* P, const float4* Nodes4, const float4* top, const uint32_t* Order,
+                                 const TreeInst* trees, v3 ro, v3 rd, float cone, float time, float wind, float best,
+                                 int* treeOut, bool shadows) {
   int foundTree = -1;
   float found = -1.0f;
   const float half = (float)kSide * kSpacing * 0.5f;
@@ -316,7 +400,7 @@ __device__ inline v2 traceForest(const float* P, const float4* Nodes4, const uin
     float broad = sphereT(ro, rd, centre, 6.6f * sc);
     if (broad < best) {
       v3 lr = toLocalPoint(ro, ti), ld = toLocalDir(rd, ti);
-      v2 h = traceTree(P, Nodes4, Order, lr, ld, cone / sc, time + ti.phase, wind, best / sc, -1, shadows);
+      v2 h = traceTree<W>(P, Nodes4, top, Order, lr, ld, cone / sc, time + ti.phase, wind, best / sc, -1, shadows);
       if (h.y >= 0.0f && h.x * sc < best) { best = h.x * sc; found = h.y; foundTree = tree; }
     }
     float next = tx < tz ? tx : tz;

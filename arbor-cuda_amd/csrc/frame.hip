@@ -1,85 +1,85 @@
 // Per-frame kernels of the ARBOR render path on gfx950: primary visibility,
 // shading, the same-surface lighting filter and the progressive resolve.
 //
-// Visibility and shading run one wave64 per 8x8 pixel tile; a block of four
-// waves builds the 324-tree instancing table in LDS once and then walks tiles
-// in a grid-stride loop, so the table and camera frame are amortised over many
-// tiles. Filter and resolve are HBM-bound per-pixel kernels over float4 rows.
+// Visibility and shading run one wave64 per 8x8 pixel tile; a block of waves
+// builds the 324-tree instancing table (and, per walk policy, the top heap
+// levels) in LDS once and then takes tiles in a grid-stride loop. The trace
+// kernel is instantiated for several walk policies and block sizes, all with
+// identical results; the tuning key "trace_variant" selects one. Filter and
+// resolve are HBM-bound per-pixel kernels over float4 rows.
 #include <string.h>
 
-#include "arbor_batched.hpp"
 #include "arbor_shade.hpp"
 #include "launch.hpp"
 #include "tuning.hpp"
 
 namespace arbor {
 
-constexpr int kTileBlock = 256;  // 4 waves, one 8x8 tile each
+constexpr int kTileBlock = 256;  // 4 waves, one 8x8 tile each (shade)
 constexpr int kWavesPerBlock = kTileBlock / 64;
 
 __device__ __forceinline__ void fillTrees(TreeInst* trees, int ws) {
   for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, ws);
-  __syncthreads();
+}
+template <class W>
+__device__ __forceinline__ void fillTop(float4* top, const float4* Nodes4) {
+  if (W::ldsTop)
+    for (int i = threadIdx.x; i < 2 * kTopNodes; i += blockDim.x) top[i] = Nodes4[i];
 }
 
 // tracePixels (Arbor.cu:252) for every pixel of the view.
-__global__ __launch_bounds__(kTileBlock) void k_tracePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
-                                                            const float* C, float4* Hit, int width, int height,
-                                                            int tilesX, int tiles) {
+template <int kWaves, class W>
+__global__ __launch_bounds__(kWaves * 64) void k_tracePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
+                                                             const float* C, float4* Hit, int width, int height,
+                                                             int tilesX, int tiles) {
   __shared__ TreeInst trees[kTrees];
+  __shared__ float4 top[W::ldsTop ? 2 * kTopNodes : 1];
   fillTrees(trees, (int)C[31]);
+  fillTop<W>(top, Nodes4);
+  __syncthreads();
   const View view = makeView(C);
   const float cone = C[14] / (float)height, time = C[8], wind = C[11];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int t = blockIdx.x * kWavesPerBlock + wave; t < tiles; t += gridDim.x * kWavesPerBlock) {
+  for (int t = blockIdx.x * kWaves + wave; t < tiles; t += gridDim.x * kWaves) {
     int x = (t % tilesX) * 8 + (lane & 7), y = (t / tilesX) * 8 + (lane >> 3);
     if (x >= width || y >= height) continue;
     v3 ro = view.eye, rd = viewRay(view, x, y, width, height);
     float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
     int tree = -1;
-    v2 h = traceForest(P, Nodes4, Order, trees, ro, rd, cone, time, wind, ground > 0.0f ? ground : kFar, &tree, false);
+    v2 h = traceForest<W>(P, Nodes4, top, Order, trees, ro, rd, cone, time, wind, ground > 0.0f ? ground : kFar, &tree,
+                          false);
     Hit[y * width + x] = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)height);
   }
 }
 
-// tracePixels with the wave-cooperative walk (leaf tests batched per wave).
-__global__ __launch_bounds__(kTileBlock) void k_tracePixelsBatched(const float* P, const float4* Nodes4,
-                                                                   const uint32_t* Order, const float* C, float4* Hit,
-                                                                   int width, int height, int tilesX, int tiles) {
-  __shared__ TreeInst trees[kTrees];
-  fillTrees(trees, (int)C[31]);
-  const View view = makeView(C);
-  const float cone = C[14] / (float)height, time = C[8], wind = C[11];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int t = blockIdx.x * kWavesPerBlock + wave; t < tiles; t += gridDim.x * kWavesPerBlock) {
-    int x = (t % tilesX) * 8 + (lane & 7), y = (t / tilesX) * 8 + (lane >> 3);
-    bool inside = x < width && y < height;
-    v3 ro = view.eye, rd = viewRay(view, inside ? x : 0, inside ? y : 0, width, height);
-    float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
-    ForestRay s;
-    forestBegin(s, ro, rd, ground > 0.0f ? ground : kFar);
-    if (!inside) s.done = true;
-    forestTraceBatched(s, P, Nodes4, Order, trees, cone, time, wind, false);
-    if (inside) Hit[y * width + x] = make_float4(s.best, s.found, (float)s.foundTree, C[14] * s.best / (float)height);
-  }
-}
-
 // Single-specimen primary rays: traceTree in tree-local space (no instancing).
-__global__ __launch_bounds__(kTileBlock) void k_traceTreePixels(const float* P, const float4* Nodes4,
-                                                                const uint32_t* Order, const float* C, float4* Hit,
-                                                                int width, int height, int tilesX, int tiles) {
+template <int kWaves, class W>
+__global__ __launch_bounds__(kWaves * 64) void k_traceTreePixels(const float* P, const float4* Nodes4,
+                                                                 const uint32_t* Order, const float* C, float4* Hit,
+                                                                 int width, int height, int tilesX, int tiles) {
+  __shared__ float4 top[W::ldsTop ? 2 * kTopNodes : 1];
+  fillTop<W>(top, Nodes4);
+  __syncthreads();
   const View view = makeView(C);
   const float cone = C[14] / (float)height, time = C[8], wind = C[11];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int t = blockIdx.x * kWavesPerBlock + wave; t < tiles; t += gridDim.x * kWavesPerBlock) {
+  for (int t = blockIdx.x * kWaves + wave; t < tiles; t += gridDim.x * kWaves) {
     int x = (t % tilesX) * 8 + (lane & 7), y = (t / tilesX) * 8 + (lane >> 3);
     if (x >= width || y >= height) continue;
     v3 ro = view.eye, rd = viewRay(view, x, y, width, height);
     float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
-    v2 h = traceTree(P, Nodes4, Order, ro, rd, cone, time, wind, ground > 0.0f ? ground : kFar, -1, false);
+    v2 h = traceTree<W>(P, Nodes4, top, Order, ro, rd, cone, time, wind, ground > 0.0f ? ground : kFar, -1, false);
     Hit[y * width + x] = make_float4(h.x, h.y, -1.0f, C[14] * h.x / (float)height);
   }
 }
+
+// The trace configurations the tuning key "trace_variant" selects from.
+using WalkPlain = Walk<false, false, false>;
+using WalkCull = Walk<true, false, false>;
+using WalkCullPrefetch = Walk<true, true, false>;
+using WalkCullTop = Walk<true, false, true>;
+using WalkAll = Walk<true, true, true>;
+using WalkShade = WalkCull;
 
 // shadePixels (Arbor.cu:277-289).
 __global__ __launch_bounds__(kTileBlock) void k_shadePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
@@ -88,6 +88,7 @@ __global__ __launch_bounds__(kTileBlock) void k_shadePixels(const float* P, cons
                                                             int tiles) {
   __shared__ TreeInst trees[kTrees];
   fillTrees(trees, (int)C[31]);
+  __syncthreads();
   const View view = makeView(C);
   const v3 sun = unit(mk(cos_cr(C[12]) * cos_cr(C[13]), sin_cr(C[13]), sin_cr(C[12]) * cos_cr(C[13])));
   const v3 su = basisU(sun), sv = cross(sun, su);
@@ -155,7 +156,8 @@ __global__ __launch_bounds__(kTileBlock) void k_shadePixels(const float* P, cons
       v3 light = unit(sun + su * (cosf(angle) * disk) + sv * (sinf(angle) * disk));
       float offset = id >= 0 ? 0.0014f : 0.004f;
       v3 visibility = mk(1.0f, 1.0f, 1.0f);
-      if (shadows) visibility = lightVisibility(P, Nodes4, Order, trees, p + light * offset, light, time, wind);
+      if (shadows)
+        visibility = lightVisibility<WalkShade>(P, Nodes4, nullptr, Order, trees, p + light * offset, light, time, wind);
       float nl = sat(dot(n, light)), back = sat(-dot(n, light));
       v3 direct = albedo * (nl / kPi);
       if (leaf) {
@@ -250,12 +252,40 @@ __global__ __launch_bounds__(256) void k_resolvePixels(const float4* Filtered, f
 using namespace arbor;
 
 namespace {
-// Grid for the tile kernels: enough blocks to fill 256 CUs several times over
-// (the cap is a tuning knob), never more than there are tiles.
-inline int tileGrid(int tiles) {
-  int blocks = (tiles + kWavesPerBlock - 1) / kWavesPerBlock;
+// Grid for the tile kernels: one wave per 8x8 tile up to the cap (a tuning
+// knob); beyond it waves take further tiles in a grid-stride loop.
+inline int tileGrid(int tiles, int wavesPerBlock) {
+  int blocks = (tiles + wavesPerBlock - 1) / wavesPerBlock;
   int cap = tuning().grid_cap;
   return blocks < cap ? blocks : cap;
+}
+
+typedef void (*TraceKernel)(const float*, const float4*, const uint32_t*, const float*, float4*, int, int, int, int);
+struct TraceVariant {
+  TraceKernel forest, tree;
+  int waves;
+};
+// trace_variant index -> kernels. Identical results for every entry.
+const TraceVariant kTraceVariants[] = {
+    {k_tracePixels<4, WalkPlain>, k_traceTreePixels<4, WalkPlain>, 4},
+    {k_tracePixels<4, WalkCull>, k_traceTreePixels<4, WalkCull>, 4},
+    {k_tracePixels<4, WalkCullPrefetch>, k_traceTreePixels<4, WalkCullPrefetch>, 4},
+    {k_tracePixels<4, WalkCullTop>, k_traceTreePixels<4, WalkCullTop>, 4},
+    {k_tracePixels<4, WalkAll>, k_traceTreePixels<4, WalkAll>, 4},
+    {k_tracePixels<1, WalkCull>, k_traceTreePixels<1, WalkCull>, 1},
+    {k_tracePixels<1, WalkCullPrefetch>, k_traceTreePixels<1, WalkCullPrefetch>, 1},
+    {k_tracePixels<2, WalkCullPrefetch>, k_traceTreePixels<2, WalkCullPrefetch>, 2},
+};
+constexpr int kTraceVariantCount = sizeof(kTraceVariants) / sizeof(kTraceVariants[0]);
+
+int launchTrace(bool forest, const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
+                int width, int height, hipStream_t stream) {
+  const TraceVariant& v = kTraceVariants[tuning().trace_variant];
+  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  hipLaunchKernelGGL(forest ? v.forest : v.tree, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), 0, stream, P,
+                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
+                     tx, tiles);
+  return launch_status();
 }
 }  // namespace
 
@@ -265,23 +295,14 @@ int arbor_tracePixels(const float* P, const float* Nodes, const unsigned* Order,
                       int width, int height, hipStream_t stream) {
   ARBOR_REQUIRE(P && Nodes && Order && C && Hit);
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
-  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
-  auto kernel = tuning().trace_variant == 1 ? k_tracePixelsBatched : k_tracePixels;
-  hipLaunchKernelGGL(kernel, dim3(tileGrid(tiles)), dim3(kTileBlock), 0, stream, P,
-                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
-                     tx, tiles);
-  return launch_status();
+  return launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream);
 }
 
 int arbor_traceTreePixels(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
                           int width, int height, hipStream_t stream) {
   ARBOR_REQUIRE(P && Nodes && Order && C && Hit);
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
-  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
-  hipLaunchKernelGGL(k_traceTreePixels, dim3(tileGrid(tiles)), dim3(kTileBlock), 0, stream, P,
-                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
-                     tx, tiles);
-  return launch_status();
+  return launchTrace(false, P, Nodes, Order, C, Hit, width, height, stream);
 }
 
 int arbor_shadePixels(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
@@ -289,7 +310,7 @@ int arbor_shadePixels(const float* P, const float* Nodes, const unsigned* Order,
   ARBOR_REQUIRE(P && Nodes && Order && C && Hit && Linear && Guide);
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
-  hipLaunchKernelGGL(k_shadePixels, dim3(tileGrid(tiles)), dim3(kTileBlock), 0, stream, P,
+  hipLaunchKernelGGL(k_shadePixels, dim3(tileGrid(tiles, kWavesPerBlock)), dim3(kTileBlock), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles);
   return launch_status();
@@ -322,7 +343,7 @@ int arbor_tune_set(const char* key, int value) {
   if (!key) return ARBOR_ERR_NULL;
   Tuning& t = tuning();
   if (!strcmp(key, "trace_variant")) {
-    if (value < 0 || value > 1) return ARBOR_ERR_ARG;
+    if (value < 0 || value >= kTraceVariantCount) return ARBOR_ERR_ARG;
     t.trace_variant = value;
     return ARBOR_OK;
   }

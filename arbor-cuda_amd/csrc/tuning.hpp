@@ -6,8 +6,8 @@
 namespace arbor {
 
 struct Tuning {
-  int trace_variant = 1;  // 0: per-lane walk, 1: wave-batched leaf tests
-  int grid_cap = 2048;    // maximum blocks of a tile kernel launch
+  int trace_variant = 1;  // index into the trace kernel table (frame.hip)
+  int grid_cap = 65535;   // maximum blocks of a tile kernel launch
 };
 
 inline Tuning& tuning() {

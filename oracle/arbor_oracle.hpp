@@ -618,9 +618,32 @@ template <class R> struct Arbor {
     }
     return best;
   }
+  // When set, woodT and leafT first reject rays that miss a padded bounding
+  // sphere of everything they can hit (the GPU's early-out); results must be
+  // bit-identical with the flag off, which the tests check on whole frames.
+  static inline bool cullSpheres = false;
+  // Padding of the rejection spheres. sphereT/tubeT decide hits from
+  // discriminants that cancel catastrophically far from the object, so the
+  // reference reports hits up to ~8e-4*|q| beyond a small sphere or tube at
+  // distance |q|; tests that end in such a discriminant are padded by
+  // 1 mm + 4e-3*|q|. The leaf slab (boxSpan) and the wood SDF march (which
+  // must reach within 2.5e-4 m of the surface) are well conditioned: 1 mm.
+  static bool sphereMiss(F3 ro, F3 rd, F3 centre, R radius, R padPerMetre) {
+    F3 q = ro - centre;
+    R b = dot(q, rd);
+    F3 p = q - rd * b;
+    R r = radius + K(0.001) + padPerMetre * len(q);
+    return dot(p, p) > r * r || b > r;
+  }
+
   static R woodT(const R* P, int id, F3 ro, F3 rd, R footprint, bool refine) {
     int b = id * PS;
     F3 a = ld3(P, b), end = ld3(P, b + 4);
+    if (cullSpheres) {
+      F3 half = (end - a) * K(0.5);
+      if (sphereMiss(ro, rd, a + half, len(half) + m_max(P[b + 7], P[b + 11]), refine ? K(0.0) : K(0.004)))
+        return FAR();
+    }
     if (!refine) {
       R ra = P[b + 7] - m_min(K(0.021), P[b + 7] * K(0.055)) * K(0.32);
       R rb = P[b + 11] - m_min(K(0.021), P[b + 11] * K(0.055)) * K(0.32);
@@ -645,10 +668,21 @@ template <class R> struct Arbor {
     F3 v = ld3(P, b + 8);
     F3 n = leafN(P, id, time, wind);
     R stem = P[b + 16], ln = P[b + 7], width = P[b + 11];
+    bool petiole = true, slab = true;
+    if (cullSpheres) {
+      // petiole capsule: radius <= 1.1 mm along u over [0, stem]
+      petiole = !sphereMiss(ro, rd, origin + u * (stem * K(0.5)), stem * K(0.5) + K(0.0011), K(0.004));
+      // Newton slab: u in [stem-0.001, stem+len+0.001], |v| <= width, n in [-0.035, 0.065]
+      R hx = ln * K(0.5) + K(0.001);
+      slab = !sphereMiss(ro, rd, origin + u * (stem + ln * K(0.5)) + n * K(0.015),
+                         m_sqrt(hx * hx + width * width + K(0.05) * K(0.05)), K(0.00001));
+      if (!petiole && !slab) return FAR();
+    }
     F3 q = ro - origin;
     R x = dot(q, u) - stem, y = dot(q, v), z = dot(q, n);
     R dx = dot(rd, u), dy = dot(rd, v), dz = dot(rd, n);
-    R best = tubeT(ro, rd, origin, origin + u * stem, K(0.0011), K(0.00065));
+    R best = petiole ? tubeT(ro, rd, origin, origin + u * stem, K(0.0011), K(0.00065)) : FAR();
+    if (!slab) return best;
     F2 span = boxSpan(c3(x, y, z), c3(dx, dy, dz), c3(K(-0.001), -width, K(-0.035)), c3(ln + K(0.001), width, K(0.065)));
     if (span.y < m_max(K(0.0001), span.x)) return best;
     R t = m_abs(dz) > K(0.000001) ? -z / dz : (span.x + span.y) * K(0.5);

@@ -96,6 +96,33 @@ int orc_count_trace_ops(const float* P, const float* Nodes, const unsigned* Orde
   return 0;
 }
 
+// Reference intersection tests vs the same tests behind the GPU's
+// bounding-sphere rejections, over whole frames (trace, and shade with sun
+// visibility on so shadow rays use the non-refined wood path): the number of
+// pixels whose Hit or Linear differs in any bit (0 expected).
+int orc_cull_mismatches(const float* P, const float* Nodes, const unsigned* Order, const float* C, int width,
+                        int height) {
+  using A = Arbor<float>;
+  size_t n = (size_t)width * height * 4;
+  std::vector<float> h[2] = {std::vector<float>(n), std::vector<float>(n)};
+  std::vector<float> l[2] = {std::vector<float>(n), std::vector<float>(n)}, g(n);
+  for (int cull = 0; cull < 2; cull++) {
+    A::cullSpheres = cull != 0;
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = 0; y < height; y++)
+      for (int x = 0; x < width; x++) A::tracePixel(P, Nodes, Order, C, h[cull].data(), width, height, x, y);
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = 0; y < height; y++)
+      for (int x = 0; x < width; x++)
+        A::shadePixel(P, Nodes, Order, C, h[cull].data(), l[cull].data(), g.data(), width, height, x, y);
+  }
+  A::cullSpheres = false;
+  int bad = 0;
+  for (size_t i = 0; i < n; i += 4)
+    if (std::memcmp(&h[0][i], &h[1][i], 16) != 0 || std::memcmp(&l[0][i], &l[1][i], 16) != 0) bad++;
+  return bad;
+}
+
 // Reference walk vs lean walk over whole frames: returns the number of
 // pixels whose Hit differs in any bit (0 expected).
 int orc_lean_walk_mismatches(const float* P, const float* Nodes, const unsigned* Order, const float* C, int width,
