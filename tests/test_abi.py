@@ -104,7 +104,8 @@ def test_engine_resize_rounding(L):
 def test_tuning_keys_validated(L):
     assert L.arbor_tune_set(None, 1) == -1
     assert L.arbor_tune_set(b"no_such_key", 1) == -2
-    assert L.arbor_tune_set(b"trace_variant", 7) == -2
+    assert L.arbor_tune_set(b"trace_variant", 99) == -2
+    assert L.arbor_tune_set(b"trace_variant", -1) == -2
     assert L.arbor_tune_set(b"grid_cap", 0) == -2
     old = L.arbor_tune_get(b"grid_cap")
     assert L.arbor_tune_set(b"grid_cap", 777) == 0 and L.arbor_tune_get(b"grid_cap") == 777
