@@ -7,9 +7,12 @@
 // kernel is instantiated for several walk policies and block sizes, all with
 // identical results; the tuning key "trace_variant" selects one. Filter and
 // resolve are HBM-bound per-pixel kernels over float4 rows.
+#include <limits.h>
+#include <math.h>
 #include <string.h>
 
 #include "arbor_shade.hpp"
+#include "arbor_wavefront.hpp"
 #include "launch.hpp"
 #include "tuning.hpp"
 
@@ -71,6 +74,32 @@ __global__ __launch_bounds__(kWaves * 64) void k_traceTreePixels(const float* P,
     v2 h = traceTree<W>(P, Nodes4, top, Order, ro, rd, cone, time, wind, ground > 0.0f ? ground : kFar, -1, false);
     Hit[y * width + x] = make_float4(h.x, h.y, -1.0f, C[14] * h.x / (float)height);
   }
+}
+
+// tracePixels with the wave-scheduled lane state machine: each block owns
+// `raysPerBlock` consecutive rays (tile-major) that its waves pull from an
+// LDS counter.
+template <int kWaves>
+__global__ __launch_bounds__(kWaves * 64) void k_tracePixelsSched(const float* P, const float4* Nodes4,
+                                                                  const uint32_t* Order, const float* C, float4* Hit,
+                                                                  int width, int height, int tilesX, int tiles,
+                                                                  int raysPerBlock, WaveSchedule ws) {
+  __shared__ TreeInst trees[kTrees];
+  __shared__ int pool;
+  fillTrees(trees, (int)C[31]);
+  const int first = blockIdx.x * raysPerBlock;
+  if (threadIdx.x == 0) pool = first;
+  __syncthreads();
+  const int end = min(tiles * 64, first + raysPerBlock);
+  ViewCtx vc;
+  vc.view = makeView(C);
+  vc.cone = C[14] / (float)height;
+  vc.time = C[8];
+  vc.wind = C[11];
+  vc.width = width;
+  vc.height = height;
+  vc.tilesX = tilesX;
+  waveTrace(&pool, end, P, Nodes4, Order, trees, vc, Hit, C[14], ws);
 }
 
 // The trace configurations the tuning key "trace_variant" selects from.
@@ -189,9 +218,41 @@ __global__ __launch_bounds__(kTileBlock) void k_shadePixels(const float* P, cons
 }
 
 // filterLighting (Arbor.cu:292-302): 5x5 cross-bilateral, same id/tree/depth.
-__global__ __launch_bounds__(256) void k_filterLighting(const float4* Linear, const float4* Guide, const float4* Hit,
-                                                        float4* Filtered, const float* C, int width, int height) {
-  int x = (int)(blockIdx.x * blockDim.x + threadIdx.x), y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+// One block filters a 32x8 tile. The tile plus a 2-pixel halo (36x12) is
+// staged in LDS as {t, id, tree, Linear/(Guide+0.02)}: the 25 taps read LDS
+// and each pixel's demodulation (three divisions) is computed once instead of
+// once per tap; the same operands and summation order give the same result.
+// The 6 distinct tap weights expf(-0.3*(j*j+k*k)) arrive precomputed.
+constexpr int kFilterW = 32, kFilterH = 8, kHalo = 2;
+constexpr int kTileW = kFilterW + 2 * kHalo, kTileH = kFilterH + 2 * kHalo;
+struct FilterWeights { float w[9]; };  // indexed by j*j+k*k (0,1,2,4,5,8 used)
+__global__ __launch_bounds__(kFilterW * kFilterH) void k_filterLighting(const float4* Linear, const float4* Guide,
+                                                                       const float4* Hit, float4* Filtered,
+                                                                       const float* C, int width, int height,
+                                                                       FilterWeights fw) {
+  __shared__ float st[kTileH][kTileW];
+  __shared__ int sid[kTileH][kTileW], stree[kTileH][kTileW];
+  __shared__ float sdr[kTileH][kTileW], sdg[kTileH][kTileW], sdb[kTileH][kTileW];
+  const int x0 = blockIdx.x * kFilterW - kHalo, y0 = blockIdx.y * kFilterH - kHalo;
+  const int tid = threadIdx.y * kFilterW + threadIdx.x;
+  for (int i = tid; i < kTileW * kTileH; i += kFilterW * kFilterH) {
+    int ty = i / kTileW, tx = i % kTileW, gx = x0 + tx, gy = y0 + ty;
+    if (gx < 0 || gy < 0 || gx >= width || gy >= height) {
+      sid[ty][tx] = INT_MIN;  // never equals a pixel's id (ids are >= -1)
+      continue;
+    }
+    int q = gy * width + gx;
+    float4 hq = Hit[q], lq = Linear[q], gq = Guide[q];
+    st[ty][tx] = hq.x;
+    sid[ty][tx] = (int)hq.y;
+    stree[ty][tx] = (int)hq.z;
+    v3 g = mk(gq.x, gq.y, gq.z) + mk(0.02f, 0.02f, 0.02f);
+    sdr[ty][tx] = lq.x / g.x;
+    sdg[ty][tx] = lq.y / g.y;
+    sdb[ty][tx] = lq.z / g.z;
+  }
+  __syncthreads();
+  int x = blockIdx.x * kFilterW + threadIdx.x, y = blockIdx.y * kFilterH + threadIdx.y;
   if (x >= width || y >= height) return;
   int b = y * width + x;
   float4 hb = Hit[b], lb = Linear[b], gb = Guide[b];
@@ -203,17 +264,15 @@ __global__ __launch_bounds__(256) void k_filterLighting(const float4* Linear, co
   float depth = hb.x;
   if (depth < kFar && C[15] < 0.5f) {
     float tol = fmaxf(0.005f, hb.w * 6.0f);
+    const int cx = threadIdx.x + kHalo, cy = threadIdx.y + kHalo;
+#pragma unroll
     for (int j = -2; j <= 2; j++)
+#pragma unroll
       for (int k = -2; k <= 2; k++) {
-        int xx = x + k, yy = y + j;
-        if (xx < 0 || yy < 0 || xx >= width || yy >= height) continue;
-        int q = yy * width + xx;
-        float4 hq = Hit[q];
-        if ((int)hq.y != id || (int)hq.z != tree || fabsf(hq.x - depth) > tol) continue;
-        float w = expf(-(float)(j * j + k * k) * 0.30f);
-        float4 lq = Linear[q], gq = Guide[q];
-        v3 g = mk(gq.x, gq.y, gq.z) + mk(0.02f, 0.02f, 0.02f);
-        sum = sum + mk(lq.x / g.x, lq.y / g.y, lq.z / g.z) * w;
+        int ty = cy + j, tx = cx + k;
+        if (sid[ty][tx] != id || stree[ty][tx] != tree || fabsf(st[ty][tx] - depth) > tol) continue;
+        float w = fw.w[j * j + k * k];
+        sum = sum + mk(sdr[ty][tx], sdg[ty][tx], sdb[ty][tx]) * w;
         weight += w;
       }
     if (weight > 0.0f) col = mix(col, (sum / weight) * base, 0.86f);
@@ -280,8 +339,20 @@ constexpr int kTraceVariantCount = sizeof(kTraceVariants) / sizeof(kTraceVariant
 
 int launchTrace(bool forest, const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
                 int width, int height, hipStream_t stream) {
-  const TraceVariant& v = kTraceVariants[tuning().trace_variant];
+  const Tuning& tu = tuning();
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  if (forest && tu.trace_variant >= kTraceVariantCount) {
+    // wave-scheduled kernels: one block per `sched_tiles` tiles
+    int rays = tu.sched_tiles * 64;
+    int blocks = (tiles * 64 + rays - 1) / rays;
+    WaveSchedule ws{tu.sched_leaf_min, tu.sched_newton_min};
+    int waves = tu.trace_variant == kTraceVariantCount ? 4 : 1;
+    auto kernel = waves == 4 ? k_tracePixelsSched<4> : k_tracePixelsSched<1>;
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(waves * 64), 0, stream, P, reinterpret_cast<const float4*>(Nodes),
+                       Order, C, reinterpret_cast<float4*>(Hit), width, height, tx, tiles, rays, ws);
+    return launch_status();
+  }
+  const TraceVariant& v = kTraceVariants[tu.trace_variant < kTraceVariantCount ? tu.trace_variant : 5];
   hipLaunchKernelGGL(forest ? v.forest : v.tree, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
                      tx, tiles);
@@ -320,10 +391,12 @@ int arbor_filterLighting(const float* Linear, const float* Guide, const float* H
                          int width, int height, hipStream_t stream) {
   ARBOR_REQUIRE(Linear && Guide && Hit && Filtered && C);
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
-  dim3 block(64, 4), grid((width + 63) / 64, (height + 3) / 4);
+  dim3 block(kFilterW, kFilterH), grid((width + kFilterW - 1) / kFilterW, (height + kFilterH - 1) / kFilterH);
+  FilterWeights fw;
+  for (int i = 0; i < 9; i++) fw.w[i] = expf(-(float)i * 0.30f);  // host libm, as the reference harness
   hipLaunchKernelGGL(k_filterLighting, grid, block, 0, stream, reinterpret_cast<const float4*>(Linear),
                      reinterpret_cast<const float4*>(Guide), reinterpret_cast<const float4*>(Hit),
-                     reinterpret_cast<float4*>(Filtered), C, width, height);
+                     reinterpret_cast<float4*>(Filtered), C, width, height, fw);
   return launch_status();
 }
 
@@ -342,16 +415,18 @@ const char* arbor_version(void) { return "arbor-mi355x 0.1 gfx950"; }
 int arbor_tune_set(const char* key, int value) {
   if (!key) return ARBOR_ERR_NULL;
   Tuning& t = tuning();
-  if (!strcmp(key, "trace_variant")) {
-    if (value < 0 || value >= kTraceVariantCount) return ARBOR_ERR_ARG;
-    t.trace_variant = value;
-    return ARBOR_OK;
-  }
-  if (!strcmp(key, "grid_cap")) {
-    if (value < 1 || value > 65535) return ARBOR_ERR_ARG;
-    t.grid_cap = value;
-    return ARBOR_OK;
-  }
+  struct { const char* name; int* slot; int lo, hi; } keys[] = {
+      {"trace_variant", &t.trace_variant, 0, kTraceVariantCount + 1},
+      {"grid_cap", &t.grid_cap, 1, 65535},
+      {"sched_tiles", &t.sched_tiles, 1, 4096},
+      {"sched_leaf_min", &t.sched_leaf_min, 1, 64},
+      {"sched_newton_min", &t.sched_newton_min, 1, 64}};
+  for (auto& k : keys)
+    if (!strcmp(key, k.name)) {
+      if (value < k.lo || value > k.hi) return ARBOR_ERR_ARG;
+      *k.slot = value;
+      return ARBOR_OK;
+    }
   return ARBOR_ERR_ARG;
 }
 
@@ -360,6 +435,9 @@ int arbor_tune_get(const char* key) {
   const Tuning& t = tuning();
   if (!strcmp(key, "trace_variant")) return t.trace_variant;
   if (!strcmp(key, "grid_cap")) return t.grid_cap;
+  if (!strcmp(key, "sched_tiles")) return t.sched_tiles;
+  if (!strcmp(key, "sched_leaf_min")) return t.sched_leaf_min;
+  if (!strcmp(key, "sched_newton_min")) return t.sched_newton_min;
   return ARBOR_ERR_ARG;
 }
 

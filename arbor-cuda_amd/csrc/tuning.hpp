@@ -6,8 +6,13 @@
 namespace arbor {
 
 struct Tuning {
-  int trace_variant = 1;  // index into the trace kernel table (frame.hip)
-  int grid_cap = 65535;   // maximum blocks of a tile kernel launch
+  int trace_variant = 5;      // index into the trace kernel table (frame.hip); the
+                              // two values past the table select the wave-scheduled
+                              // kernel with 4-wave and 1-wave blocks
+  int grid_cap = 65535;       // maximum blocks of a tile kernel launch
+  int sched_tiles = 16;       // wave-scheduled kernel: 8x8 tiles per block pool
+  int sched_leaf_min = 24;    // lanes that must be ready before a LEAF phase runs
+  int sched_newton_min = 24;  // lanes that must be ready before a NEWTON phase runs
 };
 
 inline Tuning& tuning() {

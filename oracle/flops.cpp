@@ -96,6 +96,24 @@ int orc_count_trace_ops(const float* P, const float* Nodes, const unsigned* Orde
   return 0;
 }
 
+// Operations of every pixel's primary ray (forest, lean walk) into out[w*h]:
+// the per-ray work distribution, e.g. how uneven the rays of one tile are.
+int orc_trace_ops_per_pixel(const float* P, const float* Nodes, const unsigned* Order, const float* C, int width,
+                            int height, float* out) {
+  auto p = wrap(P, (size_t)CAPACITY * PS), n = wrap(Nodes, (size_t)NODE_COUNT * 8), c = wrap(C, 64);
+  std::vector<CF> hit((size_t)width * height * 4);
+  ACF::leanWalk = true;
+  for (int y = 0; y < height; y++)
+    for (int x = 0; x < width; x++) {
+      g_ops = OpCount();
+      ACF::tracePixel(p.data(), n.data(), Order, c.data(), hit.data(), width, height, x, y);
+      const OpCount& o = g_ops;
+      out[(size_t)y * width + x] = (float)(o.add + o.mul + o.div + o.sqrt + o.minmax + o.misc + o.trans);
+    }
+  ACF::leanWalk = false;
+  return 0;
+}
+
 // Reference intersection tests vs the same tests behind the GPU's
 // bounding-sphere rejections, over whole frames (trace, and shade with sun
 // visibility on so shadow rays use the non-refined wood path): the number of

@@ -111,18 +111,23 @@ def test_trace_variants_identical(px, oracle, scene7):
     dev, _ = scene7
     w, h = 240, 136
     Cd = to_dev(_camera(oracle, 7, 2, w, h))
-    old = (px.tune_get("trace_variant"), px.tune_get("grid_cap"))
+    keys = ("trace_variant", "grid_cap", "sched_tiles", "sched_leaf_min", "sched_newton_min")
+    old = {k: px.tune_get(k) for k in keys}
+    settings = [dict(trace_variant=v, grid_cap=65535) for v in range(10)]
+    settings += [dict(trace_variant=1, grid_cap=7), dict(trace_variant=0, grid_cap=5),
+                 dict(trace_variant=8, sched_tiles=1, sched_leaf_min=64, sched_newton_min=1),
+                 dict(trace_variant=9, sched_tiles=37, sched_leaf_min=1, sched_newton_min=64)]
     outs = []
     try:
-        for variant, cap in ((0, 2048), (1, 2048), (1, 7), (0, 5)):
-            px.tune_set("trace_variant", variant)
-            px.tune_set("grid_cap", cap)
+        for st in settings:
+            for k, v in st.items():
+                px.tune_set(k, v)
             fb = px.FrameBuffers(w, h)
             px.trace(dev, Cd, fb)
             outs.append(to_host(fb.Hit))
     finally:
-        px.tune_set("trace_variant", old[0])
-        px.tune_set("grid_cap", old[1])
+        for k, v in old.items():
+            px.tune_set(k, v)
     for o in outs[1:]:
         assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32))
 
