@@ -21,12 +21,20 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "arbor-cuda_amd"))
 
-# Algorithmic FP32 operations per primary ray of this workload (walk frame 0,
-# 1920x1080, seed 7, shadows off), counted by the instrumented oracle
-# (oracle/flops.cpp: every +,-,*,/,sqrt,min,max,abs,floor and every libm call
-# counts 1). Trace only.
-TRACE_FLOP_PER_RAY = None
+# Algorithmic FP32 operations per primary ray of tracePixels on this workload,
+# averaged over every 8th frame of the 120-frame walk at every 8th pixel:
+# `python oracle/count_trace_ops.py` (the oracle's counting instantiation,
+# oracle/flops.cpp: each +,-,*,/,sqrt,min,max,abs,floor and libm call counts 1,
+# with the GPU's traversal and rejection tests).
+TRACE_FLOP_PER_RAY = 11503.7
+# HBM bytes per tracePixels launch on this workload (walk frame 0): rocprofv3
+# --pmc FETCH_SIZE 34,482 KiB x 2.0 + WRITE_SIZE 35,559 KiB x 1.0, the factors
+# being what the same counters read for a 256 MiB device copy (131,092 KiB and
+# 262,144 KiB); profiles/tracePixels_forest1080p_pmc.md.
+TRACE_HBM_BYTES = (34482.0 * 2.0 + 35559.0 * 1.0) * 1024
 FP32_PEAK_TFLOPS = 157.3
+# One non-packed FP32 VALU op per lane per clock: 256 CUs x 64 lanes x 2.4 GHz.
+FP32_SINGLE_ISSUE_TOPS = 39.32
 HBM_PEAK_GBS = 8000.0
 
 
@@ -40,7 +48,34 @@ def parse():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile-trace", type=int, default=0, metavar="N",
+                    help="profiling driver: N tracePixels launches of walk frame 0, then a 256 MiB device copy "
+                         "(the calibration kernel for FETCH_SIZE/WRITE_SIZE); prints no JSON")
     return ap.parse_args()
+
+
+def profile_trace(args):
+    import torch
+
+    import pyarbor as px
+
+    W, H = args.width, args.height
+    eng = px.Engine(W, H, args.seed)
+    inputs = torch.from_numpy(walk_inputs(px, args.seed, 1)).cuda()
+    eng.frame(1.0 / 60.0, inputs[0])  # camera of walk frame 0 (also one full frame)
+    eng.sync()
+    lib = px.lib()
+    s = lib.arbor_engine_stream(eng.h)
+    bufs = [eng.buffer_ptr(n) for n in ("P", "Nodes", "Order", "C", "Hit")]
+    for _ in range(args.profile_trace):
+        px.check(lib.arbor_tracePixels(*bufs, W, H, s), "arbor_tracePixels")
+    eng.sync()
+    a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    eng.close()
+    print("profile driver: %d tracePixels launches at %dx%d, then a 256 MiB copy" % (args.profile_trace, W, H))
 
 
 def walk_inputs(px, seed, n):
@@ -91,6 +126,9 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.profile_trace:
+        profile_trace(args)
+        return
     import numpy as np
     import torch
 
@@ -131,12 +169,16 @@ def main():
     trace_ms = float(mean[1])
 
     roofline = {"bound": "mfma", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS, "kernel": "tracePixels",
-                "compute": "FP32 vector ALU (no MFMA shape in ray traversal); peak is MI355X's FP32 157.3 TFLOP/s",
+                "compute": "FP32 vector ALU (ray traversal has no MFMA shape); peak is MI355X's FP32 157.3 TFLOP/s",
                 "traffic": None}
-    if TRACE_FLOP_PER_RAY:
+    std_res = (W, H) == (1920, 1080) and args.seed == 7
+    if std_res:
         achieved = TRACE_FLOP_PER_RAY * W * H / (trace_ms * 1e-3) / 1e12
         roofline.update({"achieved": round(achieved, 3), "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                         "flop_per_ray": TRACE_FLOP_PER_RAY})
+                         "flop_per_ray": TRACE_FLOP_PER_RAY,
+                         "frac_of_single_issue_rate": round(achieved / FP32_SINGLE_ISSUE_TOPS, 4),
+                         "traffic": int(TRACE_HBM_BYTES),
+                         "traffic_algorithmic": int(16 * W * H + 5373952)})
     else:
         roofline.update({"achieved": None, "frac": None})
 

@@ -69,6 +69,9 @@ std::vector<CF> wrap(const float* a, size_t n) {
 }  // namespace
 
 extern "C" {
+// Bounding-sphere rejections on/off for the counting instantiation.
+void orc_set_cull(int on) { ACF::cullSpheres = on != 0; }
+
 // Counts the operations of tracePixels (forest, lean walk) over every
 // stride-th pixel in x and y of a width x height view. out[0] rays, out[1]
 // total operations, out[2..8] add, mul, div, sqrt, min/max, abs/floor, libm.
