@@ -107,4 +107,122 @@ __device__ inline v3 lightVisibility(const float* P, const float4* Nodes4, const
   return transmission;
 }
 
+
+// Shading state shared by every pixel of one view (per view, not per pixel:
+// the same values shadePixels recomputes per pixel).
+struct ShadeCtx {
+  View view;
+  v3 sun, su, sv;
+  float time, wind, mode;
+  bool shadows;
+  int sample, width, height;
+};
+AD ShadeCtx makeShadeCtx(const float* C, int width, int height) {
+  ShadeCtx sc;
+  sc.view = makeView(C);
+  sc.sun = unit(mk(cos_cr(C[12]) * cos_cr(C[13]), sin_cr(C[13]), sin_cr(C[12]) * cos_cr(C[13])));
+  sc.su = basisU(sc.sun);
+  sc.sv = cross(sc.sun, sc.su);
+  sc.time = C[8];
+  sc.wind = C[11];
+  sc.mode = C[15];
+  sc.shadows = C[22] > 0.5f;
+  sc.sample = (int)C[9];
+  sc.width = width;
+  sc.height = height;
+  return sc;
+}
+
+// One pixel of shadePixels (Arbor.cu:277-289): Linear and Guide of pixel
+// (x, y) from its Hit record.
+template <class W>
+__device__ inline void shadeOne(const float* P, const float4* Nodes4, const uint32_t* Order, const TreeInst* trees,
+                                const ShadeCtx& sc, int x, int y, float4 hit, float4& outLinear, float4& outGuide) {
+  const int pixel = y * sc.width + x;
+  float tt = hit.x;
+  int id = (int)hit.y, tree = (int)hit.z;
+  v3 ro = sc.view.eye, rd = viewRay(sc.view, x, y, sc.width, sc.height);
+  v3 col = sky(rd, sc.sun);
+  v3 guide = mk(1.0f, 1.0f, 1.0f);
+  if (tt < kFar) {
+    v3 p = ro + rd * tt;
+    float fp = fmaxf(0.000005f, hit.w);
+    v3 localP = p, n = mk(0.0f, 1.0f, 0.0f);
+    float localFp = fp, phase = 0.0f;
+    Prim pr;
+    bool leaf = false;
+    if (id >= 0) {
+      pr = loadPrim(P, id);
+      leaf = pr.type() == 1.0f;
+    }
+    v3 leafUdir = mk(0.0f, 0.0f, 0.0f);
+    if (id >= 0 && tree >= 0) {
+      const TreeInst& ti = trees[tree];
+      localP = toLocalPoint(p, ti);
+      localFp = fp / ti.sc;
+      phase = ti.phase;
+      n = toWorldNormal(objectNormal(pr, localP, localFp, sc.time + phase, sc.wind), ti);
+    }
+    if (leaf) leafUdir = leafFrame(pr, sc.time + phase, sc.wind).u;
+    if (dot(n, rd) > 0.0f) n = n * -1.0f;
+    if (id < 0) {
+      float e = fmaxf(0.001f, fp);
+      float nx = (noise((p.x + e) * 18.0f, p.z * 18.0f, 133) - noise((p.x - e) * 18.0f, p.z * 18.0f, 133)) * 0.18f;
+      float nz = (noise(p.x * 18.0f, (p.z + e) * 18.0f, 133) - noise(p.x * 18.0f, (p.z - e) * 18.0f, 133)) * 0.18f;
+      n = unit(mk(nx, 1.0f, nz));
+    }
+    v3 albedo;
+    if (id < 0) albedo = groundColor(p, localFp);
+    else {
+      int type = (int)pr.type();
+      if (type == 0) albedo = barkColor(pr, localP, localFp);
+      else if (type == 1) albedo = leafColor(pr, leafUdir, localP, localFp);
+      else albedo = mix(mk(0.09f, 0.10f, 0.086f), mk(0.31f, 0.30f, 0.24f), noise3(localP * 42.0f, (int)pr.seed()));
+    }
+    if (tree >= 0) {
+      int ts = trees[tree].seed;
+      float hue = rnd(8, ts), age = rnd(9, ts), season = rnd(10, ts);
+      float dark = 0.76f + age * 0.38f;
+      v3 tint = mk(0.88f + 0.22f * hue, 0.84f + 0.28f * (1.0f - fabsf(hue - 0.55f)), 0.72f + 0.25f * (1.0f - hue));
+      if (leaf) tint = mix(tint, mk(1.12f, 0.88f, 0.48f), smooth(0.78f, 0.98f, season) * 0.38f);
+      else tint = mix(tint, mk(0.78f, 0.66f, 0.53f), season * 0.18f);
+      albedo = albedo * tint * dark;
+    }
+    guide = albedo;
+    float r1 = rnd(pixel + sc.sample * 1823, 991), r2 = rnd(pixel + sc.sample * 971, 731);
+    float angle = r1 * 6.28318f, disk = sqrtf(r2) * 0.012f;
+    v3 light = unit(sc.sun + sc.su * (cosf(angle) * disk) + sc.sv * (sinf(angle) * disk));
+    float offset = id >= 0 ? 0.0014f : 0.004f;
+    v3 visibility = mk(1.0f, 1.0f, 1.0f);
+    if (sc.shadows)
+      visibility = lightVisibility<W>(P, Nodes4, nullptr, Order, trees, p + light * offset, light, sc.time, sc.wind);
+    float nl = sat(dot(n, light)), back = sat(-dot(n, light));
+    v3 direct = albedo * (nl / kPi);
+    if (leaf) {
+      float vein = leafVein(pr, leafUdir, localP, localFp);
+      direct = direct + albedo * (back * (0.75f - 0.25f * vein) + powf(sat(dot(rd, light)), 9.0f) * 0.12f);
+    }
+    float spec = specGGX(n, rd * -1.0f, light, leaf ? 0.39f : 0.82f);
+    direct = direct + mk(spec, spec, spec);
+    const float ao = 1.0f;
+    v3 ambient = mk(0.30f, 0.37f, 0.45f) * (0.45f + 0.55f * sat(n.y)) * ao + mk(0.08f, 0.068f, 0.035f) * sat(-n.y);
+    if (leaf) ambient = ambient + mk(0.04f, 0.055f, 0.022f) * ao;
+    col = direct * visibility * mk(4.8f, 4.2f, 3.1f) + albedo * ambient;
+    float haze = 1.0f - expf(-tt * 0.0018f);
+    col = mix(col, sky(rd, sc.sun), haze);
+    if (sc.mode == 1.0f) {
+      float w0 = bandWeight(0.065f, localFp), w1 = bandWeight(0.005f, localFp), w2 = bandWeight(0.001f, localFp);
+      col = mk(w0 * 0.80f, w1 * 0.85f, w2 * 0.9f);
+    }
+    if (sc.mode == 2.0f) {
+      float hh = rnd(id + 177 + (tree + 1) * 71, 32);
+      col = id < 0 ? mk(0.10f, 0.10f, 0.10f)
+                   : (leaf ? mk(0.1f, 0.3f + hh * 0.5f, 0.17f) : mk(0.47f + hh * 0.3f, 0.25f, 0.10f));
+    }
+    if (sc.mode == 3.0f) col = n * 0.5f + mk(0.5f, 0.5f, 0.5f);
+  }
+  outLinear = make_float4(col.x, col.y, col.z, 1.0f);
+  outGuide = make_float4(guide.x, guide.y, guide.z, 1.0f);
+}
+
 }  // namespace arbor

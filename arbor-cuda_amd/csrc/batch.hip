@@ -1,0 +1,332 @@
+// Batched multi-view rendering on gfx950: many camera views of uneven
+// resolution traced and shaded in ONE launch, then filtered and resolved per
+// view, the whole sequence captured once into a hipGraph and replayed.
+//
+// Layout: a view table {width, height, tilesX, tileBase, pixelBase} and one
+// C[64] per view in device memory; Hit/Linear/Guide/Filtered/History are
+// pooled float4 arrays and Pixels a pooled u32 array, view v's pixels
+// starting at pixelBase. The fused kernel numbers all 8x8 tiles of all views
+// consecutively; each wave takes a tile, finds its view in the LDS copy of
+// the table, traces the 64 rays (traceForest) and shades them (shadeOne)
+// without a round trip through HBM. Per pixel the arithmetic is that of
+// tracePixels + shadePixels, so every view equals its single-view render.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "arbor_batch.h"
+#include "arbor_shade.hpp"
+#include "launch.hpp"
+#include "tuning.hpp"
+
+namespace arbor {
+
+struct ViewDesc {
+  int width, height, tilesX, tileBase, pixelBase, pad0, pad1, pad2;
+};
+constexpr int kMaxViews = ARBOR_BATCH_MAX_VIEWS;
+
+__device__ __forceinline__ int findView(const ViewDesc* sv, int n, int tile) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {  // last view with tileBase <= tile
+    int mid = (lo + hi + 1) >> 1;
+    if (sv[mid].tileBase <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int kWaves, class W>
+__global__ __launch_bounds__(kWaves * 64) void k_traceShadeViews(const float* P, const float4* Nodes4,
+                                                                 const uint32_t* Order, const float* Cs,
+                                                                 const ViewDesc* views, int nviews, int totalTiles,
+                                                                 float4* Hit, float4* Linear, float4* Guide) {
+  __shared__ TreeInst trees[kTrees];
+  __shared__ ViewDesc sv[kMaxViews];
+  for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, (int)Cs[31]);
+  for (int i = threadIdx.x; i < nviews; i += blockDim.x) sv[i] = views[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x * kWaves + wave; t < totalTiles; t += gridDim.x * kWaves) {
+    const int v = findView(sv, nviews, t);
+    const ViewDesc d = sv[v];
+    const float* C = Cs + 64 * v;
+    const int lt = t - d.tileBase;
+    const int x = (lt % d.tilesX) * 8 + (lane & 7), y = (lt / d.tilesX) * 8 + (lane >> 3);
+    if (x >= d.width || y >= d.height) continue;
+    const ShadeCtx sc = makeShadeCtx(C, d.width, d.height);
+    v3 ro = sc.view.eye, rd = viewRay(sc.view, x, y, d.width, d.height);
+    float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
+    int tree = -1;
+    v2 h = traceForest<W>(P, Nodes4, nullptr, Order, trees, ro, rd, C[14] / (float)d.height, C[8], C[11],
+                          ground > 0.0f ? ground : kFar, &tree, false);
+    const int pixel = d.pixelBase + y * d.width + x;
+    const float4 hit = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)d.height);
+    Hit[pixel] = hit;
+    float4 l, g;
+    shadeOne<Walk<true, false, false>>(P, Nodes4, Order, trees, sc, x, y, hit, l, g);
+    Linear[pixel] = l;
+    Guide[pixel] = g;
+  }
+}
+
+typedef void (*BatchKernel)(const float*, const float4*, const uint32_t*, const float*, const ViewDesc*, int, int,
+                            float4*, float4*, float4*);
+struct BatchVariant {
+  BatchKernel kernel;
+  int waves;
+  const char* name;
+};
+const BatchVariant kBatchVariants[] = {
+    {k_traceShadeViews<1, Walk<true, false, false>>, 1, "1 wave/block, culled walk"},
+    {k_traceShadeViews<2, Walk<true, false, false>>, 2, "2 waves/block, culled walk"},
+    {k_traceShadeViews<4, Walk<true, false, false>>, 4, "4 waves/block, culled walk"},
+    {k_traceShadeViews<1, Walk<false, false, false>>, 1, "1 wave/block, plain walk"},
+    {k_traceShadeViews<4, Walk<false, false, false>>, 4, "4 waves/block, plain walk"},
+};
+constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
+int batchVariantCount() { return kBatchVariantCount; }
+
+}  // namespace arbor
+
+using namespace arbor;
+
+struct arbor_batch {
+  int maxViews = 0;
+  long long maxPixels = 0;
+  int nviews = 0;
+  long long npixels = 0;
+  int totalTiles = 0;
+  std::vector<ViewDesc> host;
+  ViewDesc* dViews = nullptr;
+  float* dC = nullptr;
+  float4 *Hit = nullptr, *Linear = nullptr, *Guide = nullptr, *Filtered = nullptr, *History = nullptr;
+  uint32_t* Pixels = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+template <class T>
+int dalloc(T** p, size_t bytes) {
+  return hipMalloc(reinterpret_cast<void**>(p), bytes) == hipSuccess ? ARBOR_OK : ARBOR_ERR_DEVICE;
+}
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+void dropGraph(arbor_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// The launches of one batched render on `stream` (also what gets captured).
+int issue(arbor_batch* b, const float* P, const float* Nodes, const unsigned* Order, int stages, hipStream_t s) {
+  if (stages & ARBOR_BATCH_TRACE_SHADE) {
+    const BatchVariant& v = kBatchVariants[tuning().batch_variant];
+    int blocks = (b->totalTiles + v.waves - 1) / v.waves;
+    if (blocks > tuning().grid_cap) blocks = tuning().grid_cap;
+    hipLaunchKernelGGL(v.kernel, dim3(blocks), dim3(v.waves * 64), 0, s, P, reinterpret_cast<const float4*>(Nodes),
+                       Order, b->dC, b->dViews, b->nviews, b->totalTiles, b->Hit, b->Linear, b->Guide);
+    int rc = launch_status();
+    if (rc != ARBOR_OK) return rc;
+  }
+  for (int i = 0; i < b->nviews; i++) {
+    const ViewDesc& d = b->host[i];
+    const float* C = b->dC + 64 * i;
+    size_t o = (size_t)d.pixelBase;
+    if (stages & ARBOR_BATCH_FILTER) {
+      int rc = arbor_filterLighting(reinterpret_cast<const float*>(b->Linear + o),
+                                    reinterpret_cast<const float*>(b->Guide + o),
+                                    reinterpret_cast<const float*>(b->Hit + o), reinterpret_cast<float*>(b->Filtered + o),
+                                    C, d.width, d.height, s);
+      if (rc != ARBOR_OK) return rc;
+    }
+    if (stages & ARBOR_BATCH_RESOLVE) {
+      int rc = arbor_resolvePixels(reinterpret_cast<const float*>(b->Filtered + o), reinterpret_cast<float*>(b->History + o),
+                                   b->Pixels + o, C, d.width, d.height, s);
+      if (rc != ARBOR_OK) return rc;
+    }
+  }
+  return ARBOR_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int arbor_batch_create(arbor_batch** out, int max_views, long long max_pixels) {
+  if (!out) return ARBOR_ERR_NULL;
+  *out = nullptr;
+  if (max_views < 1 || max_views > kMaxViews || max_pixels < 1 || max_pixels > (1ll << 31) / 16)
+    return ARBOR_ERR_ARG;
+  arbor_batch* b = new arbor_batch();
+  b->maxViews = max_views;
+  b->maxPixels = max_pixels;
+  size_t n = (size_t)max_pixels;
+  int rc = ARBOR_OK;
+  rc |= dalloc(&b->dViews, sizeof(ViewDesc) * max_views);
+  rc |= dalloc(&b->dC, sizeof(float) * 64 * max_views);
+  rc |= dalloc(&b->Hit, 16 * n);
+  rc |= dalloc(&b->Linear, 16 * n);
+  rc |= dalloc(&b->Guide, 16 * n);
+  rc |= dalloc(&b->Filtered, 16 * n);
+  rc |= dalloc(&b->History, 16 * n);
+  rc |= dalloc(&b->Pixels, 4 * n);
+  if (rc != ARBOR_OK || hipMemset(b->History, 0, 16 * n) != hipSuccess) {
+    arbor_batch_destroy(b);
+    return ARBOR_ERR_DEVICE;
+  }
+  *out = b;
+  return ARBOR_OK;
+}
+
+int arbor_batch_destroy(arbor_batch* b) {
+  if (!b) return ARBOR_ERR_NULL;
+  (void)hipDeviceSynchronize();
+  dropGraph(b);
+  dfree(b->dViews); dfree(b->dC); dfree(b->Hit); dfree(b->Linear); dfree(b->Guide); dfree(b->Filtered);
+  dfree(b->History); dfree(b->Pixels);
+  delete b;
+  return ARBOR_OK;
+}
+
+int arbor_batch_set_views(arbor_batch* b, int nviews, const float* cameras, const int* widths, const int* heights) {
+  if (!b || !cameras || !widths || !heights) return ARBOR_ERR_NULL;
+  if (nviews < 1 || nviews > b->maxViews) return ARBOR_ERR_ARG;
+  std::vector<ViewDesc> h(nviews);
+  long long pix = 0;
+  int tiles = 0;
+  for (int i = 0; i < nviews; i++) {
+    int w = widths[i], hh = heights[i];
+    if (!dims_ok(w, hh)) return ARBOR_ERR_ARG;
+    // all views see one forest: the instancing seed C[31] must agree
+    if (cameras[64 * i + 31] != cameras[31]) return ARBOR_ERR_ARG;
+    ViewDesc d{};
+    d.width = w;
+    d.height = hh;
+    d.tilesX = (w + 7) / 8;
+    d.tileBase = tiles;
+    d.pixelBase = (int)pix;
+    tiles += d.tilesX * ((hh + 7) / 8);
+    pix += (long long)w * hh;
+    h[i] = d;
+  }
+  if (pix > b->maxPixels) return ARBOR_ERR_ARG;
+  if (hipMemcpy(b->dViews, h.data(), sizeof(ViewDesc) * nviews, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->dC, cameras, sizeof(float) * 64 * nviews, hipMemcpyHostToDevice) != hipSuccess)
+    return ARBOR_ERR_DEVICE;
+  b->host = h;
+  b->nviews = nviews;
+  b->npixels = pix;
+  b->totalTiles = tiles;
+  dropGraph(b);  // the captured sequence depends on the layout
+  return ARBOR_OK;
+}
+
+int arbor_batch_render(arbor_batch* b, const float* P, const float* Nodes, const unsigned* Order, int stages,
+                       hipStream_t stream) {
+  if (!b || !P || !Nodes || !Order) return ARBOR_ERR_NULL;
+  if (b->nviews < 1) return ARBOR_ERR_STATE;
+  if (stages <= 0 || stages > ARBOR_BATCH_ALL) return ARBOR_ERR_ARG;
+  return issue(b, P, Nodes, Order, stages, stream);
+}
+
+int arbor_batch_capture(arbor_batch* b, const float* P, const float* Nodes, const unsigned* Order, int stages,
+                        hipStream_t stream) {
+  if (!b || !P || !Nodes || !Order) return ARBOR_ERR_NULL;
+  if (b->nviews < 1) return ARBOR_ERR_STATE;
+  if (stages <= 0 || stages > ARBOR_BATCH_ALL) return ARBOR_ERR_ARG;
+  if (!stream) return ARBOR_ERR_ARG;  // capture needs a created stream, not the legacy default stream
+  dropGraph(b);
+  if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return ARBOR_ERR_DEVICE;
+  int rc = issue(b, P, Nodes, Order, stages, stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(stream, &g);
+  if (rc != ARBOR_OK || e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc != ARBOR_OK ? rc : ARBOR_ERR_DEVICE;
+  }
+  hipGraphExec_t x = nullptr;
+  if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return ARBOR_ERR_DEVICE;
+  }
+  b->graph = g;
+  b->exec = x;
+  return ARBOR_OK;
+}
+
+int arbor_batch_launch(arbor_batch* b, hipStream_t stream) {
+  if (!b) return ARBOR_ERR_NULL;
+  if (!b->exec) return ARBOR_ERR_STATE;
+  return hipGraphLaunch(b->exec, stream) == hipSuccess ? ARBOR_OK : ARBOR_ERR_LAUNCH;
+}
+
+void* arbor_batch_buffer(arbor_batch* b, int view, const char* name) {
+  if (!b || !name || view < 0 || view >= b->nviews) return nullptr;
+  size_t o = (size_t)b->host[view].pixelBase;
+  if (!strcmp(name, "Hit")) return b->Hit + o;
+  if (!strcmp(name, "Linear")) return b->Linear + o;
+  if (!strcmp(name, "Guide")) return b->Guide + o;
+  if (!strcmp(name, "Filtered")) return b->Filtered + o;
+  if (!strcmp(name, "History")) return b->History + o;
+  if (!strcmp(name, "Pixels")) return b->Pixels + o;
+  if (!strcmp(name, "C")) return b->dC + 64 * view;
+  return nullptr;
+}
+
+long long arbor_batch_pixels(const arbor_batch* b) { return b ? b->npixels : -1; }
+int arbor_batch_tiles(const arbor_batch* b) { return b ? b->totalTiles : -1; }
+
+// Times every batched-kernel variant on the batch's current views (trace +
+// shade stage, `reps` launches each after one warm-up, HIP events on
+// `stream`), keeps the fastest in the process tuning ("batch_variant"), and
+// writes {"best": i, "ms": [...], "names": [...]} to json.
+int arbor_autotune_batch(arbor_batch* b, const float* P, const float* Nodes, const unsigned* Order, int reps,
+                         hipStream_t stream, char* json, int cap) {
+  if (!b || !P || !Nodes || !Order) return ARBOR_ERR_NULL;
+  if (b->nviews < 1) return ARBOR_ERR_STATE;
+  if (reps < 1 || reps > 1000) return ARBOR_ERR_ARG;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return ARBOR_ERR_DEVICE;
+  Tuning& t = tuning();
+  const int saved = t.batch_variant;
+  std::vector<float> ms(kBatchVariantCount, 0.0f);
+  int best = 0, rc = ARBOR_OK;
+  for (int v = 0; v < kBatchVariantCount && rc == ARBOR_OK; v++) {
+    t.batch_variant = v;
+    rc = issue(b, P, Nodes, Order, ARBOR_BATCH_TRACE_SHADE, stream);
+    (void)hipEventRecord(e0, stream);
+    for (int r = 0; r < reps && rc == ARBOR_OK; r++) rc = issue(b, P, Nodes, Order, ARBOR_BATCH_TRACE_SHADE, stream);
+    (void)hipEventRecord(e1, stream);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = ARBOR_ERR_DEVICE;
+    (void)hipEventElapsedTime(&ms[v], e0, e1);
+    ms[v] /= (float)reps;
+    if (ms[v] < ms[best]) best = v;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  t.batch_variant = rc == ARBOR_OK ? best : saved;
+  dropGraph(b);
+  if (rc != ARBOR_OK) return rc;
+  if (json && cap > 0) {
+    std::string s = "{\"best\":" + std::to_string(best) + ",\"ms\":[";
+    for (int v = 0; v < kBatchVariantCount; v++) s += (v ? "," : "") + std::to_string(ms[v]);
+    s += "],\"names\":[";
+    for (int v = 0; v < kBatchVariantCount; v++) s += std::string(v ? "," : "") + "\"" + kBatchVariants[v].name + "\"";
+    s += "]}";
+    if ((int)s.size() >= cap) return ARBOR_ERR_ARG;
+    memcpy(json, s.c_str(), s.size() + 1);
+  }
+  return ARBOR_OK;
+}
+
+}  // extern "C"

@@ -8,6 +8,7 @@
 // identical results; the tuning key "trace_variant" selects one. Filter and
 // resolve are HBM-bound per-pixel kernels over float4 rows.
 #include <limits.h>
+#include <stdio.h>
 #include <math.h>
 #include <string.h>
 
@@ -118,102 +119,16 @@ __global__ __launch_bounds__(kTileBlock) void k_shadePixels(const float* P, cons
   __shared__ TreeInst trees[kTrees];
   fillTrees(trees, (int)C[31]);
   __syncthreads();
-  const View view = makeView(C);
-  const v3 sun = unit(mk(cos_cr(C[12]) * cos_cr(C[13]), sin_cr(C[13]), sin_cr(C[12]) * cos_cr(C[13])));
-  const v3 su = basisU(sun), sv = cross(sun, su);
-  const float time = C[8], wind = C[11], mode = C[15];
-  const bool shadows = C[22] > 0.5f;
-  const int sample = (int)C[9];
+  const ShadeCtx sc = makeShadeCtx(C, width, height);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int t = blockIdx.x * kWavesPerBlock + wave; t < tiles; t += gridDim.x * kWavesPerBlock) {
     int x = (t % tilesX) * 8 + (lane & 7), y = (t / tilesX) * 8 + (lane >> 3);
     if (x >= width || y >= height) continue;
     int pixel = y * width + x;
-    float4 hit = Hit[pixel];
-    float tt = hit.x;
-    int id = (int)hit.y, tree = (int)hit.z;
-    v3 ro = view.eye, rd = viewRay(view, x, y, width, height);
-    v3 col = sky(rd, sun);
-    v3 guide = mk(1.0f, 1.0f, 1.0f);
-    if (tt < kFar) {
-      v3 p = ro + rd * tt;
-      float fp = fmaxf(0.000005f, hit.w);
-      v3 localP = p, n = mk(0.0f, 1.0f, 0.0f);
-      float localFp = fp, phase = 0.0f;
-      Prim pr;
-      bool leaf = false;
-      if (id >= 0) {
-        pr = loadPrim(P, id);
-        leaf = pr.type() == 1.0f;
-      }
-      v3 leafUdir = mk(0.0f, 0.0f, 0.0f);
-      if (id >= 0 && tree >= 0) {
-        const TreeInst& ti = trees[tree];
-        localP = toLocalPoint(p, ti);
-        localFp = fp / ti.sc;
-        phase = ti.phase;
-        n = toWorldNormal(objectNormal(pr, localP, localFp, time + phase, wind), ti);
-      }
-      if (leaf) leafUdir = leafFrame(pr, time + phase, wind).u;
-      if (dot(n, rd) > 0.0f) n = n * -1.0f;
-      if (id < 0) {
-        float e = fmaxf(0.001f, fp);
-        float nx = (noise((p.x + e) * 18.0f, p.z * 18.0f, 133) - noise((p.x - e) * 18.0f, p.z * 18.0f, 133)) * 0.18f;
-        float nz = (noise(p.x * 18.0f, (p.z + e) * 18.0f, 133) - noise(p.x * 18.0f, (p.z - e) * 18.0f, 133)) * 0.18f;
-        n = unit(mk(nx, 1.0f, nz));
-      }
-      v3 albedo;
-      if (id < 0) albedo = groundColor(p, localFp);
-      else {
-        int type = (int)pr.type();
-        if (type == 0) albedo = barkColor(pr, localP, localFp);
-        else if (type == 1) albedo = leafColor(pr, leafUdir, localP, localFp);
-        else albedo = mix(mk(0.09f, 0.10f, 0.086f), mk(0.31f, 0.30f, 0.24f), noise3(localP * 42.0f, (int)pr.seed()));
-      }
-      if (tree >= 0) {
-        int ts = trees[tree].seed;
-        float hue = rnd(8, ts), age = rnd(9, ts), season = rnd(10, ts);
-        float dark = 0.76f + age * 0.38f;
-        v3 tint = mk(0.88f + 0.22f * hue, 0.84f + 0.28f * (1.0f - fabsf(hue - 0.55f)), 0.72f + 0.25f * (1.0f - hue));
-        if (leaf) tint = mix(tint, mk(1.12f, 0.88f, 0.48f), smooth(0.78f, 0.98f, season) * 0.38f);
-        else tint = mix(tint, mk(0.78f, 0.66f, 0.53f), season * 0.18f);
-        albedo = albedo * tint * dark;
-      }
-      guide = albedo;
-      float r1 = rnd(pixel + sample * 1823, 991), r2 = rnd(pixel + sample * 971, 731);
-      float angle = r1 * 6.28318f, disk = sqrtf(r2) * 0.012f;
-      v3 light = unit(sun + su * (cosf(angle) * disk) + sv * (sinf(angle) * disk));
-      float offset = id >= 0 ? 0.0014f : 0.004f;
-      v3 visibility = mk(1.0f, 1.0f, 1.0f);
-      if (shadows)
-        visibility = lightVisibility<WalkShade>(P, Nodes4, nullptr, Order, trees, p + light * offset, light, time, wind);
-      float nl = sat(dot(n, light)), back = sat(-dot(n, light));
-      v3 direct = albedo * (nl / kPi);
-      if (leaf) {
-        float vein = leafVein(pr, leafUdir, localP, localFp);
-        direct = direct + albedo * (back * (0.75f - 0.25f * vein) + powf(sat(dot(rd, light)), 9.0f) * 0.12f);
-      }
-      float spec = specGGX(n, rd * -1.0f, light, leaf ? 0.39f : 0.82f);
-      direct = direct + mk(spec, spec, spec);
-      const float ao = 1.0f;
-      v3 ambient = mk(0.30f, 0.37f, 0.45f) * (0.45f + 0.55f * sat(n.y)) * ao + mk(0.08f, 0.068f, 0.035f) * sat(-n.y);
-      if (leaf) ambient = ambient + mk(0.04f, 0.055f, 0.022f) * ao;
-      col = direct * visibility * mk(4.8f, 4.2f, 3.1f) + albedo * ambient;
-      float haze = 1.0f - expf(-tt * 0.0018f);
-      col = mix(col, sky(rd, sun), haze);
-      if (mode == 1.0f) {
-        float w0 = bandWeight(0.065f, localFp), w1 = bandWeight(0.005f, localFp), w2 = bandWeight(0.001f, localFp);
-        col = mk(w0 * 0.80f, w1 * 0.85f, w2 * 0.9f);
-      }
-      if (mode == 2.0f) {
-        float hh = rnd(id + 177 + (tree + 1) * 71, 32);
-        col = id < 0 ? mk(0.10f, 0.10f, 0.10f)
-                     : (leaf ? mk(0.1f, 0.3f + hh * 0.5f, 0.17f) : mk(0.47f + hh * 0.3f, 0.25f, 0.10f));
-      }
-      if (mode == 3.0f) col = n * 0.5f + mk(0.5f, 0.5f, 0.5f);
-    }
-    Linear[pixel] = make_float4(col.x, col.y, col.z, 1.0f);
-    Guide[pixel] = make_float4(guide.x, guide.y, guide.z, 1.0f);
+    float4 l, g;
+    shadeOne<WalkShade>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel], l, g);
+    Linear[pixel] = l;
+    Guide[pixel] = g;
   }
 }
 
@@ -420,7 +335,8 @@ int arbor_tune_set(const char* key, int value) {
       {"grid_cap", &t.grid_cap, 1, 65535},
       {"sched_tiles", &t.sched_tiles, 1, 4096},
       {"sched_leaf_min", &t.sched_leaf_min, 1, 64},
-      {"sched_newton_min", &t.sched_newton_min, 1, 64}};
+      {"sched_newton_min", &t.sched_newton_min, 1, 64},
+      {"batch_variant", &t.batch_variant, 0, batchVariantCount() - 1}};
   for (auto& k : keys)
     if (!strcmp(key, k.name)) {
       if (value < k.lo || value > k.hi) return ARBOR_ERR_ARG;
@@ -438,7 +354,81 @@ int arbor_tune_get(const char* key) {
   if (!strcmp(key, "sched_tiles")) return t.sched_tiles;
   if (!strcmp(key, "sched_leaf_min")) return t.sched_leaf_min;
   if (!strcmp(key, "sched_newton_min")) return t.sched_newton_min;
+  if (!strcmp(key, "batch_variant")) return t.batch_variant;
   return ARBOR_ERR_ARG;
+}
+
+static const char* const kTuneKeys[] = {"trace_variant", "grid_cap", "sched_tiles", "sched_leaf_min",
+                                        "sched_newton_min", "batch_variant"};
+
+// Tuning cache: one "key value" line per key; unknown keys and out-of-range
+// values in a file are rejected as a whole.
+int arbor_tune_save(const char* path) {
+  if (!path) return ARBOR_ERR_NULL;
+  FILE* f = fopen(path, "w");
+  if (!f) return ARBOR_ERR_ARG;
+  fprintf(f, "# arbor tuning (%s)\n", arbor_version());
+  for (const char* k : kTuneKeys) fprintf(f, "%s %d\n", k, arbor_tune_get(k));
+  return fclose(f) == 0 ? ARBOR_OK : ARBOR_ERR_ARG;
+}
+
+int arbor_tune_load(const char* path) {
+  if (!path) return ARBOR_ERR_NULL;
+  FILE* f = fopen(path, "r");
+  if (!f) return ARBOR_ERR_ARG;
+  const Tuning saved = tuning();
+  char line[256];
+  int rc = ARBOR_OK;
+  while (rc == ARBOR_OK && fgets(line, sizeof line, f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    char key[64];
+    int value;
+    if (sscanf(line, "%63s %d", key, &value) != 2) rc = ARBOR_ERR_ARG;
+    else rc = arbor_tune_set(key, value);
+  }
+  fclose(f);
+  if (rc != ARBOR_OK) tuning() = saved;
+  return rc;
+}
+
+// Autotuner for single-view tracing: times every trace configuration on the
+// caller's buffers (reps launches each after a warm-up, HIP events on
+// `stream`), keeps the fastest in the process tuning, writes the timings as
+// JSON {"best": i, "ms": [...]}. The wave-scheduled kernels are tried with
+// the current sched_* settings.
+int arbor_autotune_trace(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
+                         int width, int height, int reps, hipStream_t stream, char* json, int cap) {
+  ARBOR_REQUIRE(P && Nodes && Order && C && Hit);
+  if (!dims_ok(width, height) || reps < 1 || reps > 1000) return ARBOR_ERR_ARG;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return ARBOR_ERR_DEVICE;
+  Tuning& t = tuning();
+  const int saved = t.trace_variant;
+  const int n = kTraceVariantCount + 2;
+  float ms[kTraceVariantCount + 2] = {};
+  int best = saved, rc = ARBOR_OK;
+  for (int v = 0; v < n && rc == ARBOR_OK; v++) {
+    t.trace_variant = v;
+    rc = launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream);
+    (void)hipEventRecord(e0, stream);
+    for (int r = 0; r < reps && rc == ARBOR_OK; r++) rc = launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream);
+    (void)hipEventRecord(e1, stream);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = ARBOR_ERR_DEVICE;
+    (void)hipEventElapsedTime(&ms[v], e0, e1);
+    ms[v] /= (float)reps;
+    if (v == 0 || ms[v] < ms[best]) best = v;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  t.trace_variant = rc == ARBOR_OK ? best : saved;
+  if (rc != ARBOR_OK) return rc;
+  if (json && cap > 0) {
+    int used = snprintf(json, (size_t)cap, "{\"best\":%d,\"ms\":[", best);
+    for (int v = 0; v < n && used < cap; v++) used += snprintf(json + used, (size_t)(cap - used), "%s%.4f", v ? "," : "", ms[v]);
+    if (used < cap) used += snprintf(json + used, (size_t)(cap - used), "]}");
+    if (used >= cap) return ARBOR_ERR_ARG;
+  }
+  return ARBOR_OK;
 }
 
 }  // extern "C"

@@ -47,6 +47,19 @@ SIGNATURES = {
     "arbor_version": [],
     "arbor_tune_set": [ct.c_char_p, _i],
     "arbor_tune_get": [ct.c_char_p],
+    "arbor_tune_save": [ct.c_char_p],
+    "arbor_tune_load": [ct.c_char_p],
+    "arbor_autotune_trace": [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, ct.c_char_p, _i],
+    "arbor_batch_create": [ct.POINTER(_vp), _i, ct.c_longlong],
+    "arbor_batch_destroy": [_vp],
+    "arbor_batch_set_views": [_vp, _i, ct.POINTER(_f), ct.POINTER(_i), ct.POINTER(_i)],
+    "arbor_batch_render": [_vp, _vp, _vp, _vp, _i, _vp],
+    "arbor_batch_capture": [_vp, _vp, _vp, _vp, _i, _vp],
+    "arbor_batch_launch": [_vp, _vp],
+    "arbor_batch_buffer": [_vp, _i, ct.c_char_p],
+    "arbor_batch_pixels": [_vp],
+    "arbor_batch_tiles": [_vp],
+    "arbor_autotune_batch": [_vp, _vp, _vp, _vp, _i, _vp, ct.c_char_p, _i],
     "arbor_engine_round_size": [ct.POINTER(_i), ct.POINTER(_i)],
     "arbor_engine_create": [ct.POINTER(_vp), _i, _i, _i, _i, _vp],
     "arbor_engine_destroy": [_vp],
@@ -65,7 +78,8 @@ SIGNATURES = {
     "arbor_walk_input": [_i, _i, ct.POINTER(_f)],
 }
 _RESTYPES = {"arbor_version": ct.c_char_p, "arbor_engine_round_size": None, "arbor_engine_buffer": _vp,
-             "arbor_engine_stream": _vp}
+             "arbor_engine_stream": _vp, "arbor_batch_buffer": _vp, "arbor_batch_pixels": ct.c_longlong}
+BATCH_TRACE_SHADE, BATCH_FILTER, BATCH_RESOLVE, BATCH_ALL = 1, 2, 4, 7
 
 
 class ArborError(RuntimeError):
@@ -198,6 +212,86 @@ def tune_get(key):
     if v < 0:
         raise ArborError("unknown tuning key " + key)
     return v
+
+
+def autotune_trace(scene, C, fb, reps=3):
+    """Runs the single-view autotuner on these buffers; returns its JSON."""
+    import json
+
+    buf = ct.create_string_buffer(4096)
+    s = _stream()
+    rc = lib().arbor_autotune_trace(_ptr(scene.P), _ptr(scene.Nodes), _ptr(scene.Order), _ptr(C), _ptr(fb.Hit),
+                                    fb.width, fb.height, reps, s, buf, 4096)
+    check(rc, "arbor_autotune_trace")
+    return json.loads(buf.value.decode())
+
+
+class Batch:
+    """Batched multi-view rendering (include/arbor_batch.h)."""
+
+    def __init__(self, max_views, max_pixels):
+        h = ct.c_void_p()
+        check(lib().arbor_batch_create(ct.byref(h), max_views, max_pixels), "arbor_batch_create")
+        self.h = h
+        self.sizes = []
+
+    def close(self):
+        if self.h:
+            check(lib().arbor_batch_destroy(self.h), "arbor_batch_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_views(self, cameras, sizes):
+        cams = np.ascontiguousarray(cameras, np.float32).reshape(-1)
+        n = len(sizes)
+        w = (ct.c_int * n)(*[s[0] for s in sizes])
+        hh = (ct.c_int * n)(*[s[1] for s in sizes])
+        check(lib().arbor_batch_set_views(self.h, n, cams.ctypes.data_as(ct.POINTER(ct.c_float)), w, hh),
+              "arbor_batch_set_views")
+        self.sizes = list(sizes)
+
+    def render(self, scene, stages=BATCH_ALL, stream=None):
+        check(lib().arbor_batch_render(self.h, _ptr(scene.P), _ptr(scene.Nodes), _ptr(scene.Order), stages,
+                                       _stream(stream)), "arbor_batch_render")
+
+    def capture(self, scene, stream, stages=BATCH_ALL):
+        check(lib().arbor_batch_capture(self.h, _ptr(scene.P), _ptr(scene.Nodes), _ptr(scene.Order), stages,
+                                        _stream(stream)), "arbor_batch_capture")
+
+    def launch(self, stream=None):
+        check(lib().arbor_batch_launch(self.h, _stream(stream)), "arbor_batch_launch")
+
+    def autotune(self, scene, reps=2):
+        import json
+
+        buf = ct.create_string_buffer(4096)
+        check(lib().arbor_autotune_batch(self.h, _ptr(scene.P), _ptr(scene.Nodes), _ptr(scene.Order), reps,
+                                         _stream(), buf, 4096), "arbor_autotune_batch")
+        return json.loads(buf.value.decode())
+
+    def pixels(self):
+        return lib().arbor_batch_pixels(self.h)
+
+    def download(self, view, name):
+        """Host copy of view `view`'s slice of buffer `name`."""
+        import torch
+
+        torch.cuda.synchronize()
+        w, h = self.sizes[view]
+        per = {"Pixels": 4, "C": 256}.get(name, 16)
+        n = 64 * 4 if name == "C" else w * h * per
+        ptr = lib().arbor_batch_buffer(self.h, view, name.encode())
+        if not ptr:
+            raise ArborError("no buffer %s for view %d" % (name, view))
+        host = torch.empty(n, dtype=torch.uint8)
+        host.copy_(_wrap_device(ptr, n))
+        a = host.numpy()
+        return a.view(np.uint32).copy() if name == "Pixels" else a.view(np.float32).copy()
 
 
 def walk_input(seed, frame):

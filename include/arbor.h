@@ -135,8 +135,20 @@ const char* arbor_version(void);
  * wave-batched leaf tests), "grid_cap" (max blocks of a tile kernel, 1..65535).
  * Every setting gives identical results. Process-wide; set before launching. */
 int arbor_tune_set(const char* key, int value);
-/* Current value of a tuning key, or a negative status for an unknown key. */
+/* Current value of a tuning key, or a negative status for an unknown key.
+ * Further keys: "sched_tiles" (1..4096), "sched_leaf_min" and
+ * "sched_newton_min" (1..64) for the wave-scheduled trace kernel, and
+ * "batch_variant" (arbor_batch.h). */
 int arbor_tune_get(const char* key);
+/* Persisted tuning cache: writes / reads "key value" lines. A file with an
+ * unknown key or out-of-range value is rejected whole (tuning unchanged). */
+int arbor_tune_save(const char* path);
+int arbor_tune_load(const char* path);
+/* Autotuner (no counterpart in the reference): times every trace
+ * configuration on the caller's buffers and view, keeps the fastest as
+ * "trace_variant", writes {"best": i, "ms": [...]} to json. Synchronises. */
+int arbor_autotune_trace(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
+                         int width, int height, int reps, hipStream_t stream, char* json, int cap);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ DUMMY = ct.c_void_p(4096)  # never dereferenced: every case below fails validati
 
 def declared_symbols():
     names = set()
-    for h in ("arbor.h", "arbor_engine.h"):
+    for h in ("arbor.h", "arbor_engine.h", "arbor_batch.h"):
         with open(os.path.join(REPO, "include", h)) as f:
             text = f.read()
         names |= set(re.findall(r"\b(arbor_\w+)\s*\(", text))
@@ -111,6 +111,36 @@ def test_tuning_keys_validated(L):
     assert L.arbor_tune_set(b"grid_cap", 777) == 0 and L.arbor_tune_get(b"grid_cap") == 777
     assert L.arbor_tune_set(b"grid_cap", old) == 0
     assert L.arbor_tune_get(b"bogus") == -2
+
+
+def test_batch_argument_checks(L):
+    h = ct.c_void_p()
+    assert L.arbor_batch_create(None, 4, 1000) == -1
+    assert L.arbor_batch_create(ct.byref(h), 0, 1000) == -2
+    assert L.arbor_batch_create(ct.byref(h), 257, 1000) == -2
+    assert L.arbor_batch_create(ct.byref(h), 4, 0) == -2
+    assert not h.value
+    assert L.arbor_batch_destroy(None) == -1
+    assert L.arbor_batch_render(None, DUMMY, DUMMY, DUMMY, 7, None) == -1
+    assert L.arbor_batch_launch(None, None) == -1
+    assert L.arbor_batch_buffer(None, 0, b"Hit") is None
+    assert L.arbor_autotune_trace(DUMMY, DUMMY, DUMMY, DUMMY, NULL, 8, 8, 2, NULL, None, 0) == -1
+    assert L.arbor_autotune_trace(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 0, 8, 2, NULL, None, 0) == -2
+
+
+def test_tune_cache_roundtrip(L, tmp_path):
+    p = str(tmp_path / "tune.txt").encode()
+    old = L.arbor_tune_get(b"sched_tiles")
+    assert L.arbor_tune_set(b"sched_tiles", 33) == 0
+    assert L.arbor_tune_save(p) == 0
+    assert L.arbor_tune_set(b"sched_tiles", 5) == 0
+    assert L.arbor_tune_load(p) == 0 and L.arbor_tune_get(b"sched_tiles") == 33
+    bad = tmp_path / "bad.txt"
+    bad.write_text("sched_tiles 7\nno_such_key 3\n")
+    assert L.arbor_tune_load(str(bad).encode()) == -2
+    assert L.arbor_tune_get(b"sched_tiles") == 33  # rejected whole
+    assert L.arbor_tune_load(str(tmp_path / "missing").encode()) == -2
+    assert L.arbor_tune_set(b"sched_tiles", old) == 0
 
 
 def test_walk_input_deterministic(L):
