@@ -6,6 +6,9 @@
 
 #include <string>
 
+#include <math.h>
+
+#include "arbor_batch.h"
 #include "arbor_engine.h"
 #include "launch.hpp"
 
@@ -266,6 +269,55 @@ int arbor_engine_inspect(const arbor_engine* e, char* out, int cap) {
 static unsigned walk_hash(unsigned x) {
   x ^= x >> 16; x *= 2146121005u; x ^= x >> 15; x *= 2221713035u; x ^= x >> 16;
   return x;
+}
+static float walk_unit(unsigned n, unsigned seed) {  // [0,1) from the same hash as rnd
+  return (float)(walk_hash(n ^ walk_hash(seed)) & 16777215u) / 16777216.0f;
+}
+
+// Host-side camera state: initCamera's defaults (Arbor.cu:181) with fpsPose
+// at (eye, yaw, pitch) (Arbor.cu:179), evaluated with the host libm as the
+// reference harness does.
+int arbor_camera_pose(float* C, int seed, float ex, float ey, float ez, float yaw, float pitch) {
+  if (!C) return ARBOR_ERR_NULL;
+  if (seed < 0 || seed > 1000000) return ARBOR_ERR_ARG;
+  for (int i = 0; i < ARBOR_C_FLOATS; i++) C[i] = 0.0f;
+  C[12] = -0.45f; C[13] = 0.63f; C[14] = 0.90f; C[20] = 1.18f; C[22] = 1.0f; C[23] = 1.0f;
+  C[31] = (float)seed; C[44] = 1.0f;
+  C[16] = yaw; C[17] = pitch; C[27] = yaw; C[28] = pitch;
+  C[0] = ex; C[1] = ey; C[2] = ez;
+  float cp = cosf(pitch);
+  float fx = sinf(yaw) * cp, fy = sinf(pitch), fz = cosf(yaw) * cp;
+  float len = sqrtf(fx * fx + fy * fy + fz * fz);
+  float d = len > 0.0000001f ? len : 0.0000001f;
+  C[4] = ex + fx / d; C[5] = ey + fy / d; C[6] = ez + fz / d;
+  C[3] = 1.0f; C[18] = 1.0f;
+  C[9] = 0.0f; C[10] = 1.0f;
+  return ARBOR_OK;
+}
+
+// The batched-views workload: view i stands at a seeded position inside the
+// forest (|x|,|z| < 240 m, eye height 1.72 m) with a seeded yaw and a small
+// pitch, sun visibility off, at a resolution drawn uniformly from {256x256,
+// 640x360, 1280x720, 1920x1080, 2560x1440}.
+int arbor_batch_walk_views(int seed, int world_seed, int n, float* cameras, int* widths, int* heights) {
+  if (!cameras || !widths || !heights) return ARBOR_ERR_NULL;
+  if (n < 1 || n > ARBOR_BATCH_MAX_VIEWS) return ARBOR_ERR_ARG;
+  static const int res[5][2] = {{256, 256}, {640, 360}, {1280, 720}, {1920, 1080}, {2560, 1440}};
+  for (int i = 0; i < n; i++) {
+    unsigned base = (unsigned)seed * 7919u + (unsigned)i * 101u;
+    float x = (walk_unit(base + 1, 811) - 0.5f) * 480.0f;
+    float z = (walk_unit(base + 2, 811) - 0.5f) * 480.0f;
+    float yaw = walk_unit(base + 3, 811) * 6.2831853f;
+    float pitch = (walk_unit(base + 4, 811) - 0.5f) * 0.2f;
+    int r = (int)(walk_unit(base + 5, 811) * 5.0f);
+    if (r > 4) r = 4;
+    int rc = arbor_camera_pose(cameras + 64 * i, world_seed, x, 1.72f, z, yaw, pitch);
+    if (rc != ARBOR_OK) return rc;
+    cameras[64 * i + 22] = 0.0f;
+    widths[i] = res[r][0];
+    heights[i] = res[r][1];
+  }
+  return ARBOR_OK;
 }
 int arbor_walk_input(int seed, int frame, float* input32) {
   if (!input32) return ARBOR_ERR_NULL;

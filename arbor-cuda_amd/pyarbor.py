@@ -60,6 +60,8 @@ SIGNATURES = {
     "arbor_batch_pixels": [_vp],
     "arbor_batch_tiles": [_vp],
     "arbor_autotune_batch": [_vp, _vp, _vp, _vp, _i, _vp, ct.c_char_p, _i],
+    "arbor_camera_pose": [ct.POINTER(_f), _i, _f, _f, _f, _f, _f],
+    "arbor_batch_walk_views": [_i, _i, _i, ct.POINTER(_f), ct.POINTER(_i), ct.POINTER(_i)],
     "arbor_engine_round_size": [ct.POINTER(_i), ct.POINTER(_i)],
     "arbor_engine_create": [ct.POINTER(_vp), _i, _i, _i, _i, _vp],
     "arbor_engine_destroy": [_vp],
@@ -224,6 +226,22 @@ def autotune_trace(scene, C, fb, reps=3):
                                     fb.width, fb.height, reps, s, buf, 4096)
     check(rc, "arbor_autotune_trace")
     return json.loads(buf.value.decode())
+
+
+def camera_pose(seed, eye, yaw, pitch):
+    C = (ct.c_float * 64)()
+    check(lib().arbor_camera_pose(C, seed, float(eye[0]), float(eye[1]), float(eye[2]), float(yaw), float(pitch)),
+          "arbor_camera_pose")
+    return np.frombuffer(C, dtype=np.float32).copy()
+
+
+def batch_walk_views(seed, world_seed, n):
+    """The batched-views workload: (cameras [n,64], [(w, h)] * n)."""
+    cams = (ct.c_float * (64 * n))()
+    w = (ct.c_int * n)()
+    h = (ct.c_int * n)()
+    check(lib().arbor_batch_walk_views(seed, world_seed, n, cams, w, h), "arbor_batch_walk_views")
+    return np.frombuffer(cams, dtype=np.float32).reshape(n, 64).copy(), list(zip(list(w), list(h)))
 
 
 class Batch:

@@ -45,6 +45,14 @@ int arbor_batch_launch(arbor_batch* b, hipStream_t stream);
 void* arbor_batch_buffer(arbor_batch* b, int view, const char* name);
 long long arbor_batch_pixels(const arbor_batch* b);
 int arbor_batch_tiles(const arbor_batch* b);
+/* Host helpers (no GPU). arbor_camera_pose fills a C state as initCamera
+ * (Arbor.cu:181) followed by fpsPose at (eye, yaw, pitch) (Arbor.cu:179).
+ * arbor_batch_walk_views generates the batched-views workload: n views at
+ * seeded positions inside the forest (|x|,|z| < 240 m, eye height 1.72 m),
+ * seeded yaw and small pitch, sun visibility off, resolutions drawn uniformly
+ * from {256x256, 640x360, 1280x720, 1920x1080, 2560x1440}. */
+int arbor_camera_pose(float* C, int seed, float ex, float ey, float ez, float yaw, float pitch);
+int arbor_batch_walk_views(int seed, int world_seed, int n, float* cameras, int* widths, int* heights);
 /* Autotuner: times each batched-kernel variant (block waves x walk policy)
  * on the current views, keeps the fastest as the "batch_variant" tuning and
  * writes the timings as JSON. Synchronises `stream`. */

@@ -143,6 +143,26 @@ def test_tune_cache_roundtrip(L, tmp_path):
     assert L.arbor_tune_set(b"sched_tiles", old) == 0
 
 
+def test_camera_pose_matches_init_camera(L, oracle):
+    # arbor_camera_pose at initCamera's pose gives initCamera's state with the
+    # accumulation reset flag C[10] set.
+    C = pyarbor.camera_pose(7, (2.0, 1.72, 18.0), np.float32(3.14159265359), 0.0)
+    want = oracle.init_camera(7)
+    want[10] = 1.0
+    assert np.array_equal(C.view(np.uint32), want.view(np.uint32))
+
+
+def test_batch_walk_views(L):
+    cams, sizes = pyarbor.batch_walk_views(3, 7, 64)
+    assert cams.shape == (64, 64) and len(sizes) == 64
+    assert set(sizes) <= {(256, 256), (640, 360), (1280, 720), (1920, 1080), (2560, 1440)}
+    assert len(set(sizes)) >= 4
+    assert (np.abs(cams[:, 0]) < 240).all() and (np.abs(cams[:, 2]) < 240).all() and (cams[:, 1] == 1.72).all()
+    assert (cams[:, 31] == 7).all() and (cams[:, 22] == 0).all()
+    c2, s2 = pyarbor.batch_walk_views(3, 7, 64)
+    assert np.array_equal(cams, c2) and sizes == s2
+
+
 def test_walk_input_deterministic(L):
     a = pyarbor.walk_input(7, 3)
     b = pyarbor.walk_input(7, 3)
