@@ -16,7 +16,7 @@ the 3e-4 acceptance threshold). The bar, per view:
 import numpy as np
 import pytest
 
-from gpu_util import hit_stats, require_gpu, to_dev, to_host
+from gpu_util import hit_stats, require_gpu, to_dev, to_host, ulp_diff
 
 pytestmark = pytest.mark.gpu
 
@@ -174,6 +174,50 @@ def test_shade_filter_resolve(px, oracle, scene7, view, shadows):
     print("resolve max channel diff", dpx)
     assert dpx <= 1
     assert np.allclose(to_host(fb.History), hist, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("view", [1, 3])
+def test_wind_shadows_four_spp(px, oracle, scene7, view):
+    """The shadowed, windy configuration at a reduced size: breeze on
+    (C[11]=0.6) at wind time C[8]=0.75 s, sun elevation 35 deg, tree-to-tree
+    sun visibility on, four jittered samples per pixel (C[9]=s, C[10]=(s==0)),
+    each traced, shaded, filtered and accumulated by resolvePixels."""
+    print("case: wind+shadows 4 spp view", view, "192x108", flush=True)
+    dev, ref = scene7
+    w, h = 192, 108
+    C = _camera(oracle, 7, view, w, h, shadows=1)
+    C[11], C[8], C[13] = 0.6, 0.75, 0.610865
+    fb = px.FrameBuffers(w, h)
+    hist = np.zeros(w * h * 4, np.float32)
+    for s in range(4):
+        C[9], C[10] = float(s), float(s == 0)
+        Cd = to_dev(C)
+        px.trace(dev, Cd, fb)
+        h32 = oracle.trace(ref, C, w, h)
+        h64 = oracle.trace64(ref, C, w, h)
+        _check_hits("wind v%d s%d" % (view, s), to_host(fb.Hit), h32, h64)
+        # shade, filter and resolve on the oracle's hits (isolates the lighting arithmetic)
+        fb.Hit.copy_(to_dev(h32))
+        px.shade(dev, Cd, fb)
+        L, G = to_host(fb.Linear), to_host(fb.Guide)
+        L64, _ = oracle.shade64(ref, C, h32, w, h)
+        L32, G32 = oracle.shade(ref, C, h32, w, h)
+        err = np.abs(L.reshape(-1, 4)[:, :3] - L64.reshape(-1, 4)[:, :3]).max(axis=1)
+        err32 = np.abs(L32.reshape(-1, 4)[:, :3] - L64.reshape(-1, 4)[:, :3]).max(axis=1)
+        print("s%d shade frac<=1e-3 gpu %.5f fp32 oracle %.5f" % (s, (err <= 1e-3).mean(), (err32 <= 1e-3).mean()))
+        assert (err <= 1e-3).mean() >= min(0.99, (err32 <= 1e-3).mean() - 0.005)
+        px.filter_lighting(Cd, fb)
+        F = to_host(fb.Filtered)
+        assert np.abs(F - oracle.filter(L, G, h32, C, w, h)).max() <= 1e-5 * max(1.0, float(np.abs(F).max()))
+        px.resolve(Cd, fb)
+        want = oracle.resolve(F, hist, C, w, h)
+        pix = to_host(fb.Pixels, np.uint32)
+        assert np.abs(pix.view(np.uint8).astype(int) - want.view(np.uint8).astype(int)).max() <= 1
+        hu = ulp_diff(to_host(fb.History), hist).max()
+        print("s%d history max ulp %d" % (s, hu))
+        assert hu <= 2
+        fb.History.copy_(to_dev(hist))  # both sides accumulate from the same history
+    assert (hist[3::4] == 4.0).all()  # four samples accumulated
 
 
 def test_engine_frame_pipeline(px, oracle):
