@@ -19,6 +19,7 @@ struct Buffers {
   unsigned *Keys = nullptr, *Order = nullptr;
   float *Hit = nullptr, *Linear = nullptr, *Guide = nullptr, *Filtered = nullptr, *History = nullptr;
   unsigned* Pixels = nullptr;
+  void* Schedule = nullptr;  // tile schedule of the trace (arbor_tracePixelsScheduled)
 };
 
 template <class T>
@@ -52,7 +53,7 @@ size_t pixel_bytes(int w, int h) { return (size_t)w * h * 16; }
 
 void free_pixels(arbor_engine* e) {
   release(e->b.Hit); release(e->b.Linear); release(e->b.Guide); release(e->b.Filtered);
-  release(e->b.History); release(e->b.Pixels);
+  release(e->b.History); release(e->b.Pixels); release(e->b.Schedule);
 }
 
 }  // namespace
@@ -143,12 +144,16 @@ int arbor_engine_resize(arbor_engine* e, int width, int height, int round_size) 
   rc |= alloc(&e->b.Filtered, n);
   rc |= alloc(&e->b.History, n);
   rc |= alloc(&e->b.Pixels, (size_t)width * height * 4);
+  const long long sched = arbor_tile_schedule_bytes(width, height);
+  rc |= alloc(&e->b.Schedule, (size_t)sched);
   if (rc != ARBOR_OK) {
     free_pixels(e);
     e->width = e->height = 0;
     return ARBOR_ERR_DEVICE;
   }
-  if (hipMemsetAsync(e->b.History, 0, n, e->stream) != hipSuccess) return ARBOR_ERR_DEVICE;
+  if (hipMemsetAsync(e->b.History, 0, n, e->stream) != hipSuccess ||
+      hipMemsetAsync(e->b.Schedule, 0, (size_t)sched, e->stream) != hipSuccess)
+    return ARBOR_ERR_DEVICE;
   e->width = width;
   e->height = height;
   return ARBOR_OK;
@@ -190,7 +195,9 @@ int arbor_engine_frame(arbor_engine* e, float dt, const float* device_input) {
   if (e->timing) (void)hipEventRecord(e->ev[0], e->stream);
   if ((rc = arbor_updateCamera(e->b.C, I, e->b.P, delta, w, h, e->stream)) != ARBOR_OK) return rc;
   if (e->timing) (void)hipEventRecord(e->ev[1], e->stream);
-  if ((rc = arbor_tracePixels(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, w, h, e->stream)) != ARBOR_OK) return rc;
+  if ((rc = arbor_tracePixelsScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, w, h, e->b.Schedule,
+                                       e->stream)) != ARBOR_OK)
+    return rc;
   if (e->timing) (void)hipEventRecord(e->ev[2], e->stream);
   if ((rc = arbor_shadePixels(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Linear, e->b.Guide, w, h,
                               e->stream)) != ARBOR_OK)
@@ -229,7 +236,7 @@ void* arbor_engine_buffer(arbor_engine* e, const char* name) {
   struct { const char* n; void* p; } table[] = {
       {"B", b.B}, {"P", b.P}, {"Keys", b.Keys}, {"Order", b.Order}, {"Nodes", b.Nodes}, {"C", b.C}, {"I", b.I},
       {"Hit", b.Hit}, {"Linear", b.Linear}, {"Guide", b.Guide}, {"Filtered", b.Filtered}, {"History", b.History},
-      {"Pixels", b.Pixels}};
+      {"Pixels", b.Pixels}, {"Schedule", b.Schedule}};
   for (auto& t : table)
     if (strcmp(t.n, name) == 0) return t.p;
   return nullptr;

@@ -31,11 +31,15 @@ __device__ __forceinline__ void fillTop(float4* top, const float4* Nodes4) {
     for (int i = threadIdx.x; i < 2 * kTopNodes; i += blockDim.x) top[i] = Nodes4[i];
 }
 
-// tracePixels (Arbor.cu:252) for every pixel of the view.
+// tracePixels (Arbor.cu:252) for every pixel of the view. With a tile
+// schedule, work item i is tile tileOrder[i] and each wave records how long its
+// tile took (wall clock ticks, >= 1) in tileCost; the order changes only which
+// wave traces which tile, never a pixel's result.
 template <int kWaves, class W>
 __global__ __launch_bounds__(kWaves * 64) void k_tracePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
                                                              const float* C, float4* Hit, int width, int height,
-                                                             int tilesX, int tiles) {
+                                                             int tilesX, int tiles, const uint32_t* tileOrder,
+                                                             uint32_t* tileCost) {
   __shared__ TreeInst trees[kTrees];
   __shared__ float4 top[W::ldsTop ? 2 * kTopNodes : 1];
   fillTrees(trees, (int)C[31]);
@@ -44,23 +48,94 @@ __global__ __launch_bounds__(kWaves * 64) void k_tracePixels(const float* P, con
   const View view = makeView(C);
   const float cone = C[14] / (float)height, time = C[8], wind = C[11];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int t = blockIdx.x * kWaves + wave; t < tiles; t += gridDim.x * kWaves) {
+  for (int i = blockIdx.x * kWaves + wave; i < tiles; i += gridDim.x * kWaves) {
+    const int t = tileOrder ? (int)tileOrder[i] : i;
+    const long long start = tileCost ? wall_clock64() : 0;
     int x = (t % tilesX) * 8 + (lane & 7), y = (t / tilesX) * 8 + (lane >> 3);
-    if (x >= width || y >= height) continue;
-    v3 ro = view.eye, rd = viewRay(view, x, y, width, height);
-    float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
-    int tree = -1;
-    v2 h = traceForest<W>(P, Nodes4, top, Order, trees, ro, rd, cone, time, wind, ground > 0.0f ? ground : kFar, &tree,
-                          false);
-    Hit[y * width + x] = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)height);
+    if (x < width && y < height) {
+      v3 ro = view.eye, rd = viewRay(view, x, y, width, height);
+      float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
+      int tree = -1;
+      v2 h = traceForest<W>(P, Nodes4, top, Order, trees, ro, rd, cone, time, wind, ground > 0.0f ? ground : kFar,
+                            &tree, false);
+      Hit[y * width + x] = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)height);
+    }
+    if (tileCost && lane == 0) {
+      long long ticks = wall_clock64() - start;
+      tileCost[t] = ticks < 1 ? 1u : (ticks > 0x7fffffffLL ? 0x7fffffffu : (uint32_t)ticks);
+    }
   }
+}
+
+// Tile schedule (longest predicted first) for the next tracePixels launch,
+// one block of kOrderThreads. A tile's predicted cost is the largest recorded
+// cost in the 3x3 tiles around where its centre ray fell in the previous
+// frame (the direction projected through the previous camera; eye motion
+// between frames is ignored); tiles with no prediction (first frame, or
+// directions outside the previous view) go first. Tiles are bucketed by
+// log2(cost) with two fraction bits and laid out bucket by bucket, highest
+// first. Then the current camera is kept for the next frame.
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderBuckets = 128;
+struct TileSchedule {
+  float prevC[64];
+  uint32_t valid;
+  uint32_t pad[63];
+  // followed by cost[tiles], order[tiles]
+};
+__device__ inline int tileKey(const View& cur, const View& prev, bool valid, const uint32_t* cost, int t, int tilesX,
+                              int tilesY, int width, int height) {
+  if (!valid) return kOrderBuckets - 1;
+  int x = (t % tilesX) * 8 + 4, y = (t / tilesX) * 8 + 4;
+  v3 d = viewRay(cur, x, y, width, height);
+  float fz = dot(d, prev.f);
+  if (fz <= 0.000001f) return kOrderBuckets - 1;
+  float u = dot(d, prev.r) / (fz * prev.scale), w = dot(d, prev.up) / (fz * prev.scale);
+  float px = u * (float)height + (float)width * 0.5f, py = (float)height * 0.5f - w * (float)height;
+  if (!(px >= 0.0f && py >= 0.0f && px < (float)width && py < (float)height)) return kOrderBuckets - 1;
+  int tx = (int)(px * 0.125f), ty = (int)(py * 0.125f);
+  uint32_t c = 1;
+  for (int j = max(0, ty - 1); j <= min(tilesY - 1, ty + 1); j++)
+    for (int i = max(0, tx - 1); i <= min(tilesX - 1, tx + 1); i++) c = max(c, cost[j * tilesX + i]);
+  int k = 31 - __clz((int)c);
+  int sub = k >= 2 ? (int)((c >> (k - 2)) & 3u) : 0;
+  return min(kOrderBuckets - 2, 4 * k + sub);
+}
+__global__ __launch_bounds__(kOrderThreads) void k_tileOrder(const float* C, TileSchedule* s, int tilesX, int tilesY,
+                                                             int width, int height) {
+  __shared__ uint32_t count[kOrderBuckets];
+  uint32_t* cost = reinterpret_cast<uint32_t*>(s + 1);
+  const int tiles = tilesX * tilesY;
+  uint32_t* order = cost + tiles;
+  const View cur = makeView(C), prev = makeView(s->prevC);
+  const bool valid = s->valid == 1u;
+  for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) count[b] = 0;
+  __syncthreads();
+  for (int t = threadIdx.x; t < tiles; t += blockDim.x)
+    atomicAdd(&count[tileKey(cur, prev, valid, cost, t, tilesX, tilesY, width, height)], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {  // bucket starts, highest bucket first
+    uint32_t at = 0;
+    for (int b = kOrderBuckets - 1; b >= 0; b--) {
+      uint32_t n = count[b];
+      count[b] = at;
+      at += n;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < tiles; t += blockDim.x)
+    order[atomicAdd(&count[tileKey(cur, prev, valid, cost, t, tilesX, tilesY, width, height)], 1u)] = (uint32_t)t;
+  __syncthreads();
+  if (threadIdx.x < 64) s->prevC[threadIdx.x] = C[threadIdx.x];
+  if (threadIdx.x == 0) s->valid = 1u;
 }
 
 // Single-specimen primary rays: traceTree in tree-local space (no instancing).
 template <int kWaves, class W>
 __global__ __launch_bounds__(kWaves * 64) void k_traceTreePixels(const float* P, const float4* Nodes4,
                                                                  const uint32_t* Order, const float* C, float4* Hit,
-                                                                 int width, int height, int tilesX, int tiles) {
+                                                                 int width, int height, int tilesX, int tiles,
+                                                                 const uint32_t*, uint32_t*) {
   __shared__ float4 top[W::ldsTop ? 2 * kTopNodes : 1];
   fillTop<W>(top, Nodes4);
   __syncthreads();
@@ -234,7 +309,8 @@ inline int tileGrid(int tiles, int wavesPerBlock) {
   return blocks < cap ? blocks : cap;
 }
 
-typedef void (*TraceKernel)(const float*, const float4*, const uint32_t*, const float*, float4*, int, int, int, int);
+typedef void (*TraceKernel)(const float*, const float4*, const uint32_t*, const float*, float4*, int, int, int, int,
+                            const uint32_t*, uint32_t*);
 struct TraceVariant {
   TraceKernel forest, tree;
   int waves;
@@ -253,7 +329,8 @@ const TraceVariant kTraceVariants[] = {
 constexpr int kTraceVariantCount = sizeof(kTraceVariants) / sizeof(kTraceVariants[0]);
 
 int launchTrace(bool forest, const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
-                int width, int height, hipStream_t stream) {
+                int width, int height, hipStream_t stream, const uint32_t* tileOrder = nullptr,
+                uint32_t* tileCost = nullptr) {
   const Tuning& tu = tuning();
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
   if (forest && tu.trace_variant >= kTraceVariantCount) {
@@ -270,7 +347,7 @@ int launchTrace(bool forest, const float* P, const float* Nodes, const unsigned*
   const TraceVariant& v = kTraceVariants[tu.trace_variant < kTraceVariantCount ? tu.trace_variant : 5];
   hipLaunchKernelGGL(forest ? v.forest : v.tree, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
-                     tx, tiles);
+                     tx, tiles, tileOrder, tileCost);
   return launch_status();
 }
 }  // namespace
@@ -282,6 +359,27 @@ int arbor_tracePixels(const float* P, const float* Nodes, const unsigned* Order,
   ARBOR_REQUIRE(P && Nodes && Order && C && Hit);
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
   return launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream);
+}
+
+long long arbor_tile_schedule_bytes(int width, int height) {
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  long long tiles = (long long)((width + 7) / 8) * ((height + 7) / 8);
+  return (long long)sizeof(TileSchedule) + 8 * tiles;
+}
+
+int arbor_tracePixelsScheduled(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
+                               int width, int height, void* schedule, hipStream_t stream) {
+  ARBOR_REQUIRE(P && Nodes && Order && C && Hit && schedule);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  if (tuning().trace_variant >= kTraceVariantCount)  // lane-scheduled kernels keep their own order
+    return launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream);
+  int tx = (width + 7) / 8, ty = (height + 7) / 8;
+  TileSchedule* s = static_cast<TileSchedule*>(schedule);
+  hipLaunchKernelGGL(k_tileOrder, dim3(1), dim3(kOrderThreads), 0, stream, C, s, tx, ty, width, height);
+  int rc = launch_status();
+  if (rc != ARBOR_OK) return rc;
+  uint32_t* cost = reinterpret_cast<uint32_t*>(s + 1);
+  return launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream, cost + (size_t)tx * ty, cost);
 }
 
 int arbor_traceTreePixels(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,

@@ -37,6 +37,8 @@ SIGNATURES = {
     "arbor_initCamera": [_vp, _vp, _i, _vp],
     "arbor_updateCamera": [_vp, _vp, _vp, _f, _i, _i, _vp],
     "arbor_tracePixels": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "arbor_tile_schedule_bytes": [_i, _i],
+    "arbor_tracePixelsScheduled": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "arbor_shadePixels": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_filterLighting": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_resolvePixels": [_vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -80,7 +82,8 @@ SIGNATURES = {
     "arbor_walk_input": [_i, _i, ct.POINTER(_f)],
 }
 _RESTYPES = {"arbor_version": ct.c_char_p, "arbor_engine_round_size": None, "arbor_engine_buffer": _vp,
-             "arbor_engine_stream": _vp, "arbor_batch_buffer": _vp, "arbor_batch_pixels": ct.c_longlong}
+             "arbor_engine_stream": _vp, "arbor_batch_buffer": _vp, "arbor_batch_pixels": ct.c_longlong,
+             "arbor_tile_schedule_bytes": ct.c_longlong}
 BATCH_TRACE_SHADE, BATCH_FILTER, BATCH_RESOLVE, BATCH_ALL = 1, 2, 4, 7
 
 
@@ -187,6 +190,34 @@ class FrameBuffers:
 
 def trace(scene, C, fb):
     call("arbor_tracePixels", scene.P, scene.Nodes, scene.Order, C, fb.Hit, fb.width, fb.height)
+
+
+class TileSchedule:
+    """Device workspace of arbor_tracePixelsScheduled for one view size."""
+
+    def __init__(self, width, height, device="cuda"):
+        import torch
+
+        n = lib().arbor_tile_schedule_bytes(width, height)
+        check(n if n < 0 else 0, "arbor_tile_schedule_bytes")
+        self.buf = torch.zeros(n, dtype=torch.uint8, device=device)
+        self.width, self.height = width, height
+
+    def order(self):
+        """Host copy of the tile order the last call traced in."""
+        a = self.buf.cpu().numpy()
+        tiles = ((self.width + 7) // 8) * ((self.height + 7) // 8)
+        return a[512 + 4 * tiles: 512 + 8 * tiles].view(np.uint32).copy()
+
+    def cost(self):
+        a = self.buf.cpu().numpy()
+        tiles = ((self.width + 7) // 8) * ((self.height + 7) // 8)
+        return a[512: 512 + 4 * tiles].view(np.uint32).copy()
+
+
+def trace_scheduled(scene, C, fb, schedule):
+    call("arbor_tracePixelsScheduled", scene.P, scene.Nodes, scene.Order, C, fb.Hit, fb.width, fb.height,
+         schedule.buf)
 
 
 def trace_tree(scene, C, fb):

@@ -49,7 +49,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-trace", type=int, default=0, metavar="N",
-                    help="profiling driver: N tracePixels launches of walk frame 0, then a 256 MiB device copy "
+                    help="profiling driver: N scheduled tracePixels launches of walk frame 0 (the engine's trace "
+                         "call: tile-order kernel + trace kernel), then a 256 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE/WRITE_SIZE); prints no JSON")
     ap.add_argument("--views", type=int, default=0, metavar="N",
                     help="batched-views workload instead: N seeded walk views of uneven resolution rendered by one "
@@ -147,8 +148,9 @@ def profile_trace(args):
     lib = px.lib()
     s = lib.arbor_engine_stream(eng.h)
     bufs = [eng.buffer_ptr(n) for n in ("P", "Nodes", "Order", "C", "Hit")]
-    for _ in range(args.profile_trace):
-        px.check(lib.arbor_tracePixels(*bufs, W, H, s), "arbor_tracePixels")
+    sched = eng.buffer_ptr("Schedule")
+    for _ in range(args.profile_trace):  # the engine's own trace call: scheduled, same camera
+        px.check(lib.arbor_tracePixelsScheduled(*bufs, W, H, sched, s), "arbor_tracePixelsScheduled")
     eng.sync()
     a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)

@@ -121,6 +121,18 @@ int arbor_buildBVH(const float* P, unsigned* Keys, unsigned* Order, float* Nodes
 int arbor_generate(float* B, float* P, unsigned* Keys, unsigned* Order, float* Nodes, float* C, int seed,
                    float* History, int width, int height, hipStream_t stream);
 
+/* tracePixels with a tile schedule kept across frames (no counterpart in the
+ * reference; for callers that trace a moving view frame after frame). The
+ * schedule is arbor_tile_schedule_bytes(width, height) bytes of device memory
+ * the caller allocates and zeroes once; each call orders the view's 8x8 tiles
+ * longest-predicted-first (the previous call's per-tile trace times,
+ * reprojected through the camera change), traces them in that order and
+ * records this call's times. Hit is identical to arbor_tracePixels'. One
+ * schedule per view size; concurrent calls must not share it. */
+long long arbor_tile_schedule_bytes(int width, int height);
+int arbor_tracePixelsScheduled(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
+                               int width, int height, void* schedule, hipStream_t stream);
+
 /* Single-specimen primary rays: traceTree in tree-local space with the ground
  * plane as initial best and no forest instancing (the single-tree workload of
  * native_harness.cpp:84 per pixel). Hit layout as tracePixels with tree = -1. */

@@ -17,7 +17,7 @@ namespace orc {
 struct OpCount {
   uint64_t add = 0, mul = 0, div = 0, sqrt = 0, minmax = 0, misc = 0, trans = 0;
 };
-static OpCount g_ops;  // single-threaded use only
+static thread_local OpCount g_ops;  // per thread: counting runs may use OpenMP
 
 struct CF {
   float v;
@@ -106,6 +106,7 @@ int orc_trace_ops_per_pixel(const float* P, const float* Nodes, const unsigned* 
   auto p = wrap(P, (size_t)CAPACITY * PS), n = wrap(Nodes, (size_t)NODE_COUNT * 8), c = wrap(C, 64);
   std::vector<CF> hit((size_t)width * height * 4);
   ACF::leanWalk = true;
+#pragma omp parallel for schedule(dynamic, 2)
   for (int y = 0; y < height; y++)
     for (int x = 0; x < width; x++) {
       g_ops = OpCount();

@@ -143,6 +143,15 @@ def test_tune_cache_roundtrip(L, tmp_path):
     assert L.arbor_tune_set(b"sched_tiles", old) == 0
 
 
+def test_tile_schedule_size_and_checks(L):
+    # 512-byte header (previous camera, valid flag) + cost and order per 8x8 tile
+    assert L.arbor_tile_schedule_bytes(1920, 1080) == 512 + 8 * 240 * 135
+    assert L.arbor_tile_schedule_bytes(33, 9) == 512 + 8 * 5 * 2
+    assert L.arbor_tile_schedule_bytes(0, 10) == -2
+    assert L.arbor_tracePixelsScheduled(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 64, 64, NULL, NULL) == -1
+    assert L.arbor_tracePixelsScheduled(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 0, 64, DUMMY, NULL) == -2
+
+
 def test_camera_pose_matches_init_camera(L, oracle):
     # arbor_camera_pose at initCamera's pose gives initCamera's state with the
     # accumulation reset flag C[10] set.

@@ -132,6 +132,31 @@ def test_trace_variants_identical(px, oracle, scene7):
         assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32))
 
 
+def test_scheduled_trace_identical(px, oracle, scene7):
+    """tracePixelsScheduled over a moving camera (walk frames, then a preset
+    jump) gives Hit bit-identical to tracePixels on every frame; its order is
+    a permutation of the tiles and every tile gets a recorded cost."""
+    print("case: scheduled trace, 6 frames 328x184", flush=True)
+    dev, _ = scene7
+    w, h = 328, 184  # ragged: 41 x 23 tiles
+    sched = px.TileSchedule(w, h)
+    a, b = px.FrameBuffers(w, h), px.FrameBuffers(w, h)
+    C = oracle.init_camera(7)
+    tiles = 41 * 23
+    for f in range(6):
+        I = px.walk_input(7, f)
+        if f == 4:
+            I[5] = 3  # jump to a view preset: no overlap with the previous view
+        oracle.update_camera(C, I, 1.0 / 60.0, w, h)
+        Cd = to_dev(C)
+        px.trace(dev, Cd, a)
+        px.trace_scheduled(dev, Cd, b, sched)
+        assert np.array_equal(to_host(a.Hit).view(np.uint32), to_host(b.Hit).view(np.uint32)), f
+        order = sched.order()
+        assert np.array_equal(np.sort(order), np.arange(tiles, dtype=np.uint32)), f
+        assert (sched.cost() >= 1).all()
+
+
 @pytest.mark.parametrize("view,shadows", [(1, 0), (3, 1), (4, 0)])
 def test_shade_filter_resolve(px, oracle, scene7, view, shadows):
     """Shading on identical hits (the oracle's FP32 Hit uploaded) isolates the
