@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #include "arbor.h"
+#include "arbor_math.hpp"
 
 namespace arbor {
 
@@ -251,40 +252,44 @@ AD v2 woodUV(const float* P, int id, v3 p) {
 }
 AD float windAngle(float seedField, float time, float wind) {
   float phase = (float)(((int)seedField) % 8192) * 0.618f;
-  return wind * (0.11f * sinf(time * 2.1f + phase) + 0.045f * sinf(time * 5.8f + phase * 1.7f));
+  return wind * (0.11f * sinBounded(time * 2.1f + phase) + 0.045f * sinBounded(time * 5.8f + phase * 1.7f));
 }
 AD v3 leafU(const float* P, int id, float time, float wind) {
   int b = id * kPS;
-  float a = windAngle(P[b + 15], time, wind);
-  return ld3(P, b + 4) * cosf(a) + ld3(P, b + 12) * sinf(a);
+  SinCos a = sincosBounded(windAngle(P[b + 15], time, wind));
+  return ld3(P, b + 4) * a.c + ld3(P, b + 12) * a.s;
 }
 AD v3 leafN(const float* P, int id, float time, float wind) {
   int b = id * kPS;
-  float a = windAngle(P[b + 15], time, wind);
-  return ld3(P, b + 12) * cosf(a) - ld3(P, b + 4) * sinf(a);
+  SinCos a = sincosBounded(windAngle(P[b + 15], time, wind));
+  return ld3(P, b + 12) * a.c - ld3(P, b + 4) * a.s;
 }
 AD float leafWidth(float u, float len, float halfWidth, float footprint) {
   float q = sat(u / len);
-  float s = sinf(kPi * q);
-  float lobes = 0.77f + 0.23f * cosf(q * kPi * 10.0f + 0.45f);
-  float fine = bandWeight(0.002f, footprint) * 0.015f * sinf(q * kPi * 160.0f);
+  float s = sinBounded(kPi * q);
+  float lobes = 0.77f + 0.23f * cosBounded(q * kPi * 10.0f + 0.45f);
+  float fine = bandWeight(0.002f, footprint) * 0.015f * sinBounded(q * kPi * 160.0f);
   return halfWidth * powf(fmaxf(0.0f, s), 0.72f) * (lobes + fine);
 }
 AD float leafHeight(float u, float v, float len, float halfWidth, float curl, float twist, float footprint) {
   float q = clampf(u / len, -0.2f, 1.2f);
-  float s = sinf(kPi * q);
+  float s = sinBounded(kPi * q);
   float coarse = curl * s + twist * (q - 0.4f) * v + 0.009f * (v * v / (halfWidth * halfWidth)) * s;
   float mid = 0.0012f * expf(-fabsf(v) / 0.005f) * s * bandWeight(0.007f, footprint);
   return coarse + mid;
 }
-// leafHeight split into its u-only and v-only factors (same expressions).
+// leafHeight split into its u-only and v-only factors (same expressions; the
+// divisions by the leaf's length and squared half-width use their reciprocals
+// rlen = 1/len and rhw2 = 1/hw2 and give the same correctly rounded quotients).
 struct LeafU { float s, tw; };  // sin(PI*q), twist*(q-0.4)
 struct LeafV { float v, w2, m; };  // v, 0.009*(v*v/hw^2), 0.0012*exp(-|v|/0.005)
-AD LeafU leafUTerms(float u, float len, float twist) {
-  float q = clampf(u / len, -0.2f, 1.2f);
-  return {sinf(kPi * q), twist * (q - 0.4f)};
+AD LeafU leafUTerms(float u, float len, float rlen, float twist) {
+  float q = clampf(divKnown(u, len, rlen), -0.2f, 1.2f);
+  return {sinBounded(kPi * q), twist * (q - 0.4f)};
 }
-AD LeafV leafVTerms(float v, float hw2) { return {v, 0.009f * (v * v / hw2), 0.0012f * expf(-fabsf(v) / 0.005f)}; }
+AD LeafV leafVTerms(float v, float hw2, float rhw2) {
+  return {v, 0.009f * divKnown(v * v, hw2, rhw2), 0.0012f * expf(-divKnown(fabsf(v), 0.005f, 1.0f / 0.005f))};
+}
 AD float leafHeightOf(LeafU a, LeafV b, float curl, float band) {
   float coarse = curl * a.s + a.tw * b.v + b.w2 * a.s;
   float mid = b.m * a.s * band;

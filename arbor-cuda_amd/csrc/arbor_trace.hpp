@@ -50,44 +50,88 @@ AD void primBounds(const Prim& p, v3& lo, v3& hi) {
 }
 
 // ---- wood surface (Arbor.cu:148-170) ----
-AD v2 woodUV(const Prim& pr, v3 p) {
-  v3 a = pr.a();
-  v3 d = unit(pr.b() - a);
+// The terms of woodField, barkCut and woodUV that depend on the primitive and
+// the footprint but not on the sample point, each computed by the same
+// operations as in those functions. A march or a gradient evaluates many
+// points of one primitive with one frame.
+struct WoodFrame {
+  v3 a, ab;          // a, b - a
+  float fieldLen2, rField;  // fmaxf(1e-7, dot(ab, ab)) and its reciprocal: woodField's segment parameter
+  float cutLen2, rCut;      // fmaxf(1e-6, dot(ab, ab)): barkCut's
+  float ra, rb, arc0;
+  v3 d, u, v;        // woodUV's axis frame
+  int columns;
+  float rColumns;  // 1 / columns
+  uint32_t seedHash, seedHash31;  // hashU(seed), hashU(seed + 31) of the Voronoi hashes
+  int seed;
+  float fine, grain, footprint;
+};
+AD WoodFrame woodFrame(const Prim& pr, float footprint) {
+  WoodFrame f;
+  f.a = pr.a();
+  f.ab = pr.b() - f.a;
+  float len2 = dot(f.ab, f.ab);
+  f.fieldLen2 = fmaxf(0.0000001f, len2);
+  f.cutLen2 = fmaxf(0.000001f, len2);
+  f.rField = 1.0f / f.fieldLen2;
+  f.rCut = 1.0f / f.cutLen2;
+  f.ra = pr.ra();
+  f.rb = pr.rb();
+  f.arc0 = pr.f16();
+  f.d = unit(f.ab);
   v3 initial = pr.c();
-  v3 u = unit(initial - d * dot(initial, d));
-  v3 v = cross(d, u);
-  v3 q = p - a;
-  float z = dot(q, d);
-  float theta = atan2f(dot(q, v), dot(q, u));
-  return {theta, pr.f16() + z};
+  f.u = unit(initial - f.d * dot(initial, f.d));
+  f.v = cross(f.d, f.u);
+  f.columns = (int)fmaxf(7.0f, floorf(pr.f18() * 2.0f * kPi / 0.065f + 0.5f));
+  f.rColumns = 1.0f / (float)f.columns;
+  f.seed = (int)pr.seed();
+  f.seedHash = hashU((uint32_t)f.seed);
+  f.seedHash31 = hashU((uint32_t)(f.seed + 31));
+  f.fine = bandWeight(0.010f, footprint);
+  f.grain = bandWeight(0.0025f, footprint);
+  f.footprint = footprint;
+  return f;
 }
-__device__ inline float barkCut(const Prim& pr, v3 p, float footprint) {
-  v3 a = pr.a();
-  v3 d = pr.b() - a;
-  float along = sat(dot(p - a, d) / fmaxf(0.000001f, dot(d, d)));
-  float radius = lerp(pr.ra(), pr.rb(), along);
+// hash2(x, y, seed) given hashU(seed)
+AD float hash2h(int x, int y, uint32_t seedHash) {
+  return (float)(hashU((uint32_t)wadd(wmul(x, 1973), wmul(y, 9277)) ^ seedHash) & 16777215u) / 16777216.0f;
+}
+AD v2 woodUV(const WoodFrame& f, v3 p) {
+  v3 q = p - f.a;
+  float z = dot(q, f.d);
+  float theta = atan2f(dot(q, f.v), dot(q, f.u));
+  return {theta, f.arc0 + z};
+}
+__device__ inline float barkCut(const WoodFrame& f, v3 p) {
+  float along = sat(divKnown(dot(p - f.a, f.ab), f.cutLen2, f.rCut));
+  float radius = lerp(f.ra, f.rb, along);
   float amp = fminf(0.021f, radius * 0.055f);
-  int columns = (int)fmaxf(7.0f, floorf(pr.f18() * 2.0f * kPi / 0.065f + 0.5f));
-  float period = radius * 2.0f * kPi / (float)columns;
-  float coarse = bandWeight(period, footprint);
-  float fine = bandWeight(0.010f, footprint);
-  float grain = bandWeight(0.0025f, footprint);
-  if (coarse < 0.000001f && fine < 0.000001f) return amp * 0.32f;
-  v2 uv = woodUV(pr, p);
+  const int columns = f.columns;
+  float period = divKnown(radius * 2.0f * kPi, (float)columns, f.rColumns);
+  float coarse = bandWeight(period, f.footprint);
+  if (coarse < 0.000001f && f.fine < 0.000001f) return amp * 0.32f;
+  v2 uv = woodUV(f, p);
   float theta = uv.x, arc = uv.y;
-  int seed = (int)pr.seed();
-  float st = sinf(theta), ct = cosf(theta);
-  float x = (theta / (2.0f * kPi) + 0.5f) * (float)columns + 0.32f * sinf(arc * 6.7f + ct * 3.0f) +
-            0.15f * sinf(arc * 19.0f + st * 2.0f);
-  float y = arc / 0.19f + 0.25f * sinf(theta * 7.0f) + 0.1f * sinf(theta * 19.0f);
+  const SinCos tc = sincosBounded(theta);
+  float st = tc.s, ct = tc.c;
+  float x = (divKnown(theta, 2.0f * kPi, 1.0f / (2.0f * kPi)) + 0.5f) * (float)columns +
+            0.32f * sinBounded(arc * 6.7f + ct * 3.0f) + 0.15f * sinBounded(arc * 19.0f + st * 2.0f);
+  float y = divKnown(arc, 0.19f, 1.0f / 0.19f) + 0.25f * sinBounded(theta * 7.0f) + 0.1f * sinBounded(theta * 19.0f);
   int ix = (int)floorf(x), iy = (int)floorf(y);
+  // the three columns ix-1, ix, ix+1 wrapped into [0, columns)
+  int wrap[3];
+  wrap[0] = (((ix - 1) % columns) + columns) % columns;
+  wrap[1] = wrap[0] + 1 == columns ? 0 : wrap[0] + 1;
+  wrap[2] = wrap[1] + 1 == columns ? 0 : wrap[1] + 1;
   float first = 100.0f, second = 100.0f;
+#pragma unroll
   for (int j = -1; j <= 1; j++)
+#pragma unroll
     for (int k = -1; k <= 1; k++) {
       int cell = ix + k;
-      int wrapped = ((cell % columns) + columns) % columns;
-      float xx = (float)cell + 0.18f + 0.64f * hash2(wrapped, iy + j, seed) - x;
-      float yy = (float)(iy + j) + 0.18f + 0.64f * hash2(wrapped + columns, iy + j, seed + 31) - y;
+      int wrapped = wrap[k + 1];
+      float xx = (float)cell + 0.18f + 0.64f * hash2h(wrapped, iy + j, f.seedHash) - x;
+      float yy = (float)(iy + j) + 0.18f + 0.64f * hash2h(wrapped + columns, iy + j, f.seedHash31) - y;
       float dd = xx * xx + yy * yy;
       if (dd < first) { second = first; first = dd; }
       else if (dd < second) second = dd;
@@ -96,19 +140,19 @@ __device__ inline float barkCut(const Prim& pr, v3 p, float footprint) {
   float fissure = 1.0f - smooth(0.015f, 0.17f, gap);
   float plates = fissure * fissure;
   v3 coord = mk(ct * 9.0f, st * 9.0f, arc * 23.0f);
-  float splinter = noise3(coord, seed);
-  float pores = noise3(coord * 4.0f, seed + 731);
-  return amp * clampf(0.32f + coarse * (plates - 0.37f) * 0.69f + fine * (splinter - 0.5f) * 0.24f +
-                          grain * (pores - 0.5f) * 0.08f,
+  float splinter = noise3(coord, f.seed);
+  float pores = noise3(coord * 4.0f, f.seed + 731);
+  return amp * clampf(0.32f + coarse * (plates - 0.37f) * 0.69f + f.fine * (splinter - 0.5f) * 0.24f +
+                          f.grain * (pores - 0.5f) * 0.08f,
                       0.005f, 1.0f);
 }
-__device__ inline float woodField(const Prim& pr, v3 p, float footprint) {
-  v3 a = pr.a();
-  v3 ab = pr.b() - a;
-  float t = sat(dot(p - a, ab) / fmaxf(0.0000001f, dot(ab, ab)));
-  float radius = lerp(pr.ra(), pr.rb(), t);
-  return length(p - a - ab * t) - radius + barkCut(pr, p, footprint);
+__device__ inline float woodField(const WoodFrame& f, v3 p) {
+  float t = sat(divKnown(dot(p - f.a, f.ab), f.fieldLen2, f.rField));
+  float radius = lerp(f.ra, f.rb, t);
+  return length(p - f.a - f.ab * t) - radius + barkCut(f, p);
 }
+AD v2 woodUV(const Prim& pr, v3 p) { return woodUV(woodFrame(pr, 1.0f), p); }
+__device__ inline float barkCut(const Prim& pr, v3 p, float footprint) { return barkCut(woodFrame(pr, footprint), p); }
 
 // ---- leaf frame (Arbor.cu:171-172) ----
 struct LeafFrame { v3 u, n; };
@@ -117,8 +161,8 @@ AD LeafFrame leafFrame(const Prim& pr, float time, float wind) {
   // Breeze off: the rotation angle is wind*(...) = 0, cos = 1 and sin = 0, so
   // the rotated basis equals the stored one.
   if (wind == 0.0f) { f.u = pr.b(); f.n = pr.n(); return f; }
-  float a = windAngle(pr.seed(), time, wind);
-  float ca = cosf(a), sa = sinf(a);
+  const SinCos w = sincosBounded(windAngle(pr.seed(), time, wind));
+  float ca = w.c, sa = w.s;
   f.u = pr.b() * ca + pr.n() * sa;
   f.n = pr.n() * ca - pr.b() * sa;
   return f;
@@ -169,7 +213,7 @@ __device__ inline float tubeT(v3 ro, v3 rd, v3 a, v3 b, float ra, float rb) {
   }
   return best;
 }
-__device__ inline float woodT(const Prim& pr, v3 ro, v3 rd, float footprint, bool refine, bool cull) {
+__device__ inline float woodT(const Prim& pr, v3 ro, v3 rd, float footprint, bool refine, bool cull, float best) {
   v3 a = pr.a(), end = pr.b();
   // Every surface woodT can return lies inside the radius-max(ra,rb) capsule
   // (bark only cuts inward); a ray missing its bounding sphere returns FAR.
@@ -185,19 +229,65 @@ __device__ inline float woodT(const Prim& pr, v3 ro, v3 rd, float footprint, boo
   }
   float t = tubeT(ro, rd, a, end, pr.ra(), pr.rb());
   if (t >= kFar) return kFar;
+  // The march only moves t forward from the tube hit: from t >= best it can
+  // only return a value the caller discards (it keeps strictly nearer hits).
+  if (t >= best) return kFar;
   v3 lo, hi;
   primBounds(pr, lo, hi);
   v2 span = boxSpan(ro, rd, lo, hi);
   float eps = fmaxf(0.000025f, fminf(0.00025f, footprint * 0.13f));
+  const WoodFrame wf = woodFrame(pr, footprint);
   for (int k = 0; k < 64; k++) {
-    float f = woodField(pr, ro + rd * t, footprint);
+    float f = woodField(wf, ro + rd * t);
     if (f < eps) return t;
     t += fmaxf(eps * 0.35f, f * 0.27f);
     if (t > span.y) return kFar;
   }
   return kFar;
 }
-__device__ inline float leafT(const Prim& pr, v3 ro, v3 rd, float footprint, float time, float wind, bool cull) {
+// The leaf's Newton solve and acceptance test (Arbor.cu:205-214) in its local
+// frame (x along u from the petiole end, y along v, z along n; d* the ray
+// direction there), from the clamped start t inside the slab span: seven
+// steps of central-difference Newton on z + dz*t = leafHeight(x + dx*t,
+// y + dy*t); returns t when the end point lies on the blade within 3e-4, else
+// FAR. leafHeight(u, v) = curl*s(u) + tw(u)*v + w2(v)*s(u) + m(v)*s(u)*band
+// with s, tw functions of u only and w2, m of v only, so the five evaluations
+// of a step share 3 sines and 3 exponentials; each term is computed as
+// leafHeight computes it, and the divisions by len, width^2, 0.005 and 2e by
+// their reciprocals (same correctly rounded quotients).
+struct LeafLocal {
+  float x, y, z, dx, dy, dz, len, width, curl, twist, footprint;
+};
+__device__ inline float leafNewton(const LeafLocal& L, float t, float spanLo, float spanHi) {
+  const float band = bandWeight(0.007f, L.footprint);
+  const float hw2 = L.width * L.width;
+  const float rlen = 1.0f / L.len, rhw2 = 1.0f / hw2;
+  const float e = 0.00015f;
+  const float twoE = 2.0f * e, rTwoE = 1.0f / (2.0f * e);
+  for (int k = 0; k < 7; k++) {
+    float xx = L.x + L.dx * t, yy = L.y + L.dy * t;
+    LeafU u0 = leafUTerms(xx, L.len, rlen, L.twist), up = leafUTerms(xx + e, L.len, rlen, L.twist),
+          um = leafUTerms(xx - e, L.len, rlen, L.twist);
+    LeafV v0 = leafVTerms(yy, hw2, rhw2), vp = leafVTerms(yy + e, hw2, rhw2), vm = leafVTerms(yy - e, hw2, rhw2);
+    float height = leafHeightOf(u0, v0, L.curl, band);
+    float f = L.z + L.dz * t - height;
+    float hx = divKnown(leafHeightOf(up, v0, L.curl, band) - leafHeightOf(um, v0, L.curl, band), twoE, rTwoE);
+    float hy = divKnown(leafHeightOf(u0, vp, L.curl, band) - leafHeightOf(u0, vm, L.curl, band), twoE, rTwoE);
+    float deriv = L.dz - hx * L.dx - hy * L.dy;
+    if (fabsf(deriv) < 0.00001f) break;
+    t -= clampf(f / deriv, -0.13f, 0.13f);
+    t = clampf(t, spanLo, spanHi);
+  }
+  float xx = L.x + L.dx * t, yy = L.y + L.dy * t;
+  float err = fabsf(L.z + L.dz * t -
+                    leafHeightOf(leafUTerms(xx, L.len, rlen, L.twist), leafVTerms(yy, hw2, rhw2), L.curl, band));
+  // same predicate as the reference, cheapest terms first
+  if (t > 0.0001f && xx > 0.0f && xx < L.len && err < 0.0003f && fabsf(yy) < leafWidth(xx, L.len, L.width, L.footprint))
+    return t;
+  return kFar;
+}
+__device__ inline float leafT(const Prim& pr, v3 ro, v3 rd, float footprint, float time, float wind, bool cull,
+                             float bestSoFar) {
   v3 origin = pr.a();
   LeafFrame fr = leafFrame(pr, time, wind);
   v3 u = fr.u, v = pr.c(), n = fr.n;
@@ -221,41 +311,24 @@ __device__ inline float leafT(const Prim& pr, v3 ro, v3 rd, float footprint, flo
   if (!slab) return best;
   v2 span = boxSpan(mk(x, y, z), mk(dx, dy, dz), mk(-0.001f, -width, -0.035f), mk(len + 0.001f, width, 0.065f));
   if (span.y < fmaxf(0.0001f, span.x)) return best;
+  // The Newton solution is clamped to [span.x, span.y]: from span.x >= the
+  // caller's best it cannot be kept, and min(petiole, it) is decided by the
+  // petiole alone.
+  if (span.x >= bestSoFar) return best;
   float t = fabsf(dz) > 0.000001f ? -z / dz : (span.x + span.y) * 0.5f;
   t = clampf(t, fmaxf(0.0001f, span.x), span.y);
-  // leafHeight(u, v) = curl*s(u) + tw(u)*v + w2(v)*s(u) + m(v)*s(u)*band with
-  // s, tw functions of u only and w2, m of v only; the five evaluations of a
-  // Newton step share them (3 sines and 3 exponentials instead of 5 + 5),
-  // each term still computed exactly as leafHeight computes it.
-  const float band = bandWeight(0.007f, footprint);
-  const float hw2 = width * width;
-  for (int k = 0; k < 7; k++) {
-    float xx = x + dx * t, yy = y + dy * t;
-    const float e = 0.00015f;
-    LeafU u0 = leafUTerms(xx, len, twist), up = leafUTerms(xx + e, len, twist), um = leafUTerms(xx - e, len, twist);
-    LeafV v0 = leafVTerms(yy, hw2), vp = leafVTerms(yy + e, hw2), vm = leafVTerms(yy - e, hw2);
-    float height = leafHeightOf(u0, v0, curl, band);
-    float f = z + dz * t - height;
-    float hx = (leafHeightOf(up, v0, curl, band) - leafHeightOf(um, v0, curl, band)) / (2.0f * e);
-    float hy = (leafHeightOf(u0, vp, curl, band) - leafHeightOf(u0, vm, curl, band)) / (2.0f * e);
-    float deriv = dz - hx * dx - hy * dy;
-    if (fabsf(deriv) < 0.00001f) break;
-    t -= clampf(f / deriv, -0.13f, 0.13f);
-    t = clampf(t, span.x, span.y);
-  }
-  float xx = x + dx * t, yy = y + dy * t;
-  float err = fabsf(z + dz * t - leafHeightOf(leafUTerms(xx, len, twist), leafVTerms(yy, hw2), curl, band));
-  // same predicate as the reference, cheapest terms first
-  if (t > 0.0001f && xx > 0.0f && xx < len && err < 0.0003f && fabsf(yy) < leafWidth(xx, len, width, footprint))
-    best = fminf(best, t);
-  return best;
+  LeafLocal L{x, y, z, dx, dy, dz, len, width, curl, twist, footprint};
+  return fminf(best, leafNewton(L, t, span.x, span.y));
 }
+// `best` is the caller's current nearest hit: a result >= best is discarded
+// by every caller, so a test may give up (returning FAR) once its answer is
+// known to be >= best; any result < best is exact.
 __device__ inline float objectT(const Prim& pr, v3 ro, v3 rd, float footprint, float time, float wind, bool refine,
-                               bool cull) {
+                               bool cull, float best = kFar) {
   int type = (int)pr.type();
   if (type < 0) return kFar;
-  if (type == 0) return woodT(pr, ro, rd, footprint, refine, cull);
-  if (type == 1) return leafT(pr, ro, rd, footprint, time, wind, cull);
+  if (type == 0) return woodT(pr, ro, rd, footprint, refine, cull, best);
+  if (type == 1) return leafT(pr, ro, rd, footprint, time, wind, cull, best);
   v3 h = pr.b();
   v3 q = ro - pr.a();
   v3 p = mk(q.x / h.x, q.y / h.y, q.z / h.z);
@@ -352,7 +425,7 @@ __device__ inline v2 traceTree(const float* P, const float4* Nodes4, const float
         if (id != ignore) {
           float footprint = fmaxf(0.000005f, cone * fmaxf(0.05f, span.x));
           Prim pr = loadPrim(P, id);
-          float t = objectT(pr, ro, rd, footprint, time, wind, !shadows, W::cull);
+          float t = objectT(pr, ro, rd, footprint, time, wind, !shadows, W::cull, best);
           if (t < best) { best = t; found = id; }
         }
       }
@@ -417,9 +490,9 @@ __device__ inline v3 objectNormal(const Prim& pr, v3 p, float footprint, float t
   if (type == 0) {
     float e = fmaxf(0.000035f, fminf(0.0005f, footprint * 0.18f));
     v3 dx = mk(e, 0.0f, 0.0f), dy = mk(0.0f, e, 0.0f), dz = mk(0.0f, 0.0f, e);
-    return unit(mk(woodField(pr, p + dx, footprint) - woodField(pr, p - dx, footprint),
-                   woodField(pr, p + dy, footprint) - woodField(pr, p - dy, footprint),
-                   woodField(pr, p + dz, footprint) - woodField(pr, p - dz, footprint)));
+    const WoodFrame wf = woodFrame(pr, footprint);
+    return unit(mk(woodField(wf, p + dx) - woodField(wf, p - dx), woodField(wf, p + dy) - woodField(wf, p - dy),
+                   woodField(wf, p + dz) - woodField(wf, p - dz)));
   }
   if (type == 1) {
     LeafFrame fr = leafFrame(pr, time, wind);

@@ -235,27 +235,8 @@ AD void laneNewton(LaneRay& s, const float* P) {
   float len = r[7], width = r[11], curl = r[17], twist = r[18];
   const float x = s.nx, y = s.ny, z = s.nz, dx = s.ndx, dy = s.ndy, dz = s.ndz;
   float t = s.nt;
-  const float band = bandWeight(0.007f, s.fp);
-  const float hw2 = width * width;
-  for (int k = 0; k < 7; k++) {
-    float xx = x + dx * t, yy = y + dy * t;
-    const float e = 0.00015f;
-    LeafU u0 = leafUTerms(xx, len, twist), up = leafUTerms(xx + e, len, twist), um = leafUTerms(xx - e, len, twist);
-    LeafV v0 = leafVTerms(yy, hw2), vp = leafVTerms(yy + e, hw2), vm = leafVTerms(yy - e, hw2);
-    float height = leafHeightOf(u0, v0, curl, band);
-    float f = z + dz * t - height;
-    float hx = (leafHeightOf(up, v0, curl, band) - leafHeightOf(um, v0, curl, band)) / (2.0f * e);
-    float hy = (leafHeightOf(u0, vp, curl, band) - leafHeightOf(u0, vm, curl, band)) / (2.0f * e);
-    float deriv = dz - hx * dx - hy * dy;
-    if (fabsf(deriv) < 0.00001f) break;
-    t -= clampf(f / deriv, -0.13f, 0.13f);
-    t = clampf(t, s.spanLo, s.spanHi);
-  }
-  float xx = x + dx * t, yy = y + dy * t;
-  float err = fabsf(z + dz * t - leafHeightOf(leafUTerms(xx, len, twist), leafVTerms(yy, hw2), curl, band));
-  float best = s.pbest;
-  if (t > 0.0001f && xx > 0.0f && xx < len && err < 0.0003f && fabsf(yy) < leafWidth(xx, len, width, s.fp))
-    best = fminf(best, t);
+  LeafLocal L{x, y, z, dx, dy, dz, len, width, curl, twist, s.fp};
+  float best = fminf(s.pbest, leafNewton(L, t, s.spanLo, s.spanHi));
   laneObjectResult(s, best);
 }
 

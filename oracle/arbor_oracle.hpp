@@ -636,7 +636,13 @@ template <class R> struct Arbor {
     return dot(p, p) > r * r || b > r;
   }
 
-  static R woodT(const R* P, int id, F3 ro, F3 rd, R footprint, bool refine) {
+  // When set, woodT and leafT stop once their answer is known to be >= the
+  // caller's current best (a result the caller discards: it keeps strictly
+  // nearer hits): the wood march never moves before the tube hit, the leaf
+  // Newton solution never before the slab entry. Results must be
+  // bit-identical with the flag off (checked on whole frames).
+  static inline bool skipBeyondBest = false;
+  static R woodT(const R* P, int id, F3 ro, F3 rd, R footprint, bool refine, R bestSoFar = FAR()) {
     int b = id * PS;
     F3 a = ld3(P, b), end = ld3(P, b + 4);
     if (cullSpheres) {
@@ -651,6 +657,7 @@ template <class R> struct Arbor {
     }
     R t = tubeT(ro, rd, a, end, P[b + 7], P[b + 11]);
     if (t >= FAR()) return FAR();
+    if (skipBeyondBest && t >= bestSoFar) return FAR();
     F2 span = boxSpan(ro, rd, boundLow(P, id), boundHigh(P, id));
     R eps = m_max(K(0.000025), m_min(K(0.00025), footprint * K(0.13)));
     for (int k = 0; k < 64; k++) {
@@ -661,7 +668,7 @@ template <class R> struct Arbor {
     }
     return FAR();
   }
-  static R leafT(const R* P, int id, F3 ro, F3 rd, R footprint, R time, R wind) {
+  static R leafT(const R* P, int id, F3 ro, F3 rd, R footprint, R time, R wind, R bestSoFar = FAR()) {
     int b = id * PS;
     F3 origin = ld3(P, b);
     F3 u = leafU(P, id, time, wind);
@@ -685,6 +692,7 @@ template <class R> struct Arbor {
     if (!slab) return best;
     F2 span = boxSpan(c3(x, y, z), c3(dx, dy, dz), c3(K(-0.001), -width, K(-0.035)), c3(ln + K(0.001), width, K(0.065)));
     if (span.y < m_max(K(0.0001), span.x)) return best;
+    if (skipBeyondBest && span.x >= bestSoFar) return best;
     R t = m_abs(dz) > K(0.000001) ? -z / dz : (span.x + span.y) * K(0.5);
     t = clampf(t, m_max(K(0.0001), span.x), span.y);
     const R curl = P[b + 17], twist = P[b + 18];
@@ -708,11 +716,11 @@ template <class R> struct Arbor {
       best = m_min(best, t);
     return best;
   }
-  static R objectT(const R* P, int id, F3 ro, F3 rd, R footprint, R time, R wind, bool refine) {
+  static R objectT(const R* P, int id, F3 ro, F3 rd, R footprint, R time, R wind, bool refine, R best = FAR()) {
     int b = id * PS, type = (int)P[b + 3];
     if (type < 0) return FAR();
-    if (type == 0) return woodT(P, id, ro, rd, footprint, refine);
-    if (type == 1) return leafT(P, id, ro, rd, footprint, time, wind);
+    if (type == 0) return woodT(P, id, ro, rd, footprint, refine, best);
+    if (type == 1) return leafT(P, id, ro, rd, footprint, time, wind, best);
     F3 h = ld3(P, b + 4);
     F3 q = ro - ld3(P, b);
     F3 p = c3(q.x / h.x, q.y / h.y, q.z / h.z);
@@ -769,7 +777,7 @@ template <class R> struct Arbor {
           int id = (int)Order[node - CAPACITY];
           if (id != ignore) {
             R footprint = m_max(K(0.000005), cone * m_max(K(0.05), span.x));
-            R t = objectT(P, id, ro, rd, footprint, time, wind, !shadows);
+            R t = objectT(P, id, ro, rd, footprint, time, wind, !shadows, best);
             if (t < best) { best = t; found = id; }
           }
         }
@@ -812,7 +820,7 @@ template <class R> struct Arbor {
         int id = (int)Order[node - CAPACITY];
         if (id != ignore) {
           R footprint = m_max(K(0.000005), cone * m_max(K(0.05), span.x));
-          R t = objectT(P, id, ro, rd, footprint, time, wind, !shadows);
+          R t = objectT(P, id, ro, rd, footprint, time, wind, !shadows, best);
           if (t < best) { best = t; found = id; }
         }
       }

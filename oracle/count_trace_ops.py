@@ -50,6 +50,7 @@ def main():
     o = Oracle()
     lib = o.lib
     lib.orc_set_cull.argtypes = [ct.c_int]
+    lib.orc_set_skip.argtypes = [ct.c_int]
     W, H, seed = 1920, 1080, 7
     s = o.generate(seed)
     C = o.init_camera(seed)
@@ -63,8 +64,10 @@ def main():
         if f % args.frame_step:
             continue
         lib.orc_set_cull(1)
+        lib.orc_set_skip(1)
         r = o.count_trace_ops(s, C, W, H, args.stride)
         lib.orc_set_cull(0)
+        lib.orc_set_skip(0)
         per_frame.append(r["ops_per_ray"])
         kinds = r if kinds is None else {k: kinds[k] + r[k] for k in r}
         print("frame %3d: %.1f ops/ray" % (f, r["ops_per_ray"]), flush=True)

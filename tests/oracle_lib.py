@@ -175,6 +175,8 @@ class Oracle(_Common):
                                 [_f32p, _f32p, _u32p, _f32p, _i, _i, _i, _i,
                                  np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")])
         self._lean_mismatch = _bind(L, "orc_lean_walk_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i, _i])
+        self._cull_mismatch = _bind(L, "orc_cull_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
+        self._skip_mismatch = _bind(L, "orc_skip_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
         self._generate_cr = _bind(L, "orc_generate_cr", _i, [_i, _f32p, _f32p, _u32p, _u32p, _f32p])
 
     def generate_cr(self, seed):
@@ -194,6 +196,16 @@ class Oracle(_Common):
         kinds = ("add", "mul", "div", "sqrt", "minmax", "absfloor", "libm")
         return {"rays": int(rays), "ops_per_ray": out[1] / rays,
                 **{k: out[2 + i] / rays for i, k in enumerate(kinds)}}
+
+    def cull_mismatches(self, s, C, w, h):
+        """Pixels whose Hit or Linear changes when woodT/leafT first reject
+        rays missing their padded bounding spheres (the GPU's culls)."""
+        return self._cull_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h)
+
+    def skip_mismatches(self, s, C, w, h):
+        """Pixels whose Hit or Linear changes when woodT/leafT stop once their
+        answer is known to be >= the caller's best (the GPU's early exits)."""
+        return self._skip_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h)
 
     def lean_walk_mismatches(self, s, C, w, h, single_tree=False):
         return self._lean_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h,

@@ -265,6 +265,48 @@ def test_lean_walk_is_the_reference_walk(oracle, scenes, view, single):
     assert oracle.lean_walk_mismatches(scenes["7"], C, 200, 120, single) == 0
 
 
+def _walk_camera(oracle, frame, w, h, shadows):
+    import pyarbor
+
+    C = oracle.init_camera(7)
+    for f in range(frame + 1):
+        I = pyarbor.walk_input(7, f)
+        oracle.update_camera(C, I, 1.0 / 60, w, h)
+    C[22] = shadows
+    return C
+
+
+@pytest.mark.parametrize("view,wind", [(1, 0.0), (3, 0.6), (5, 0.0)])
+def test_sphere_culls_change_nothing(oracle, scenes, view, wind):
+    """The GPU's bounding-sphere rejections in woodT/leafT (padded for the
+    reference's discriminant cancellation at distance) leave every Hit and
+    Linear bit unchanged: trace, and shade with sun visibility on (shadow rays
+    take the non-refined wood path)."""
+    C = oracle.init_camera(7)
+    I = np.zeros(32, np.float32)
+    I[5] = view
+    oracle.update_camera(C, I, 1.0 / 60, 200, 120)
+    C[22], C[11], C[8] = 1, wind, 0.75
+    assert oracle.cull_mismatches(scenes["7"], C, 200, 120) == 0
+
+
+@pytest.mark.parametrize("frame,view", [(0, 0), (40, 0), (0, 2), (0, 4)])
+def test_early_exits_change_nothing(oracle, scenes, frame, view):
+    """woodT/leafT stopping once their answer is known to be >= the caller's
+    best (march from a tube hit at or behind best; Newton slab entered at or
+    behind best) leave every Hit and Linear bit unchanged: trace, and shade
+    with sun visibility on."""
+    if view:
+        C = oracle.init_camera(7)
+        I = np.zeros(32, np.float32)
+        I[5] = view
+        oracle.update_camera(C, I, 1.0 / 60, 200, 120)
+        C[22] = 1
+    else:
+        C = _walk_camera(oracle, frame, 200, 120, 1)
+    assert oracle.skip_mismatches(scenes["7"], C, 200, 120) == 0
+
+
 def test_operation_count_is_stable(oracle, scenes):
     """The op counter is deterministic and reports the workload's mix."""
     C = oracle.init_camera(7)
