@@ -67,22 +67,27 @@ __global__ __launch_bounds__(kWaves * 64) void k_tracePixels(const float* P, con
   }
 }
 
-// Tile schedule (longest predicted first) for the next tracePixels launch,
-// one block of kOrderThreads. A tile's predicted cost is the largest recorded
-// cost in the 3x3 tiles around where its centre ray fell in the previous
-// frame (the direction projected through the previous camera; eye motion
-// between frames is ignored); tiles with no prediction (first frame, or
-// directions outside the previous view) go first. Tiles are bucketed by
-// log2(cost) with two fraction bits and laid out bucket by bucket, highest
-// first. Then the current camera is kept for the next frame.
-constexpr int kOrderThreads = 1024;
+// Tile schedule (longest predicted first) for the next tracePixels launch. A
+// tile's predicted cost is the largest recorded cost in the 3x3 tiles around
+// where its centre ray fell in the previous frame (the direction projected
+// through the previous camera; eye motion between frames is ignored); tiles
+// with no prediction (first frame, or directions outside the previous view)
+// go first. Tiles are bucketed by log2(cost) with two fraction bits and laid
+// out bucket by bucket, highest first, by three grid-wide passes: keys and a
+// bucket histogram, bucket starts, scatter. The last pass keeps the current
+// camera for the next frame and leaves the histogram zeroed.
 constexpr int kOrderBuckets = 128;
+constexpr int kOrderBlock = 256;
 struct TileSchedule {
   float prevC[64];
   uint32_t valid;
   uint32_t pad[63];
-  // followed by cost[tiles], order[tiles]
+  uint32_t hist[kOrderBuckets];    // zero between calls
+  uint32_t starts[kOrderBuckets];  // next free slot of each bucket during the scatter
+  uint32_t pad2[128];
+  // followed by cost[tiles], order[tiles], key[tiles]
 };
+static_assert(sizeof(TileSchedule) == 2048, "schedule header");
 __device__ inline int tileKey(const View& cur, const View& prev, bool valid, const uint32_t* cost, int t, int tilesX,
                               int tilesY, int width, int height) {
   if (!valid) return kOrderBuckets - 1;
@@ -101,33 +106,63 @@ __device__ inline int tileKey(const View& cur, const View& prev, bool valid, con
   int sub = k >= 2 ? (int)((c >> (k - 2)) & 3u) : 0;
   return min(kOrderBuckets - 2, 4 * k + sub);
 }
-__global__ __launch_bounds__(kOrderThreads) void k_tileOrder(const float* C, TileSchedule* s, int tilesX, int tilesY,
-                                                             int width, int height) {
+__global__ __launch_bounds__(kOrderBlock) void k_tileKeys(const float* C, TileSchedule* s, int tilesX, int tilesY,
+                                                          int width, int height) {
   __shared__ uint32_t count[kOrderBuckets];
-  uint32_t* cost = reinterpret_cast<uint32_t*>(s + 1);
   const int tiles = tilesX * tilesY;
-  uint32_t* order = cost + tiles;
-  const View cur = makeView(C), prev = makeView(s->prevC);
-  const bool valid = s->valid == 1u;
+  const uint32_t* cost = reinterpret_cast<const uint32_t*>(s + 1);
+  uint32_t* key = const_cast<uint32_t*>(cost) + 2 * (size_t)tiles;
   for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) count[b] = 0;
   __syncthreads();
-  for (int t = threadIdx.x; t < tiles; t += blockDim.x)
-    atomicAdd(&count[tileKey(cur, prev, valid, cost, t, tilesX, tilesY, width, height)], 1u);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < tiles) {
+    const View cur = makeView(C), prev = makeView(s->prevC);
+    int k = tileKey(cur, prev, s->valid == 1u, cost, t, tilesX, tilesY, width, height);
+    key[t] = (uint32_t)k;
+    atomicAdd(&count[k], 1u);
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {  // bucket starts, highest bucket first
+  for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x)
+    if (count[b]) atomicAdd(&s->hist[b], count[b]);
+}
+__global__ __launch_bounds__(kOrderBuckets) void k_tileStarts(TileSchedule* s) {
+  __shared__ uint32_t h[kOrderBuckets];
+  h[threadIdx.x] = s->hist[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x == 0) {  // highest bucket first
     uint32_t at = 0;
     for (int b = kOrderBuckets - 1; b >= 0; b--) {
-      uint32_t n = count[b];
-      count[b] = at;
+      uint32_t n = h[b];
+      h[b] = at;
       at += n;
     }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < tiles; t += blockDim.x)
-    order[atomicAdd(&count[tileKey(cur, prev, valid, cost, t, tilesX, tilesY, width, height)], 1u)] = (uint32_t)t;
+  s->starts[threadIdx.x] = h[threadIdx.x];
+  s->hist[threadIdx.x] = 0u;
+}
+__global__ __launch_bounds__(kOrderBlock) void k_tileScatter(const float* C, TileSchedule* s, int tiles) {
+  __shared__ uint32_t count[kOrderBuckets], base[kOrderBuckets];
+  const uint32_t* key = reinterpret_cast<const uint32_t*>(s + 1) + 2 * (size_t)tiles;
+  uint32_t* order = reinterpret_cast<uint32_t*>(s + 1) + tiles;
+  for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) count[b] = 0;
   __syncthreads();
-  if (threadIdx.x < 64) s->prevC[threadIdx.x] = C[threadIdx.x];
-  if (threadIdx.x == 0) s->valid = 1u;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t k = 0, local = 0;
+  if (t < tiles) {
+    k = key[t];
+    local = atomicAdd(&count[k], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x)
+    base[b] = count[b] ? atomicAdd(&s->starts[b], count[b]) : 0u;
+  __syncthreads();
+  const uint32_t at = base[k] + local;
+  if (t < tiles && at < (uint32_t)tiles) order[at] = (uint32_t)t;  // always true unless an earlier pass failed
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < 64) s->prevC[threadIdx.x] = C[threadIdx.x];
+    if (threadIdx.x == 0) s->valid = 1u;
+  }
 }
 
 // Single-specimen primary rays: traceTree in tree-local space (no instancing).
@@ -364,7 +399,7 @@ int arbor_tracePixels(const float* P, const float* Nodes, const unsigned* Order,
 long long arbor_tile_schedule_bytes(int width, int height) {
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
   long long tiles = (long long)((width + 7) / 8) * ((height + 7) / 8);
-  return (long long)sizeof(TileSchedule) + 8 * tiles;
+  return (long long)sizeof(TileSchedule) + 12 * tiles;
 }
 
 int arbor_tracePixelsScheduled(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
@@ -375,7 +410,10 @@ int arbor_tracePixelsScheduled(const float* P, const float* Nodes, const unsigne
     return launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream);
   int tx = (width + 7) / 8, ty = (height + 7) / 8;
   TileSchedule* s = static_cast<TileSchedule*>(schedule);
-  hipLaunchKernelGGL(k_tileOrder, dim3(1), dim3(kOrderThreads), 0, stream, C, s, tx, ty, width, height);
+  const int tiles = tx * ty, blocks = (tiles + kOrderBlock - 1) / kOrderBlock;
+  hipLaunchKernelGGL(k_tileKeys, dim3(blocks), dim3(kOrderBlock), 0, stream, C, s, tx, ty, width, height);
+  hipLaunchKernelGGL(k_tileStarts, dim3(1), dim3(kOrderBuckets), 0, stream, s);
+  hipLaunchKernelGGL(k_tileScatter, dim3(blocks), dim3(kOrderBlock), 0, stream, C, s, tiles);
   int rc = launch_status();
   if (rc != ARBOR_OK) return rc;
   uint32_t* cost = reinterpret_cast<uint32_t*>(s + 1);

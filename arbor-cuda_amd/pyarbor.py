@@ -193,7 +193,11 @@ def trace(scene, C, fb):
 
 
 class TileSchedule:
-    """Device workspace of arbor_tracePixelsScheduled for one view size."""
+    """Device workspace of arbor_tracePixelsScheduled for one view size:
+    a 2048-byte header, then per 8x8 tile its recorded cost, its place in the
+    order and its bucket key (u32 each)."""
+
+    HEADER = 2048
 
     def __init__(self, width, height, device="cuda"):
         import torch
@@ -207,12 +211,12 @@ class TileSchedule:
         """Host copy of the tile order the last call traced in."""
         a = self.buf.cpu().numpy()
         tiles = ((self.width + 7) // 8) * ((self.height + 7) // 8)
-        return a[512 + 4 * tiles: 512 + 8 * tiles].view(np.uint32).copy()
+        return a[self.HEADER + 4 * tiles: self.HEADER + 8 * tiles].view(np.uint32).copy()
 
     def cost(self):
         a = self.buf.cpu().numpy()
         tiles = ((self.width + 7) // 8) * ((self.height + 7) // 8)
-        return a[512: 512 + 4 * tiles].view(np.uint32).copy()
+        return a[self.HEADER: self.HEADER + 4 * tiles].view(np.uint32).copy()
 
 
 def trace_scheduled(scene, C, fb, schedule):

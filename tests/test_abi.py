@@ -144,9 +144,10 @@ def test_tune_cache_roundtrip(L, tmp_path):
 
 
 def test_tile_schedule_size_and_checks(L):
-    # 512-byte header (previous camera, valid flag) + cost and order per 8x8 tile
-    assert L.arbor_tile_schedule_bytes(1920, 1080) == 512 + 8 * 240 * 135
-    assert L.arbor_tile_schedule_bytes(33, 9) == 512 + 8 * 5 * 2
+    # 2048-byte header (previous camera, valid flag, bucket histogram and starts)
+    # + cost, order and key per 8x8 tile
+    assert L.arbor_tile_schedule_bytes(1920, 1080) == 2048 + 12 * 240 * 135
+    assert L.arbor_tile_schedule_bytes(33, 9) == 2048 + 12 * 5 * 2
     assert L.arbor_tile_schedule_bytes(0, 10) == -2
     assert L.arbor_tracePixelsScheduled(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 64, 64, NULL, NULL) == -1
     assert L.arbor_tracePixelsScheduled(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 0, 64, DUMMY, NULL) == -2
