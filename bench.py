@@ -27,11 +27,11 @@ sys.path.insert(0, os.path.join(HERE, "arbor-cuda_amd"))
 # oracle/flops.cpp: each +,-,*,/,sqrt,min,max,abs,floor and libm call counts 1,
 # with the GPU's traversal, rejection tests and early exits).
 TRACE_FLOP_PER_RAY = 10998.7
-# HBM bytes per tracePixels launch on this workload (walk frame 0): rocprofv3
-# --pmc FETCH_SIZE 34,760 KiB x 2.0 + WRITE_SIZE 35,560 KiB x 1.0, the factors
-# being what the same counters read for a 256 MiB device copy (131,097 KiB and
-# 262,144 KiB); profiles/tracePixels_forest1080p_pmc.md.
-TRACE_HBM_BYTES = (34760.0 * 2.0 + 35560.0 * 1.0) * 1024
+# HBM bytes per scheduled tracePixels launch on this workload (walk frame 0):
+# rocprofv3 --pmc FETCH_SIZE 32,120 KiB x 2.0 + WRITE_SIZE 33,345 KiB x 1.0, the
+# factors being what the same counters read for a 256 MiB device copy
+# (131,097 KiB and 262,144 KiB); profiles/tracePixels_forest1080p_pmc.md.
+TRACE_HBM_BYTES = (32120.0 * 2.0 + 33345.0 * 1.0) * 1024
 FP32_PEAK_TFLOPS = 157.3
 # One non-packed FP32 VALU op per lane per clock: 256 CUs x 64 lanes x 2.4 GHz.
 FP32_SINGLE_ISSUE_TOPS = 39.32
