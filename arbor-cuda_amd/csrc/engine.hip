@@ -326,6 +326,72 @@ int arbor_batch_walk_views(int seed, int world_seed, int n, float* cameras, int*
   }
   return ARBOR_OK;
 }
+namespace {
+// Per-sample camera slots of the harness render loop (native_harness.cpp:62).
+__global__ void k_setSample(float* C, int sample, int light) {
+  if (threadIdx.x == 0) {
+    C[9] = (float)sample;
+    C[10] = sample == 0 ? 1.0f : 0.0f;
+    C[22] = light ? 1.0f : 0.0f;
+  }
+}
+}  // namespace
+
+int arbor_engine_render(arbor_engine* e, int view, int samples, int light) {
+  if (!e) return ARBOR_ERR_NULL;
+  if (!e->generated || !e->b.Hit) return ARBOR_ERR_STATE;
+  if (view < 0 || view > 5 || samples < 1 || samples > 512) return ARBOR_ERR_ARG;
+  float input[ARBOR_I_FLOATS] = {};
+  input[5] = (float)view;  // Renderer::view (native_harness.cpp:50)
+  if (hipMemcpyAsync(e->b.I, input, sizeof(input), hipMemcpyHostToDevice, e->stream) != hipSuccess)
+    return ARBOR_ERR_DEVICE;
+  if (hipStreamSynchronize(e->stream) != hipSuccess) return ARBOR_ERR_DEVICE;  // `input` is on this stack
+  const int w = e->width, h = e->height;
+  int rc = arbor_updateCamera(e->b.C, e->b.I, e->b.P, 1.0f / 60.0f, w, h, e->stream);
+  for (int i = 0; i < samples && rc == ARBOR_OK; i++) {
+    hipLaunchKernelGGL(k_setSample, dim3(1), dim3(64), 0, e->stream, e->b.C, i, light);
+    if ((rc = arbor::launch_status()) != ARBOR_OK) break;
+    rc = arbor_tracePixelsScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, w, h, e->b.Schedule, e->stream);
+    if (rc == ARBOR_OK)
+      rc = arbor_shadePixels(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Linear, e->b.Guide, w, h, e->stream);
+    if (rc == ARBOR_OK) rc = arbor_filterLighting(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.C, w, h, e->stream);
+    if (rc == ARBOR_OK) rc = arbor_resolvePixels(e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w, h, e->stream);
+  }
+  if (rc == ARBOR_OK) e->frames += samples;
+  return rc;
+}
+
+int arbor_write_ppm(const char* path, const unsigned* pixels, int width, int height) {
+  if (!path || !pixels) return ARBOR_ERR_NULL;
+  if (width < 1 || height < 1) return ARBOR_ERR_ARG;
+  FILE* f = fopen(path, "wb");
+  if (!f) return ARBOR_ERR_ARG;
+  fprintf(f, "P6\n%d %d\n255\n", width, height);
+  std::string row((size_t)width * 3, '\0');
+  bool ok = true;
+  for (int y = 0; y < height && ok; y++) {
+    for (int x = 0; x < width; x++) {
+      unsigned p = pixels[(size_t)y * width + x];
+      row[3 * x] = (char)(p & 255u);
+      row[3 * x + 1] = (char)((p >> 8) & 255u);
+      row[3 * x + 2] = (char)((p >> 16) & 255u);
+    }
+    ok = fwrite(row.data(), 1, row.size(), f) == row.size();
+  }
+  ok = fclose(f) == 0 && ok;
+  return ok ? ARBOR_OK : ARBOR_ERR_DEVICE;
+}
+
+int arbor_engine_save_ppm(arbor_engine* e, const char* path) {
+  if (!e || !path) return ARBOR_ERR_NULL;
+  if (!e->b.Pixels) return ARBOR_ERR_STATE;
+  std::string host((size_t)e->width * e->height * 4, '\0');
+  if (hipMemcpyAsync(&host[0], e->b.Pixels, host.size(), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+      hipStreamSynchronize(e->stream) != hipSuccess)
+    return ARBOR_ERR_DEVICE;
+  return arbor_write_ppm(path, reinterpret_cast<const unsigned*>(host.data()), e->width, e->height);
+}
+
 int arbor_walk_input(int seed, int frame, float* input32) {
   if (!input32) return ARBOR_ERR_NULL;
   if (frame < 0) return ARBOR_ERR_ARG;

@@ -37,6 +37,9 @@ SIGNATURES = {
     "arbor_initCamera": [_vp, _vp, _i, _vp],
     "arbor_updateCamera": [_vp, _vp, _vp, _f, _i, _i, _vp],
     "arbor_tracePixels": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "arbor_engine_render": [_vp, _i, _i, _i],
+    "arbor_engine_save_ppm": [_vp, ct.c_char_p],
+    "arbor_write_ppm": [ct.c_char_p, ct.POINTER(ct.c_uint), _i, _i],
     "arbor_tile_schedule_bytes": [_i, _i],
     "arbor_tracePixelsScheduled": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "arbor_shadePixels": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -347,6 +350,13 @@ class Batch:
         return a.view(np.uint32).copy() if name == "Pixels" else a.view(np.float32).copy()
 
 
+def write_ppm(path, pixels, width, height):
+    """Binary PPM (P6) of packed RGBA8 host pixels, as native_harness.cpp:53."""
+    px = np.ascontiguousarray(pixels, dtype=np.uint32)
+    check(lib().arbor_write_ppm(str(path).encode(), px.ctypes.data_as(ct.POINTER(ct.c_uint)), width, height),
+          "arbor_write_ppm")
+
+
 def walk_input(seed, frame):
     out = (ct.c_float * 32)()
     check(lib().arbor_walk_input(seed, frame, out), "arbor_walk_input")
@@ -410,6 +420,14 @@ class Engine:
 
     def buffer_ptr(self, name):
         return self._lib.arbor_engine_buffer(self.h, name.encode())
+
+    def render(self, view, samples, light=True):
+        """The reference harness's render mode (view preset, `samples`
+        jittered samples accumulated by resolvePixels)."""
+        check(self._lib.arbor_engine_render(self.h, view, samples, int(bool(light))), "arbor_engine_render")
+
+    def save_ppm(self, path):
+        check(self._lib.arbor_engine_save_ppm(self.h, str(path).encode()), "arbor_engine_save_ppm")
 
     def inspect(self):
         import json

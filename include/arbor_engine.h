@@ -48,6 +48,17 @@ int arbor_engine_size(const arbor_engine* e, int* width, int* height);
 int arbor_engine_inspect(const arbor_engine* e, char* out, int cap);
 /* Seeded walk-path input for frame `frame` (32 floats). */
 int arbor_walk_input(int seed, int frame, float* input32);
+/* The reference harness's render mode (native_harness.cpp:60-63): camera to
+ * view preset `view` (0..5, one updateCamera of 1/60 s with input[5] = view),
+ * sun visibility C[22] = light, then `samples` (1..512) jittered samples
+ * C[9] = i, C[10] = (i == 0), each traced, shaded, filtered and accumulated by
+ * resolvePixels (History holds their mean, Pixels the tone-mapped image). */
+int arbor_engine_render(arbor_engine* e, int view, int samples, int light);
+/* native_harness.cpp:53 save(): binary PPM (P6) of the engine's Pixels, RGB
+ * from the low three bytes of each packed pixel. */
+int arbor_engine_save_ppm(arbor_engine* e, const char* path);
+/* The same PPM from host pixels (no GPU). */
+int arbor_write_ppm(const char* path, const unsigned* pixels, int width, int height);
 
 #ifdef __cplusplus
 }

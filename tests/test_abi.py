@@ -153,6 +153,19 @@ def test_tile_schedule_size_and_checks(L):
     assert L.arbor_tracePixelsScheduled(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 0, 64, DUMMY, NULL) == -2
 
 
+def test_ppm_writer_format(L, tmp_path):
+    # native_harness.cpp:53: "P6\n<w> <h>\n255\n" then R, G, B = the low three bytes
+    px = np.array([0xFF030201, 0x00FFFEFD, 0x12345678, 0x0], np.uint32)
+    path = tmp_path / "a.ppm"
+    pyarbor.write_ppm(path, px, 2, 2)
+    data = path.read_bytes()
+    assert data == b"P6\n2 2\n255\n" + bytes([1, 2, 3, 0xFD, 0xFE, 0xFF, 0x78, 0x56, 0x34, 0, 0, 0])
+    assert L.arbor_write_ppm(None, None, 2, 2) == -1
+    assert L.arbor_write_ppm(str(path).encode(), px.ctypes.data_as(ct.POINTER(ct.c_uint)), 0, 2) == -2
+    assert L.arbor_engine_render(None, 1, 4, 1) == -1
+    assert L.arbor_engine_save_ppm(None, b"x.ppm") == -1
+
+
 def test_camera_pose_matches_init_camera(L, oracle):
     # arbor_camera_pose at initCamera's pose gives initCamera's state with the
     # accumulation reset flag C[10] set.

@@ -245,6 +245,41 @@ def test_wind_shadows_four_spp(px, oracle, scene7, view):
     assert (hist[3::4] == 4.0).all()  # four samples accumulated
 
 
+def test_engine_render_mode_and_ppm(px, oracle, tmp_path):
+    """The harness's render mode through the engine (native_harness.cpp:60-63:
+    view preset 2, sun visibility on, 4 samples accumulated by resolve) against
+    the oracle's same loop; the PPM carries the engine's Pixels."""
+    print("case: engine render view 2, 4 samples, 320x192", flush=True)
+    w, h = 320, 192
+    eng = px.Engine(w, h, 7)
+    eng.render(2, 4, light=True)
+    got = eng.download("Pixels", w * h, np.uint32)
+    ref = oracle.generate(7)
+    C = oracle.init_camera(7)
+    I = np.zeros(32, np.float32)
+    I[5] = 2
+    oracle.update_camera(C, I, 1.0 / 60.0, w, h)
+    C[22] = 1
+    hist = np.zeros(w * h * 4, np.float32)
+    for s in range(4):
+        C[9], C[10] = float(s), float(s == 0)
+        H = oracle.trace(ref, C, w, h)
+        L, G = oracle.shade(ref, C, H, w, h)
+        F = oracle.filter(L, G, H, C, w, h)
+        want = oracle.resolve(F, hist, C, w, h)
+    d = np.abs(got.view(np.uint8).astype(int) - want.view(np.uint8).astype(int)).reshape(-1, 4).max(axis=1)
+    print("render pixels: frac<=2 %.5f max %d" % ((d <= 2).mean(), d.max()))
+    assert (d <= 2).mean() >= 0.98
+    path = tmp_path / "render.ppm"
+    eng.save_ppm(path)
+    data = path.read_bytes()
+    head = b"P6\n%d %d\n255\n" % (w, h)
+    assert data[:len(head)] == head and len(data) == len(head) + 3 * w * h
+    rgb = np.frombuffer(data[len(head):], np.uint8).reshape(-1, 3)
+    assert np.array_equal(rgb, got.view(np.uint8).reshape(-1, 4)[:, :3])
+    eng.close()
+
+
 def test_engine_frame_pipeline(px, oracle):
     """The engine's 5-dispatch frame against the oracle pipeline (same camera
     path): pixels within 2/255 on >= 98% of pixels (near-tie rays may flip)."""
