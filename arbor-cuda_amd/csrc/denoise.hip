@@ -1,0 +1,418 @@
+// Kernels of the learned denoiser on gfx950: a 3x3 convolution encoder-decoder
+// in BF16 with FP32 accumulation on the matrix cores, and its training pieces.
+//
+// Tensors are NHWC bf16 with the channel count a multiple of 8, so a lane's
+// 8 consecutive channels are one 16-byte load. A 3x3 convolution is the
+// implicit GEMM Y[p, co] = sum_{tap, ci} X[p + off(tap), ci] * W[tap, co, ci]
+// (zero outside the image), K = 9*Cin walked in k-groups of 8 channels of one
+// tap, four k-groups per v_mfma_f32_16x16x32_bf16. Operand layout of that
+// instruction (checked on the device): A row = lane % 16, B column = lane % 16,
+// both with k = 8*(lane/16) + j for their 8 bf16; D column = lane % 16, rows
+// 4*(lane/16) + i for its 4 floats.
+//
+// The gradient with respect to X is the same convolution of dY with the
+// weights flipped in the taps and transposed in the channels. The weight
+// gradient is a GEMM over pixels: tiles of dY and X are staged transposed in
+// LDS so each operand fragment is one 16-byte LDS read; blocks write partial
+// sums per pixel chunk, reduced by a second kernel (deterministic).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "arbor_denoise.h"
+#include "launch.hpp"
+
+namespace arbor {
+namespace dn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
+__device__ __forceinline__ __bf16 f2bf(float v) { return (__bf16)v; }  // round to nearest even
+__device__ __forceinline__ bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int j = 0; j < 8; j++) z[j] = (__bf16)0.0f;
+  return z;
+}
+
+// ---- 3x3 convolution, stride 1, zero padding 1 ----
+// Block: 4 waves; wave w computes 16*MSUB pixels x 16*NSUB output channels,
+// the channels [co0, co0 + 16*NSUB) of a CoutTotal-channel output.
+template <int NSUB, int MSUB>
+__global__ __launch_bounds__(256) void k_conv3x3(const __bf16* __restrict__ X, int N, int H, int W, int Cin,
+                                                 const __bf16* __restrict__ Wt, const float* __restrict__ bias,
+                                                 int relu, __bf16* __restrict__ Y, int CoutTotal, int co0) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int M = N * H * W, HW = H * W;
+  const int pbase = (blockIdx.x * 4 + wave) * (16 * MSUB);
+  if (pbase >= M) return;
+  const int row = lane & 15, g = lane >> 4;
+  int pn[MSUB], py[MSUB], px[MSUB];
+  bool pv[MSUB];
+#pragma unroll
+  for (int m = 0; m < MSUB; m++) {
+    int p = pbase + m * 16 + row;
+    pv[m] = p < M;
+    int q = pv[m] ? p : 0;
+    pn[m] = q / HW;
+    int r = q - pn[m] * HW;
+    py[m] = r / W;
+    px[m] = r - py[m] * W;
+  }
+  f32x4 acc[MSUB][NSUB];
+#pragma unroll
+  for (int m = 0; m < MSUB; m++)
+#pragma unroll
+    for (int n = 0; n < NSUB; n++) acc[m][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int G = Cin >> 3;  // k-groups per tap
+  const int groups = 9 * G;
+  // this lane's k-group, advanced by 4 per step as (tap, channel group)
+  int tap = g / G, cg = g - tap * G;
+  for (int kg0 = 0; kg0 < groups; kg0 += 4) {
+    const bool live = kg0 + g < groups;
+    const int ci = cg << 3;
+    const int dy = tap / 3 - 1, dx = tap - (tap / 3) * 3 - 1;
+    bf16x8 a[MSUB], b[NSUB];
+#pragma unroll
+    for (int m = 0; m < MSUB; m++) {
+      const int yy = py[m] + dy, xx = px[m] + dx;
+      const bool in = live && pv[m] && yy >= 0 && yy < H && xx >= 0 && xx < W;
+      a[m] = in ? *reinterpret_cast<const bf16x8*>(X + ((size_t)(pn[m] * H + yy) * W + xx) * Cin + ci) : zero8();
+    }
+#pragma unroll
+    for (int n = 0; n < NSUB; n++)
+      b[n] = live ? *reinterpret_cast<const bf16x8*>(Wt + ((size_t)tap * CoutTotal + co0 + n * 16 + row) * Cin + ci)
+                  : zero8();
+#pragma unroll
+    for (int m = 0; m < MSUB; m++)
+#pragma unroll
+      for (int n = 0; n < NSUB; n++) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[n], acc[m][n], 0, 0, 0);
+    cg += 4;
+    while (cg >= G) { cg -= G; tap++; }
+  }
+#pragma unroll
+  for (int m = 0; m < MSUB; m++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int p = pbase + m * 16 + 4 * g + i;
+      if (p >= M) continue;
+#pragma unroll
+      for (int n = 0; n < NSUB; n++) {
+        const int co = co0 + n * 16 + row;
+        float v = acc[m][n][i] + (bias ? bias[co] : 0.0f);
+        if (relu) v = fmaxf(v, 0.0f);
+        Y[(size_t)p * CoutTotal + co] = f2bf(v);
+      }
+    }
+}
+
+// ---- weight and bias gradients ----
+// Block (chunk c, tap, 64x64 tile of (co, ci)): sum over the chunk's pixels of
+// dY[p, co] * X[p + off(tap), ci] into partial[c][tap][co][ci] (fp32); the
+// tap-0 blocks of the first ci tile also sum dY[p, co] into partialB[c][co].
+constexpr int kWgTile = 64, kWgK = 32;
+__global__ __launch_bounds__(256) void k_conv3x3_wgrad(const __bf16* __restrict__ X, const __bf16* __restrict__ dY,
+                                                       int N, int H, int W, int Cin, int Cout, int chunk,
+                                                       float* __restrict__ partial, float* __restrict__ partialB) {
+  __shared__ __attribute__((aligned(16))) __bf16 sA[kWgTile][kWgK + 8];  // [co][px], rows padded
+  __shared__ __attribute__((aligned(16))) __bf16 sB[kWgTile][kWgK + 8];  // [ci][px]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, t = threadIdx.x;
+  const int c = blockIdx.x, tap = blockIdx.y;
+  const int tilesCi = (Cin + kWgTile - 1) / kWgTile;
+  const int co0 = (blockIdx.z / tilesCi) * kWgTile, ci0 = (blockIdx.z % tilesCi) * kWgTile;
+  const int M = N * H * W, HW = H * W;
+  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  const int p0 = c * chunk, p1 = min(M, p0 + chunk);
+  const bool doBias = tap == 0 && ci0 == 0;
+  float biasAcc = 0.0f;  // thread t < 64 sums channel co0 + t
+  f32x4 acc[4];
+#pragma unroll
+  for (int n = 0; n < 4; n++) acc[n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int row = lane & 15, g = lane >> 4;
+  // staging: thread t loads 8 channels (group t % 8) of pixel t / 8 of the 32-pixel step
+  const int sp = t >> 3, sc = (t & 7) << 3;
+  for (int pb = p0; pb < p1; pb += kWgK) {
+    const int p = pb + sp;
+    bf16x8 vy = zero8(), vx = zero8();
+    if (p < p1) {
+      if (co0 + sc < Cout) vy = *reinterpret_cast<const bf16x8*>(dY + (size_t)p * Cout + co0 + sc);
+      const int n = p / HW, r = p - n * HW, y = r / W, x = r - y * W;
+      const int yy = y + dy, xx = x + dx;
+      if (ci0 + sc < Cin && yy >= 0 && yy < H && xx >= 0 && xx < W)
+        vx = *reinterpret_cast<const bf16x8*>(X + ((size_t)(n * H + yy) * W + xx) * Cin + ci0 + sc);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      sA[sc + j][sp] = vy[j];
+      sB[sc + j][sp] = vx[j];
+    }
+    __syncthreads();
+    if (doBias && t < kWgTile) {
+#pragma unroll 8
+      for (int k = 0; k < kWgK; k++) biasAcc += bf2f(sA[t][k]);
+    }
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[wave * 16 + row][8 * g]);
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[n * 16 + row][8 * g]);
+      acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[n], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int n = 0; n < 4; n++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int co = co0 + wave * 16 + 4 * g + i, ci = ci0 + n * 16 + row;
+      if (co < Cout && ci < Cin) partial[(((size_t)c * 9 + tap) * Cout + co) * Cin + ci] = acc[n][i];
+    }
+  if (doBias && t < kWgTile && co0 + t < Cout) partialB[(size_t)c * Cout + co0 + t] = biasAcc;
+}
+
+// out[i] = sum over c of part[c * n + i]
+__global__ void k_sumParts(const float* __restrict__ part, int chunks, long long n, float* __restrict__ out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    float s = 0.0f;
+    for (int c = 0; c < chunks; c++) s += part[c * n + i];
+    out[i] = s;
+  }
+}
+
+// Wf[tap][ci][co] = W[8 - tap][co][ci]
+__global__ void k_flipWeights(const __bf16* __restrict__ Wt, int Cin, int Cout, __bf16* __restrict__ Wf) {
+  const long long n = 9LL * Cin * Cout;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int co = (int)(i % Cout);
+    const long long r = i / Cout;
+    const int ci = (int)(r % Cin), tap = (int)(r / Cin);
+    Wf[i] = Wt[((size_t)(8 - tap) * Cout + co) * Cin + ci];
+  }
+}
+
+// ---- resampling, concatenation, activation gradient ----
+__global__ void k_pool2(const __bf16* __restrict__ X, int N, int H, int W, int C, __bf16* __restrict__ Y) {
+  const int Ho = H / 2, Wo = W / 2;
+  const long long n = (long long)N * Ho * Wo * C;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    long long r = i / C;
+    const int x = (int)(r % Wo);
+    r /= Wo;
+    const int y = (int)(r % Ho), b = (int)(r / Ho);
+    const size_t base = (((size_t)b * H + 2 * y) * W + 2 * x) * C + c;
+    float s = bf2f(X[base]) + bf2f(X[base + C]) + bf2f(X[base + (size_t)W * C]) + bf2f(X[base + (size_t)W * C + C]);
+    Y[i] = f2bf(s * 0.25f);
+  }
+}
+__global__ void k_pool2Bwd(const __bf16* __restrict__ dY, int N, int H, int W, int C, __bf16* __restrict__ dX) {
+  const long long n = (long long)N * H * W * C;  // H, W: the input (fine) size
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    long long r = i / C;
+    const int x = (int)(r % W);
+    r /= W;
+    const int y = (int)(r % H), b = (int)(r / H);
+    dX[i] = f2bf(bf2f(dY[(((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c]) * 0.25f);
+  }
+}
+__global__ void k_up2(const __bf16* __restrict__ X, int N, int H, int W, int C, __bf16* __restrict__ Y) {
+  const long long n = (long long)N * 2 * H * 2 * W * C;  // H, W: the input (coarse) size
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    long long r = i / C;
+    const int x = (int)(r % (2 * W));
+    r /= 2 * W;
+    const int y = (int)(r % (2 * H)), b = (int)(r / (2 * H));
+    Y[i] = X[(((size_t)b * H + y / 2) * W + x / 2) * C + c];
+  }
+}
+__global__ void k_up2Bwd(const __bf16* __restrict__ dY, int N, int H, int W, int C, __bf16* __restrict__ dX) {
+  const long long n = (long long)N * H * W * C;  // H, W: the coarse size
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    long long r = i / C;
+    const int x = (int)(r % W);
+    r /= W;
+    const int y = (int)(r % H), b = (int)(r / H);
+    const size_t base = (((size_t)b * 2 * H + 2 * y) * 2 * W + 2 * x) * C + c, row = (size_t)2 * W * C;
+    dX[i] = f2bf(bf2f(dY[base]) + bf2f(dY[base + C]) + bf2f(dY[base + row]) + bf2f(dY[base + row + C]));
+  }
+}
+// Y[p] = [A[p] (Ca channels), B[p] (Cb channels)]; the backward splits.
+__global__ void k_concat(const __bf16* __restrict__ A, int Ca, const __bf16* __restrict__ B, int Cb, long long pixels,
+                         __bf16* __restrict__ Y) {
+  const int C = Ca + Cb;
+  const long long n = pixels * C;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const long long p = i / C;
+    const int c = (int)(i - p * C);
+    Y[i] = c < Ca ? A[p * Ca + c] : B[p * Cb + c - Ca];
+  }
+}
+__global__ void k_split(const __bf16* __restrict__ Y, int Ca, int Cb, long long pixels, __bf16* __restrict__ A,
+                        __bf16* __restrict__ B) {
+  const int C = Ca + Cb;
+  const long long n = pixels * C;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const long long p = i / C;
+    const int c = (int)(i - p * C);
+    if (c < Ca) A[p * Ca + c] = Y[i];
+    else B[p * Cb + c - Ca] = Y[i];
+  }
+}
+// G *= (Yact > 0): the gradient through ReLU given its output.
+__global__ void k_reluBwd(__bf16* __restrict__ G, const __bf16* __restrict__ Yact, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    if (!(bf2f(Yact[i]) > 0.0f)) G[i] = (__bf16)0.0f;
+}
+__global__ void k_add(__bf16* __restrict__ A, const __bf16* __restrict__ B, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    A[i] = f2bf(bf2f(A[i]) + bf2f(B[i]));
+}
+
+inline int gridFor(long long n, int block = 256) {
+  long long b = (n + block - 1) / block;
+  return (int)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
+}
+
+}  // namespace dn
+}  // namespace arbor
+
+using namespace arbor;
+using namespace arbor::dn;
+
+namespace {
+template <int NSUB>
+void launchConv(const __bf16* X, int N, int H, int W, int Cin, const __bf16* Wt, const float* bias, int relu, __bf16* Y,
+                int coutTotal, int co0, hipStream_t s) {
+  constexpr int MSUB = 2;
+  const long long M = (long long)N * H * W;
+  const int blocks = (int)((M + 4 * 16 * MSUB - 1) / (4 * 16 * MSUB));
+  hipLaunchKernelGGL((k_conv3x3<NSUB, MSUB>), dim3(blocks), dim3(256), 0, s, X, N, H, W, Cin, Wt, bias, relu, Y,
+                     coutTotal, co0);
+}
+bool dnShapeOk(int n, int h, int w, int c) { return n >= 1 && h >= 1 && w >= 1 && c >= 8 && (c & 7) == 0; }
+}  // namespace
+
+extern "C" {
+
+long long arbor_dn_wgrad_partial_floats(int n, int h, int w, int cin, int cout) {
+  if (!dnShapeOk(n, h, w, cin) || !dnShapeOk(1, 1, 1, cout)) return ARBOR_ERR_ARG;
+  const long long M = (long long)n * h * w;
+  const long long chunks = (M + ARBOR_DN_WGRAD_CHUNK - 1) / ARBOR_DN_WGRAD_CHUNK;
+  return chunks * (9LL * cin * cout + cout);
+}
+
+int arbor_dn_conv3x3(const uint16_t* X, int n, int h, int w, int cin, const uint16_t* Wt, const float* bias, int relu,
+                     uint16_t* Y, int cout, hipStream_t stream) {
+  ARBOR_REQUIRE(X && Wt && Y);
+  if (!dnShapeOk(n, h, w, cin) || cout < 16 || cout % 16 != 0 || cout > ARBOR_DN_MAX_CHANNELS) return ARBOR_ERR_ARG;
+  if (cin > ARBOR_DN_MAX_CHANNELS || (long long)n * h * w > (1LL << 31) / 256) return ARBOR_ERR_ARG;
+  auto x = reinterpret_cast<const __bf16*>(X);
+  auto wt = reinterpret_cast<const __bf16*>(Wt);
+  auto y = reinterpret_cast<__bf16*>(Y);
+  // output channels in launches of at most 128
+  for (int co0 = 0; co0 < cout; co0 += 128) {
+    const int width = cout - co0 < 128 ? cout - co0 : 128;
+    switch (width / 16) {
+      case 1: launchConv<1>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 2: launchConv<2>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 3: launchConv<3>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 4: launchConv<4>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 5: launchConv<5>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 6: launchConv<6>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 7: launchConv<7>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 8: launchConv<8>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+    }
+  }
+  return launch_status();
+}
+
+int arbor_dn_conv3x3_wgrad(const uint16_t* X, const uint16_t* dY, int n, int h, int w, int cin, int cout,
+                           float* partial, long long partial_floats, float* dW, float* dB, hipStream_t stream) {
+  ARBOR_REQUIRE(X && dY && partial && dW && dB);
+  const long long need = arbor_dn_wgrad_partial_floats(n, h, w, cin, cout);
+  if (need < 0 || partial_floats < need) return ARBOR_ERR_ARG;
+  const long long M = (long long)n * h * w;
+  const int chunks = (int)((M + ARBOR_DN_WGRAD_CHUNK - 1) / ARBOR_DN_WGRAD_CHUNK);
+  if (chunks > 65535) return ARBOR_ERR_ARG;
+  const int tiles = ((cout + kWgTile - 1) / kWgTile) * ((cin + kWgTile - 1) / kWgTile);
+  float* partialB = partial + (size_t)chunks * 9 * cin * cout;
+  hipLaunchKernelGGL(k_conv3x3_wgrad, dim3(chunks, 9, tiles), dim3(256), 0, stream, reinterpret_cast<const __bf16*>(X),
+                     reinterpret_cast<const __bf16*>(dY), n, h, w, cin, cout, ARBOR_DN_WGRAD_CHUNK, partial, partialB);
+  const long long nw = 9LL * cin * cout;
+  hipLaunchKernelGGL(k_sumParts, dim3(gridFor(nw)), dim3(256), 0, stream, partial, chunks, nw, dW);
+  hipLaunchKernelGGL(k_sumParts, dim3(gridFor(cout)), dim3(256), 0, stream, partialB, chunks, (long long)cout, dB);
+  return launch_status();
+}
+
+int arbor_dn_flip_weights(const uint16_t* Wt, int cin, int cout, uint16_t* Wflip, hipStream_t stream) {
+  ARBOR_REQUIRE(Wt && Wflip);
+  if (cin < 1 || cout < 1) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_flipWeights, dim3(gridFor(9LL * cin * cout)), dim3(256), 0, stream,
+                     reinterpret_cast<const __bf16*>(Wt), cin, cout, reinterpret_cast<__bf16*>(Wflip));
+  return launch_status();
+}
+
+int arbor_dn_pool2(const uint16_t* X, int n, int h, int w, int c, uint16_t* Y, hipStream_t stream) {
+  ARBOR_REQUIRE(X && Y);
+  if (n < 1 || h < 2 || w < 2 || (h & 1) || (w & 1) || c < 1) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_pool2, dim3(gridFor((long long)n * h * w * c / 4)), dim3(256), 0, stream,
+                     reinterpret_cast<const __bf16*>(X), n, h, w, c, reinterpret_cast<__bf16*>(Y));
+  return launch_status();
+}
+int arbor_dn_pool2_bwd(const uint16_t* dY, int n, int h, int w, int c, uint16_t* dX, hipStream_t stream) {
+  ARBOR_REQUIRE(dY && dX);
+  if (n < 1 || h < 2 || w < 2 || (h & 1) || (w & 1) || c < 1) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_pool2Bwd, dim3(gridFor((long long)n * h * w * c)), dim3(256), 0, stream,
+                     reinterpret_cast<const __bf16*>(dY), n, h, w, c, reinterpret_cast<__bf16*>(dX));
+  return launch_status();
+}
+int arbor_dn_up2(const uint16_t* X, int n, int h, int w, int c, uint16_t* Y, hipStream_t stream) {
+  ARBOR_REQUIRE(X && Y);
+  if (n < 1 || h < 1 || w < 1 || c < 1) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_up2, dim3(gridFor((long long)n * h * w * c * 4)), dim3(256), 0, stream,
+                     reinterpret_cast<const __bf16*>(X), n, h, w, c, reinterpret_cast<__bf16*>(Y));
+  return launch_status();
+}
+int arbor_dn_up2_bwd(const uint16_t* dY, int n, int h, int w, int c, uint16_t* dX, hipStream_t stream) {
+  ARBOR_REQUIRE(dY && dX);
+  if (n < 1 || h < 1 || w < 1 || c < 1) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_up2Bwd, dim3(gridFor((long long)n * h * w * c)), dim3(256), 0, stream,
+                     reinterpret_cast<const __bf16*>(dY), n, h, w, c, reinterpret_cast<__bf16*>(dX));
+  return launch_status();
+}
+int arbor_dn_concat(const uint16_t* A, int ca, const uint16_t* B, int cb, long long pixels, uint16_t* Y,
+                    hipStream_t stream) {
+  ARBOR_REQUIRE(A && B && Y);
+  if (ca < 1 || cb < 1 || pixels < 1) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_concat, dim3(gridFor(pixels * (ca + cb))), dim3(256), 0, stream,
+                     reinterpret_cast<const __bf16*>(A), ca, reinterpret_cast<const __bf16*>(B), cb, pixels,
+                     reinterpret_cast<__bf16*>(Y));
+  return launch_status();
+}
+int arbor_dn_split(const uint16_t* Y, int ca, int cb, long long pixels, uint16_t* A, uint16_t* B, hipStream_t stream) {
+  ARBOR_REQUIRE(Y && A && B);
+  if (ca < 1 || cb < 1 || pixels < 1) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_split, dim3(gridFor(pixels * (ca + cb))), dim3(256), 0, stream,
+                     reinterpret_cast<const __bf16*>(Y), ca, cb, pixels, reinterpret_cast<__bf16*>(A),
+                     reinterpret_cast<__bf16*>(B));
+  return launch_status();
+}
+int arbor_dn_relu_bwd(uint16_t* G, const uint16_t* Yact, long long count, hipStream_t stream) {
+  ARBOR_REQUIRE(G && Yact);
+  if (count < 1) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_reluBwd, dim3(gridFor(count)), dim3(256), 0, stream, reinterpret_cast<__bf16*>(G),
+                     reinterpret_cast<const __bf16*>(Yact), count);
+  return launch_status();
+}
+int arbor_dn_add(uint16_t* A, const uint16_t* B, long long count, hipStream_t stream) {
+  ARBOR_REQUIRE(A && B);
+  if (count < 1) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_add, dim3(gridFor(count)), dim3(256), 0, stream, reinterpret_cast<__bf16*>(A),
+                     reinterpret_cast<const __bf16*>(B), count);
+  return launch_status();
+}
+
+}  // extern "C"

@@ -37,6 +37,18 @@ SIGNATURES = {
     "arbor_initCamera": [_vp, _vp, _i, _vp],
     "arbor_updateCamera": [_vp, _vp, _vp, _f, _i, _i, _vp],
     "arbor_tracePixels": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "arbor_dn_conv3x3": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp],
+    "arbor_dn_wgrad_partial_floats": [_i, _i, _i, _i, _i],
+    "arbor_dn_conv3x3_wgrad": [_vp, _vp, _i, _i, _i, _i, _i, _vp, ct.c_longlong, _vp, _vp, _vp],
+    "arbor_dn_flip_weights": [_vp, _i, _i, _vp, _vp],
+    "arbor_dn_pool2": [_vp, _i, _i, _i, _i, _vp, _vp],
+    "arbor_dn_pool2_bwd": [_vp, _i, _i, _i, _i, _vp, _vp],
+    "arbor_dn_up2": [_vp, _i, _i, _i, _i, _vp, _vp],
+    "arbor_dn_up2_bwd": [_vp, _i, _i, _i, _i, _vp, _vp],
+    "arbor_dn_concat": [_vp, _i, _vp, _i, ct.c_longlong, _vp, _vp],
+    "arbor_dn_split": [_vp, _i, _i, ct.c_longlong, _vp, _vp, _vp],
+    "arbor_dn_relu_bwd": [_vp, _vp, ct.c_longlong, _vp],
+    "arbor_dn_add": [_vp, _vp, ct.c_longlong, _vp],
     "arbor_engine_render": [_vp, _i, _i, _i],
     "arbor_engine_save_ppm": [_vp, ct.c_char_p],
     "arbor_write_ppm": [ct.c_char_p, ct.POINTER(ct.c_uint), _i, _i],
@@ -86,7 +98,7 @@ SIGNATURES = {
 }
 _RESTYPES = {"arbor_version": ct.c_char_p, "arbor_engine_round_size": None, "arbor_engine_buffer": _vp,
              "arbor_engine_stream": _vp, "arbor_batch_buffer": _vp, "arbor_batch_pixels": ct.c_longlong,
-             "arbor_tile_schedule_bytes": ct.c_longlong}
+             "arbor_tile_schedule_bytes": ct.c_longlong, "arbor_dn_wgrad_partial_floats": ct.c_longlong}
 BATCH_TRACE_SHADE, BATCH_FILTER, BATCH_RESOLVE, BATCH_ALL = 1, 2, 4, 7
 
 

@@ -18,7 +18,7 @@ DUMMY = ct.c_void_p(4096)  # never dereferenced: every case below fails validati
 
 def declared_symbols():
     names = set()
-    for h in ("arbor.h", "arbor_engine.h", "arbor_batch.h"):
+    for h in ("arbor.h", "arbor_engine.h", "arbor_batch.h", "arbor_denoise.h"):
         with open(os.path.join(REPO, "include", h)) as f:
             text = f.read()
         names |= set(re.findall(r"\b(arbor_\w+)\s*\(", text))
