@@ -134,16 +134,19 @@ AD ShadeCtx makeShadeCtx(const float* C, int width, int height) {
 }
 
 // One pixel of shadePixels (Arbor.cu:277-289): Linear and Guide of pixel
-// (x, y) from its Hit record.
+// (x, y) from its Hit record; with outNormal, also the shading normal (facing
+// the camera; zero for sky).
 template <class W>
 __device__ inline void shadeOne(const float* P, const float4* Nodes4, const uint32_t* Order, const TreeInst* trees,
-                                const ShadeCtx& sc, int x, int y, float4 hit, float4& outLinear, float4& outGuide) {
+                                const ShadeCtx& sc, int x, int y, float4 hit, float4& outLinear, float4& outGuide,
+                                v3* outNormal = nullptr) {
   const int pixel = y * sc.width + x;
   float tt = hit.x;
   int id = (int)hit.y, tree = (int)hit.z;
   v3 ro = sc.view.eye, rd = viewRay(sc.view, x, y, sc.width, sc.height);
   v3 col = sky(rd, sc.sun);
   v3 guide = mk(1.0f, 1.0f, 1.0f);
+  if (outNormal) *outNormal = mk(0.0f, 0.0f, 0.0f);
   if (tt < kFar) {
     v3 p = ro + rd * tt;
     float fp = fmaxf(0.000005f, hit.w);
@@ -171,6 +174,7 @@ __device__ inline void shadeOne(const float* P, const float4* Nodes4, const uint
       float nz = (noise(p.x * 18.0f, (p.z + e) * 18.0f, 133) - noise(p.x * 18.0f, (p.z - e) * 18.0f, 133)) * 0.18f;
       n = unit(mk(nx, 1.0f, nz));
     }
+    if (outNormal) *outNormal = n;
     v3 albedo;
     if (id < 0) albedo = groundColor(p, localFp);
     else {

@@ -1,0 +1,541 @@
+// The learned denoiser's network: a 5-level 3x3-convolution encoder-decoder
+// (U-Net) with 32/48/64/96/128 channels, BF16 activations and weights, FP32
+// accumulation, master weights and Adam state, trained with an L1 loss on
+// log(1 + radiance), and used for inference on whole frames.
+//
+// Input per pixel (16 channels, 10 used): log(1 + 1-spp radiance) rgb,
+// albedo rgb (Guide), shading normal xyz, and depth log(1 + min(t, 1000))/7
+// (1 for sky). Output: log(1 + radiance) rgb.
+//
+// Graph (s = ReLU conv pairs kept as skips, p = 2x2 average pool, u = nearest
+// 2x upsample, c = channel concatenation [u, s]):
+//   in -> L0 -> L1 = s0 -> p -> L2 -> L3 = s1 -> p -> L4 -> L5 = s2 -> p ->
+//   L6 -> L7 = s3 -> p -> L8 -> L9 = e4 -> u,c(s3) -> L10 -> L11 -> u,c(s2) ->
+//   L12 -> L13 -> u,c(s1) -> L14 -> L15 -> u,c(s0) -> L16 -> L17 -> L18 (linear)
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <random>
+#include <string>
+#include <vector>
+
+#include "arbor_denoise.h"
+#include "launch.hpp"
+
+namespace {
+
+using bf16 = __bf16;
+
+constexpr int kInC = 16, kOutC = 16, kLevels = 5;
+constexpr int kWidths[kLevels] = {32, 48, 64, 96, 128};
+constexpr float kFar = 100000.0f;
+
+// ---- data kernels ----
+__global__ void k_features(const float4* __restrict__ Linear, const float4* __restrict__ Guide,
+                           const float4* __restrict__ Normal, const float4* __restrict__ Hit, int w, int h,
+                           bf16* __restrict__ X, int H, int W) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= W) return;
+  float f[kInC];
+  for (int c = 0; c < kInC; c++) f[c] = 0.0f;
+  if (x < w && y < h) {
+    const int i = y * w + x;
+    const float4 l = Linear[i], g = Guide[i], n = Normal[i], t = Hit[i];
+    f[0] = log1pf(fmaxf(l.x, 0.0f));
+    f[1] = log1pf(fmaxf(l.y, 0.0f));
+    f[2] = log1pf(fmaxf(l.z, 0.0f));
+    f[3] = g.x;
+    f[4] = g.y;
+    f[5] = g.z;
+    f[6] = n.x;
+    f[7] = n.y;
+    f[8] = n.z;
+    f[9] = t.x < kFar ? log1pf(fminf(t.x, 1000.0f)) * (1.0f / 7.0f) : 1.0f;
+  }
+  bf16* o = X + ((size_t)y * W + x) * kInC;
+  for (int c = 0; c < kInC; c++) o[c] = (bf16)f[c];
+}
+// Target slot: radiance rgb and a validity mask in w.
+__global__ void k_target(const float4* __restrict__ R, int w, int h, float4* __restrict__ T, int H, int W) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= W) return;
+  float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (x < w && y < h) {
+    const float4 r = R[y * w + x];
+    v = make_float4(log1pf(fmaxf(r.x, 0.0f)), log1pf(fmaxf(r.y, 0.0f)), log1pf(fmaxf(r.z, 0.0f)), 1.0f);
+  }
+  T[(size_t)y * W + x] = v;
+}
+// L1 in log space over the masked pixels: gradient sign(o - t) * scale into G
+// (channels 0..2; the rest 0); per-block partial sums of |o - t| into part.
+__global__ __launch_bounds__(256) void k_l1Loss(const bf16* __restrict__ Out, const float4* __restrict__ T, long long n,
+                                                float scale, bf16* __restrict__ G, float* __restrict__ part) {
+  __shared__ float red[256];
+  float s = 0.0f;
+  for (long long p = blockIdx.x * 256LL + threadIdx.x; p < n; p += (long long)gridDim.x * 256) {
+    const float4 t = T[p];
+    const float tv[3] = {t.x, t.y, t.z};
+    for (int c = 0; c < kOutC; c++) {
+      float g = 0.0f;
+      if (c < 3 && t.w > 0.0f) {
+        const float d = (float)Out[p * kOutC + c] - tv[c];
+        s += fabsf(d);
+        g = d > 0.0f ? scale : (d < 0.0f ? -scale : 0.0f);
+      }
+      G[p * kOutC + c] = (bf16)g;
+    }
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ void k_sumTo(const float* __restrict__ part, int n, float scale, float* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    float s = 0.0f;
+    for (int i = 0; i < n; i++) s += part[i];
+    *out = s * scale;
+  }
+}
+// Output slot -> radiance (expm1 of the prediction, clamped at 0), alpha 1.
+__global__ void k_output(const bf16* __restrict__ Y, int H, int W, float4* __restrict__ Out, int w, int h) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w || y >= h) return;
+  const bf16* o = Y + ((size_t)y * W + x) * kOutC;
+  Out[y * w + x] = make_float4(fmaxf(expm1f((float)o[0]), 0.0f), fmaxf(expm1f((float)o[1]), 0.0f),
+                               fmaxf(expm1f((float)o[2]), 0.0f), 1.0f);
+}
+// Adam on fp32 master values; also writes the bf16 copy when w16 != NULL.
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       long long n, float lr, float b1, float b2, float eps, bf16* __restrict__ w16) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi, vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * mi / (sqrtf(vi) + eps);
+    w[i] = wi;
+    if (w16) w16[i] = (bf16)wi;
+  }
+}
+__global__ void k_toBf16(const float* __restrict__ a, long long n, bf16* __restrict__ b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    b[i] = (bf16)a[i];
+}
+
+int grid1(long long n) {
+  long long b = (n + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
+}
+
+struct Conv {
+  int cin, cout, level;
+  bool relu;
+  float *w = nullptr, *b = nullptr, *gw = nullptr, *gb = nullptr, *mw = nullptr, *vw = nullptr, *mb = nullptr,
+        *vb = nullptr;
+  uint16_t *w16 = nullptr, *wflip = nullptr;
+  long long nw() const { return 9LL * cin * cout; }
+};
+struct Act {
+  int c, level;
+  uint16_t* p = nullptr;  // activation
+  uint16_t* g = nullptr;  // gradient (training)
+};
+
+}  // namespace
+
+struct arbor_denoiser {
+  int maxBatch = 0, H = 0, W = 0;
+  bool train = false;
+  int step = 0;
+  std::vector<Conv> conv;
+  // activations: index by name
+  Act in, a[9], s[4], p[4], e4, u[4], c[4], dA[4], d[4], out;
+  uint16_t* gpool = nullptr;  // pool-gradient temporary (largest: level 0 x 32)
+  float* target = nullptr;    // [maxBatch][H][W] float4
+  float* partial = nullptr;
+  long long partialFloats = 0;
+  float* lossPart = nullptr;
+  float* loss = nullptr;
+  std::vector<void*> owned;
+
+  int hl(int level) const { return H >> level; }
+  int wl(int level) const { return W >> level; }
+  long long count(const Act& t, int n) const { return (long long)n * hl(t.level) * wl(t.level) * t.c; }
+  template <class T>
+  int alloc(T** ptr, size_t bytes) {
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess) return ARBOR_ERR_DEVICE;
+    if (hipMemset(q, 0, bytes) != hipSuccess) return ARBOR_ERR_DEVICE;
+    owned.push_back(q);
+    *ptr = static_cast<T*>(q);
+    return ARBOR_OK;
+  }
+  int allocAct(Act& t, int c, int level) {
+    t.c = c;
+    t.level = level;
+    size_t bytes = (size_t)maxBatch * hl(level) * wl(level) * c * 2;
+    int rc = alloc(&t.p, bytes);
+    if (rc == ARBOR_OK && train) rc = alloc(&t.g, bytes);
+    return rc;
+  }
+};
+
+namespace {
+
+// Layer table: (cin, cout, level, relu) in graph order L0..L18.
+void buildLayers(arbor_denoiser* d) {
+  const int* c = kWidths;
+  int spec[19][4] = {
+      {kInC, c[0], 0, 1}, {c[0], c[0], 0, 1}, {c[0], c[1], 1, 1}, {c[1], c[1], 1, 1}, {c[1], c[2], 2, 1},
+      {c[2], c[2], 2, 1}, {c[2], c[3], 3, 1}, {c[3], c[3], 3, 1}, {c[3], c[4], 4, 1}, {c[4], c[4], 4, 1},
+      {c[4] + c[3], c[3], 3, 1}, {c[3], c[3], 3, 1}, {c[3] + c[2], c[2], 2, 1}, {c[2], c[2], 2, 1},
+      {c[2] + c[1], c[1], 1, 1}, {c[1], c[1], 1, 1}, {c[1] + c[0], c[0], 0, 1}, {c[0], c[0], 0, 1},
+      {c[0], kOutC, 0, 0}};
+  for (auto& sp : spec) {
+    Conv cv;
+    cv.cin = sp[0];
+    cv.cout = sp[1];
+    cv.level = sp[2];
+    cv.relu = sp[3] != 0;
+    d->conv.push_back(cv);
+  }
+}
+
+int initWeights(arbor_denoiser* d, int seed) {
+  std::mt19937 rng((uint32_t)seed * 2654435761u + 12345u);
+  for (size_t li = 0; li < d->conv.size(); li++) {
+    Conv& cv = d->conv[li];
+    const bool last = li + 1 == d->conv.size();
+    const int fanIn = 9 * (li == 0 ? 10 : cv.cin);
+    std::normal_distribution<float> nd(0.0f, last ? sqrtf(1.0f / fanIn) : sqrtf(2.0f / fanIn));
+    std::vector<float> w((size_t)cv.nw());
+    for (int tap = 0; tap < 9; tap++)
+      for (int co = 0; co < cv.cout; co++)
+        for (int ci = 0; ci < cv.cin; ci++) {
+          float v = nd(rng);
+          if ((li == 0 && ci >= 10) || (last && co >= 3)) v = 0.0f;  // unused input / output channels
+          w[((size_t)tap * cv.cout + co) * cv.cin + ci] = v;
+        }
+    if (hipMemcpy(cv.w, w.data(), w.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return ARBOR_ERR_DEVICE;
+  }
+  return ARBOR_OK;
+}
+
+int refreshBf16(arbor_denoiser* d, hipStream_t s) {
+  for (Conv& cv : d->conv) {
+    hipLaunchKernelGGL(k_toBf16, dim3(grid1(cv.nw())), dim3(256), 0, s, cv.w, cv.nw(), reinterpret_cast<bf16*>(cv.w16));
+    int rc = arbor_dn_flip_weights(cv.w16, cv.cin, cv.cout, cv.wflip, s);
+    if (rc != ARBOR_OK) return rc;
+  }
+  return arbor::launch_status();
+}
+
+#define DN_TRY(x)                      \
+  do {                                 \
+    int rc_ = (x);                     \
+    if (rc_ != ARBOR_OK) return rc_;   \
+  } while (0)
+
+int runConv(arbor_denoiser* d, int li, const Act& x, Act& y, int n, hipStream_t s) {
+  const Conv& cv = d->conv[li];
+  return arbor_dn_conv3x3(x.p, n, d->hl(cv.level), d->wl(cv.level), cv.cin, cv.w16, cv.b, cv.relu ? 1 : 0, y.p,
+                          cv.cout, s);
+}
+
+int forward(arbor_denoiser* d, int n, hipStream_t s) {
+  // encoder
+  const Act* x = &d->in;
+  for (int k = 0; k < kLevels; k++) {
+    Act& first = d->a[k];
+    Act& second = k < 4 ? d->s[k] : d->e4;
+    DN_TRY(runConv(d, 2 * k, *x, first, n, s));
+    DN_TRY(runConv(d, 2 * k + 1, first, second, n, s));
+    if (k < 4) {
+      DN_TRY(arbor_dn_pool2(second.p, n, d->hl(k), d->wl(k), second.c, d->p[k].p, s));
+      x = &d->p[k];
+    }
+  }
+  // decoder: level k = 3..0
+  const Act* low = &d->e4;
+  for (int k = 3; k >= 0; k--) {
+    const int j = 3 - k;  // decoder stage
+    DN_TRY(arbor_dn_up2(low->p, n, d->hl(k + 1), d->wl(k + 1), low->c, d->u[k].p, s));
+    DN_TRY(arbor_dn_concat(d->u[k].p, d->u[k].c, d->s[k].p, d->s[k].c, (long long)n * d->hl(k) * d->wl(k), d->c[k].p,
+                           s));
+    DN_TRY(runConv(d, 10 + 2 * j, d->c[k], d->dA[k], n, s));
+    DN_TRY(runConv(d, 11 + 2 * j, d->dA[k], d->d[k], n, s));
+    low = &d->d[k];
+  }
+  return runConv(d, 18, d->d[0], d->out, n, s);
+}
+
+// dX of conv li from dY (into gx; gx may be NULL for the input layer), and
+// the layer's weight and bias gradients from its input x.
+int backConv(arbor_denoiser* d, int li, const Act& x, const Act& y, uint16_t* gx, int n, hipStream_t s) {
+  const Conv& cv = d->conv[li];
+  const int h = d->hl(cv.level), w = d->wl(cv.level);
+  if (cv.relu) DN_TRY(arbor_dn_relu_bwd(y.g, y.p, (long long)n * h * w * cv.cout, s));
+  DN_TRY(arbor_dn_conv3x3_wgrad(x.p, y.g, n, h, w, cv.cin, cv.cout, d->partial, d->partialFloats, cv.gw, cv.gb, s));
+  if (gx) DN_TRY(arbor_dn_conv3x3(y.g, n, h, w, cv.cout, cv.wflip, nullptr, 0, gx, cv.cin, s));
+  return ARBOR_OK;
+}
+
+int backward(arbor_denoiser* d, int n, hipStream_t s) {
+  // output layer and decoder
+  DN_TRY(backConv(d, 18, d->d[0], d->out, d->d[0].g, n, s));
+  for (int k = 0; k <= 3; k++) {
+    const int j = 3 - k;
+    DN_TRY(backConv(d, 11 + 2 * j, d->dA[k], d->d[k], d->dA[k].g, n, s));
+    DN_TRY(backConv(d, 10 + 2 * j, d->c[k], d->dA[k], d->c[k].g, n, s));
+    const long long px = (long long)n * d->hl(k) * d->wl(k);
+    DN_TRY(arbor_dn_split(d->c[k].g, d->u[k].c, d->s[k].c, px, d->u[k].g, d->s[k].g, s));
+    Act& low = k < 3 ? d->d[k + 1] : d->e4;
+    DN_TRY(arbor_dn_up2_bwd(d->u[k].g, n, d->hl(k + 1), d->wl(k + 1), low.c, low.g, s));
+  }
+  // bottleneck and encoder, deepest first
+  for (int k = 4; k >= 0; k--) {
+    Act& second = k < 4 ? d->s[k] : d->e4;
+    const Act& input = k == 0 ? d->in : d->p[k - 1];
+    DN_TRY(backConv(d, 2 * k + 1, d->a[k], second, d->a[k].g, n, s));
+    DN_TRY(backConv(d, 2 * k, input, d->a[k], k == 0 ? nullptr : d->p[k - 1].g, n, s));
+    if (k > 0) {  // the skip below receives the pooled path's gradient too
+      Act& skip = d->s[k - 1];
+      DN_TRY(arbor_dn_pool2_bwd(d->p[k - 1].g, n, d->hl(k - 1), d->wl(k - 1), skip.c, d->gpool, s));
+      DN_TRY(arbor_dn_add(skip.g, d->gpool, (long long)n * d->hl(k - 1) * d->wl(k - 1) * skip.c, s));
+    }
+  }
+  return ARBOR_OK;
+}
+
+bool slotOk(const arbor_denoiser* d, int index, int w, int h) {
+  return index >= 0 && index < d->maxBatch && w >= 1 && h >= 1 && w <= d->W && h <= d->H;
+}
+
+}  // namespace
+
+extern "C" {
+
+int arbor_denoiser_create(arbor_denoiser** out, int max_batch, int height, int width, int train, int seed) {
+  if (!out) return ARBOR_ERR_NULL;
+  *out = nullptr;
+  if (max_batch < 1 || max_batch > 1024 || height < 16 || width < 16 || height % 16 || width % 16 ||
+      height > ARBOR_MAX_DIM || width > ARBOR_MAX_DIM)
+    return ARBOR_ERR_ARG;
+  arbor_denoiser* d = new arbor_denoiser();
+  d->maxBatch = max_batch;
+  d->H = height;
+  d->W = width;
+  d->train = train != 0;
+  buildLayers(d);
+  int rc = ARBOR_OK;
+  for (Conv& cv : d->conv) {
+    rc |= d->alloc(&cv.w, cv.nw() * 4);
+    rc |= d->alloc(&cv.b, cv.cout * 4);
+    rc |= d->alloc(&cv.w16, cv.nw() * 2);
+    rc |= d->alloc(&cv.wflip, cv.nw() * 2);
+    if (d->train) {
+      rc |= d->alloc(&cv.gw, cv.nw() * 4);
+      rc |= d->alloc(&cv.gb, cv.cout * 4);
+      rc |= d->alloc(&cv.mw, cv.nw() * 4);
+      rc |= d->alloc(&cv.vw, cv.nw() * 4);
+      rc |= d->alloc(&cv.mb, cv.cout * 4);
+      rc |= d->alloc(&cv.vb, cv.cout * 4);
+    }
+  }
+  const int* c = kWidths;
+  rc |= d->allocAct(d->in, kInC, 0);
+  for (int k = 0; k < kLevels; k++) rc |= d->allocAct(d->a[k], c[k], k);
+  for (int k = 0; k < 4; k++) {
+    rc |= d->allocAct(d->s[k], c[k], k);
+    rc |= d->allocAct(d->p[k], c[k], k + 1);
+    rc |= d->allocAct(d->u[k], c[k + 1], k);
+    rc |= d->allocAct(d->c[k], c[k + 1] + c[k], k);
+    rc |= d->allocAct(d->dA[k], c[k], k);
+    rc |= d->allocAct(d->d[k], c[k], k);
+  }
+  rc |= d->allocAct(d->e4, c[4], 4);
+  rc |= d->allocAct(d->out, kOutC, 0);
+  if (d->train) {
+    rc |= d->alloc(&d->gpool, (size_t)max_batch * height * width * c[0] * 2);
+    rc |= d->alloc(&d->target, (size_t)max_batch * height * width * 16);
+    long long most = 0;
+    for (Conv& cv : d->conv) {
+      long long need = arbor_dn_wgrad_partial_floats(max_batch, d->hl(cv.level), d->wl(cv.level), cv.cin, cv.cout);
+      most = need > most ? need : most;
+    }
+    d->partialFloats = most;
+    rc |= d->alloc(&d->partial, (size_t)most * 4);
+    rc |= d->alloc(&d->lossPart, 1024 * 4);
+    rc |= d->alloc(&d->loss, 4);
+  }
+  if (rc == ARBOR_OK) rc = initWeights(d, seed);
+  if (rc == ARBOR_OK) rc = refreshBf16(d, nullptr);
+  if (rc == ARBOR_OK && hipDeviceSynchronize() != hipSuccess) rc = ARBOR_ERR_DEVICE;
+  if (rc != ARBOR_OK) {
+    arbor_denoiser_destroy(d);
+    return ARBOR_ERR_DEVICE;
+  }
+  *out = d;
+  return ARBOR_OK;
+}
+
+void* arbor_denoiser_tensor(arbor_denoiser* d, const char* name) {
+  if (!d || !name) return nullptr;
+  if (strcmp(name, "input") == 0) return d->in.p;
+  if (strcmp(name, "output") == 0) return d->out.p;
+  if (strcmp(name, "target") == 0) return d->target;
+  const char* colon = strchr(name, ':');
+  if (!colon) return nullptr;
+  char* end = nullptr;
+  long li = strtol(colon + 1, &end, 10);
+  if (end == colon + 1 || *end != '\0' || li < 0 || li >= (long)d->conv.size()) return nullptr;
+  const Conv& cv = d->conv[(size_t)li];
+  const std::string kind(name, colon - name);
+  if (kind == "weight") return cv.w;
+  if (kind == "bias") return cv.b;
+  if (kind == "wgrad") return cv.gw;
+  if (kind == "bgrad") return cv.gb;
+  return nullptr;
+}
+
+int arbor_denoiser_destroy(arbor_denoiser* d) {
+  if (!d) return ARBOR_ERR_NULL;
+  (void)hipDeviceSynchronize();
+  for (void* q : d->owned) (void)hipFree(q);
+  delete d;
+  return ARBOR_OK;
+}
+
+long long arbor_denoiser_parameters(const arbor_denoiser* d) {
+  if (!d) return ARBOR_ERR_NULL;
+  long long n = 0;
+  for (const Conv& cv : d->conv) n += cv.nw() + cv.cout;
+  return n;
+}
+
+int arbor_denoiser_set_input(arbor_denoiser* d, int index, const float* Linear, const float* Guide,
+                             const float* Normal, const float* Hit, int w, int h, hipStream_t stream) {
+  ARBOR_REQUIRE(d && Linear && Guide && Normal && Hit);
+  if (!slotOk(d, index, w, h)) return ARBOR_ERR_ARG;
+  auto X = reinterpret_cast<bf16*>(d->in.p) + (size_t)index * d->H * d->W * kInC;
+  hipLaunchKernelGGL(k_features, dim3((d->W + 63) / 64, d->H), dim3(64), 0, stream,
+                     reinterpret_cast<const float4*>(Linear), reinterpret_cast<const float4*>(Guide),
+                     reinterpret_cast<const float4*>(Normal), reinterpret_cast<const float4*>(Hit), w, h, X, d->H,
+                     d->W);
+  return arbor::launch_status();
+}
+
+int arbor_denoiser_set_target(arbor_denoiser* d, int index, const float* Radiance, int w, int h, hipStream_t stream) {
+  ARBOR_REQUIRE(d && Radiance);
+  if (!d->train) return ARBOR_ERR_STATE;
+  if (!slotOk(d, index, w, h)) return ARBOR_ERR_ARG;
+  auto T = reinterpret_cast<float4*>(d->target) + (size_t)index * d->H * d->W;
+  hipLaunchKernelGGL(k_target, dim3((d->W + 63) / 64, d->H), dim3(64), 0, stream,
+                     reinterpret_cast<const float4*>(Radiance), w, h, T, d->H, d->W);
+  return arbor::launch_status();
+}
+
+int arbor_denoiser_forward(arbor_denoiser* d, int n, hipStream_t stream) {
+  ARBOR_REQUIRE(d);
+  if (n < 1 || n > d->maxBatch) return ARBOR_ERR_ARG;
+  return forward(d, n, stream);
+}
+
+int arbor_denoiser_get_output(arbor_denoiser* d, int index, float* Out, int w, int h, hipStream_t stream) {
+  ARBOR_REQUIRE(d && Out);
+  if (!slotOk(d, index, w, h)) return ARBOR_ERR_ARG;
+  auto Y = reinterpret_cast<const bf16*>(d->out.p) + (size_t)index * d->H * d->W * kOutC;
+  hipLaunchKernelGGL(k_output, dim3((w + 63) / 64, h), dim3(64), 0, stream, Y, d->H, d->W,
+                     reinterpret_cast<float4*>(Out), w, h);
+  return arbor::launch_status();
+}
+
+int arbor_denoiser_train_step(arbor_denoiser* d, int n, float lr, hipStream_t stream) {
+  ARBOR_REQUIRE(d);
+  if (!d->train) return ARBOR_ERR_STATE;
+  if (n < 1 || n > d->maxBatch || !(lr > 0.0f)) return ARBOR_ERR_ARG;
+  DN_TRY(forward(d, n, stream));
+  const long long px = (long long)n * d->H * d->W;
+  const int blocks = (int)((px + 255) / 256 < 1024 ? (px + 255) / 256 : 1024);
+  // mean over every pixel and channel of the batch (padding counts as zero loss)
+  const float scale = 1.0f / (float)(px * 3);
+  hipLaunchKernelGGL(k_l1Loss, dim3(blocks), dim3(256), 0, stream, reinterpret_cast<const bf16*>(d->out.p),
+                     reinterpret_cast<const float4*>(d->target), px, scale, reinterpret_cast<bf16*>(d->out.g),
+                     d->lossPart);
+  hipLaunchKernelGGL(k_sumTo, dim3(1), dim3(64), 0, stream, d->lossPart, blocks, scale, d->loss);
+  DN_TRY(arbor::launch_status());
+  DN_TRY(backward(d, n, stream));
+  d->step++;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float lrT = lr * sqrtf(1.0f - powf(b2, (float)d->step)) / (1.0f - powf(b1, (float)d->step));
+  for (Conv& cv : d->conv) {
+    hipLaunchKernelGGL(k_adam, dim3(grid1(cv.nw())), dim3(256), 0, stream, cv.w, cv.gw, cv.mw, cv.vw, cv.nw(), lrT, b1,
+                       b2, eps, reinterpret_cast<bf16*>(cv.w16));
+    hipLaunchKernelGGL(k_adam, dim3(grid1(cv.cout)), dim3(256), 0, stream, cv.b, cv.gb, cv.mb, cv.vb,
+                       (long long)cv.cout, lrT, b1, b2, eps, (bf16*)nullptr);
+    DN_TRY(arbor_dn_flip_weights(cv.w16, cv.cin, cv.cout, cv.wflip, stream));
+  }
+  return arbor::launch_status();
+}
+
+int arbor_denoiser_loss(arbor_denoiser* d, float* host_loss) {
+  ARBOR_REQUIRE(d && host_loss);
+  if (!d->train) return ARBOR_ERR_STATE;
+  if (hipMemcpy(host_loss, d->loss, 4, hipMemcpyDeviceToHost) != hipSuccess) return ARBOR_ERR_DEVICE;
+  return ARBOR_OK;
+}
+
+// File: "ARBORDN1", int32 layer count, per layer int32 cin, cout, then fp32
+// weights [9][cout][cin] and biases [cout].
+int arbor_denoiser_save(const arbor_denoiser* d, const char* path) {
+  ARBOR_REQUIRE(d && path);
+  if (hipDeviceSynchronize() != hipSuccess) return ARBOR_ERR_DEVICE;
+  FILE* f = fopen(path, "wb");
+  if (!f) return ARBOR_ERR_ARG;
+  bool ok = fwrite("ARBORDN1", 1, 8, f) == 8;
+  int32_t layers = (int32_t)d->conv.size();
+  ok = ok && fwrite(&layers, 4, 1, f) == 1;
+  for (const Conv& cv : d->conv) {
+    int32_t dims[2] = {cv.cin, cv.cout};
+    std::vector<float> w((size_t)cv.nw()), b((size_t)cv.cout);
+    ok = ok && hipMemcpy(w.data(), cv.w, w.size() * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(b.data(), cv.b, b.size() * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    ok = ok && fwrite(dims, 4, 2, f) == 2 && fwrite(w.data(), 4, w.size(), f) == w.size() &&
+         fwrite(b.data(), 4, b.size(), f) == b.size();
+  }
+  ok = fclose(f) == 0 && ok;
+  return ok ? ARBOR_OK : ARBOR_ERR_DEVICE;
+}
+
+int arbor_denoiser_load(arbor_denoiser* d, const char* path) {
+  ARBOR_REQUIRE(d && path);
+  FILE* f = fopen(path, "rb");
+  if (!f) return ARBOR_ERR_ARG;
+  char magic[8];
+  int32_t layers = 0;
+  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "ARBORDN1", 8) == 0 && fread(&layers, 4, 1, f) == 1 &&
+            layers == (int32_t)d->conv.size();
+  for (size_t li = 0; ok && li < d->conv.size(); li++) {
+    Conv& cv = d->conv[li];
+    int32_t dims[2];
+    ok = fread(dims, 4, 2, f) == 2 && dims[0] == cv.cin && dims[1] == cv.cout;
+    std::vector<float> w((size_t)cv.nw()), b((size_t)cv.cout);
+    ok = ok && fread(w.data(), 4, w.size(), f) == w.size() && fread(b.data(), 4, b.size(), f) == b.size();
+    ok = ok && hipMemcpy(cv.w, w.data(), w.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(cv.b, b.data(), b.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+  }
+  fclose(f);
+  if (!ok) return ARBOR_ERR_ARG;
+  int rc = refreshBf16(d, nullptr);
+  if (rc == ARBOR_OK && hipDeviceSynchronize() != hipSuccess) rc = ARBOR_ERR_DEVICE;
+  return rc;
+}
+
+}  // extern "C"

@@ -225,7 +225,7 @@ using WalkShade = WalkCull;
 __global__ __launch_bounds__(kTileBlock) void k_shadePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
                                                             const float* C, const float4* Hit, float4* Linear,
                                                             float4* Guide, int width, int height, int tilesX,
-                                                            int tiles) {
+                                                            int tiles, float4* Normal) {
   __shared__ TreeInst trees[kTrees];
   fillTrees(trees, (int)C[31]);
   __syncthreads();
@@ -236,9 +236,11 @@ __global__ __launch_bounds__(kTileBlock) void k_shadePixels(const float* P, cons
     if (x >= width || y >= height) continue;
     int pixel = y * width + x;
     float4 l, g;
-    shadeOne<WalkShade>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel], l, g);
+    v3 n;
+    shadeOne<WalkShade>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel], l, g, Normal ? &n : nullptr);
     Linear[pixel] = l;
     Guide[pixel] = g;
+    if (Normal) Normal[pixel] = make_float4(n.x, n.y, n.z, 0.0f);
   }
 }
 
@@ -434,7 +436,20 @@ int arbor_shadePixels(const float* P, const float* Nodes, const unsigned* Order,
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
   hipLaunchKernelGGL(k_shadePixels, dim3(tileGrid(tiles, kWavesPerBlock)), dim3(kTileBlock), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
-                     reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles);
+                     reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
+                     nullptr);
+  return launch_status();
+}
+
+int arbor_shadePixelsAOV(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                         float* Linear, float* Guide, float* Normal, int width, int height, hipStream_t stream) {
+  ARBOR_REQUIRE(P && Nodes && Order && C && Hit && Linear && Guide && Normal);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  hipLaunchKernelGGL(k_shadePixels, dim3(tileGrid(tiles, kWavesPerBlock)), dim3(kTileBlock), 0, stream, P,
+                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
+                     reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
+                     reinterpret_cast<float4*>(Normal));
   return launch_status();
 }
 

@@ -49,6 +49,19 @@ SIGNATURES = {
     "arbor_dn_split": [_vp, _i, _i, ct.c_longlong, _vp, _vp, _vp],
     "arbor_dn_relu_bwd": [_vp, _vp, ct.c_longlong, _vp],
     "arbor_dn_add": [_vp, _vp, ct.c_longlong, _vp],
+    "arbor_shadePixelsAOV": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "arbor_denoiser_create": [ct.POINTER(_vp), _i, _i, _i, _i, _i],
+    "arbor_denoiser_destroy": [_vp],
+    "arbor_denoiser_parameters": [_vp],
+    "arbor_denoiser_set_input": [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "arbor_denoiser_set_target": [_vp, _i, _vp, _i, _i, _vp],
+    "arbor_denoiser_forward": [_vp, _i, _vp],
+    "arbor_denoiser_get_output": [_vp, _i, _vp, _i, _i, _vp],
+    "arbor_denoiser_train_step": [_vp, _i, ct.c_float, _vp],
+    "arbor_denoiser_loss": [_vp, ct.POINTER(ct.c_float)],
+    "arbor_denoiser_save": [_vp, ct.c_char_p],
+    "arbor_denoiser_load": [_vp, ct.c_char_p],
+    "arbor_denoiser_tensor": [_vp, ct.c_char_p],
     "arbor_engine_render": [_vp, _i, _i, _i],
     "arbor_engine_save_ppm": [_vp, ct.c_char_p],
     "arbor_write_ppm": [ct.c_char_p, ct.POINTER(ct.c_uint), _i, _i],
@@ -98,7 +111,8 @@ SIGNATURES = {
 }
 _RESTYPES = {"arbor_version": ct.c_char_p, "arbor_engine_round_size": None, "arbor_engine_buffer": _vp,
              "arbor_engine_stream": _vp, "arbor_batch_buffer": _vp, "arbor_batch_pixels": ct.c_longlong,
-             "arbor_tile_schedule_bytes": ct.c_longlong, "arbor_dn_wgrad_partial_floats": ct.c_longlong}
+             "arbor_tile_schedule_bytes": ct.c_longlong, "arbor_dn_wgrad_partial_floats": ct.c_longlong,
+             "arbor_denoiser_parameters": ct.c_longlong, "arbor_denoiser_tensor": _vp}
 BATCH_TRACE_SHADE, BATCH_FILTER, BATCH_RESOLVE, BATCH_ALL = 1, 2, 4, 7
 
 
@@ -367,6 +381,74 @@ def write_ppm(path, pixels, width, height):
     px = np.ascontiguousarray(pixels, dtype=np.uint32)
     check(lib().arbor_write_ppm(str(path).encode(), px.ctypes.data_as(ct.POINTER(ct.c_uint)), width, height),
           "arbor_write_ppm")
+
+
+def shade_aov(scene, C, fb, normal):
+    call("arbor_shadePixelsAOV", scene.P, scene.Nodes, scene.Order, C, fb.Hit, fb.Linear, fb.Guide, normal,
+         fb.width, fb.height)
+
+
+class Denoiser:
+    """The learned denoiser's network runner (include/arbor_denoise.h)."""
+
+    def __init__(self, max_batch, height, width, train=True, seed=1):
+        h = ct.c_void_p()
+        check(lib().arbor_denoiser_create(ct.byref(h), max_batch, height, width, int(train), seed),
+              "arbor_denoiser_create")
+        self.h, self.max_batch, self.height, self.width, self.train = h, max_batch, height, width, train
+
+    def close(self):
+        if self.h:
+            check(lib().arbor_denoiser_destroy(self.h), "arbor_denoiser_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def parameters(self):
+        return lib().arbor_denoiser_parameters(self.h)
+
+    def set_input(self, index, linear, guide, normal, hit, w, h):
+        call("arbor_denoiser_set_input", self.h, index, linear, guide, normal, hit, w, h)
+
+    def set_target(self, index, radiance, w, h):
+        call("arbor_denoiser_set_target", self.h, index, radiance, w, h)
+
+    def forward(self, n):
+        call("arbor_denoiser_forward", self.h, n)
+
+    def output(self, index, out, w, h):
+        call("arbor_denoiser_get_output", self.h, index, out, w, h)
+
+    def train_step(self, n, lr):
+        call("arbor_denoiser_train_step", self.h, n, ct.c_float(lr))
+
+    def loss(self):
+        v = ct.c_float()
+        check(lib().arbor_denoiser_loss(self.h, ct.byref(v)), "arbor_denoiser_loss")
+        return v.value
+
+    def save(self, path):
+        check(lib().arbor_denoiser_save(self.h, str(path).encode()), "arbor_denoiser_save")
+
+    def load(self, path):
+        check(lib().arbor_denoiser_load(self.h, str(path).encode()), "arbor_denoiser_load")
+
+    def tensor(self, name, count, dtype):
+        """Host copy of `count` elements of a named device tensor."""
+        import torch
+
+        ptr = lib().arbor_denoiser_tensor(self.h, name.encode())
+        if not ptr:
+            raise ArborError("no denoiser tensor " + name)
+        nbytes = count * np.dtype(dtype).itemsize
+        torch.cuda.synchronize()
+        host = torch.empty(nbytes, dtype=torch.uint8)
+        host.copy_(_wrap_device(ptr, nbytes))
+        return host.numpy().view(dtype).copy()
 
 
 def walk_input(seed, frame):

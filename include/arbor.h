@@ -121,6 +121,13 @@ int arbor_buildBVH(const float* P, unsigned* Keys, unsigned* Order, float* Nodes
 int arbor_generate(float* B, float* P, unsigned* Keys, unsigned* Order, float* Nodes, float* C, int seed,
                    float* History, int width, int height, hipStream_t stream);
 
+/* shadePixels plus the shading normal per pixel (no counterpart in the
+ * reference; an auxiliary output for the learned denoiser): Normal[4p..] =
+ * {n, 0}, the normal shadePixels lights with, facing the camera; zero for sky.
+ * Linear and Guide are identical to arbor_shadePixels'. */
+int arbor_shadePixelsAOV(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                         float* Linear, float* Guide, float* Normal, int width, int height, hipStream_t stream);
+
 /* tracePixels with a tile schedule kept across frames (no counterpart in the
  * reference; for callers that trace a moving view frame after frame). The
  * schedule is arbor_tile_schedule_bytes(width, height) bytes of device memory

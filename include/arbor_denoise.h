@@ -52,6 +52,47 @@ int arbor_dn_split(const uint16_t* Y, int ca, int cb, long long pixels, uint16_t
 int arbor_dn_relu_bwd(uint16_t* G, const uint16_t* Yact, long long count, hipStream_t stream);
 int arbor_dn_add(uint16_t* A, const uint16_t* B, long long count, hipStream_t stream);
 
+/* ---- the network ----
+ * 5-level U-Net, 3x3 convolutions with 32/48/64/96/128 channels (ReLU), 2x2
+ * average pooling, nearest upsampling and skip concatenation, a linear 3x3
+ * output layer: 1,021,904 parameters stored (1,016,419 excluding the zero
+ * padding of the input and output channels). Input per pixel: log(1 + radiance) rgb
+ * of one sample, albedo rgb (Guide), shading normal (arbor_shadePixelsAOV)
+ * and depth log(1 + min(t, 1000))/7 (1 for sky); output log(1 + radiance).
+ * The runner holds a batch of max_batch slots of height x width (multiples
+ * of 16); images smaller than a slot are zero-padded. */
+typedef struct arbor_denoiser arbor_denoiser;
+/* train != 0 also allocates gradients, Adam state and targets. Weights are
+ * He-initialised from seed. */
+int arbor_denoiser_create(arbor_denoiser** out, int max_batch, int height, int width, int train, int seed);
+int arbor_denoiser_destroy(arbor_denoiser* d);
+long long arbor_denoiser_parameters(const arbor_denoiser* d);
+/* Input features of slot `index` from one frame's buffers (float4 per pixel,
+ * w x h): Linear, Guide, Normal, Hit. */
+int arbor_denoiser_set_input(arbor_denoiser* d, int index, const float* Linear, const float* Guide,
+                             const float* Normal, const float* Hit, int w, int h, hipStream_t stream);
+/* Training target of slot `index`: radiance float4 per pixel (w x h). */
+int arbor_denoiser_set_target(arbor_denoiser* d, int index, const float* Radiance, int w, int h, hipStream_t stream);
+/* Forward pass over slots 0..n-1. */
+int arbor_denoiser_forward(arbor_denoiser* d, int n, hipStream_t stream);
+/* Radiance of slot `index` (float4, alpha 1, w x h): a drop-in for Filtered
+ * ahead of resolvePixels. */
+int arbor_denoiser_get_output(arbor_denoiser* d, int index, float* Out, int w, int h, hipStream_t stream);
+/* Forward, L1 loss on log(1 + radiance) against the targets, backward, one
+ * Adam step (beta 0.9 / 0.999) over slots 0..n-1. */
+int arbor_denoiser_train_step(arbor_denoiser* d, int n, float lr, hipStream_t stream);
+/* Mean L1 loss of the last train step (synchronises). */
+int arbor_denoiser_loss(arbor_denoiser* d, float* host_loss);
+/* Weights as "ARBORDN1", layer count, then per layer cin, cout (int32) and
+ * fp32 weights [9][cout][cin] and biases [cout]. */
+int arbor_denoiser_save(const arbor_denoiser* d, const char* path);
+int arbor_denoiser_load(arbor_denoiser* d, const char* path);
+/* Device pointer of "input" (bf16 NHWC, 16 channels), "output" (bf16 NHWC,
+ * 16 channels, 3 used), "target" (float4: log(1 + radiance), mask), or of
+ * layer i's fp32 "weight:i" / "bias:i" and, after a train step, their
+ * gradients "wgrad:i" / "bgrad:i". */
+void* arbor_denoiser_tensor(arbor_denoiser* d, const char* name);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,175 @@
+"""GPU checks of the denoiser network runner (include/arbor_denoise.h) against
+a plain PyTorch fp32 restatement of the same graph (CPU), fed the runner's own
+bf16 input tensor and weights, with each layer's output rounded to bf16 as the
+runner stores it:
+
+* forward: relative L2 error of the output <= 2e-2;
+* one training step: every layer's weight and bias gradient against autograd
+  of the same L1 loss (the runner keeps bf16 gradients between layers):
+  cosine >= 0.99 and relative norm error <= 0.1;
+* training reduces the loss on a small fixed problem; save/load round trip
+  reproduces the outputs bit for bit.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import require_gpu
+
+pytestmark = pytest.mark.gpu
+
+# (cin, cout, level, relu) of L0..L18, as the runner builds them
+LAYERS = [(16, 32, 0, 1), (32, 32, 0, 1), (32, 48, 1, 1), (48, 48, 1, 1), (48, 64, 2, 1), (64, 64, 2, 1),
+          (64, 96, 3, 1), (96, 96, 3, 1), (96, 128, 4, 1), (128, 128, 4, 1), (224, 96, 3, 1), (96, 96, 3, 1),
+          (160, 64, 2, 1), (64, 64, 2, 1), (112, 48, 1, 1), (48, 48, 1, 1), (80, 32, 0, 1), (32, 32, 0, 1),
+          (32, 16, 0, 0)]
+N, H, W, w_img, h_img = 2, 64, 96, 90, 61
+
+
+@pytest.fixture(scope="module")
+def env():
+    torch = require_gpu()
+    import pyarbor
+
+    return torch, pyarbor
+
+
+def _frames(torch, seed):
+    g = torch.Generator().manual_seed(seed)
+    n = w_img * h_img
+    lin = torch.rand(N, n, 4, generator=g) * 3.0
+    gui = torch.rand(N, n, 4, generator=g)
+    nor = torch.nn.functional.normalize(torch.randn(N, n, 4, generator=g)[..., :3], dim=-1)
+    nor = torch.cat([nor, torch.zeros(N, n, 1)], -1)
+    hit = torch.rand(N, n, 4, generator=g) * 60.0
+    hit[:, ::7, 0] = 1e5  # sky
+    # target: a smoothed version of the radiance
+    img = lin.reshape(N, h_img, w_img, 4).permute(0, 3, 1, 2)
+    tgt = torch.nn.functional.avg_pool2d(img, 5, 1, 2, count_include_pad=False).permute(0, 2, 3, 1).reshape(N, n, 4)
+    return [t.contiguous().cuda() for t in (lin, gui, nor, hit, tgt)]
+
+
+def _load(dn, torch):
+    Ws, Bs = [], []
+    for i, (ci, co, _, _) in enumerate(LAYERS):
+        w = torch.from_numpy(dn.tensor("weight:%d" % i, 9 * co * ci, np.float32))
+        b = torch.from_numpy(dn.tensor("bias:%d" % i, co, np.float32))
+        Ws.append(w.reshape(3, 3, co, ci).permute(2, 3, 0, 1).contiguous())
+        Bs.append(b)
+    return Ws, Bs
+
+
+def _ref_forward(torch, x, Ws, Bs):
+    F = torch.nn.functional
+
+    def bfr(t):
+        return t.to(torch.bfloat16).float()
+
+    def conv(i, t):
+        y = F.conv2d(t, bfr(Ws[i]), Bs[i], padding=1)
+        return bfr(F.relu(y) if LAYERS[i][3] else y)
+
+    def up(t):
+        return t.repeat_interleave(2, 2).repeat_interleave(2, 3)
+
+    skips, t = [], x
+    for k in range(5):
+        a = conv(2 * k, t)
+        s = conv(2 * k + 1, a)
+        if k < 4:
+            skips.append(s)
+            t = bfr(F.avg_pool2d(s, 2))
+        else:
+            t = s
+    for j, k in enumerate(range(3, -1, -1)):
+        c = torch.cat([up(t), skips[k]], 1)
+        t = conv(11 + 2 * j, conv(10 + 2 * j, c))
+    return conv(18, t)
+
+
+def _input(dn, torch):
+    x = torch.from_numpy(dn.tensor("input", N * H * W * 16, np.uint16).astype(np.int16).view(np.int16))
+    return x.view(torch.bfloat16).float().reshape(N, H, W, 16).permute(0, 3, 1, 2)
+
+
+def _output(dn, torch):
+    y = torch.from_numpy(dn.tensor("output", N * H * W * 16, np.uint16).astype(np.int16).view(np.int16))
+    return y.view(torch.bfloat16).float().reshape(N, H, W, 16).permute(0, 3, 1, 2)
+
+
+def test_forward_and_gradients_match_torch(env):
+    torch, px = env
+    print("case: denoiser forward + gradients, batch 2, 64x96 slots", flush=True)
+    dn = px.Denoiser(N, H, W, train=True, seed=3)
+    assert dn.parameters() == 1021904
+    lin, gui, nor, hit, tgt = _frames(torch, 1)
+    for i in range(N):
+        dn.set_input(i, lin[i], gui[i], nor[i], hit[i], w_img, h_img)
+        dn.set_target(i, tgt[i], w_img, h_img)
+    dn.forward(N)
+    x = _input(dn, torch)
+    assert x[:, 10:].abs().max() == 0 and x[:, :, h_img:].abs().max() == 0 and x[:, :, :, w_img:].abs().max() == 0
+    Ws, Bs = _load(dn, torch)
+    ref = _ref_forward(torch, x, Ws, Bs)
+    got = _output(dn, torch)
+    rel = ((got[:, :3] - ref[:, :3]).norm() / ref[:, :3].norm()).item()
+    print("forward relative L2 error %.3g" % rel)
+    assert rel <= 2e-2
+    assert got[:, 3:].abs().max() == 0
+    # gradients of one step (taken before the step's weight update)
+    Ws_g = [w.clone().requires_grad_(True) for w in Ws]
+    Bs_g = [b.clone().requires_grad_(True) for b in Bs]
+    out = _ref_forward(torch, x, Ws_g, Bs_g)
+    t = torch.from_numpy(dn.tensor("target", N * H * W * 4, np.float32)).reshape(N, H, W, 4).permute(0, 3, 1, 2)
+    loss = ((out[:, :3] - t[:, :3]).abs() * t[:, 3:4]).sum() / (N * H * W * 3)
+    loss.backward()
+    dn.train_step(N, 1e-3)
+    print("loss runner %.5f torch %.5f" % (dn.loss(), loss.item()))
+    assert abs(dn.loss() - loss.item()) <= 0.02 * loss.item()
+    for i, (ci, co, _, _) in enumerate(LAYERS):
+        gw = torch.from_numpy(dn.tensor("wgrad:%d" % i, 9 * co * ci, np.float32)).reshape(3, 3, co, ci).permute(
+            2, 3, 0, 1)
+        gb = torch.from_numpy(dn.tensor("bgrad:%d" % i, co, np.float32))
+        for name, a, b in (("w", gw, Ws_g[i].grad), ("b", gb, Bs_g[i].grad)):
+            nb = b.norm().item()
+            if nb == 0:
+                assert a.norm().item() == 0
+                continue
+            cos = (a * b).sum().item() / (a.norm().item() * nb)
+            relerr = (a - b).norm().item() / nb
+            print("L%d %s grad cos %.5f relerr %.4f" % (i, name, cos, relerr))
+            assert cos >= 0.99 and relerr <= 0.1, (i, name)
+    dn.close()
+
+
+def test_training_reduces_loss_and_checkpoint_roundtrip(env, tmp_path):
+    torch, px = env
+    print("case: denoiser training 60 steps + save/load", flush=True)
+    dn = px.Denoiser(N, H, W, train=True, seed=5)
+    lin, gui, nor, hit, tgt = _frames(torch, 2)
+    for i in range(N):
+        dn.set_input(i, lin[i], gui[i], nor[i], hit[i], w_img, h_img)
+        dn.set_target(i, tgt[i], w_img, h_img)
+    losses = []
+    for step in range(60):
+        dn.train_step(N, 2e-3)
+        losses.append(dn.loss())
+    print("loss", ["%.4f" % v for v in losses[::10]], "final %.4f" % losses[-1])
+    assert losses[-1] < 0.5 * losses[0]
+    path = tmp_path / "dn.bin"
+    dn.save(path)
+    dn.forward(N)
+    want = _output(dn, torch)
+    inf = px.Denoiser(N, H, W, train=False, seed=99)
+    inf.load(path)
+    for i in range(N):
+        inf.set_input(i, lin[i], gui[i], nor[i], hit[i], w_img, h_img)
+    inf.forward(N)
+    assert torch.equal(_output(inf, torch), want)
+    out = torch.zeros(w_img * h_img * 4, dtype=torch.float32, device="cuda")
+    inf.output(1, out, w_img, h_img)
+    o = out.cpu().reshape(-1, 4)
+    assert torch.allclose(o[:, :3], torch.expm1(want[1, :3, :h_img, :w_img].permute(1, 2, 0).reshape(-1, 3)).clamp(
+        min=0), rtol=1e-6, atol=1e-6)
+    assert (o[:, 3] == 1).all()
+    inf.close()
+    dn.close()
