@@ -4,8 +4,8 @@
 // Tensors are NHWC bf16 with the channel count a multiple of 8, so a lane's
 // 8 consecutive channels are one 16-byte load. A 3x3 convolution is the
 // implicit GEMM Y[p, co] = sum_{tap, ci} X[p + off(tap), ci] * W[tap, co, ci]
-// (zero outside the image), K = 9*Cin walked in k-groups of 8 channels of one
-// tap, four k-groups per v_mfma_f32_16x16x32_bf16. Operand layout of that
+// (zero outside the image): per tap and 32-channel chunk one
+// v_mfma_f32_16x16x32_bf16 per 16x16 output subtile. Operand layout of that
 // instruction (checked on the device): A row = lane % 16, B column = lane % 16,
 // both with k = 8*(lane/16) + j for their 8 bf16; D column = lane % 16, rows
 // 4*(lane/16) + i for its 4 floats.
@@ -37,74 +37,82 @@ __device__ __forceinline__ bf16x8 zero8() {
 }
 
 // ---- 3x3 convolution, stride 1, zero padding 1 ----
-// Block: 4 waves; wave w computes 16*MSUB pixels x 16*NSUB output channels,
-// the channels [co0, co0 + 16*NSUB) of a CoutTotal-channel output.
-template <int NSUB, int MSUB>
+// A block computes an 8x16-pixel output tile of one image for the output
+// channels [co0, co0 + 16*NSUB) of a CoutTotal-channel output. The input
+// tile with its 1-pixel halo (10x18 pixels) is staged in LDS 32 channels at a
+// time, so the nine taps read their A fragments from LDS: each input value
+// leaves HBM/L2 about 1.3 times instead of nine. Wave w owns output rows 2w
+// and 2w+1 (one 16-pixel M-subtile each); B fragments (weights) come from
+// global memory, shared by every block through the caches.
+constexpr int kTileH = 8, kTileW = 16, kHaloH = kTileH + 2, kHaloW = kTileW + 2, kChunk = 32;
+constexpr int kLdsStride = kChunk + 8;  // bf16 per staged pixel (80 bytes: 16-byte aligned, staggered banks)
+template <int NSUB>
 __global__ __launch_bounds__(256) void k_conv3x3(const __bf16* __restrict__ X, int N, int H, int W, int Cin,
                                                  const __bf16* __restrict__ Wt, const float* __restrict__ bias,
                                                  int relu, __bf16* __restrict__ Y, int CoutTotal, int co0) {
+  __shared__ __attribute__((aligned(16))) __bf16 tile[kHaloH * kHaloW * kLdsStride];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int M = N * H * W, HW = H * W;
-  const int pbase = (blockIdx.x * 4 + wave) * (16 * MSUB);
-  if (pbase >= M) return;
   const int row = lane & 15, g = lane >> 4;
-  int pn[MSUB], py[MSUB], px[MSUB];
-  bool pv[MSUB];
+  const int tilesX = (W + kTileW - 1) / kTileW, tilesY = (H + kTileH - 1) / kTileH;
+  int b = blockIdx.x;
+  const int tx = b % tilesX;
+  b /= tilesX;
+  const int ty = b % tilesY, n = b / tilesY;
+  const int x0 = tx * kTileW, y0 = ty * kTileH;
+  f32x4 acc[2][NSUB];
 #pragma unroll
-  for (int m = 0; m < MSUB; m++) {
-    int p = pbase + m * 16 + row;
-    pv[m] = p < M;
-    int q = pv[m] ? p : 0;
-    pn[m] = q / HW;
-    int r = q - pn[m] * HW;
-    py[m] = r / W;
-    px[m] = r - py[m] * W;
-  }
-  f32x4 acc[MSUB][NSUB];
+  for (int m = 0; m < 2; m++)
 #pragma unroll
-  for (int m = 0; m < MSUB; m++)
-#pragma unroll
-    for (int n = 0; n < NSUB; n++) acc[m][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  const int G = Cin >> 3;  // k-groups per tap
-  const int groups = 9 * G;
-  // this lane's k-group, advanced by 4 per step as (tap, channel group)
-  int tap = g / G, cg = g - tap * G;
-  for (int kg0 = 0; kg0 < groups; kg0 += 4) {
-    const bool live = kg0 + g < groups;
-    const int ci = cg << 3;
-    const int dy = tap / 3 - 1, dx = tap - (tap / 3) * 3 - 1;
-    bf16x8 a[MSUB], b[NSUB];
-#pragma unroll
-    for (int m = 0; m < MSUB; m++) {
-      const int yy = py[m] + dy, xx = px[m] + dx;
-      const bool in = live && pv[m] && yy >= 0 && yy < H && xx >= 0 && xx < W;
-      a[m] = in ? *reinterpret_cast<const bf16x8*>(X + ((size_t)(pn[m] * H + yy) * W + xx) * Cin + ci) : zero8();
+    for (int j = 0; j < NSUB; j++) acc[m][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int c0 = 0; c0 < Cin; c0 += kChunk) {
+    // stage (10 x 18 pixels) x 32 channels: 720 16-byte groups
+    for (int i = threadIdx.x; i < kHaloH * kHaloW * 4; i += 256) {
+      const int pix = i >> 2, grp = i & 3;
+      const int yy = y0 - 1 + pix / kHaloW, xx = x0 - 1 + pix % kHaloW, ci = c0 + grp * 8;
+      bf16x8 v = zero8();
+      if (yy >= 0 && yy < H && xx >= 0 && xx < W && ci < Cin)
+        v = *reinterpret_cast<const bf16x8*>(X + ((size_t)(n * H + yy) * W + xx) * Cin + ci);
+      *reinterpret_cast<bf16x8*>(&tile[pix * kLdsStride + grp * 8]) = v;
     }
+    __syncthreads();
+    const int ci = c0 + g * 8;
+    const bool chan = ci < Cin;
+#pragma unroll 3
+    for (int tap = 0; tap < 9; tap++) {
+      const int dy = tap / 3, dx = tap - dy * 3;
+      bf16x8 a[2], w8[NSUB];
 #pragma unroll
-    for (int n = 0; n < NSUB; n++)
-      b[n] = live ? *reinterpret_cast<const bf16x8*>(Wt + ((size_t)tap * CoutTotal + co0 + n * 16 + row) * Cin + ci)
-                  : zero8();
+      for (int m = 0; m < 2; m++)
+        a[m] = *reinterpret_cast<const bf16x8*>(&tile[((2 * wave + m + dy) * kHaloW + row + dx) * kLdsStride + g * 8]);
 #pragma unroll
-    for (int m = 0; m < MSUB; m++)
+      for (int j = 0; j < NSUB; j++)
+        w8[j] = chan ? *reinterpret_cast<const bf16x8*>(Wt + ((size_t)tap * CoutTotal + co0 + j * 16 + row) * Cin + ci)
+                     : zero8();
 #pragma unroll
-      for (int n = 0; n < NSUB; n++) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[n], acc[m][n], 0, 0, 0);
-    cg += 4;
-    while (cg >= G) { cg -= G; tap++; }
+      for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int j = 0; j < NSUB; j++) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], w8[j], acc[m][j], 0, 0, 0);
+    }
+    __syncthreads();
   }
 #pragma unroll
-  for (int m = 0; m < MSUB; m++)
+  for (int m = 0; m < 2; m++) {
+    const int y = y0 + 2 * wave + m;
+    if (y >= H) continue;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int p = pbase + m * 16 + 4 * g + i;
-      if (p >= M) continue;
+      const int x = x0 + 4 * g + i;
+      if (x >= W) continue;
+      const size_t p = (size_t)(n * H + y) * W + x;
 #pragma unroll
-      for (int n = 0; n < NSUB; n++) {
-        const int co = co0 + n * 16 + row;
-        float v = acc[m][n][i] + (bias ? bias[co] : 0.0f);
+      for (int j = 0; j < NSUB; j++) {
+        const int co = co0 + j * 16 + row;
+        float v = acc[m][j][i] + (bias ? bias[co] : 0.0f);
         if (relu) v = fmaxf(v, 0.0f);
-        Y[(size_t)p * CoutTotal + co] = f2bf(v);
+        Y[p * CoutTotal + co] = f2bf(v);
       }
     }
+  }
 }
 
 // ---- weight and bias gradients ----
@@ -286,10 +294,8 @@ namespace {
 template <int NSUB>
 void launchConv(const __bf16* X, int N, int H, int W, int Cin, const __bf16* Wt, const float* bias, int relu, __bf16* Y,
                 int coutTotal, int co0, hipStream_t s) {
-  constexpr int MSUB = 2;
-  const long long M = (long long)N * H * W;
-  const int blocks = (int)((M + 4 * 16 * MSUB - 1) / (4 * 16 * MSUB));
-  hipLaunchKernelGGL((k_conv3x3<NSUB, MSUB>), dim3(blocks), dim3(256), 0, s, X, N, H, W, Cin, Wt, bias, relu, Y,
+  const long long blocks = (long long)N * ((H + kTileH - 1) / kTileH) * ((W + kTileW - 1) / kTileW);
+  hipLaunchKernelGGL((k_conv3x3<NSUB>), dim3((unsigned)blocks), dim3(256), 0, s, X, N, H, W, Cin, Wt, bias, relu, Y,
                      coutTotal, co0);
 }
 bool dnShapeOk(int n, int h, int w, int c) { return n >= 1 && h >= 1 && w >= 1 && c >= 8 && (c & 7) == 0; }

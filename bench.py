@@ -56,7 +56,95 @@ def parse():
                     help="batched-views workload instead: N seeded walk views of uneven resolution rendered by one "
                          "hipGraph-captured batch per step (about 0.3 s per step at N=64; use --steps 10)")
     ap.add_argument("--views-seed", type=int, default=3)
+    ap.add_argument("--denoiser", nargs="?", const=DENOISER_WEIGHTS, default="", metavar="CHECKPOINT",
+                    help="learned-denoiser workload instead: held-out 1920x1080 frames (sun visibility on), "
+                         "1-sample inputs against 64-sample targets; filterLighting vs the network")
     return ap.parse_args()
+
+
+DENOISER_WEIGHTS = os.path.join(HERE, "arbor-cuda_amd", "weights", "denoiser_forest.bin")
+# BF16 dense MFMA peak of MI355X (TFLOP/s).
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def denoiser_flops(height, width):
+    """Algorithmic FLOPs of one forward pass at height x width (multiples of 16):
+    2 * 9 * cin * cout per output pixel of every convolution, counting the 10
+    used input and 3 used output channels."""
+    layers = [(10, 32, 0), (32, 32, 0), (32, 48, 1), (48, 48, 1), (48, 64, 2), (64, 64, 2), (64, 96, 3), (96, 96, 3),
+              (96, 128, 4), (128, 128, 4), (224, 96, 3), (96, 96, 3), (160, 64, 2), (64, 64, 2), (112, 48, 1),
+              (48, 48, 1), (80, 32, 0), (32, 32, 0), (32, 3, 0)]
+    return sum(2.0 * 9 * ci * co * (height >> lv) * (width >> lv) for ci, co, lv in layers)
+
+
+def bench_denoiser(args):
+    """Learned denoiser vs filterLighting on held-out frames: quality against
+    the 64-sample image (PSNR, SSIM after resolvePixels' display mapping) and
+    time per 1920x1080 frame (HIP events); the network's FLOP rate against the
+    BF16 MFMA peak."""
+    import numpy as np
+    import torch
+
+    import dn_train
+    import pyarbor as px
+
+    W, H, spp = 1920, 1080, 64
+    scene = px.DeviceScene().generate(args.seed)
+    inf = px.Denoiser(1, 1088, 1920, train=False, seed=0)
+    inf.load(args.denoiser)
+    eng = px.Engine(W, H, args.seed)
+    eng.set_input(px.walk_input(args.seed, 0))
+    eng.frame(1.0 / 60.0)
+    cams = [("walk frame 0", eng.download("C", 64))]
+    eng.close()
+    vc, _ = px.batch_walk_views(11 + 777777, args.seed, 2)
+    cams += [("held-out view %d" % i, vc[i]) for i in range(2)]
+    rows = []
+    for name, cam in cams:
+        cam = cam.copy()
+        cam[22] = 1.0
+        fb, normal, Cd = dn_train.render_inputs(px, torch, scene, cam, W, H)
+        target = dn_train.render_targets_batched(px, torch, scene, cam[None, :], W, H, spp)[0]
+        out = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+
+        def net():
+            inf.set_input(0, fb.Linear, fb.Guide, normal, fb.Hit, W, H)
+            inf.forward(1)
+            inf.output(0, out, W, H)
+
+        net()
+        ref = dn_train.display(px, torch, target, Cd, W, H)
+        row = {"camera": name}
+        for k, buf in (("1 spp", fb.Linear), ("filterLighting", fb.Filtered), ("learned", out)):
+            img = dn_train.display(px, torch, buf, Cd, W, H)
+            row[k] = {"psnr": round(dn_train.psnr(img, ref), 3), "ssim": round(dn_train.ssim(img, ref), 4)}
+        row["filter_ms"] = round(dn_train.event_ms(torch, lambda: px.filter_lighting(Cd, fb)), 4)
+        row["denoiser_ms"] = round(dn_train.event_ms(torch, net, reps=10), 4)
+        row["forward_ms"] = round(dn_train.event_ms(torch, lambda: inf.forward(1), reps=10), 4)
+        rows.append(row)
+    fwd = float(np.mean([r["forward_ms"] for r in rows]))
+    den = float(np.mean([r["denoiser_ms"] for r in rows]))
+    flt = float(np.mean([r["filter_ms"] for r in rows]))
+    flops = denoiser_flops(1088, 1920)
+    achieved = flops / (fwd * 1e-3) / 1e12
+    result = {
+        "metric": "learned vs filter denoise ms at equal PSNR",
+        "value": round(den, 4),
+        "unit": "ms/frame (learned denoiser, 1920x1080)",
+        "n_gpus": 1, "steps": len(rows), "warmup": 1, "ms_per_step": round(den, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic (tracer-rendered held-out frames of the seed-7 forest, sun visibility on)",
+        "config": {"workload": "learned denoiser (5-level U-Net, 32/48/64/96/128 channels, BF16 MFMA) vs "
+                               "filterLighting on 1-sample 1920x1080 frames, against 64-sample targets",
+                   "checkpoint": os.path.basename(args.denoiser)},
+        "filter_ms": round(flt, 4),
+        "learned_vs_filter_time": round(den / flt, 2),
+        "quality": rows,
+        "roofline": {"bound": "mfma", "kernel": "denoiser forward (19 convolutions + resampling)",
+                     "flop_per_frame": flops, "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": None},
+    }
+    print(json.dumps(result), flush=True)
 
 
 def bench_views(args):
@@ -213,6 +301,9 @@ def main():
         return
     if args.views:
         bench_views(args)
+        return
+    if args.denoiser:
+        bench_denoiser(args)
         return
     import numpy as np
     import torch
