@@ -37,19 +37,27 @@ __device__ __forceinline__ bf16x8 zero8() {
 }
 
 // ---- 3x3 convolution, stride 1, zero padding 1 ----
-// A block computes an 8x16-pixel output tile of one image for the output
-// channels [co0, co0 + 16*NSUB) of a CoutTotal-channel output. The input
-// tile with its 1-pixel halo (10x18 pixels) is staged in LDS 32 channels at a
-// time, so the nine taps read their A fragments from LDS: each input value
-// leaves HBM/L2 about 1.3 times instead of nine. Wave w owns output rows 2w
-// and 2w+1 (one 16-pixel M-subtile each); B fragments (weights) come from
-// global memory, shared by every block through the caches.
-constexpr int kTileH = 8, kTileW = 16, kHaloH = kTileH + 2, kHaloW = kTileW + 2, kChunk = 32;
+// A block computes a (4*MSUB)x16-pixel output tile of one image for the
+// output channels [co0, co0 + 16*NSUB) of a CoutTotal-channel output. The
+// input tile with its 1-pixel halo is staged in LDS 32 channels at a time, so
+// the nine taps read their A fragments from LDS: each input value leaves
+// HBM/L2 about 1.3 times instead of nine. Wave w owns output rows
+// MSUB*w .. MSUB*w + MSUB-1 (one 16-pixel M-subtile each); B fragments
+// (weights) come from global memory, shared by every block through the
+// caches, and each feeds MSUB matrix instructions.
+//
+// With Xc non-NULL the input is the channel concatenation [up2(Xc), X] of a
+// half-resolution tensor Xc (Cc channels, read at (y/2, x/2)) and X (Cin - Cc
+// channels): the decoder's upsampling and skip concatenation happen in the
+// staging, never in memory.
+constexpr int kTileW = 16, kHaloW = kTileW + 2, kChunk = 32;
 constexpr int kLdsStride = kChunk + 8;  // bf16 per staged pixel (80 bytes: 16-byte aligned, staggered banks)
-template <int NSUB>
-__global__ __launch_bounds__(256) void k_conv3x3(const __bf16* __restrict__ X, int N, int H, int W, int Cin,
-                                                 const __bf16* __restrict__ Wt, const float* __restrict__ bias,
-                                                 int relu, __bf16* __restrict__ Y, int CoutTotal, int co0) {
+template <int NSUB, int MSUB>
+__global__ __launch_bounds__(256) void k_conv3x3(const __bf16* __restrict__ X, const __bf16* __restrict__ Xc, int Cc,
+                                                 int N, int H, int W, int Cin, const __bf16* __restrict__ Wt,
+                                                 const float* __restrict__ bias, int relu, __bf16* __restrict__ Y,
+                                                 int CoutTotal, int co0) {
+  constexpr int kTileH = 4 * MSUB, kHaloH = kTileH + 2;
   __shared__ __attribute__((aligned(16))) __bf16 tile[kHaloH * kHaloW * kLdsStride];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = lane & 15, g = lane >> 4;
@@ -59,45 +67,49 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __bf16* __restrict__ X, i
   b /= tilesX;
   const int ty = b % tilesY, n = b / tilesY;
   const int x0 = tx * kTileW, y0 = ty * kTileH;
-  f32x4 acc[2][NSUB];
+  const int Cs = Cin - Cc;  // channels read from X
+  f32x4 acc[MSUB][NSUB];
 #pragma unroll
-  for (int m = 0; m < 2; m++)
+  for (int m = 0; m < MSUB; m++)
 #pragma unroll
     for (int j = 0; j < NSUB; j++) acc[m][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   for (int c0 = 0; c0 < Cin; c0 += kChunk) {
-    // stage (10 x 18 pixels) x 32 channels: 720 16-byte groups
+    // stage (kHaloH x 18 pixels) x 32 channels as 16-byte groups
     for (int i = threadIdx.x; i < kHaloH * kHaloW * 4; i += 256) {
       const int pix = i >> 2, grp = i & 3;
       const int yy = y0 - 1 + pix / kHaloW, xx = x0 - 1 + pix % kHaloW, ci = c0 + grp * 8;
       bf16x8 v = zero8();
-      if (yy >= 0 && yy < H && xx >= 0 && xx < W && ci < Cin)
-        v = *reinterpret_cast<const bf16x8*>(X + ((size_t)(n * H + yy) * W + xx) * Cin + ci);
+      if (yy >= 0 && yy < H && xx >= 0 && xx < W && ci < Cin) {
+        if (ci < Cc)
+          v = *reinterpret_cast<const bf16x8*>(Xc + ((size_t)(n * (H >> 1) + (yy >> 1)) * (W >> 1) + (xx >> 1)) * Cc + ci);
+        else
+          v = *reinterpret_cast<const bf16x8*>(X + ((size_t)(n * H + yy) * W + xx) * Cs + ci - Cc);
+      }
       *reinterpret_cast<bf16x8*>(&tile[pix * kLdsStride + grp * 8]) = v;
     }
     __syncthreads();
     const int ci = c0 + g * 8;
     const bool chan = ci < Cin;
-#pragma unroll 3
     for (int tap = 0; tap < 9; tap++) {
       const int dy = tap / 3, dx = tap - dy * 3;
-      bf16x8 a[2], w8[NSUB];
-#pragma unroll
-      for (int m = 0; m < 2; m++)
-        a[m] = *reinterpret_cast<const bf16x8*>(&tile[((2 * wave + m + dy) * kHaloW + row + dx) * kLdsStride + g * 8]);
+      bf16x8 w8[NSUB];
 #pragma unroll
       for (int j = 0; j < NSUB; j++)
         w8[j] = chan ? *reinterpret_cast<const bf16x8*>(Wt + ((size_t)tap * CoutTotal + co0 + j * 16 + row) * Cin + ci)
                      : zero8();
 #pragma unroll
-      for (int m = 0; m < 2; m++)
+      for (int m = 0; m < MSUB; m++) {
+        const bf16x8 a =
+            *reinterpret_cast<const bf16x8*>(&tile[((MSUB * wave + m + dy) * kHaloW + row + dx) * kLdsStride + g * 8]);
 #pragma unroll
-        for (int j = 0; j < NSUB; j++) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], w8[j], acc[m][j], 0, 0, 0);
+        for (int j = 0; j < NSUB; j++) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w8[j], acc[m][j], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int m = 0; m < 2; m++) {
-    const int y = y0 + 2 * wave + m;
+  for (int m = 0; m < MSUB; m++) {
+    const int y = y0 + MSUB * wave + m;
     if (y >= H) continue;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -199,84 +211,107 @@ __global__ void k_flipWeights(const __bf16* __restrict__ Wt, int Cin, int Cout, 
 }
 
 // ---- resampling, concatenation, activation gradient ----
+__device__ __forceinline__ bf16x8 ld8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+__device__ __forceinline__ void st8(__bf16* p, bf16x8 v) { *reinterpret_cast<bf16x8*>(p) = v; }
+// Elementwise kernels work on 8-channel groups (16 bytes); C is a multiple of
+// 8 and every count fits 32 bits (checked by the host wrappers).
 __global__ void k_pool2(const __bf16* __restrict__ X, int N, int H, int W, int C, __bf16* __restrict__ Y) {
-  const int Ho = H / 2, Wo = W / 2;
-  const long long n = (long long)N * Ho * Wo * C;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    long long r = i / C;
-    const int x = (int)(r % Wo);
+  const int Ho = H / 2, Wo = W / 2, G = C >> 3;
+  const int n = N * Ho * Wo * G;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = (i % G) << 3;
+    int r = i / G;
+    const int x = r % Wo;
     r /= Wo;
-    const int y = (int)(r % Ho), b = (int)(r / Ho);
-    const size_t base = (((size_t)b * H + 2 * y) * W + 2 * x) * C + c;
-    float s = bf2f(X[base]) + bf2f(X[base + C]) + bf2f(X[base + (size_t)W * C]) + bf2f(X[base + (size_t)W * C + C]);
-    Y[i] = f2bf(s * 0.25f);
+    const int y = r % Ho, b = r / Ho;
+    const size_t base = (((size_t)b * H + 2 * y) * W + 2 * x) * C + c, down = (size_t)W * C;
+    const bf16x8 p = ld8(X + base), q = ld8(X + base + C), u = ld8(X + base + down), v = ld8(X + base + down + C);
+    bf16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = f2bf((bf2f(p[k]) + bf2f(q[k]) + bf2f(u[k]) + bf2f(v[k])) * 0.25f);
+    st8(Y + (size_t)i * 8, o);
   }
 }
 __global__ void k_pool2Bwd(const __bf16* __restrict__ dY, int N, int H, int W, int C, __bf16* __restrict__ dX) {
-  const long long n = (long long)N * H * W * C;  // H, W: the input (fine) size
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    long long r = i / C;
-    const int x = (int)(r % W);
+  const int G = C >> 3, n = N * H * W * G;  // H, W: the input (fine) size
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = (i % G) << 3;
+    int r = i / G;
+    const int x = r % W;
     r /= W;
-    const int y = (int)(r % H), b = (int)(r / H);
-    dX[i] = f2bf(bf2f(dY[(((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c]) * 0.25f);
+    const int y = r % H, b = r / H;
+    const bf16x8 g = ld8(dY + (((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c);
+    bf16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = f2bf(bf2f(g[k]) * 0.25f);
+    st8(dX + (size_t)i * 8, o);
   }
 }
 __global__ void k_up2(const __bf16* __restrict__ X, int N, int H, int W, int C, __bf16* __restrict__ Y) {
-  const long long n = (long long)N * 2 * H * 2 * W * C;  // H, W: the input (coarse) size
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    long long r = i / C;
-    const int x = (int)(r % (2 * W));
+  const int G = C >> 3, n = N * 2 * H * 2 * W * G;  // H, W: the input (coarse) size
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = (i % G) << 3;
+    int r = i / G;
+    const int x = r % (2 * W);
     r /= 2 * W;
-    const int y = (int)(r % (2 * H)), b = (int)(r / (2 * H));
-    Y[i] = X[(((size_t)b * H + y / 2) * W + x / 2) * C + c];
+    const int y = r % (2 * H), b = r / (2 * H);
+    st8(Y + (size_t)i * 8, ld8(X + (((size_t)b * H + y / 2) * W + x / 2) * C + c));
   }
 }
 __global__ void k_up2Bwd(const __bf16* __restrict__ dY, int N, int H, int W, int C, __bf16* __restrict__ dX) {
-  const long long n = (long long)N * H * W * C;  // H, W: the coarse size
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    long long r = i / C;
-    const int x = (int)(r % W);
+  const int G = C >> 3, n = N * H * W * G;  // H, W: the coarse size
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = (i % G) << 3;
+    int r = i / G;
+    const int x = r % W;
     r /= W;
-    const int y = (int)(r % H), b = (int)(r / H);
-    const size_t base = (((size_t)b * 2 * H + 2 * y) * 2 * W + 2 * x) * C + c, row = (size_t)2 * W * C;
-    dX[i] = f2bf(bf2f(dY[base]) + bf2f(dY[base + C]) + bf2f(dY[base + row]) + bf2f(dY[base + row + C]));
+    const int y = r % H, b = r / H;
+    const size_t base = (((size_t)b * 2 * H + 2 * y) * 2 * W + 2 * x) * C + c, down = (size_t)2 * W * C;
+    const bf16x8 p = ld8(dY + base), q = ld8(dY + base + C), u = ld8(dY + base + down), v = ld8(dY + base + down + C);
+    bf16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = f2bf(bf2f(p[k]) + bf2f(q[k]) + bf2f(u[k]) + bf2f(v[k]));
+    st8(dX + (size_t)i * 8, o);
   }
 }
 // Y[p] = [A[p] (Ca channels), B[p] (Cb channels)]; the backward splits.
-__global__ void k_concat(const __bf16* __restrict__ A, int Ca, const __bf16* __restrict__ B, int Cb, long long pixels,
+__global__ void k_concat(const __bf16* __restrict__ A, int Ca, const __bf16* __restrict__ B, int Cb, int pixels,
                          __bf16* __restrict__ Y) {
-  const int C = Ca + Cb;
-  const long long n = pixels * C;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const long long p = i / C;
-    const int c = (int)(i - p * C);
-    Y[i] = c < Ca ? A[p * Ca + c] : B[p * Cb + c - Ca];
+  const int Ga = Ca >> 3, G = (Ca + Cb) >> 3, n = pixels * G;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int p = i / G, k = i - p * G;
+    st8(Y + (size_t)i * 8, k < Ga ? ld8(A + ((size_t)p * Ca + 8 * k)) : ld8(B + ((size_t)p * Cb + 8 * (k - Ga))));
   }
 }
-__global__ void k_split(const __bf16* __restrict__ Y, int Ca, int Cb, long long pixels, __bf16* __restrict__ A,
+__global__ void k_split(const __bf16* __restrict__ Y, int Ca, int Cb, int pixels, __bf16* __restrict__ A,
                         __bf16* __restrict__ B) {
-  const int C = Ca + Cb;
-  const long long n = pixels * C;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const long long p = i / C;
-    const int c = (int)(i - p * C);
-    if (c < Ca) A[p * Ca + c] = Y[i];
-    else B[p * Cb + c - Ca] = Y[i];
+  const int Ga = Ca >> 3, G = (Ca + Cb) >> 3, n = pixels * G;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int p = i / G, k = i - p * G;
+    const bf16x8 v = ld8(Y + (size_t)i * 8);
+    if (k < Ga) st8(A + ((size_t)p * Ca + 8 * k), v);
+    else st8(B + ((size_t)p * Cb + 8 * (k - Ga)), v);
   }
 }
 // G *= (Yact > 0): the gradient through ReLU given its output.
-__global__ void k_reluBwd(__bf16* __restrict__ G, const __bf16* __restrict__ Yact, long long n) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    if (!(bf2f(Yact[i]) > 0.0f)) G[i] = (__bf16)0.0f;
+__global__ void k_reluBwd(__bf16* __restrict__ G, const __bf16* __restrict__ Yact, int groups) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < groups; i += gridDim.x * blockDim.x) {
+    bf16x8 g = ld8(G + (size_t)i * 8);
+    const bf16x8 y = ld8(Yact + (size_t)i * 8);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (!(bf2f(y[k]) > 0.0f)) g[k] = (__bf16)0.0f;
+    st8(G + (size_t)i * 8, g);
+  }
 }
-__global__ void k_add(__bf16* __restrict__ A, const __bf16* __restrict__ B, long long n) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    A[i] = f2bf(bf2f(A[i]) + bf2f(B[i]));
+__global__ void k_add(__bf16* __restrict__ A, const __bf16* __restrict__ B, int groups) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < groups; i += gridDim.x * blockDim.x) {
+    bf16x8 a = ld8(A + (size_t)i * 8);
+    const bf16x8 b = ld8(B + (size_t)i * 8);
+#pragma unroll
+    for (int k = 0; k < 8; k++) a[k] = f2bf(bf2f(a[k]) + bf2f(b[k]));
+    st8(A + (size_t)i * 8, a);
+  }
 }
 
 inline int gridFor(long long n, int block = 256) {
@@ -292,13 +327,35 @@ using namespace arbor::dn;
 
 namespace {
 template <int NSUB>
-void launchConv(const __bf16* X, int N, int H, int W, int Cin, const __bf16* Wt, const float* bias, int relu, __bf16* Y,
-                int coutTotal, int co0, hipStream_t s) {
-  const long long blocks = (long long)N * ((H + kTileH - 1) / kTileH) * ((W + kTileW - 1) / kTileW);
-  hipLaunchKernelGGL((k_conv3x3<NSUB>), dim3((unsigned)blocks), dim3(256), 0, s, X, N, H, W, Cin, Wt, bias, relu, Y,
-                     coutTotal, co0);
+void launchConv(const __bf16* X, const __bf16* Xc, int Cc, int N, int H, int W, int Cin, const __bf16* Wt,
+                const float* bias, int relu, __bf16* Y, int coutTotal, int co0, hipStream_t s) {
+  // narrow outputs: 16-row tiles (each weight fragment feeds 4 MFMAs); wide: 8-row tiles
+  constexpr int MSUB = NSUB <= 4 ? 4 : 2;
+  const long long blocks = (long long)N * ((H + 4 * MSUB - 1) / (4 * MSUB)) * ((W + kTileW - 1) / kTileW);
+  hipLaunchKernelGGL((k_conv3x3<NSUB, MSUB>), dim3((unsigned)blocks), dim3(256), 0, s, X, Xc, Cc, N, H, W, Cin, Wt,
+                     bias, relu, Y, coutTotal, co0);
+}
+int conv(const __bf16* x, const __bf16* xc, int cc, int n, int h, int w, int cin, const __bf16* wt, const float* bias,
+         int relu, __bf16* y, int cout, hipStream_t stream) {
+  // output channels in launches of at most 128
+  for (int co0 = 0; co0 < cout; co0 += 128) {
+    const int width = cout - co0 < 128 ? cout - co0 : 128;
+    switch (width / 16) {
+      case 1: launchConv<1>(x, xc, cc, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 2: launchConv<2>(x, xc, cc, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 3: launchConv<3>(x, xc, cc, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 4: launchConv<4>(x, xc, cc, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 5: launchConv<5>(x, xc, cc, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 6: launchConv<6>(x, xc, cc, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 7: launchConv<7>(x, xc, cc, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+      case 8: launchConv<8>(x, xc, cc, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
+    }
+  }
+  return launch_status();
 }
 bool dnShapeOk(int n, int h, int w, int c) { return n >= 1 && h >= 1 && w >= 1 && c >= 8 && (c & 7) == 0; }
+// element counts of the vectorised kernels stay below 2^31
+bool fits32(long long elements) { return elements > 0 && elements < (1LL << 31); }
 }  // namespace
 
 extern "C" {
@@ -315,24 +372,19 @@ int arbor_dn_conv3x3(const uint16_t* X, int n, int h, int w, int cin, const uint
   ARBOR_REQUIRE(X && Wt && Y);
   if (!dnShapeOk(n, h, w, cin) || cout < 16 || cout % 16 != 0 || cout > ARBOR_DN_MAX_CHANNELS) return ARBOR_ERR_ARG;
   if (cin > ARBOR_DN_MAX_CHANNELS || (long long)n * h * w > (1LL << 31) / 256) return ARBOR_ERR_ARG;
-  auto x = reinterpret_cast<const __bf16*>(X);
-  auto wt = reinterpret_cast<const __bf16*>(Wt);
-  auto y = reinterpret_cast<__bf16*>(Y);
-  // output channels in launches of at most 128
-  for (int co0 = 0; co0 < cout; co0 += 128) {
-    const int width = cout - co0 < 128 ? cout - co0 : 128;
-    switch (width / 16) {
-      case 1: launchConv<1>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
-      case 2: launchConv<2>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
-      case 3: launchConv<3>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
-      case 4: launchConv<4>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
-      case 5: launchConv<5>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
-      case 6: launchConv<6>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
-      case 7: launchConv<7>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
-      case 8: launchConv<8>(x, n, h, w, cin, wt, bias, relu, y, cout, co0, stream); break;
-    }
-  }
-  return launch_status();
+  return conv(reinterpret_cast<const __bf16*>(X), nullptr, 0, n, h, w, cin, reinterpret_cast<const __bf16*>(Wt), bias,
+              relu, reinterpret_cast<__bf16*>(Y), cout, stream);
+}
+
+int arbor_dn_conv3x3_cat(const uint16_t* Xc, int cc, const uint16_t* X, int cs, int n, int h, int w, const uint16_t* Wt,
+                         const float* bias, int relu, uint16_t* Y, int cout, hipStream_t stream) {
+  ARBOR_REQUIRE(Xc && X && Wt && Y);
+  const int cin = cc + cs;
+  if (!dnShapeOk(n, h, w, cin) || cc < 8 || (cc & 7) || cs < 8 || (cs & 7) || (h & 1) || (w & 1)) return ARBOR_ERR_ARG;
+  if (cout < 16 || cout % 16 != 0 || cout > ARBOR_DN_MAX_CHANNELS || cin > ARBOR_DN_MAX_CHANNELS) return ARBOR_ERR_ARG;
+  if ((long long)n * h * w > (1LL << 31) / 256) return ARBOR_ERR_ARG;
+  return conv(reinterpret_cast<const __bf16*>(X), reinterpret_cast<const __bf16*>(Xc), cc, n, h, w, cin,
+              reinterpret_cast<const __bf16*>(Wt), bias, relu, reinterpret_cast<__bf16*>(Y), cout, stream);
 }
 
 int arbor_dn_conv3x3_wgrad(const uint16_t* X, const uint16_t* dY, int n, int h, int w, int cin, int cout,
@@ -363,61 +415,63 @@ int arbor_dn_flip_weights(const uint16_t* Wt, int cin, int cout, uint16_t* Wflip
 
 int arbor_dn_pool2(const uint16_t* X, int n, int h, int w, int c, uint16_t* Y, hipStream_t stream) {
   ARBOR_REQUIRE(X && Y);
-  if (n < 1 || h < 2 || w < 2 || (h & 1) || (w & 1) || c < 1) return ARBOR_ERR_ARG;
-  hipLaunchKernelGGL(k_pool2, dim3(gridFor((long long)n * h * w * c / 4)), dim3(256), 0, stream,
+  if (!dnShapeOk(n, h, w, c) || h < 2 || w < 2 || (h & 1) || (w & 1) || !fits32((long long)n * h * w * c))
+    return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_pool2, dim3(gridFor((long long)n * h * w * c / 32)), dim3(256), 0, stream,
                      reinterpret_cast<const __bf16*>(X), n, h, w, c, reinterpret_cast<__bf16*>(Y));
   return launch_status();
 }
 int arbor_dn_pool2_bwd(const uint16_t* dY, int n, int h, int w, int c, uint16_t* dX, hipStream_t stream) {
   ARBOR_REQUIRE(dY && dX);
-  if (n < 1 || h < 2 || w < 2 || (h & 1) || (w & 1) || c < 1) return ARBOR_ERR_ARG;
-  hipLaunchKernelGGL(k_pool2Bwd, dim3(gridFor((long long)n * h * w * c)), dim3(256), 0, stream,
+  if (!dnShapeOk(n, h, w, c) || h < 2 || w < 2 || (h & 1) || (w & 1) || !fits32((long long)n * h * w * c))
+    return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_pool2Bwd, dim3(gridFor((long long)n * h * w * c / 8)), dim3(256), 0, stream,
                      reinterpret_cast<const __bf16*>(dY), n, h, w, c, reinterpret_cast<__bf16*>(dX));
   return launch_status();
 }
 int arbor_dn_up2(const uint16_t* X, int n, int h, int w, int c, uint16_t* Y, hipStream_t stream) {
   ARBOR_REQUIRE(X && Y);
-  if (n < 1 || h < 1 || w < 1 || c < 1) return ARBOR_ERR_ARG;
-  hipLaunchKernelGGL(k_up2, dim3(gridFor((long long)n * h * w * c * 4)), dim3(256), 0, stream,
+  if (!dnShapeOk(n, h, w, c) || !fits32((long long)n * h * w * c * 4)) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_up2, dim3(gridFor((long long)n * h * w * c / 2)), dim3(256), 0, stream,
                      reinterpret_cast<const __bf16*>(X), n, h, w, c, reinterpret_cast<__bf16*>(Y));
   return launch_status();
 }
 int arbor_dn_up2_bwd(const uint16_t* dY, int n, int h, int w, int c, uint16_t* dX, hipStream_t stream) {
   ARBOR_REQUIRE(dY && dX);
-  if (n < 1 || h < 1 || w < 1 || c < 1) return ARBOR_ERR_ARG;
-  hipLaunchKernelGGL(k_up2Bwd, dim3(gridFor((long long)n * h * w * c)), dim3(256), 0, stream,
+  if (!dnShapeOk(n, h, w, c) || !fits32((long long)n * h * w * c * 4)) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_up2Bwd, dim3(gridFor((long long)n * h * w * c / 8)), dim3(256), 0, stream,
                      reinterpret_cast<const __bf16*>(dY), n, h, w, c, reinterpret_cast<__bf16*>(dX));
   return launch_status();
 }
 int arbor_dn_concat(const uint16_t* A, int ca, const uint16_t* B, int cb, long long pixels, uint16_t* Y,
                     hipStream_t stream) {
   ARBOR_REQUIRE(A && B && Y);
-  if (ca < 1 || cb < 1 || pixels < 1) return ARBOR_ERR_ARG;
-  hipLaunchKernelGGL(k_concat, dim3(gridFor(pixels * (ca + cb))), dim3(256), 0, stream,
-                     reinterpret_cast<const __bf16*>(A), ca, reinterpret_cast<const __bf16*>(B), cb, pixels,
+  if (ca < 8 || cb < 8 || (ca & 7) || (cb & 7) || !fits32(pixels * (ca + cb))) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_concat, dim3(gridFor(pixels * (ca + cb) / 8)), dim3(256), 0, stream,
+                     reinterpret_cast<const __bf16*>(A), ca, reinterpret_cast<const __bf16*>(B), cb, (int)pixels,
                      reinterpret_cast<__bf16*>(Y));
   return launch_status();
 }
 int arbor_dn_split(const uint16_t* Y, int ca, int cb, long long pixels, uint16_t* A, uint16_t* B, hipStream_t stream) {
   ARBOR_REQUIRE(Y && A && B);
-  if (ca < 1 || cb < 1 || pixels < 1) return ARBOR_ERR_ARG;
-  hipLaunchKernelGGL(k_split, dim3(gridFor(pixels * (ca + cb))), dim3(256), 0, stream,
-                     reinterpret_cast<const __bf16*>(Y), ca, cb, pixels, reinterpret_cast<__bf16*>(A),
+  if (ca < 8 || cb < 8 || (ca & 7) || (cb & 7) || !fits32(pixels * (ca + cb))) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_split, dim3(gridFor(pixels * (ca + cb) / 8)), dim3(256), 0, stream,
+                     reinterpret_cast<const __bf16*>(Y), ca, cb, (int)pixels, reinterpret_cast<__bf16*>(A),
                      reinterpret_cast<__bf16*>(B));
   return launch_status();
 }
 int arbor_dn_relu_bwd(uint16_t* G, const uint16_t* Yact, long long count, hipStream_t stream) {
   ARBOR_REQUIRE(G && Yact);
-  if (count < 1) return ARBOR_ERR_ARG;
-  hipLaunchKernelGGL(k_reluBwd, dim3(gridFor(count)), dim3(256), 0, stream, reinterpret_cast<__bf16*>(G),
-                     reinterpret_cast<const __bf16*>(Yact), count);
+  if (!fits32(count) || (count & 7)) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_reluBwd, dim3(gridFor(count / 8)), dim3(256), 0, stream, reinterpret_cast<__bf16*>(G),
+                     reinterpret_cast<const __bf16*>(Yact), (int)(count / 8));
   return launch_status();
 }
 int arbor_dn_add(uint16_t* A, const uint16_t* B, long long count, hipStream_t stream) {
   ARBOR_REQUIRE(A && B);
-  if (count < 1) return ARBOR_ERR_ARG;
-  hipLaunchKernelGGL(k_add, dim3(gridFor(count)), dim3(256), 0, stream, reinterpret_cast<__bf16*>(A),
-                     reinterpret_cast<const __bf16*>(B), count);
+  if (!fits32(count) || (count & 7)) return ARBOR_ERR_ARG;
+  hipLaunchKernelGGL(k_add, dim3(gridFor(count / 8)), dim3(256), 0, stream, reinterpret_cast<__bf16*>(A),
+                     reinterpret_cast<const __bf16*>(B), (int)(count / 8));
   return launch_status();
 }
 

@@ -266,10 +266,16 @@ int forward(arbor_denoiser* d, int n, hipStream_t s) {
   const Act* low = &d->e4;
   for (int k = 3; k >= 0; k--) {
     const int j = 3 - k;  // decoder stage
-    DN_TRY(arbor_dn_up2(low->p, n, d->hl(k + 1), d->wl(k + 1), low->c, d->u[k].p, s));
-    DN_TRY(arbor_dn_concat(d->u[k].p, d->u[k].c, d->s[k].p, d->s[k].c, (long long)n * d->hl(k) * d->wl(k), d->c[k].p,
-                           s));
-    DN_TRY(runConv(d, 10 + 2 * j, d->c[k], d->dA[k], n, s));
+    if (d->train) {  // the weight gradient needs the concatenated input in memory
+      DN_TRY(arbor_dn_up2(low->p, n, d->hl(k + 1), d->wl(k + 1), low->c, d->u[k].p, s));
+      DN_TRY(arbor_dn_concat(d->u[k].p, d->u[k].c, d->s[k].p, d->s[k].c, (long long)n * d->hl(k) * d->wl(k),
+                             d->c[k].p, s));
+      DN_TRY(runConv(d, 10 + 2 * j, d->c[k], d->dA[k], n, s));
+    } else {  // upsampling and concatenation inside the convolution's staging
+      const Conv& cv = d->conv[10 + 2 * j];
+      DN_TRY(arbor_dn_conv3x3_cat(low->p, low->c, d->s[k].p, d->s[k].c, n, d->hl(k), d->wl(k), cv.w16, cv.b, 1,
+                                  d->dA[k].p, cv.cout, s));
+    }
     DN_TRY(runConv(d, 11 + 2 * j, d->dA[k], d->d[k], n, s));
     low = &d->d[k];
   }
@@ -355,8 +361,15 @@ int arbor_denoiser_create(arbor_denoiser** out, int max_batch, int height, int w
   for (int k = 0; k < 4; k++) {
     rc |= d->allocAct(d->s[k], c[k], k);
     rc |= d->allocAct(d->p[k], c[k], k + 1);
-    rc |= d->allocAct(d->u[k], c[k + 1], k);
-    rc |= d->allocAct(d->c[k], c[k + 1] + c[k], k);
+    if (d->train) {
+      rc |= d->allocAct(d->u[k], c[k + 1], k);
+      rc |= d->allocAct(d->c[k], c[k + 1] + c[k], k);
+    } else {  // the fused convolution reads the coarse and skip tensors directly
+      d->u[k].c = c[k + 1];
+      d->u[k].level = k;
+      d->c[k].c = c[k + 1] + c[k];
+      d->c[k].level = k;
+    }
     rc |= d->allocAct(d->dA[k], c[k], k);
     rc |= d->allocAct(d->d[k], c[k], k);
   }

@@ -39,6 +39,7 @@ SIGNATURES = {
     "arbor_tracePixels": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_dn_conv3x3": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp],
     "arbor_dn_wgrad_partial_floats": [_i, _i, _i, _i, _i],
+    "arbor_dn_conv3x3_cat": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp],
     "arbor_dn_conv3x3_wgrad": [_vp, _vp, _i, _i, _i, _i, _i, _vp, ct.c_longlong, _vp, _vp, _vp],
     "arbor_dn_flip_weights": [_vp, _i, _i, _vp, _vp],
     "arbor_dn_pool2": [_vp, _i, _i, _i, _i, _vp, _vp],

@@ -29,6 +29,11 @@ extern "C" {
  * relu != 0; bias may be NULL. */
 int arbor_dn_conv3x3(const uint16_t* X, int n, int h, int w, int cin, const uint16_t* Wt, const float* bias, int relu,
                      uint16_t* Y, int cout, hipStream_t stream);
+/* The same convolution of the channel concatenation [up2(Xc), X]: Xc is
+ * n x (h/2) x (w/2) x cc (nearest 2x upsampling), X is n x h x w x cs; Wt has
+ * cin = cc + cs input channels. Equals conv3x3 of the materialised tensor. */
+int arbor_dn_conv3x3_cat(const uint16_t* Xc, int cc, const uint16_t* X, int cs, int n, int h, int w, const uint16_t* Wt,
+                         const float* bias, int relu, uint16_t* Y, int cout, hipStream_t stream);
 /* Weight and bias gradients of that convolution: dW [9][cout][cin] and dB
  * [cout] (fp32) from X and dY; `partial` is scratch of at least
  * arbor_dn_wgrad_partial_floats(...) floats. */
@@ -38,6 +43,8 @@ int arbor_dn_conv3x3_wgrad(const uint16_t* X, const uint16_t* dY, int n, int h, 
 /* Wflip[tap][ci][co] = Wt[8 - tap][co][ci]: conv3x3(dY, Wflip) is the gradient
  * of the convolution with respect to X. */
 int arbor_dn_flip_weights(const uint16_t* Wt, int cin, int cout, uint16_t* Wflip, hipStream_t stream);
+/* The elementwise operations below take channel counts (and counts) that are
+ * multiples of 8, and tensors below 2^31 elements. */
 /* 2x2 average pooling (h, w even) and its gradient (h, w: the fine size). */
 int arbor_dn_pool2(const uint16_t* X, int n, int h, int w, int c, uint16_t* Y, hipStream_t stream);
 int arbor_dn_pool2_bwd(const uint16_t* dY, int n, int h, int w, int c, uint16_t* dX, hipStream_t stream);

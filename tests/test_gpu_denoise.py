@@ -97,6 +97,27 @@ def test_conv3x3_gradients(env, n, h, w, cin, cout):
     assert ((got - ref).abs() <= ref.abs() * 2.0 ** -8 + ref.abs().max() * 2.0 ** -12).all()
 
 
+@pytest.mark.parametrize("n,h,w,cc,cs,cout", [(1, 10, 12, 128, 96, 96), (2, 8, 18, 48, 32, 32), (1, 34, 16, 64, 48, 48)])
+def test_conv3x3_cat_equals_materialised_concat(env, n, h, w, cc, cs, cout):
+    """The fused [up2(Xc), X] convolution gives the same bits as up2 + concat + conv."""
+    torch, px = env
+    print("case: conv3x3_cat n%d %dx%d %d+%d->%d" % (n, h, w, cc, cs, cout), flush=True)
+    g = torch.Generator().manual_seed(cc + cs)
+    xc = _bf(torch, torch.randn(n, h // 2, w // 2, cc, generator=g)).cuda()
+    xs = _bf(torch, torch.randn(n, h, w, cs, generator=g)).cuda()
+    wt = _bf(torch, torch.randn(9, cout, cc + cs, generator=g) * 0.05).cuda()
+    b = (torch.randn(cout, generator=g) * 0.1).cuda()
+    up = torch.zeros(n, h, w, cc, dtype=torch.bfloat16, device="cuda")
+    px.call("arbor_dn_up2", xc, n, h // 2, w // 2, cc, up)
+    cat = torch.zeros(n, h, w, cc + cs, dtype=torch.bfloat16, device="cuda")
+    px.call("arbor_dn_concat", up, cc, xs, cs, n * h * w, cat)
+    want = torch.zeros(n, h, w, cout, dtype=torch.bfloat16, device="cuda")
+    px.call("arbor_dn_conv3x3", cat, n, h, w, cc + cs, wt, b, 1, want, cout)
+    got = torch.zeros_like(want)
+    px.call("arbor_dn_conv3x3_cat", xc, cc, xs, cs, n, h, w, wt, b, 1, got, cout)
+    assert torch.equal(got.cpu(), want.cpu())
+
+
 def test_resampling_concat_and_elementwise(env):
     torch, px = env
     print("case: pool/up/concat/split/relu/add", flush=True)
