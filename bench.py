@@ -56,6 +56,9 @@ def parse():
                     help="batched-views workload instead: N seeded walk views of uneven resolution rendered by one "
                          "hipGraph-captured batch per step (about 0.3 s per step at N=64; use --steps 10)")
     ap.add_argument("--views-seed", type=int, default=3)
+    ap.add_argument("--profile-denoiser", type=int, default=0, metavar="N",
+                    help="profiling driver: N learned-denoiser passes (features, forward, output) at 1920x1080 "
+                         "on fixed random inputs with the committed checkpoint; prints no JSON")
     ap.add_argument("--denoiser", nargs="?", const=DENOISER_WEIGHTS, default="", metavar="CHECKPOINT",
                     help="learned-denoiser workload instead: held-out 1920x1080 frames (sun visibility on), "
                          "1-sample inputs against 64-sample targets; filterLighting vs the network")
@@ -75,6 +78,26 @@ def denoiser_flops(height, width):
               (96, 128, 4), (128, 128, 4), (224, 96, 3), (96, 96, 3), (160, 64, 2), (64, 64, 2), (112, 48, 1),
               (48, 48, 1), (80, 32, 0), (32, 32, 0), (32, 3, 0)]
     return sum(2.0 * 9 * ci * co * (height >> lv) * (width >> lv) for ci, co, lv in layers)
+
+
+def profile_denoiser(args):
+    import torch
+
+    import pyarbor as px
+
+    W, H = 1920, 1080
+    inf = px.Denoiser(1, 1088, 1920, train=False, seed=0)
+    inf.load(DENOISER_WEIGHTS)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    bufs = [torch.rand(W * H * 4, device="cuda", generator=g) for _ in range(4)]
+    out = torch.zeros(W * H * 4, device="cuda")
+    for _ in range(args.profile_denoiser):
+        inf.set_input(0, bufs[0], bufs[1], bufs[2], bufs[3], W, H)
+        inf.forward(1)
+        inf.output(0, out, W, H)
+    torch.cuda.synchronize()
+    inf.close()
+    print("profile driver: %d denoiser passes at %dx%d" % (args.profile_denoiser, W, H))
 
 
 def bench_denoiser(args):
@@ -298,6 +321,9 @@ def main():
         sys.exit(2)
     if args.profile_trace:
         profile_trace(args)
+        return
+    if args.profile_denoiser:
+        profile_denoiser(args)
         return
     if args.views:
         bench_views(args)
