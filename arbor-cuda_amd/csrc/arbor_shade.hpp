@@ -89,9 +89,12 @@ AD float specGGX(v3 n, v3 v, v3 l, float rough) {
   float F = 0.035f + 0.965f * powf(1.0f - vh, 5.0f);
   return D * G * F / (4.0f * nv + 0.0001f);
 }
-// Up to four shadow traversals through leaves (Arbor.cu:271-274).
+// Up to four shadow traversals through leaves (Arbor.cu:271-274). The
+// sun-visibility pass inlines this (its kernel then needs 121 VGPRs, 4 waves
+// per SIMD); the fused shading paths call the out-of-line lightVisibility
+// below, so their own register budget is not added to the traversal's.
 template <class W>
-__device__ inline v3 lightVisibility(const float* P, const float4* Nodes4, const float4* top, const uint32_t* Order,
+__device__ __forceinline__ v3 lightVisibilityInline(const float* P, const float4* Nodes4, const float4* top, const uint32_t* Order,
                                      const TreeInst* trees, v3 p, v3 direction, float time, float wind) {
   v3 transmission = mk(1.0f, 1.0f, 1.0f);
   v3 ro = p;
@@ -105,6 +108,11 @@ __device__ inline v3 lightVisibility(const float* P, const float4* Nodes4, const
     ro = ro + direction * (h.x + 0.0018f);
   }
   return transmission;
+}
+template <class W>
+__device__ __noinline__ v3 lightVisibility(const float* P, const float4* Nodes4, const float4* top, const uint32_t* Order,
+                                           const TreeInst* trees, v3 p, v3 direction, float time, float wind) {
+  return lightVisibilityInline<W>(P, Nodes4, top, Order, trees, p, direction, time, wind);
 }
 
 
@@ -133,13 +141,41 @@ AD ShadeCtx makeShadeCtx(const float* C, int width, int height) {
   return sc;
 }
 
+// The jittered sun direction of pixel `pixel` for the current sample and the
+// shadow ray's origin off hit point p (Arbor.cu:282-284): shading and the
+// separate sun-visibility pass both take them from here.
+struct SunRay { v3 light, origin; };
+AD SunRay sunRayOf(const ShadeCtx& sc, int pixel, v3 p, int id) {
+  float r1 = rnd(pixel + sc.sample * 1823, 991), r2 = rnd(pixel + sc.sample * 971, 731);
+  float angle = r1 * 6.28318f, disk = sqrtf(r2) * 0.012f;
+  SunRay s;
+  s.light = unit(sc.sun + sc.su * (cosf(angle) * disk) + sc.sv * (sinf(angle) * disk));
+  float offset = id >= 0 ? 0.0014f : 0.004f;
+  s.origin = p + s.light * offset;
+  return s;
+}
+
+// Sun visibility of one pixel from its Hit record: lightVisibility along its
+// sun ray, or 1 for sky or with tree-to-tree visibility off.
+template <class W>
+__device__ inline v3 sunVisibilityOne(const float* P, const float4* Nodes4, const uint32_t* Order,
+                                      const TreeInst* trees, const ShadeCtx& sc, int x, int y, float4 hit) {
+  if (!sc.shadows || !(hit.x < kFar)) return mk(1.0f, 1.0f, 1.0f);
+  v3 rd = viewRay(sc.view, x, y, sc.width, sc.height);
+  v3 p = sc.view.eye + rd * hit.x;
+  SunRay s = sunRayOf(sc, y * sc.width + x, p, (int)hit.y);
+  return lightVisibilityInline<W>(P, Nodes4, nullptr, Order, trees, s.origin, s.light, sc.time, sc.wind);
+}
+
 // One pixel of shadePixels (Arbor.cu:277-289): Linear and Guide of pixel
 // (x, y) from its Hit record; with outNormal, also the shading normal (facing
-// the camera; zero for sky).
+// the camera; zero for sky). With vis, the sun visibility comes from the
+// separate pass (sunVisibilityOne: same rays, same value) instead of being
+// traced here.
 template <class W>
 __device__ inline void shadeOne(const float* P, const float4* Nodes4, const uint32_t* Order, const TreeInst* trees,
                                 const ShadeCtx& sc, int x, int y, float4 hit, float4& outLinear, float4& outGuide,
-                                v3* outNormal = nullptr) {
+                                v3* outNormal = nullptr, const float4* vis = nullptr) {
   const int pixel = y * sc.width + x;
   float tt = hit.x;
   int id = (int)hit.y, tree = (int)hit.z;
@@ -193,13 +229,17 @@ __device__ inline void shadeOne(const float* P, const float4* Nodes4, const uint
       albedo = albedo * tint * dark;
     }
     guide = albedo;
-    float r1 = rnd(pixel + sc.sample * 1823, 991), r2 = rnd(pixel + sc.sample * 971, 731);
-    float angle = r1 * 6.28318f, disk = sqrtf(r2) * 0.012f;
-    v3 light = unit(sc.sun + sc.su * (cosf(angle) * disk) + sc.sv * (sinf(angle) * disk));
-    float offset = id >= 0 ? 0.0014f : 0.004f;
+    const SunRay sray = sunRayOf(sc, pixel, p, id);
+    const v3 light = sray.light;
     v3 visibility = mk(1.0f, 1.0f, 1.0f);
-    if (sc.shadows)
-      visibility = lightVisibility<W>(P, Nodes4, nullptr, Order, trees, p + light * offset, light, sc.time, sc.wind);
+    if (sc.shadows) {
+      if (vis) {
+        const float4 v = vis[pixel];
+        visibility = mk(v.x, v.y, v.z);
+      } else {
+        visibility = lightVisibility<W>(P, Nodes4, nullptr, Order, trees, sray.origin, light, sc.time, sc.wind);
+      }
+    }
     float nl = sat(dot(n, light)), back = sat(-dot(n, light));
     v3 direct = albedo * (nl / kPi);
     if (leaf) {

@@ -19,7 +19,9 @@ struct Buffers {
   unsigned *Keys = nullptr, *Order = nullptr;
   float *Hit = nullptr, *Linear = nullptr, *Guide = nullptr, *Filtered = nullptr, *History = nullptr;
   unsigned* Pixels = nullptr;
-  void* Schedule = nullptr;  // tile schedule of the trace (arbor_tracePixelsScheduled)
+  void* Schedule = nullptr;     // tile schedule of the trace (arbor_tracePixelsScheduled)
+  void* VisSchedule = nullptr;  // tile schedule of the sun-visibility pass
+  float* Vis = nullptr;         // sun visibility per pixel
 };
 
 template <class T>
@@ -53,10 +55,20 @@ size_t pixel_bytes(int w, int h) { return (size_t)w * h * 16; }
 
 void free_pixels(arbor_engine* e) {
   release(e->b.Hit); release(e->b.Linear); release(e->b.Guide); release(e->b.Filtered);
-  release(e->b.History); release(e->b.Pixels); release(e->b.Schedule);
+  release(e->b.History); release(e->b.Pixels); release(e->b.Schedule); release(e->b.VisSchedule); release(e->b.Vis);
 }
 
 }  // namespace
+
+// shadePixels for the engine: the sun-visibility pass (tile-scheduled) and
+// shading from it; Linear and Guide equal arbor_shadePixels'.
+int shadeEngine(arbor_engine* e, int w, int h, hipStream_t stream) {
+  int rc = arbor_sunVisibilityScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, w, h,
+                                        e->b.VisSchedule, stream);
+  if (rc != ARBOR_OK) return rc;
+  return arbor_shadePixelsVis(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, e->b.Linear, e->b.Guide, w,
+                              h, stream);
+}
 
 extern "C" {
 
@@ -146,13 +158,16 @@ int arbor_engine_resize(arbor_engine* e, int width, int height, int round_size) 
   rc |= alloc(&e->b.Pixels, (size_t)width * height * 4);
   const long long sched = arbor_tile_schedule_bytes(width, height);
   rc |= alloc(&e->b.Schedule, (size_t)sched);
+  rc |= alloc(&e->b.VisSchedule, (size_t)sched);
+  rc |= alloc(&e->b.Vis, n);
   if (rc != ARBOR_OK) {
     free_pixels(e);
     e->width = e->height = 0;
     return ARBOR_ERR_DEVICE;
   }
   if (hipMemsetAsync(e->b.History, 0, n, e->stream) != hipSuccess ||
-      hipMemsetAsync(e->b.Schedule, 0, (size_t)sched, e->stream) != hipSuccess)
+      hipMemsetAsync(e->b.Schedule, 0, (size_t)sched, e->stream) != hipSuccess ||
+      hipMemsetAsync(e->b.VisSchedule, 0, (size_t)sched, e->stream) != hipSuccess)
     return ARBOR_ERR_DEVICE;
   e->width = width;
   e->height = height;
@@ -199,8 +214,7 @@ int arbor_engine_frame(arbor_engine* e, float dt, const float* device_input) {
                                        e->stream)) != ARBOR_OK)
     return rc;
   if (e->timing) (void)hipEventRecord(e->ev[2], e->stream);
-  if ((rc = arbor_shadePixels(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Linear, e->b.Guide, w, h,
-                              e->stream)) != ARBOR_OK)
+  if ((rc = shadeEngine(e, w, h, e->stream)) != ARBOR_OK)
     return rc;
   if (e->timing) (void)hipEventRecord(e->ev[3], e->stream);
   if ((rc = arbor_filterLighting(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.C, w, h, e->stream)) !=
@@ -328,11 +342,12 @@ int arbor_batch_walk_views(int seed, int world_seed, int n, float* cameras, int*
 }
 namespace {
 // Per-sample camera slots of the harness render loop (native_harness.cpp:62).
+// light < 0 keeps the sun visibility flag.
 __global__ void k_setSample(float* C, int sample, int light) {
   if (threadIdx.x == 0) {
     C[9] = (float)sample;
     C[10] = sample == 0 ? 1.0f : 0.0f;
-    C[22] = light ? 1.0f : 0.0f;
+    if (light >= 0) C[22] = light ? 1.0f : 0.0f;
   }
 }
 }  // namespace
@@ -353,11 +368,37 @@ int arbor_engine_render(arbor_engine* e, int view, int samples, int light) {
     if ((rc = arbor::launch_status()) != ARBOR_OK) break;
     rc = arbor_tracePixelsScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, w, h, e->b.Schedule, e->stream);
     if (rc == ARBOR_OK)
-      rc = arbor_shadePixels(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Linear, e->b.Guide, w, h, e->stream);
+      rc = shadeEngine(e, w, h, e->stream);
     if (rc == ARBOR_OK) rc = arbor_filterLighting(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.C, w, h, e->stream);
     if (rc == ARBOR_OK) rc = arbor_resolvePixels(e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w, h, e->stream);
   }
   if (rc == ARBOR_OK) e->frames += samples;
+  return rc;
+}
+
+int arbor_engine_frame_samples(arbor_engine* e, float dt, const float* device_input, int samples) {
+  if (!e) return ARBOR_ERR_NULL;
+  if (!e->generated || !e->b.Hit) return ARBOR_ERR_STATE;
+  if (samples < 1 || samples > 512) return ARBOR_ERR_ARG;
+  const float* I = device_input;
+  if (!I) {
+    if (hipMemcpyAsync(e->b.I, e->input, sizeof(e->input), hipMemcpyHostToDevice, e->stream) != hipSuccess)
+      return ARBOR_ERR_DEVICE;
+    I = e->b.I;
+  }
+  float delta = dt < 0.0f ? 0.0f : (dt > 0.1f ? 0.1f : dt);
+  const int w = e->width, h = e->height;
+  int rc = arbor_updateCamera(e->b.C, I, e->b.P, delta, w, h, e->stream);
+  for (int i = 0; i < samples && rc == ARBOR_OK; i++) {
+    hipLaunchKernelGGL(k_setSample, dim3(1), dim3(64), 0, e->stream, e->b.C, i, -1);
+    if ((rc = arbor::launch_status()) != ARBOR_OK) break;
+    rc = arbor_tracePixelsScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, w, h, e->b.Schedule, e->stream);
+    if (rc == ARBOR_OK)
+      rc = shadeEngine(e, w, h, e->stream);
+    if (rc == ARBOR_OK) rc = arbor_filterLighting(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.C, w, h, e->stream);
+    if (rc == ARBOR_OK) rc = arbor_resolvePixels(e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w, h, e->stream);
+  }
+  if (rc == ARBOR_OK) e->frames++;
   return rc;
 }
 

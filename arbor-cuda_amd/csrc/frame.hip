@@ -67,6 +67,40 @@ __global__ __launch_bounds__(kWaves * 64) void k_tracePixels(const float* P, con
   }
 }
 
+// Sun visibility of every pixel (the shadow traversals of shadePixels,
+// Arbor.cu:271-274) as a pass of its own: Vis[p] = {transmission, 1}. It
+// runs the trace kernel's launch shape (one wave64 per 8x8 tile, the walk
+// inlined, optional tile schedule), so shading's register budget no longer
+// limits how many shadow traversals are in flight.
+template <int kWaves, class W>
+__global__ __launch_bounds__(kWaves * 64) void k_sunVisibility(const float* P, const float4* Nodes4,
+                                                               const uint32_t* Order, const float* C, const float4* Hit,
+                                                               float4* Vis, int width, int height, int tilesX,
+                                                               int tiles, const uint32_t* tileOrder,
+                                                               uint32_t* tileCost) {
+  __shared__ TreeInst trees[kTrees];
+  if (!(C[22] > 0.5f)) return;  // visibility off: shading never reads Vis
+  fillTrees(trees, (int)C[31]);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = blockIdx.x * kWaves + wave; i < tiles; i += gridDim.x * kWaves) {
+    const int t = tileOrder ? (int)tileOrder[i] : i;
+    const long long start = tileCost ? wall_clock64() : 0;
+    int x = (t % tilesX) * 8 + (lane & 7), y = (t / tilesX) * 8 + (lane >> 3);
+    if (x < width && y < height) {
+      const int pixel = y * width + x;
+      // per tile from C: the context is dead during the traversal instead of live across the loop
+      const ShadeCtx sc = makeShadeCtx(C, width, height);
+      v3 v = sunVisibilityOne<W>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel]);
+      Vis[pixel] = make_float4(v.x, v.y, v.z, 1.0f);
+    }
+    if (tileCost && lane == 0) {
+      long long ticks = wall_clock64() - start;
+      tileCost[t] = ticks < 1 ? 1u : (ticks > 0x7fffffffLL ? 0x7fffffffu : (uint32_t)ticks);
+    }
+  }
+}
+
 // Tile schedule (longest predicted first) for the next tracePixels launch. A
 // tile's predicted cost is the largest recorded cost in the 3x3 tiles around
 // where its centre ray fell in the previous frame (the direction projected
@@ -225,7 +259,7 @@ using WalkShade = WalkCull;
 __global__ __launch_bounds__(kTileBlock) void k_shadePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
                                                             const float* C, const float4* Hit, float4* Linear,
                                                             float4* Guide, int width, int height, int tilesX,
-                                                            int tiles, float4* Normal) {
+                                                            int tiles, float4* Normal, const float4* Vis) {
   __shared__ TreeInst trees[kTrees];
   fillTrees(trees, (int)C[31]);
   __syncthreads();
@@ -237,7 +271,7 @@ __global__ __launch_bounds__(kTileBlock) void k_shadePixels(const float* P, cons
     int pixel = y * width + x;
     float4 l, g;
     v3 n;
-    shadeOne<WalkShade>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel], l, g, Normal ? &n : nullptr);
+    shadeOne<WalkShade>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel], l, g, Normal ? &n : nullptr, Vis);
     Linear[pixel] = l;
     Guide[pixel] = g;
     if (Normal) Normal[pixel] = make_float4(n.x, n.y, n.z, 0.0f);
@@ -398,6 +432,26 @@ int arbor_tracePixels(const float* P, const float* Nodes, const unsigned* Order,
   return launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream);
 }
 
+}  // extern "C"
+
+namespace {
+// The three ordering passes over a schedule; its cost and order arrays out.
+int orderTiles(const float* C, void* schedule, int width, int height, hipStream_t stream, const uint32_t** order,
+               uint32_t** cost) {
+  const int tx = (width + 7) / 8, ty = (height + 7) / 8;
+  TileSchedule* s = static_cast<TileSchedule*>(schedule);
+  const int tiles = tx * ty, blocks = (tiles + kOrderBlock - 1) / kOrderBlock;
+  hipLaunchKernelGGL(k_tileKeys, dim3(blocks), dim3(kOrderBlock), 0, stream, C, s, tx, ty, width, height);
+  hipLaunchKernelGGL(k_tileStarts, dim3(1), dim3(kOrderBuckets), 0, stream, s);
+  hipLaunchKernelGGL(k_tileScatter, dim3(blocks), dim3(kOrderBlock), 0, stream, C, s, tiles);
+  *cost = reinterpret_cast<uint32_t*>(s + 1);
+  *order = *cost + (size_t)tiles;
+  return launch_status();
+}
+}  // namespace
+
+extern "C" {
+
 long long arbor_tile_schedule_bytes(int width, int height) {
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
   long long tiles = (long long)((width + 7) / 8) * ((height + 7) / 8);
@@ -410,16 +464,11 @@ int arbor_tracePixelsScheduled(const float* P, const float* Nodes, const unsigne
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
   if (tuning().trace_variant >= kTraceVariantCount)  // lane-scheduled kernels keep their own order
     return launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream);
-  int tx = (width + 7) / 8, ty = (height + 7) / 8;
-  TileSchedule* s = static_cast<TileSchedule*>(schedule);
-  const int tiles = tx * ty, blocks = (tiles + kOrderBlock - 1) / kOrderBlock;
-  hipLaunchKernelGGL(k_tileKeys, dim3(blocks), dim3(kOrderBlock), 0, stream, C, s, tx, ty, width, height);
-  hipLaunchKernelGGL(k_tileStarts, dim3(1), dim3(kOrderBuckets), 0, stream, s);
-  hipLaunchKernelGGL(k_tileScatter, dim3(blocks), dim3(kOrderBlock), 0, stream, C, s, tiles);
-  int rc = launch_status();
+  uint32_t* cost = nullptr;
+  const uint32_t* order = nullptr;
+  int rc = orderTiles(C, schedule, width, height, stream, &order, &cost);
   if (rc != ARBOR_OK) return rc;
-  uint32_t* cost = reinterpret_cast<uint32_t*>(s + 1);
-  return launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream, cost + (size_t)tx * ty, cost);
+  return launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream, order, cost);
 }
 
 int arbor_traceTreePixels(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
@@ -437,8 +486,51 @@ int arbor_shadePixels(const float* P, const float* Nodes, const unsigned* Order,
   hipLaunchKernelGGL(k_shadePixels, dim3(tileGrid(tiles, kWavesPerBlock)), dim3(kTileBlock), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
-                     nullptr);
+                     nullptr, nullptr);
   return launch_status();
+}
+
+int arbor_shadePixelsVis(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                         const float* Vis, float* Linear, float* Guide, int width, int height, hipStream_t stream) {
+  ARBOR_REQUIRE(P && Nodes && Order && C && Hit && Vis && Linear && Guide);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  hipLaunchKernelGGL(k_shadePixels, dim3(tileGrid(tiles, kWavesPerBlock)), dim3(kTileBlock), 0, stream, P,
+                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
+                     reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
+                     nullptr, reinterpret_cast<const float4*>(Vis));
+  return launch_status();
+}
+
+namespace {
+int launchVisibility(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                     float* Vis, int width, int height, const uint32_t* tileOrder, uint32_t* tileCost,
+                     hipStream_t stream) {
+  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  hipLaunchKernelGGL((k_sunVisibility<1, WalkCull>), dim3(tileGrid(tiles, 1)), dim3(64), 0, stream, P,
+                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
+                     reinterpret_cast<float4*>(Vis), width, height, tx, tiles, tileOrder, tileCost);
+  return launch_status();
+}
+}  // namespace
+
+int arbor_sunVisibility(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                        float* Vis, int width, int height, hipStream_t stream) {
+  ARBOR_REQUIRE(P && Nodes && Order && C && Hit && Vis);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  return launchVisibility(P, Nodes, Order, C, Hit, Vis, width, height, nullptr, nullptr, stream);
+}
+
+int arbor_sunVisibilityScheduled(const float* P, const float* Nodes, const unsigned* Order, const float* C,
+                                 const float* Hit, float* Vis, int width, int height, void* schedule,
+                                 hipStream_t stream) {
+  ARBOR_REQUIRE(P && Nodes && Order && C && Hit && Vis && schedule);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  uint32_t* cost = nullptr;
+  const uint32_t* order = nullptr;
+  int rc = orderTiles(C, schedule, width, height, stream, &order, &cost);
+  if (rc != ARBOR_OK) return rc;
+  return launchVisibility(P, Nodes, Order, C, Hit, Vis, width, height, order, cost, stream);
 }
 
 int arbor_shadePixelsAOV(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
@@ -449,7 +541,7 @@ int arbor_shadePixelsAOV(const float* P, const float* Nodes, const unsigned* Ord
   hipLaunchKernelGGL(k_shadePixels, dim3(tileGrid(tiles, kWavesPerBlock)), dim3(kTileBlock), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
-                     reinterpret_cast<float4*>(Normal));
+                     reinterpret_cast<float4*>(Normal), nullptr);
   return launch_status();
 }
 

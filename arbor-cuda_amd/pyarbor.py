@@ -64,10 +64,14 @@ SIGNATURES = {
     "arbor_denoiser_load": [_vp, ct.c_char_p],
     "arbor_denoiser_tensor": [_vp, ct.c_char_p],
     "arbor_engine_render": [_vp, _i, _i, _i],
+    "arbor_engine_frame_samples": [_vp, ct.c_float, _vp, _i],
     "arbor_engine_save_ppm": [_vp, ct.c_char_p],
     "arbor_write_ppm": [ct.c_char_p, ct.POINTER(ct.c_uint), _i, _i],
     "arbor_tile_schedule_bytes": [_i, _i],
     "arbor_tracePixelsScheduled": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
+    "arbor_sunVisibility": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "arbor_sunVisibilityScheduled": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
+    "arbor_shadePixelsVis": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_shadePixels": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_filterLighting": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_resolvePixels": [_vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -384,6 +388,20 @@ def write_ppm(path, pixels, width, height):
           "arbor_write_ppm")
 
 
+def sun_visibility(scene, C, fb, vis, schedule=None):
+    """Sun-visibility pass into vis (float4 per pixel); schedule: a TileSchedule."""
+    if schedule is None:
+        call("arbor_sunVisibility", scene.P, scene.Nodes, scene.Order, C, fb.Hit, vis, fb.width, fb.height)
+    else:
+        call("arbor_sunVisibilityScheduled", scene.P, scene.Nodes, scene.Order, C, fb.Hit, vis, fb.width,
+             fb.height, schedule.buf)
+
+
+def shade_vis(scene, C, fb, vis):
+    call("arbor_shadePixelsVis", scene.P, scene.Nodes, scene.Order, C, fb.Hit, vis, fb.Linear, fb.Guide,
+         fb.width, fb.height)
+
+
 def shade_aov(scene, C, fb, normal):
     call("arbor_shadePixelsAOV", scene.P, scene.Nodes, scene.Order, C, fb.Hit, fb.Linear, fb.Guide, normal,
          fb.width, fb.height)
@@ -515,6 +533,10 @@ class Engine:
 
     def buffer_ptr(self, name):
         return self._lib.arbor_engine_buffer(self.h, name.encode())
+
+    def frame_samples(self, samples, dt=1.0 / 60.0, device_input=None):
+        ptr = None if device_input is None else ct.c_void_p(device_input.data_ptr())
+        check(self._lib.arbor_engine_frame_samples(self.h, ct.c_float(dt), ptr, samples), "arbor_engine_frame_samples")
 
     def render(self, view, samples, light=True):
         """The reference harness's render mode (view preset, `samples`

@@ -56,6 +56,9 @@ def parse():
                     help="batched-views workload instead: N seeded walk views of uneven resolution rendered by one "
                          "hipGraph-captured batch per step (about 0.3 s per step at N=64; use --steps 10)")
     ap.add_argument("--views-seed", type=int, default=3)
+    ap.add_argument("--shadows4k", action="store_true",
+                    help="sun-visibility workload instead: 3840x2160, 4 jittered samples per frame, tree-to-tree "
+                         "sun visibility on at 35 deg elevation, breeze on, static initial camera (use --steps 10)")
     ap.add_argument("--profile-denoiser", type=int, default=0, metavar="N",
                     help="profiling driver: N learned-denoiser passes (features, forward, output) at 1920x1080 "
                          "on fixed random inputs with the committed checkpoint; prints no JSON")
@@ -78,6 +81,50 @@ def denoiser_flops(height, width):
               (96, 128, 4), (128, 128, 4), (224, 96, 3), (96, 96, 3), (160, 64, 2), (64, 64, 2), (112, 48, 1),
               (48, 48, 1), (80, 32, 0), (32, 32, 0), (32, 3, 0)]
     return sum(2.0 * 9 * ci * co * (height >> lv) * (width >> lv) for ci, co, lv in layers)
+
+
+def bench_shadows4k(args):
+    """3840x2160 at 4 samples per frame with sun visibility (one shadow ray,
+    and up to 3 more through leaves, per hit pixel), sun at 35 deg, breeze on
+    (wind time +1/60 s per frame), through the engine's multi-sample frame."""
+    import numpy as np
+    import torch
+
+    import pyarbor as px
+
+    W, H, S, K, Wm = 3840, 2160, 4, args.steps, args.warmup
+    eng = px.Engine(W, H, args.seed)
+    C = px._wrap_device(eng.buffer_ptr("C"), 256).view(torch.float32)
+    C[13] = 0.610865  # 35 deg sun elevation
+    torch.cuda.synchronize()
+    first = torch.zeros(32, dtype=torch.float32, device="cuda")
+    first[7] = 1.0  # breeze on (updateCamera I[7])
+    still = torch.zeros(32, dtype=torch.float32, device="cuda")
+    eng.frame_samples(S, 1.0 / 60.0, first)
+    for _ in range(Wm):
+        eng.frame_samples(S, 1.0 / 60.0, still)
+    eng.sync()
+    c = eng.download("C", 64)
+    assert c[22] == 1.0 and abs(c[11] - 0.6) < 1e-6, "sun visibility and breeze must be on"
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        eng.frame_samples(S, 1.0 / 60.0, still)
+    eng.sync()
+    elapsed = time.perf_counter() - t0
+    eng.close()
+    rays = W * H * S
+    result = {
+        "metric": "Mrays/s, ms/frame @3840x2160 4 spp sun visibility on",
+        "value": round(rays * K / elapsed / 1e6, 3), "unit": "Mrays/s (primary samples)",
+        "n_gpus": 1, "steps": K, "warmup": Wm, "ms_per_step": round(elapsed / K * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (procedural forest from seed, static initial camera)",
+        "config": {"workload": "324 trees, 3840x2160, 4 jittered samples per frame, tree-to-tree sun visibility on, "
+                               "sun elevation 35 deg, breeze on (wind time +1/60 s per frame)",
+                   "width": W, "height": H, "samples": S, "seed": args.seed, "frames": K},
+    }
+    print(json.dumps(result), flush=True)
 
 
 def profile_denoiser(args):
@@ -324,6 +371,9 @@ def main():
         return
     if args.profile_denoiser:
         profile_denoiser(args)
+        return
+    if args.shadows4k:
+        bench_shadows4k(args)
         return
     if args.views:
         bench_views(args)

@@ -140,6 +140,21 @@ long long arbor_tile_schedule_bytes(int width, int height);
 int arbor_tracePixelsScheduled(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
                                int width, int height, void* schedule, hipStream_t stream);
 
+/* Sun visibility as a pass of its own (no counterpart in the reference; the
+ * shadow traversals of shadePixels, Arbor.cu:271-274, with the trace's launch
+ * shape): Vis[4p..] = {transmission rgb, 1} for every pixel (1 for sky); with
+ * C[22] turning tree-to-tree visibility off it writes nothing, as shading
+ * then reads nothing. arbor_shadePixelsVis then shades
+ * from it; Linear and Guide equal arbor_shadePixels' bit for bit. The
+ * scheduled form keeps its own tile schedule (arbor_tile_schedule_bytes). */
+int arbor_sunVisibility(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                        float* Vis, int width, int height, hipStream_t stream);
+int arbor_sunVisibilityScheduled(const float* P, const float* Nodes, const unsigned* Order, const float* C,
+                                 const float* Hit, float* Vis, int width, int height, void* schedule,
+                                 hipStream_t stream);
+int arbor_shadePixelsVis(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                         const float* Vis, float* Linear, float* Guide, int width, int height, hipStream_t stream);
+
 /* Single-specimen primary rays: traceTree in tree-local space with the ground
  * plane as initial best and no forest instancing (the single-tree workload of
  * native_harness.cpp:84 per pixel). Hit layout as tracePixels with tree = -1. */

@@ -54,6 +54,11 @@ int arbor_walk_input(int seed, int frame, float* input32);
  * C[9] = i, C[10] = (i == 0), each traced, shaded, filtered and accumulated by
  * resolvePixels (History holds their mean, Pixels the tone-mapped image). */
 int arbor_engine_render(arbor_engine* e, int view, int samples, int light);
+/* One frame of `samples` (1..512) jittered samples: updateCamera once, then per
+ * sample C[9] = i, C[10] = (i == 0), tracePixels, shadePixels, filterLighting,
+ * resolvePixels (History: the frame's sample mean). The multi-sample frame
+ * of the 4-sample, sun-visibility configuration. */
+int arbor_engine_frame_samples(arbor_engine* e, float dt, const float* device_input, int samples);
 /* native_harness.cpp:53 save(): binary PPM (P6) of the engine's Pixels, RGB
  * from the low three bytes of each packed pixel. */
 int arbor_engine_save_ppm(arbor_engine* e, const char* path);

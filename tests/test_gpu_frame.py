@@ -157,6 +157,48 @@ def test_scheduled_trace_identical(px, oracle, scene7):
         assert (sched.cost() >= 1).all()
 
 
+def test_visibility_pass_identical(px, oracle, scene7):
+    """sunVisibility (plain and tile-scheduled) followed by shadePixelsVis gives
+    Linear and Guide bit-identical to the fused shadePixels: sun visibility on,
+    breeze on, over moving frames and jittered samples; with visibility off
+    the pass writes nothing and shading reads nothing (Vis filled with NaN)."""
+    import torch
+
+    print("case: visibility pass, 5 frames 328x184, wind", flush=True)
+    dev, _ = scene7
+    w, h = 328, 184
+    sched = px.TileSchedule(w, h)
+    a, b = px.FrameBuffers(w, h), px.FrameBuffers(w, h)
+    vis = torch.zeros(w * h * 4, dtype=torch.float32, device="cuda")
+    C = oracle.init_camera(7)
+    for f in range(5):
+        oracle.update_camera(C, px.walk_input(7, f), 1.0 / 60.0, w, h)
+        c = C.copy()
+        c[22], c[11], c[8], c[13] = float(f != 2), 0.6, 0.25 * f, 0.610865
+        c[9], c[10] = float(f % 3), float(f % 3 == 0)
+        Cd = to_dev(c)
+        px.trace(dev, Cd, a)
+        b.Hit.copy_(a.Hit)
+        px.shade(dev, Cd, a)
+        if f == 2:
+            vis.fill_(float("nan"))
+        if f % 2:
+            px.sun_visibility(dev, Cd, b, vis)
+        else:
+            px.sun_visibility(dev, Cd, b, vis, sched)
+        px.shade_vis(dev, Cd, b, vis)
+        for name in ("Linear", "Guide"):
+            want = to_host(getattr(a, name)).view(np.uint32)
+            got = to_host(getattr(b, name)).view(np.uint32)
+            assert np.array_equal(want, got), (f, name, int((want != got).sum()))
+        if f == 2:
+            assert torch.isnan(vis).all()
+        else:
+            v = to_host(vis).reshape(-1, 4)
+            assert (v[:, 3] == 1).all() and (v[:, :3] >= 0).all() and (v[:, :3] <= 1).all()
+            print("frame %d mean visibility %.4f" % (f, float(v[:, :3].mean())))
+
+
 @pytest.mark.parametrize("view,shadows", [(1, 0), (3, 1), (4, 0)])
 def test_shade_filter_resolve(px, oracle, scene7, view, shadows):
     """Shading on identical hits (the oracle's FP32 Hit uploaded) isolates the
