@@ -264,6 +264,13 @@ __device__ inline float leafNewton(const LeafLocal& L, float t, float spanLo, fl
   const float rlen = 1.0f / L.len, rhw2 = 1.0f / hw2;
   const float e = 0.00015f;
   const float twoE = 2.0f * e, rTwoE = 1.0f / (2.0f * e);
+  // Each iteration is a pure function of t, so the solve stops once the rest
+  // is known: an update that leaves t's bits unchanged repeats forever, and
+  // one that brings t back to its value two updates earlier alternates
+  // between the last two values through the 7th iteration, whose value the
+  // parity of the remaining count picks. The derivative break cannot fire
+  // later: every value the rest would visit has already passed it.
+  float tPrev = t;
   for (int k = 0; k < 7; k++) {
     float xx = L.x + L.dx * t, yy = L.y + L.dy * t;
     LeafU u0 = leafUTerms(xx, L.len, rlen, L.twist), up = leafUTerms(xx + e, L.len, rlen, L.twist),
@@ -275,8 +282,15 @@ __device__ inline float leafNewton(const LeafLocal& L, float t, float spanLo, fl
     float hy = divKnown(leafHeightOf(u0, vp, L.curl, band) - leafHeightOf(u0, vm, L.curl, band), twoE, rTwoE);
     float deriv = L.dz - hx * L.dx - hy * L.dy;
     if (fabsf(deriv) < 0.00001f) break;
-    t -= clampf(f / deriv, -0.13f, 0.13f);
-    t = clampf(t, spanLo, spanHi);
+    const float tn = clampf(t - clampf(f / deriv, -0.13f, 0.13f), spanLo, spanHi);
+    const uint32_t bn = __float_as_uint(tn);
+    if (bn == __float_as_uint(t)) break;
+    if (bn == __float_as_uint(tPrev)) {
+      if (((6 - k) & 1) == 0) t = tn;
+      break;
+    }
+    tPrev = t;
+    t = tn;
   }
   float xx = L.x + L.dx * t, yy = L.y + L.dy * t;
   float err = fabsf(L.z + L.dz * t -

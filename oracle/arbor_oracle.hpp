@@ -642,6 +642,15 @@ template <class R> struct Arbor {
   // Newton solution never before the slab entry. Results must be
   // bit-identical with the flag off (checked on whole frames).
   static inline bool skipBeyondBest = false;
+  // When set, the leaf Newton solve stops as soon as its remaining iterations
+  // are known: an update that leaves t unchanged bit for bit repeats forever
+  // (each iteration is a pure function of t), and one that returns t to its
+  // value two updates earlier alternates between the last two values until
+  // the 7th, whose value the parity of the remaining count selects. The
+  // derivative break cannot fire later in either case: every value involved
+  // already passed it. Results must be bit-identical with the flag off.
+  static inline bool newtonExits = false;
+  static bool sameBits(R a, R b) { return std::memcmp(&a, &b, sizeof(R)) == 0; }
   static R woodT(const R* P, int id, F3 ro, F3 rd, R footprint, bool refine, R bestSoFar = FAR()) {
     int b = id * PS;
     F3 a = ld3(P, b), end = ld3(P, b + 4);
@@ -696,6 +705,7 @@ template <class R> struct Arbor {
     R t = m_abs(dz) > K(0.000001) ? -z / dz : (span.x + span.y) * K(0.5);
     t = clampf(t, m_max(K(0.0001), span.x), span.y);
     const R curl = P[b + 17], twist = P[b + 18];
+    R tPrev = t;  // t before the previous update
     for (int k = 0; k < 7; k++) {
       R xx = x + dx * t, yy = y + dy * t;
       R height = leafHeight(xx, yy, ln, width, curl, twist, footprint);
@@ -707,8 +717,14 @@ template <class R> struct Arbor {
               leafHeight(xx, yy - e, ln, width, curl, twist, footprint)) / (K(2.0) * e);
       R deriv = dz - hx * dx - hy * dy;
       if (m_abs(deriv) < K(0.00001)) break;
-      t -= clampf(f / deriv, K(-0.13), K(0.13));
-      t = clampf(t, span.x, span.y);
+      R tn = t - clampf(f / deriv, K(-0.13), K(0.13));
+      tn = clampf(tn, span.x, span.y);
+      if (newtonExits) {
+        if (sameBits(tn, t)) break;                          // fixed point
+        if (sameBits(tn, tPrev)) { if (((6 - k) & 1) == 0) t = tn; break; }  // 2-cycle
+      }
+      tPrev = t;
+      t = tn;
     }
     R xx = x + dx * t, yy = y + dy * t;
     R err = m_abs(z + dz * t - leafHeight(xx, yy, ln, width, curl, twist, footprint));

@@ -73,6 +73,8 @@ extern "C" {
 void orc_set_cull(int on) { ACF::cullSpheres = on != 0; }
 // Early exits of woodT/leafT past the caller's best on/off for the counting instantiation.
 void orc_set_skip(int on) { ACF::skipBeyondBest = on != 0; }
+// Early exits of the leaf Newton solve (fixed point, 2-cycle) on/off for the counting instantiation.
+void orc_set_newton_exits(int on) { ACF::newtonExits = on != 0; }
 
 // Counts the operations of tracePixels (forest, lean walk) over every
 // stride-th pixel in x and y of a width x height view. out[0] rays, out[1]
@@ -141,6 +143,33 @@ int orc_skip_mismatches(const float* P, const float* Nodes, const unsigned* Orde
         A::shadePixel(P, Nodes, Order, C, h[skip].data(), l[skip].data(), g.data(), width, height, x, y);
   }
   A::skipBeyondBest = false;
+  int bad = 0;
+  for (size_t i = 0; i < n; i += 4)
+    if (std::memcmp(&h[0][i], &h[1][i], 16) != 0 || std::memcmp(&l[0][i], &l[1][i], 16) != 0) bad++;
+  return bad;
+}
+
+// Full 7-iteration leaf Newton solves vs solves that stop at a bit-exact
+// fixed point or 2-cycle (the GPU's leafNewton), over whole frames (trace,
+// and shade with sun visibility on): pixels whose Hit or Linear differs in
+// any bit (0 expected).
+int orc_newton_exit_mismatches(const float* P, const float* Nodes, const unsigned* Order, const float* C, int width,
+                               int height) {
+  using A = Arbor<float>;
+  size_t n = (size_t)width * height * 4;
+  std::vector<float> h[2] = {std::vector<float>(n), std::vector<float>(n)};
+  std::vector<float> l[2] = {std::vector<float>(n), std::vector<float>(n)}, g(n);
+  for (int on = 0; on < 2; on++) {
+    A::newtonExits = on != 0;
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = 0; y < height; y++)
+      for (int x = 0; x < width; x++) A::tracePixel(P, Nodes, Order, C, h[on].data(), width, height, x, y);
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = 0; y < height; y++)
+      for (int x = 0; x < width; x++)
+        A::shadePixel(P, Nodes, Order, C, h[on].data(), l[on].data(), g.data(), width, height, x, y);
+  }
+  A::newtonExits = false;
   int bad = 0;
   for (size_t i = 0; i < n; i += 4)
     if (std::memcmp(&h[0][i], &h[1][i], 16) != 0 || std::memcmp(&l[0][i], &l[1][i], 16) != 0) bad++;

@@ -177,6 +177,7 @@ class Oracle(_Common):
         self._lean_mismatch = _bind(L, "orc_lean_walk_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i, _i])
         self._cull_mismatch = _bind(L, "orc_cull_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
         self._skip_mismatch = _bind(L, "orc_skip_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
+        self._newton_mismatch = _bind(L, "orc_newton_exit_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
         self._generate_cr = _bind(L, "orc_generate_cr", _i, [_i, _f32p, _f32p, _u32p, _u32p, _f32p])
 
     def generate_cr(self, seed):
@@ -201,6 +202,11 @@ class Oracle(_Common):
         """Pixels whose Hit or Linear changes when woodT/leafT first reject
         rays missing their padded bounding spheres (the GPU's culls)."""
         return self._cull_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h)
+
+    def newton_exit_mismatches(self, s, C, w, h):
+        """Pixels whose Hit or Linear changes when the leaf Newton solve stops
+        at a bit-exact fixed point or 2-cycle (0 expected)."""
+        return self._newton_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h)
 
     def skip_mismatches(self, s, C, w, h):
         """Pixels whose Hit or Linear changes when woodT/leafT stop once their

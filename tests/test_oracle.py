@@ -307,6 +307,23 @@ def test_early_exits_change_nothing(oracle, scenes, frame, view):
     assert oracle.skip_mismatches(scenes["7"], C, 200, 120) == 0
 
 
+@pytest.mark.parametrize("frame,view,wind", [(0, 0, 0.0), (40, 0, 0.6), (0, 2, 0.6), (0, 4, 0.0)])
+def test_newton_exits_change_nothing(oracle, scenes, frame, view, wind):
+    """The leaf Newton solve stopping at a bit-exact fixed point or 2-cycle
+    (its remaining iterations then known) leaves every Hit and Linear bit
+    unchanged: trace, and shade with sun visibility on, with and without
+    wind."""
+    if view:
+        C = oracle.init_camera(7)
+        I = np.zeros(32, np.float32)
+        I[5] = view
+        oracle.update_camera(C, I, 1.0 / 60, 200, 120)
+    else:
+        C = _walk_camera(oracle, frame, 200, 120, 1)
+    C[22], C[11], C[8] = 1, wind, 0.75
+    assert oracle.newton_exit_mismatches(scenes["7"], C, 200, 120) == 0
+
+
 def test_operation_count_is_stable(oracle, scenes):
     """The op counter is deterministic and reports the workload's mix."""
     C = oracle.init_camera(7)
