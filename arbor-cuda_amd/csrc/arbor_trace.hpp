@@ -229,8 +229,9 @@ __device__ inline float woodT(const Prim& pr, v3 ro, v3 rd, float footprint, boo
   }
   float t = tubeT(ro, rd, a, end, pr.ra(), pr.rb());
   if (t >= kFar) return kFar;
-  // The march only moves t forward from the tube hit: from t >= best it can
-  // only return a value the caller discards (it keeps strictly nearer hits).
+  // The march only moves t forward from the tube hit: from t >= best (at the
+  // tube hit or at any later step) it can only return a value the caller
+  // discards (it keeps strictly nearer hits).
   if (t >= best) return kFar;
   v3 lo, hi;
   primBounds(pr, lo, hi);
@@ -241,7 +242,7 @@ __device__ inline float woodT(const Prim& pr, v3 ro, v3 rd, float footprint, boo
     float f = woodField(wf, ro + rd * t);
     if (f < eps) return t;
     t += fmaxf(eps * 0.35f, f * 0.27f);
-    if (t > span.y) return kFar;
+    if (t > span.y || t >= best) return kFar;
   }
   return kFar;
 }

@@ -638,8 +638,8 @@ template <class R> struct Arbor {
 
   // When set, woodT and leafT stop once their answer is known to be >= the
   // caller's current best (a result the caller discards: it keeps strictly
-  // nearer hits): the wood march never moves before the tube hit, the leaf
-  // Newton solution never before the slab entry. Results must be
+  // nearer hits): the wood march never moves before the tube hit and never
+  // back from any step, the leaf Newton solution never before the slab entry. Results must be
   // bit-identical with the flag off (checked on whole frames).
   static inline bool skipBeyondBest = false;
   // When set, the leaf Newton solve stops as soon as its remaining iterations
@@ -674,6 +674,7 @@ template <class R> struct Arbor {
       if (f < eps) return t;
       t += m_max(eps * K(0.35), f * K(0.27));
       if (t > span.y) return FAR();
+      if (skipBeyondBest && t >= bestSoFar) return FAR();  // t only grows: any hit is >= best
     }
     return FAR();
   }
