@@ -158,13 +158,13 @@ AD SunRay sunRayOf(const ShadeCtx& sc, int pixel, v3 p, int id) {
 // Sun visibility of one pixel from its Hit record: lightVisibility along its
 // sun ray, or 1 for sky or with tree-to-tree visibility off.
 template <class W>
-__device__ inline v3 sunVisibilityOne(const float* P, const float4* Nodes4, const uint32_t* Order,
+__device__ inline v3 sunVisibilityOne(const float* P, const float4* Nodes4, const float4* top, const uint32_t* Order,
                                       const TreeInst* trees, const ShadeCtx& sc, int x, int y, float4 hit) {
   if (!sc.shadows || !(hit.x < kFar)) return mk(1.0f, 1.0f, 1.0f);
   v3 rd = viewRay(sc.view, x, y, sc.width, sc.height);
   v3 p = sc.view.eye + rd * hit.x;
   SunRay s = sunRayOf(sc, y * sc.width + x, p, (int)hit.y);
-  return lightVisibilityInline<W>(P, Nodes4, nullptr, Order, trees, s.origin, s.light, sc.time, sc.wind);
+  return lightVisibilityInline<W>(P, Nodes4, top, Order, trees, s.origin, s.light, sc.time, sc.wind);
 }
 
 // One pixel of shadePixels (Arbor.cu:277-289): Linear and Guide of pixel

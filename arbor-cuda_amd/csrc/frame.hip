@@ -72,15 +72,17 @@ __global__ __launch_bounds__(kWaves * 64) void k_tracePixels(const float* P, con
 // runs the trace kernel's launch shape (one wave64 per 8x8 tile, the walk
 // inlined, optional tile schedule), so shading's register budget no longer
 // limits how many shadow traversals are in flight.
-template <int kWaves, class W>
-__global__ __launch_bounds__(kWaves * 64) void k_sunVisibility(const float* P, const float4* Nodes4,
+template <int kWaves, class W, int kMinWaves = 1>
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(kMinWaves))) void k_sunVisibility(const float* P, const float4* Nodes4,
                                                                const uint32_t* Order, const float* C, const float4* Hit,
                                                                float4* Vis, int width, int height, int tilesX,
                                                                int tiles, const uint32_t* tileOrder,
                                                                uint32_t* tileCost) {
   __shared__ TreeInst trees[kTrees];
+  __shared__ float4 top[W::ldsTop ? 2 * kTopNodes : 1];
   if (!(C[22] > 0.5f)) return;  // visibility off: shading never reads Vis
   fillTrees(trees, (int)C[31]);
+  fillTop<W>(top, Nodes4);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = blockIdx.x * kWaves + wave; i < tiles; i += gridDim.x * kWaves) {
@@ -91,7 +93,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_sunVisibility(const float* P, c
       const int pixel = y * width + x;
       // per tile from C: the context is dead during the traversal instead of live across the loop
       const ShadeCtx sc = makeShadeCtx(C, width, height);
-      v3 v = sunVisibilityOne<W>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel]);
+      v3 v = sunVisibilityOne<W>(P, Nodes4, top, Order, trees, sc, x, y, Hit[pixel]);
       Vis[pixel] = make_float4(v.x, v.y, v.z, 1.0f);
     }
     if (tileCost && lane == 0) {
@@ -503,11 +505,30 @@ int arbor_shadePixelsVis(const float* P, const float* Nodes, const unsigned* Ord
 }
 
 namespace {
+typedef void (*VisKernel)(const float*, const float4*, const uint32_t*, const float*, const float4*, float4*, int, int,
+                          int, int, const uint32_t*, uint32_t*);
+struct VisVariant {
+  VisKernel kernel;
+  int waves;
+};
+// vis_variant index -> kernel. Identical results for every entry. The
+// kernel keeps the 324-entry tree table (11.7 KB) in LDS per block, so
+// one-wave blocks stop at 14 waves per CU; two-wave blocks with a 96-VGPR
+// budget (entry 6, the default) reach 5 waves per SIMD.
+const VisVariant kVisVariants[] = {
+    {k_sunVisibility<1, WalkCull>, 1},         {k_sunVisibility<1, WalkCullPrefetch>, 1},
+    {k_sunVisibility<1, WalkCullTop>, 1},      {k_sunVisibility<2, WalkCull>, 2},
+    {k_sunVisibility<4, WalkCull>, 4},         {k_sunVisibility<1, WalkPlain>, 1},
+    {k_sunVisibility<2, WalkCull, 5>, 2},      {k_sunVisibility<4, WalkCull, 5>, 4},
+};
+constexpr int kVisVariantCount = sizeof(kVisVariants) / sizeof(kVisVariants[0]);
+
 int launchVisibility(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
                      float* Vis, int width, int height, const uint32_t* tileOrder, uint32_t* tileCost,
                      hipStream_t stream) {
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
-  hipLaunchKernelGGL((k_sunVisibility<1, WalkCull>), dim3(tileGrid(tiles, 1)), dim3(64), 0, stream, P,
+  const VisVariant& v = kVisVariants[tuning().vis_variant];
+  hipLaunchKernelGGL(v.kernel, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Vis), width, height, tx, tiles, tileOrder, tileCost);
   return launch_status();
@@ -575,11 +596,12 @@ int arbor_tune_set(const char* key, int value) {
   Tuning& t = tuning();
   struct { const char* name; int* slot; int lo, hi; } keys[] = {
       {"trace_variant", &t.trace_variant, 0, kTraceVariantCount + 1},
-      {"grid_cap", &t.grid_cap, 1, 65535},
+      {"grid_cap", &t.grid_cap, 1, 1 << 20},
       {"sched_tiles", &t.sched_tiles, 1, 4096},
       {"sched_leaf_min", &t.sched_leaf_min, 1, 64},
       {"sched_newton_min", &t.sched_newton_min, 1, 64},
-      {"batch_variant", &t.batch_variant, 0, batchVariantCount() - 1}};
+      {"batch_variant", &t.batch_variant, 0, batchVariantCount() - 1},
+      {"vis_variant", &t.vis_variant, 0, kVisVariantCount - 1}};
   for (auto& k : keys)
     if (!strcmp(key, k.name)) {
       if (value < k.lo || value > k.hi) return ARBOR_ERR_ARG;
@@ -598,11 +620,12 @@ int arbor_tune_get(const char* key) {
   if (!strcmp(key, "sched_leaf_min")) return t.sched_leaf_min;
   if (!strcmp(key, "sched_newton_min")) return t.sched_newton_min;
   if (!strcmp(key, "batch_variant")) return t.batch_variant;
+  if (!strcmp(key, "vis_variant")) return t.vis_variant;
   return ARBOR_ERR_ARG;
 }
 
 static const char* const kTuneKeys[] = {"trace_variant", "grid_cap", "sched_tiles", "sched_leaf_min",
-                                        "sched_newton_min", "batch_variant"};
+                                        "sched_newton_min", "batch_variant", "vis_variant"};
 
 // Tuning cache: one "key value" line per key; unknown keys and out-of-range
 // values in a file are rejected as a whole.

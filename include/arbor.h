@@ -166,13 +166,14 @@ const char* arbor_version(void);
 
 /* Launch tuning (no counterpart in the reference, whose launch shapes are
  * fixed by its WebGPU host). Keys: "trace_variant" (0 per-lane walk, 1
- * wave-batched leaf tests), "grid_cap" (max blocks of a tile kernel, 1..65535).
+ * wave-batched leaf tests), "grid_cap" (max blocks of a tile kernel, 1..1048576).
  * Every setting gives identical results. Process-wide; set before launching. */
 int arbor_tune_set(const char* key, int value);
 /* Current value of a tuning key, or a negative status for an unknown key.
  * Further keys: "sched_tiles" (1..4096), "sched_leaf_min" and
- * "sched_newton_min" (1..64) for the wave-scheduled trace kernel, and
- * "batch_variant" (arbor_batch.h). */
+ * "sched_newton_min" (1..64) for the wave-scheduled trace kernel,
+ * "batch_variant" (arbor_batch.h) and "vis_variant" (0..7, the
+ * sun-visibility kernel's launch shape and register budget). */
 int arbor_tune_get(const char* key);
 /* Persisted tuning cache: writes / reads "key value" lines. A file with an
  * unknown key or out-of-range value is rejected whole (tuning unchanged). */

@@ -199,6 +199,34 @@ def test_visibility_pass_identical(px, oracle, scene7):
             print("frame %d mean visibility %.4f" % (f, float(v[:, :3].mean())))
 
 
+def test_visibility_variants_identical(px, oracle, scene7):
+    """Every vis_variant (launch shape, walk options, register budget) gives
+    the same Vis bit for bit."""
+    import torch
+
+    print("case: visibility variants seed 7 view 3 240x136, wind", flush=True)
+    dev, _ = scene7
+    w, h = 240, 136
+    C = _camera(oracle, 7, 3, w, h, shadows=1)
+    C[11], C[8] = 0.6, 0.5
+    Cd = to_dev(C)
+    fb = px.FrameBuffers(w, h)
+    px.trace(dev, Cd, fb)
+    old = px.tune_get("vis_variant")
+    outs = []
+    try:
+        for v in range(8):
+            px.tune_set("vis_variant", v)
+            vis = torch.zeros(w * h * 4, dtype=torch.float32, device="cuda")
+            px.sun_visibility(dev, Cd, fb, vis)
+            outs.append(to_host(vis))
+    finally:
+        px.tune_set("vis_variant", old)
+    assert (outs[0].reshape(-1, 4)[:, :3] < 1).mean() > 0.05  # some pixels in shadow
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32))
+
+
 @pytest.mark.parametrize("view,shadows", [(1, 0), (3, 1), (4, 0)])
 def test_shade_filter_resolve(px, oracle, scene7, view, shadows):
     """Shading on identical hits (the oracle's FP32 Hit uploaded) isolates the
