@@ -47,6 +47,7 @@ struct arbor_engine {
   bool timing = false;
   float stage_ms[6] = {0, 0, 0, 0, 0, 0};  // update, trace, shade, filter, resolve, whole frame
   float input[ARBOR_I_FLOATS] = {};
+  FILE* rec = nullptr;  // input recording (arbor_engine_record)
 };
 
 namespace {
@@ -59,6 +60,35 @@ void free_pixels(arbor_engine* e) {
 }
 
 }  // namespace
+
+// Input recording: "ARBORIN1", then int32 version, width, height, seed; then
+// per frame float dt, int32 samples, float input[32] (little-endian, 136 B).
+constexpr char kRecMagic[8] = {'A', 'R', 'B', 'O', 'R', 'I', 'N', '1'};
+constexpr int kRecVersion = 1;
+struct RecFrame {
+  float dt;
+  int32_t samples;
+  float input[ARBOR_I_FLOATS];
+};
+static_assert(sizeof(RecFrame) == 136, "recorded frame");
+
+// Appends the frame about to run to the recording (a device input is copied
+// back first, which waits for the stream).
+int recordFrame(arbor_engine* e, float dt, const float* device_input, int samples) {
+  if (!e->rec) return ARBOR_OK;
+  RecFrame r;
+  r.dt = dt;
+  r.samples = samples;
+  if (device_input) {
+    if (hipMemcpyAsync(r.input, device_input, sizeof(r.input), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+        hipStreamSynchronize(e->stream) != hipSuccess)
+      return ARBOR_ERR_DEVICE;
+  } else {
+    memcpy(r.input, e->input, sizeof(r.input));
+  }
+  if (fwrite(&r, sizeof(r), 1, e->rec) != 1) return ARBOR_ERR_ARG;
+  return ARBOR_OK;
+}
 
 // shadePixels for the engine: the sun-visibility pass (tile-scheduled) and
 // shading from it; Linear and Guide equal arbor_shadePixels'.
@@ -118,6 +148,7 @@ int arbor_engine_create(arbor_engine** out, int width, int height, int seed, int
 
 int arbor_engine_destroy(arbor_engine* e) {
   if (!e) return ARBOR_ERR_NULL;
+  if (e->rec) fclose(e->rec);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   free_pixels(e);
   release(e->b.B); release(e->b.P); release(e->b.Keys); release(e->b.Order); release(e->b.Nodes);
@@ -198,6 +229,8 @@ int arbor_engine_set_timing(arbor_engine* e, int on) {
 int arbor_engine_frame(arbor_engine* e, float dt, const float* device_input) {
   if (!e) return ARBOR_ERR_NULL;
   if (!e->generated || !e->b.Hit) return ARBOR_ERR_STATE;
+  int rc0 = recordFrame(e, dt, device_input, 0);
+  if (rc0 != ARBOR_OK) return rc0;
   const float* I = device_input;
   if (!I) {
     if (hipMemcpyAsync(e->b.I, e->input, sizeof(e->input), hipMemcpyHostToDevice, e->stream) != hipSuccess)
@@ -380,6 +413,8 @@ int arbor_engine_frame_samples(arbor_engine* e, float dt, const float* device_in
   if (!e) return ARBOR_ERR_NULL;
   if (!e->generated || !e->b.Hit) return ARBOR_ERR_STATE;
   if (samples < 1 || samples > 512) return ARBOR_ERR_ARG;
+  int rc0 = recordFrame(e, dt, device_input, samples);
+  if (rc0 != ARBOR_OK) return rc0;
   const float* I = device_input;
   if (!I) {
     if (hipMemcpyAsync(e->b.I, e->input, sizeof(e->input), hipMemcpyHostToDevice, e->stream) != hipSuccess)
@@ -399,6 +434,58 @@ int arbor_engine_frame_samples(arbor_engine* e, float dt, const float* device_in
     if (rc == ARBOR_OK) rc = arbor_resolvePixels(e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w, h, e->stream);
   }
   if (rc == ARBOR_OK) e->frames++;
+  return rc;
+}
+
+int arbor_engine_record(arbor_engine* e, const char* path) {
+  if (!e) return ARBOR_ERR_NULL;
+  if (e->rec) {
+    int bad = fclose(e->rec);
+    e->rec = nullptr;
+    if (bad) return ARBOR_ERR_ARG;
+  }
+  if (!path) return ARBOR_OK;
+  FILE* f = fopen(path, "wb");
+  if (!f) return ARBOR_ERR_ARG;
+  const int32_t head[4] = {kRecVersion, e->width, e->height, e->seed};
+  if (fwrite(kRecMagic, 8, 1, f) != 1 || fwrite(head, sizeof(head), 1, f) != 1) {
+    fclose(f);
+    return ARBOR_ERR_ARG;
+  }
+  e->rec = f;
+  return ARBOR_OK;
+}
+
+int arbor_engine_replay(arbor_engine* e, const char* path, int first, int count, int* played) {
+  if (!e || !path) return ARBOR_ERR_NULL;
+  if (played) *played = 0;
+  if (first < 0) return ARBOR_ERR_ARG;
+  if (!e->generated || !e->b.Hit) return ARBOR_ERR_STATE;
+  FILE* f = fopen(path, "rb");
+  if (!f) return ARBOR_ERR_ARG;
+  char magic[8];
+  int32_t head[4];
+  int rc = ARBOR_OK;
+  if (fread(magic, 8, 1, f) != 1 || memcmp(magic, kRecMagic, 8) != 0 || fread(head, sizeof(head), 1, f) != 1 ||
+      head[0] != kRecVersion)
+    rc = ARBOR_ERR_ARG;
+  if (rc == ARBOR_OK && fseek(f, (long)first * (long)sizeof(RecFrame), SEEK_CUR) != 0) rc = ARBOR_ERR_ARG;
+  int n = 0;
+  RecFrame r;
+  FILE* keep = e->rec;  // a replay is not recorded again
+  e->rec = nullptr;
+  while (rc == ARBOR_OK && (count <= 0 || n < count) && fread(&r, sizeof(r), 1, f) == 1) {
+    if (r.samples < 0 || r.samples > 512 || !(r.dt == r.dt)) {
+      rc = ARBOR_ERR_ARG;
+      break;
+    }
+    memcpy(e->input, r.input, sizeof(e->input));
+    rc = r.samples == 0 ? arbor_engine_frame(e, r.dt, nullptr) : arbor_engine_frame_samples(e, r.dt, nullptr, r.samples);
+    if (rc == ARBOR_OK) n++;
+  }
+  e->rec = keep;
+  fclose(f);
+  if (played) *played = n;
   return rc;
 }
 

@@ -66,6 +66,8 @@ SIGNATURES = {
     "arbor_engine_render": [_vp, _i, _i, _i],
     "arbor_engine_frame_samples": [_vp, ct.c_float, _vp, _i],
     "arbor_engine_save_ppm": [_vp, ct.c_char_p],
+    "arbor_engine_record": [_vp, ct.c_char_p],
+    "arbor_engine_replay": [_vp, ct.c_char_p, _i, _i, ct.POINTER(ct.c_int)],
     "arbor_write_ppm": [ct.c_char_p, ct.POINTER(ct.c_uint), _i, _i],
     "arbor_tile_schedule_bytes": [_i, _i],
     "arbor_tracePixelsScheduled": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
@@ -545,6 +547,18 @@ class Engine:
 
     def save_ppm(self, path):
         check(self._lib.arbor_engine_save_ppm(self.h, str(path).encode()), "arbor_engine_save_ppm")
+
+    def record(self, path):
+        """Record every later frame's inputs to path (None stops)."""
+        check(self._lib.arbor_engine_record(self.h, None if path is None else str(path).encode()),
+              "arbor_engine_record")
+
+    def replay(self, path, first=0, count=0):
+        """Run recorded frames; returns how many ran."""
+        n = ct.c_int(0)
+        check(self._lib.arbor_engine_replay(self.h, str(path).encode(), first, count, ct.byref(n)),
+              "arbor_engine_replay")
+        return n.value
 
     def inspect(self):
         import json

@@ -327,23 +327,14 @@ def walk_inputs(px, seed, n):
     return np.stack(rows).astype(np.float32)
 
 
-def cpu_baseline(seed, width, height, threads):
-    """The reference CPU harness (compiled from its sources into
-    oracle/_ref/libarborref.so) on this host's cores, one frame (walk frame 0)
-    of the same workload: trace + shade + filter + resolve."""
+def _cpu_frame(cpu, seed, width, height, threads):
+    """One frame (walk frame 0) of the workload on a CPU implementation:
+    (seconds for trace+shade+filter+resolve, seconds for the trace)."""
     import numpy as np
-
-    sys.path.insert(0, os.path.join(HERE, "tests"))
-    import oracle_lib
-
-    kind = "reference" if oracle_lib.ref_available() else "port"
-    cpu = oracle_lib.RefHarness() if kind == "reference" else oracle_lib.Oracle()
-    if threads <= 0:
-        threads = min(16, len(os.sched_getaffinity(0)))
-    scene = cpu.generate(seed)
-    C = cpu.init_camera(seed)
     import pyarbor
 
+    scene = cpu.generate(seed)
+    C = cpu.init_camera(seed)
     I = walk_inputs(pyarbor, seed, 1)[0]
     cpu.update_camera(C, I, 1.0 / 60.0, width, height)
     t0 = time.perf_counter()
@@ -354,11 +345,32 @@ def cpu_baseline(seed, width, height, threads):
     hist = np.zeros_like(F)
     cpu.resolve(F, hist, C, width, height, threads)
     t2 = time.perf_counter()
+    return t2 - t0, t1 - t0
+
+
+def cpu_baseline(seed, width, height, threads):
+    """The reference CPU harness (compiled from its sources into
+    oracle/_ref/libarborref.so) on this host's cores, one frame (walk frame 0)
+    of the same workload: trace + shade + filter + resolve. The oracle's float
+    restatement (same outputs bit for bit) is timed beside it as "port"."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import oracle_lib
+
+    if threads <= 0:
+        threads = min(16, len(os.sched_getaffinity(0)))
     rays = width * height
-    return {"value": round(rays / (t2 - t0) / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": kind,
+    port_s, port_trace_s = _cpu_frame(oracle_lib.Oracle(), seed, width, height, threads)
+    kind = "reference" if oracle_lib.ref_available() else "port"
+    if kind == "reference":
+        frame_s, trace_s = _cpu_frame(oracle_lib.RefHarness(), seed, width, height, threads)
+    else:
+        frame_s, trace_s = port_s, port_trace_s
+    return {"value": round(rays / frame_s / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": kind,
             "sample": "walk frame 0 of the same workload (%dx%d, seed %d, shadows off), trace+shade+filter+resolve, "
                       "OpenMP schedule(dynamic,2) rows" % (width, height, seed),
-            "ms_per_frame": round((t2 - t0) * 1e3, 1), "trace_ms": round((t1 - t0) * 1e3, 1)}
+            "ms_per_frame": round(frame_s * 1e3, 1), "trace_ms": round(trace_s * 1e3, 1),
+            "port": {"value": round(rays / port_s / 1e6, 4), "ms_per_frame": round(port_s * 1e3, 1),
+                     "trace_ms": round(port_trace_s * 1e3, 1)}}
 
 
 def main():
@@ -471,6 +483,7 @@ def main():
         cb = cpu_baseline(args.seed, W, H, args.cpu_threads)
         result["cpu_baseline"] = cb
         result["gpu_vs_cpu"] = round(result["value"] / cb["value"], 1)
+        result["gpu_vs_cpu_port"] = round(result["value"] / cb["port"]["value"], 1)
     print(json.dumps(result), flush=True)
 
 

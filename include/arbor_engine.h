@@ -59,6 +59,18 @@ int arbor_engine_render(arbor_engine* e, int view, int samples, int light);
  * resolvePixels (History: the frame's sample mean). The multi-sample frame
  * of the 4-sample, sun-visibility configuration. */
 int arbor_engine_frame_samples(arbor_engine* e, float dt, const float* device_input, int samples);
+/* Input recording (no counterpart in the reference, whose inputs come from
+ * the browser each frame): with path non-NULL every later frame's dt, sample
+ * count and 32 input floats are appended to the file ("ARBORIN1", version,
+ * width, height, seed, then 136 bytes per frame); a frame given a device
+ * input copies it back first, which waits for the stream. NULL stops. */
+int arbor_engine_record(arbor_engine* e, const char* path);
+/* Replays a recording: frames first .. first+count-1 (count <= 0: to the end)
+ * through arbor_engine_frame / arbor_engine_frame_samples as recorded, with
+ * the recorded inputs. *played (may be NULL) = frames run. A replay on an
+ * engine of the recording's size and seed, from the same state, reproduces
+ * its Pixels bit for bit. ARBOR_ERR_ARG for a file that is not a recording. */
+int arbor_engine_replay(arbor_engine* e, const char* path, int first, int count, int* played);
 /* native_harness.cpp:53 save(): binary PPM (P6) of the engine's Pixels, RGB
  * from the low three bytes of each packed pixel. */
 int arbor_engine_save_ppm(arbor_engine* e, const char* path);

@@ -101,6 +101,13 @@ def test_engine_resize_rounding(L):
         assert (cw.value, ch.value) == want
 
 
+def test_engine_record_replay_argument_checks(L, tmp_path):
+    assert L.arbor_engine_record(None, b"x.rec") == -1
+    n = ct.c_int(5)
+    assert L.arbor_engine_replay(None, b"x.rec", 0, 0, ct.byref(n)) == -1
+    assert L.arbor_engine_replay(DUMMY, None, 0, 0, ct.byref(n)) == -1
+
+
 def test_tuning_keys_validated(L):
     assert L.arbor_tune_set(None, 1) == -1
     assert L.arbor_tune_set(b"no_such_key", 1) == -2

@@ -350,6 +350,48 @@ def test_engine_render_mode_and_ppm(px, oracle, tmp_path):
     eng.close()
 
 
+def test_engine_record_replay(px, tmp_path):
+    """Frames recorded on one engine (host and device inputs, a breeze toggle,
+    a multi-sample frame) replay on a fresh engine of the same size and seed
+    to the same camera state and bit-identical Pixels; partial replays and a
+    file that is not a recording are handled."""
+    import torch
+
+    print("case: engine record/replay 320x192 seed 7", flush=True)
+    w, h = 320, 192
+    rec = tmp_path / "walk.rec"
+    a = px.Engine(w, h, 7)
+    a.record(rec)
+    for f in range(5):
+        I = px.walk_input(7, f)
+        if f == 1:
+            I[7] = 1.0  # breeze on
+        if f % 2:
+            a.frame(1.0 / 60.0, torch.from_numpy(I).cuda())
+        else:
+            a.set_input(I)
+            a.frame(1.0 / 60.0)
+    a.set_input(px.walk_input(7, 5))
+    a.frame_samples(3, 1.0 / 60.0)
+    a.record(None)
+    want_c = a.download("C", 64).view(np.uint32).copy()
+    want_px = a.download("Pixels", w * h, np.uint32).copy()
+    a.frame(1.0 / 60.0)  # not recorded
+    assert rec.stat().st_size == 24 + 6 * 136
+    b = px.Engine(w, h, 7)
+    assert b.replay(rec) == 6
+    assert np.array_equal(b.download("C", 64).view(np.uint32), want_c)
+    assert np.array_equal(b.download("Pixels", w * h, np.uint32), want_px)
+    c = px.Engine(w, h, 7)
+    assert c.replay(rec, 4, 1) == 1 and c.replay(rec, 10) == 0
+    bad = tmp_path / "bad.rec"
+    bad.write_bytes(b"not a recording")
+    with pytest.raises(px.ArborError):
+        c.replay(bad)
+    for e in (a, b, c):
+        e.close()
+
+
 def test_engine_frame_pipeline(px, oracle):
     """The engine's 5-dispatch frame against the oracle pipeline (same camera
     path): pixels within 2/255 on >= 98% of pixels (near-tie rays may flip)."""
