@@ -268,14 +268,19 @@ AD float leafWidth(float u, float len, float halfWidth, float footprint) {
   float q = sat(u / len);
   float s = sinBounded(kPi * q);
   float lobes = 0.77f + 0.23f * cosBounded(q * kPi * 10.0f + 0.45f);
-  float fine = bandWeight(0.002f, footprint) * 0.015f * sinBounded(q * kPi * 160.0f);
+  // with a zero band weight the fine lobes are a signed zero and
+  // lobes + (+-0) = lobes (lobes >= 0.54): the sine is not evaluated
+  const float bw = bandWeight(0.002f, footprint);
+  float fine = bw == 0.0f ? 0.0f : bw * 0.015f * sinBounded(q * kPi * 160.0f);
   return halfWidth * powf(fmaxf(0.0f, s), 0.72f) * (lobes + fine);
 }
 AD float leafHeight(float u, float v, float len, float halfWidth, float curl, float twist, float footprint) {
   float q = clampf(u / len, -0.2f, 1.2f);
   float s = sinBounded(kPi * q);
   float coarse = curl * s + twist * (q - 0.4f) * v + 0.009f * (v * v / (halfWidth * halfWidth)) * s;
-  float mid = 0.0012f * expf(-fabsf(v) / 0.005f) * s * bandWeight(0.007f, footprint);
+  const float bw = bandWeight(0.007f, footprint);  // 0: (m * s) * 0 is s * 0 for any m > 0
+  float m = bw == 0.0f ? 1.0f : 0.0012f * expf(-fabsf(v) / 0.005f);
+  float mid = m * s * bw;
   return coarse + mid;
 }
 // leafHeight split into its u-only and v-only factors (same expressions; the
@@ -287,8 +292,12 @@ AD LeafU leafUTerms(float u, float len, float rlen, float twist) {
   float q = clampf(divKnown(u, len, rlen), -0.2f, 1.2f);
   return {sinBounded(kPi * q), twist * (q - 0.4f)};
 }
-AD LeafV leafVTerms(float v, float hw2, float rhw2) {
-  return {v, 0.009f * divKnown(v * v, hw2, rhw2), 0.0012f * expf(-divKnown(fabsf(v), 0.005f, 1.0f / 0.005f))};
+// mid = false when the midrib's band weight is 0: the midrib term is then
+// (m * s) * 0, a zero with the sign of s for any positive m, so m = 1 stands
+// in for the exponential.
+AD LeafV leafVTerms(float v, float hw2, float rhw2, bool mid = true) {
+  return {v, 0.009f * divKnown(v * v, hw2, rhw2),
+          mid ? 0.0012f * expf(-divKnown(fabsf(v), 0.005f, 1.0f / 0.005f)) : 1.0f};
 }
 AD float leafHeightOf(LeafU a, LeafV b, float curl, float band) {
   float coarse = curl * a.s + a.tw * b.v + b.w2 * a.s;

@@ -261,6 +261,7 @@ struct LeafLocal {
 };
 __device__ inline float leafNewton(const LeafLocal& L, float t, float spanLo, float spanHi) {
   const float band = bandWeight(0.007f, L.footprint);
+  const bool mid = band != 0.0f;
   const float hw2 = L.width * L.width;
   const float rlen = 1.0f / L.len, rhw2 = 1.0f / hw2;
   const float e = 0.00015f;
@@ -276,7 +277,8 @@ __device__ inline float leafNewton(const LeafLocal& L, float t, float spanLo, fl
     float xx = L.x + L.dx * t, yy = L.y + L.dy * t;
     LeafU u0 = leafUTerms(xx, L.len, rlen, L.twist), up = leafUTerms(xx + e, L.len, rlen, L.twist),
           um = leafUTerms(xx - e, L.len, rlen, L.twist);
-    LeafV v0 = leafVTerms(yy, hw2, rhw2), vp = leafVTerms(yy + e, hw2, rhw2), vm = leafVTerms(yy - e, hw2, rhw2);
+    LeafV v0 = leafVTerms(yy, hw2, rhw2, mid), vp = leafVTerms(yy + e, hw2, rhw2, mid),
+          vm = leafVTerms(yy - e, hw2, rhw2, mid);
     float height = leafHeightOf(u0, v0, L.curl, band);
     float f = L.z + L.dz * t - height;
     float hx = divKnown(leafHeightOf(up, v0, L.curl, band) - leafHeightOf(um, v0, L.curl, band), twoE, rTwoE);
@@ -295,7 +297,7 @@ __device__ inline float leafNewton(const LeafLocal& L, float t, float spanLo, fl
   }
   float xx = L.x + L.dx * t, yy = L.y + L.dy * t;
   float err = fabsf(L.z + L.dz * t -
-                    leafHeightOf(leafUTerms(xx, L.len, rlen, L.twist), leafVTerms(yy, hw2, rhw2), L.curl, band));
+                    leafHeightOf(leafUTerms(xx, L.len, rlen, L.twist), leafVTerms(yy, hw2, rhw2, mid), L.curl, band));
   // same predicate as the reference, cheapest terms first
   if (t > 0.0001f && xx > 0.0f && xx < L.len && err < 0.0003f && fabsf(yy) < leafWidth(xx, L.len, L.width, L.footprint))
     return t;

@@ -513,19 +513,28 @@ template <class R> struct Arbor {
     R a = windAngle(P, id, time, wind);
     return n * m_cos(a) - u * m_sin(a);
   }
+  // When set, a band-weighted detail term whose band weight is exactly 0 is
+  // not evaluated: leafWidth's fine lobes (bw * 0.015 * sin(...) is a signed
+  // zero, and lobes + (+-0) = lobes as lobes >= 0.54), and leafHeight's
+  // midrib, whose value (m * s) * 0 with m > 0 is s * 0 for any positive m.
+  // Results must be bit-identical with the flag off.
+  static inline bool bandSkips = false;
   static R leafWidth(R u, R length, R halfWidth, R footprint) {
     const R pi = PI();
     R q = sat(u / length);
     R s = m_sin(pi * q);
     R lobes = K(0.77) + K(0.23) * m_cos(q * pi * K(10.0) + K(0.45));
-    R fine = bandWeight(K(0.002), footprint) * K(0.015) * m_sin(q * pi * K(160.0));
+    R bw = bandWeight(K(0.002), footprint);
+    R fine = bandSkips && bw == K(0.0) ? K(0.0) : bw * K(0.015) * m_sin(q * pi * K(160.0));
     return halfWidth * m_pow(m_max(K(0.0), s), K(0.72)) * (lobes + fine);
   }
   static R leafHeight(R u, R v, R ln, R halfWidth, R curl, R twist, R footprint) {
     R q = clampf(u / ln, K(-0.2), K(1.2));
     R s = m_sin(PI() * q);
     R coarse = curl * s + twist * (q - K(0.4)) * v + K(0.009) * (v * v / (halfWidth * halfWidth)) * s;
-    R mid = K(0.0012) * m_exp(-m_abs(v) / K(0.005)) * s * bandWeight(K(0.007), footprint);
+    R bw = bandWeight(K(0.007), footprint);
+    R m = bandSkips && bw == K(0.0) ? K(1.0) : K(0.0012) * m_exp(-m_abs(v) / K(0.005));
+    R mid = m * s * bw;
     return coarse + mid;
   }
 

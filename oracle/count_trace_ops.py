@@ -4,7 +4,8 @@ Runs the oracle's counting instantiation (oracle/flops.cpp) on the
 full-forest workload bench.py times: seed 7, 324 trees, 1920x1080, shadows
 off, the seeded 120-frame walk. Counts tracePixels with the GPU's traversal
 (lean walk), bounding-sphere rejections, early exits past the caller's best
-and the leaf Newton solve's fixed-point/2-cycle exits, sampling every `--frame-step`-th
+the leaf Newton solve's fixed-point/2-cycle exits and the skips of
+band-weighted leaf terms with a zero band weight, sampling every `--frame-step`-th
 walk frame and every `--stride`-th pixel in x and y, and prints the mean.
 
 Test / measurement infrastructure: it calls the oracle, never the product.
@@ -53,6 +54,7 @@ def main():
     lib.orc_set_cull.argtypes = [ct.c_int]
     lib.orc_set_skip.argtypes = [ct.c_int]
     lib.orc_set_newton_exits.argtypes = [ct.c_int]
+    lib.orc_set_band_skips.argtypes = [ct.c_int]
     W, H, seed = 1920, 1080, 7
     s = o.generate(seed)
     C = o.init_camera(seed)
@@ -68,10 +70,12 @@ def main():
         lib.orc_set_cull(1)
         lib.orc_set_skip(1)
         lib.orc_set_newton_exits(1)
+        lib.orc_set_band_skips(1)
         r = o.count_trace_ops(s, C, W, H, args.stride)
         lib.orc_set_cull(0)
         lib.orc_set_skip(0)
         lib.orc_set_newton_exits(0)
+        lib.orc_set_band_skips(0)
         per_frame.append(r["ops_per_ray"])
         kinds = r if kinds is None else {k: kinds[k] + r[k] for k in r}
         print("frame %3d: %.1f ops/ray" % (f, r["ops_per_ray"]), flush=True)

@@ -75,6 +75,8 @@ void orc_set_cull(int on) { ACF::cullSpheres = on != 0; }
 void orc_set_skip(int on) { ACF::skipBeyondBest = on != 0; }
 // Early exits of the leaf Newton solve (fixed point, 2-cycle) on/off for the counting instantiation.
 void orc_set_newton_exits(int on) { ACF::newtonExits = on != 0; }
+// Skips of band-weighted terms whose band weight is 0 on/off for the counting instantiation.
+void orc_set_band_skips(int on) { ACF::bandSkips = on != 0; }
 
 // Counts the operations of tracePixels (forest, lean walk) over every
 // stride-th pixel in x and y of a width x height view. out[0] rays, out[1]
@@ -173,6 +175,63 @@ int orc_newton_exit_mismatches(const float* P, const float* Nodes, const unsigne
   int bad = 0;
   for (size_t i = 0; i < n; i += 4)
     if (std::memcmp(&h[0][i], &h[1][i], 16) != 0 || std::memcmp(&l[0][i], &l[1][i], 16) != 0) bad++;
+  return bad;
+}
+
+// Band-weighted leaf terms always evaluated vs skipped when their band weight
+// is 0 (the GPU's leafNewton and leafWidth), over whole frames (trace, and
+// shade with sun visibility on): pixels whose Hit or Linear differs in any
+// bit (0 expected).
+int orc_band_skip_mismatches(const float* P, const float* Nodes, const unsigned* Order, const float* C, int width,
+                               int height) {
+  using A = Arbor<float>;
+  size_t n = (size_t)width * height * 4;
+  std::vector<float> h[2] = {std::vector<float>(n), std::vector<float>(n)};
+  std::vector<float> l[2] = {std::vector<float>(n), std::vector<float>(n)}, g(n);
+  for (int on = 0; on < 2; on++) {
+    A::bandSkips = on != 0;
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = 0; y < height; y++)
+      for (int x = 0; x < width; x++) A::tracePixel(P, Nodes, Order, C, h[on].data(), width, height, x, y);
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = 0; y < height; y++)
+      for (int x = 0; x < width; x++)
+        A::shadePixel(P, Nodes, Order, C, h[on].data(), l[on].data(), g.data(), width, height, x, y);
+  }
+  A::bandSkips = false;
+  int bad = 0;
+  for (size_t i = 0; i < n; i += 4)
+    if (std::memcmp(&h[0][i], &h[1][i], 16) != 0 || std::memcmp(&l[0][i], &l[1][i], 16) != 0) bad++;
+  return bad;
+}
+
+// The same check on single leaves: n leaf tests with seeded leaves, rays
+// aimed at the leaf surface from 0.02-0.5 m off it, footprints log-uniform in
+// [1e-5, 1e-2] (across both bands' transitions) and wind on, plus
+// leafHeight/leafWidth at seeded (u, v): the number of results that differ
+// in any bit between skips off and on (0 expected).
+int orc_band_skip_leaf_mismatches(const float* P, int n) {
+  using A = Arbor<float>;
+  int bad = 0;
+  for (int i = 0; i < n; i++) {
+    auto r = [&](int k) { return A::rnd(i * 7 + k, 4242); };
+    const int id = WOOD_COUNT + (int)(r(0) * (float)LEAF_COUNT) % LEAF_COUNT, b = id * PS;
+    const float time = r(1) * 10.0f, wind = 0.6f, fp = std::pow(10.0f, -5.0f + 3.0f * r(2));
+    V3<float> u = A::leafU(P, id, time, wind), n3 = A::leafN(P, id, time, wind), v = A::ld3(P, b + 8);
+    V3<float> target = A::ld3(P, b) + u * (P[b + 16] + P[b + 7] * r(3)) + v * (P[b + 11] * (r(4) - 0.5f));
+    V3<float> off = n3 * (0.02f + 0.48f * r(5)) + u * (r(6) - 0.5f) * 0.1f + v * (r(7) - 0.5f) * 0.1f;
+    V3<float> ro = target + off, rd = A::unit(target - ro);
+    const float uu = P[b + 7] * (1.4f * r(8) - 0.2f), vv = P[b + 11] * (2.0f * r(9) - 1.0f);
+    float res[2][3];
+    for (int on = 0; on < 2; on++) {
+      A::bandSkips = on != 0;
+      res[on][0] = A::leafT(P, id, ro, rd, fp, time, wind);
+      res[on][1] = A::leafHeight(uu, vv, P[b + 7], P[b + 11], P[b + 17], P[b + 18], fp);
+      res[on][2] = A::leafWidth(uu, P[b + 7], P[b + 11], fp);
+    }
+    A::bandSkips = false;
+    if (std::memcmp(res[0], res[1], sizeof(res[0])) != 0) bad++;
+  }
   return bad;
 }
 

@@ -178,6 +178,8 @@ class Oracle(_Common):
         self._cull_mismatch = _bind(L, "orc_cull_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
         self._skip_mismatch = _bind(L, "orc_skip_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
         self._newton_mismatch = _bind(L, "orc_newton_exit_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
+        self._band_mismatch = _bind(L, "orc_band_skip_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
+        self._band_leaf_mismatch = _bind(L, "orc_band_skip_leaf_mismatches", _i, [_f32p, _i])
         self._generate_cr = _bind(L, "orc_generate_cr", _i, [_i, _f32p, _f32p, _u32p, _u32p, _f32p])
 
     def generate_cr(self, seed):
@@ -207,6 +209,16 @@ class Oracle(_Common):
         """Pixels whose Hit or Linear changes when the leaf Newton solve stops
         at a bit-exact fixed point or 2-cycle (0 expected)."""
         return self._newton_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h)
+
+    def band_skip_mismatches(self, s, C, w, h):
+        """Pixels whose Hit or Linear changes when band-weighted leaf terms
+        with a zero band weight are skipped (0 expected)."""
+        return self._band_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h)
+
+    def band_skip_leaf_mismatches(self, s, n):
+        """Single-leaf form of band_skip_mismatches: n seeded leaf tests (and
+        leafHeight/leafWidth evaluations) across the bands' transitions."""
+        return self._band_leaf_mismatch(s.P, n)
 
     def skip_mismatches(self, s, C, w, h):
         """Pixels whose Hit or Linear changes when woodT/leafT stop once their

@@ -324,6 +324,31 @@ def test_newton_exits_change_nothing(oracle, scenes, frame, view, wind):
     assert oracle.newton_exit_mismatches(scenes["7"], C, 200, 120) == 0
 
 
+@pytest.mark.parametrize("frame,view,wind,fov", [(0, 0, 0.0, 0.9), (40, 0, 0.6, 0.9), (0, 2, 0.0, 0.1),
+                                                  (0, 4, 0.6, 0.1), (0, 1, 0.0, 0.1)])
+def test_band_skips_change_nothing(oracle, scenes, frame, view, wind, fov):
+    """Skipping band-weighted leaf terms whose band weight is exactly 0 (the
+    midrib's exponential, the fine lobes' sine) leaves every Hit and Linear
+    bit unchanged: trace, and shade with sun visibility on. The narrow views
+    (FOV scale 0.1) put many leaves inside the bands' transition, where the
+    terms must still be evaluated."""
+    if view:
+        C = oracle.init_camera(7)
+        I = np.zeros(32, np.float32)
+        I[5] = view
+        oracle.update_camera(C, I, 1.0 / 60, 200, 120)
+    else:
+        C = _walk_camera(oracle, frame, 200, 120, 1)
+    C[22], C[11], C[8], C[14] = 1, wind, 0.75, fov
+    assert oracle.band_skip_mismatches(scenes["7"], C, 200, 120) == 0
+
+
+def test_band_skips_change_nothing_per_leaf(oracle, scenes):
+    """The same on 20,000 single leaf tests whose footprints span both bands'
+    transitions (whole frames rarely put leaves there)."""
+    assert oracle.band_skip_leaf_mismatches(scenes["7"], 20000) == 0
+
+
 def test_operation_count_is_stable(oracle, scenes):
     """The op counter is deterministic and reports the workload's mix."""
     C = oracle.init_camera(7)
