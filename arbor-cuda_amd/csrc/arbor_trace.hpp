@@ -361,7 +361,12 @@ __device__ inline float objectT(const Prim& pr, v3 ro, v3 rd, float footprint, f
 // Node n's record is two float4 {lo, count} {hi, id}; children 2n, 2n+1 are
 // adjacent, so an interior visit reads one 64-byte pair.
 struct NodeBox { float4 lo, hi; };
-AD bool spanHit(v2 s, float count, float best) { return count > 0.0f && s.y >= fmaxf(0.0001f, s.x) && s.x < best; }
+// Evaluated without short-circuit branches (same truth value): the three
+// tests are cheap, and a branch between them splits the slab test into two
+// blocks, where the compiler re-canonicalises its operands for the max.
+AD bool spanHit(v2 s, float count, float best) {
+  return (count > 0.0f) & (s.y >= fmaxf(0.0001f, s.x)) & (s.x < best);
+}
 AD int heapDepth(uint32_t node) { return 31 - __builtin_clz(node); }
 
 // Compile-time configuration of the walk. Every configuration visits the same
