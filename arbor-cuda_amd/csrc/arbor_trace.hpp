@@ -114,37 +114,43 @@ __device__ inline float barkCut(const WoodFrame& f, v3 p) {
   float theta = uv.x, arc = uv.y;
   const SinCos tc = sincosBounded(theta);
   float st = tc.s, ct = tc.c;
-  float x = (divKnown(theta, 2.0f * kPi, 1.0f / (2.0f * kPi)) + 0.5f) * (float)columns +
-            0.32f * sinBounded(arc * 6.7f + ct * 3.0f) + 0.15f * sinBounded(arc * 19.0f + st * 2.0f);
-  float y = divKnown(arc, 0.19f, 1.0f / 0.19f) + 0.25f * sinBounded(theta * 7.0f) + 0.1f * sinBounded(theta * 19.0f);
-  int ix = (int)floorf(x), iy = (int)floorf(y);
-  // the three columns ix-1, ix, ix+1 wrapped into [0, columns)
-  int wrap[3];
-  wrap[0] = (((ix - 1) % columns) + columns) % columns;
-  wrap[1] = wrap[0] + 1 == columns ? 0 : wrap[0] + 1;
-  wrap[2] = wrap[1] + 1 == columns ? 0 : wrap[1] + 1;
-  float first = 100.0f, second = 100.0f;
+  // A term whose band weight is 0 is a signed zero, and the running sum
+  // (from 0.32, never -0) is unchanged by adding it: the plates (the cell
+  // coordinates and the 3x3 Voronoi search) and the two noises are only
+  // evaluated under a nonzero band weight.
+  float sum = 0.32f;
+  if (coarse != 0.0f) {
+    float x = (divKnown(theta, 2.0f * kPi, 1.0f / (2.0f * kPi)) + 0.5f) * (float)columns +
+              0.32f * sinBounded(arc * 6.7f + ct * 3.0f) + 0.15f * sinBounded(arc * 19.0f + st * 2.0f);
+    float y = divKnown(arc, 0.19f, 1.0f / 0.19f) + 0.25f * sinBounded(theta * 7.0f) + 0.1f * sinBounded(theta * 19.0f);
+    int ix = (int)floorf(x), iy = (int)floorf(y);
+    // the three columns ix-1, ix, ix+1 wrapped into [0, columns)
+    int wrap[3];
+    wrap[0] = (((ix - 1) % columns) + columns) % columns;
+    wrap[1] = wrap[0] + 1 == columns ? 0 : wrap[0] + 1;
+    wrap[2] = wrap[1] + 1 == columns ? 0 : wrap[1] + 1;
+    float first = 100.0f, second = 100.0f;
 #pragma unroll
-  for (int j = -1; j <= 1; j++)
+    for (int j = -1; j <= 1; j++)
 #pragma unroll
-    for (int k = -1; k <= 1; k++) {
-      int cell = ix + k;
-      int wrapped = wrap[k + 1];
-      float xx = (float)cell + 0.18f + 0.64f * hash2h(wrapped, iy + j, f.seedHash) - x;
-      float yy = (float)(iy + j) + 0.18f + 0.64f * hash2h(wrapped + columns, iy + j, f.seedHash31) - y;
-      float dd = xx * xx + yy * yy;
-      if (dd < first) { second = first; first = dd; }
-      else if (dd < second) second = dd;
-    }
-  float gap = sqrtf(second) - sqrtf(first);
-  float fissure = 1.0f - smooth(0.015f, 0.17f, gap);
-  float plates = fissure * fissure;
+      for (int k = -1; k <= 1; k++) {
+        int cell = ix + k;
+        int wrapped = wrap[k + 1];
+        float xx = (float)cell + 0.18f + 0.64f * hash2h(wrapped, iy + j, f.seedHash) - x;
+        float yy = (float)(iy + j) + 0.18f + 0.64f * hash2h(wrapped + columns, iy + j, f.seedHash31) - y;
+        float dd = xx * xx + yy * yy;
+        if (dd < first) { second = first; first = dd; }
+        else if (dd < second) second = dd;
+      }
+    float gap = sqrtf(second) - sqrtf(first);
+    float fissure = 1.0f - smooth(0.015f, 0.17f, gap);
+    float plates = fissure * fissure;
+    sum = sum + coarse * (plates - 0.37f) * 0.69f;
+  }
   v3 coord = mk(ct * 9.0f, st * 9.0f, arc * 23.0f);
-  float splinter = noise3(coord, f.seed);
-  float pores = noise3(coord * 4.0f, f.seed + 731);
-  return amp * clampf(0.32f + coarse * (plates - 0.37f) * 0.69f + f.fine * (splinter - 0.5f) * 0.24f +
-                          f.grain * (pores - 0.5f) * 0.08f,
-                      0.005f, 1.0f);
+  if (f.fine != 0.0f) sum = sum + f.fine * (noise3(coord, f.seed) - 0.5f) * 0.24f;
+  if (f.grain != 0.0f) sum = sum + f.grain * (noise3(coord * 4.0f, f.seed + 731) - 0.5f) * 0.08f;
+  return amp * clampf(sum, 0.005f, 1.0f);
 }
 __device__ inline float woodField(const WoodFrame& f, v3 p) {
   float t = sat(divKnown(dot(p - f.a, f.ab), f.fieldLen2, f.rField));

@@ -26,9 +26,9 @@ sys.path.insert(0, os.path.join(HERE, "arbor-cuda_amd"))
 # `python oracle/count_trace_ops.py` (the oracle's counting instantiation,
 # oracle/flops.cpp: each +,-,*,/,sqrt,min,max,abs,floor and libm call counts 1,
 # with the GPU's traversal, rejection tests and early exits, including the leaf
-# Newton solve's fixed-point and 2-cycle exits and the skipped band-weighted
-# leaf terms whose band weight is 0).
-TRACE_FLOP_PER_RAY = 9713.8
+# Newton solve's fixed-point and 2-cycle exits, the skipped band-weighted
+# terms whose band weight is 0, and the stored leaf basis with the breeze off).
+TRACE_FLOP_PER_RAY = 8785.9
 # HBM bytes per scheduled tracePixels launch on this workload (walk frame 0):
 # rocprofv3 --pmc FETCH_SIZE 30,601 KiB x 2.0 + WRITE_SIZE 33,341 KiB x 1.0, the
 # factors being what the same counters read for a 256 MiB device copy

@@ -463,30 +463,36 @@ template <class R> struct Arbor {
     F2 uv = woodUV(P, id, p);
     R theta = uv.x, arc = uv.y;
     int seed = (int)P[b + 15];
-    R x = (theta / (K(2.0) * pi) + K(0.5)) * R(columns) + K(0.32) * m_sin(arc * K(6.7) + m_cos(theta) * K(3.0)) +
-          K(0.15) * m_sin(arc * K(19.0) + m_sin(theta) * K(2.0));
-    R y = arc / K(0.19) + K(0.25) * m_sin(theta * K(7.0)) + K(0.1) * m_sin(theta * K(19.0));
-    int ix = (int)m_floor(x), iy = (int)m_floor(y);
-    R first = K(100.0), second = K(100.0);
-    for (int j = -1; j <= 1; j++)
-      for (int k = -1; k <= 1; k++) {
-        int cell = ix + k;
-        int wrapped = ((cell % columns) + columns) % columns;
-        R xx = R(cell) + K(0.18) + K(0.64) * hash2(wrapped, iy + j, seed) - x;
-        R yy = R(iy + j) + K(0.18) + K(0.64) * hash2(wrapped + columns, iy + j, seed + 31) - y;
-        R dd = xx * xx + yy * yy;
-        if (dd < first) { second = first; first = dd; }
-        else if (dd < second) second = dd;
-      }
-    R gap = m_sqrt(second) - m_sqrt(first);
-    R fissure = K(1.0) - smooth(K(0.015), K(0.17), gap);
-    R plates = fissure * fissure;
+    // With bandSkips, a term whose band weight is 0 is a signed zero, and the
+    // running sum (from 0.32, never -0) is unchanged by adding it: the term's
+    // inputs (the plates' cell coordinates and Voronoi search, or a noise)
+    // are then not evaluated.
+    R sum = K(0.32);
+    if (!(bandSkips && coarse == K(0.0))) {
+      R x = (theta / (K(2.0) * pi) + K(0.5)) * R(columns) + K(0.32) * m_sin(arc * K(6.7) + m_cos(theta) * K(3.0)) +
+            K(0.15) * m_sin(arc * K(19.0) + m_sin(theta) * K(2.0));
+      R y = arc / K(0.19) + K(0.25) * m_sin(theta * K(7.0)) + K(0.1) * m_sin(theta * K(19.0));
+      int ix = (int)m_floor(x), iy = (int)m_floor(y);
+      R first = K(100.0), second = K(100.0);
+      for (int j = -1; j <= 1; j++)
+        for (int k = -1; k <= 1; k++) {
+          int cell = ix + k;
+          int wrapped = ((cell % columns) + columns) % columns;
+          R xx = R(cell) + K(0.18) + K(0.64) * hash2(wrapped, iy + j, seed) - x;
+          R yy = R(iy + j) + K(0.18) + K(0.64) * hash2(wrapped + columns, iy + j, seed + 31) - y;
+          R dd = xx * xx + yy * yy;
+          if (dd < first) { second = first; first = dd; }
+          else if (dd < second) second = dd;
+        }
+      R gap = m_sqrt(second) - m_sqrt(first);
+      R fissure = K(1.0) - smooth(K(0.015), K(0.17), gap);
+      R plates = fissure * fissure;
+      sum = sum + coarse * (plates - K(0.37)) * K(0.69);
+    }
     F3 coord = c3(m_cos(theta) * K(9.0), m_sin(theta) * K(9.0), arc * K(23.0));
-    R splinter = noise3(coord, seed);
-    R pores = noise3(coord * K(4.0), seed + 731);
-    return amp * clampf(K(0.32) + coarse * (plates - K(0.37)) * K(0.69) + fine * (splinter - K(0.5)) * K(0.24) +
-                            grain * (pores - K(0.5)) * K(0.08),
-                        K(0.005), K(1.0));
+    if (!(bandSkips && fine == K(0.0))) sum = sum + fine * (noise3(coord, seed) - K(0.5)) * K(0.24);
+    if (!(bandSkips && grain == K(0.0))) sum = sum + grain * (noise3(coord * K(4.0), seed + 731) - K(0.5)) * K(0.08);
+    return amp * clampf(sum, K(0.005), K(1.0));
   }
   static R woodField(const R* P, int id, F3 p, R footprint) {
     int b = id * PS;
@@ -501,22 +507,30 @@ template <class R> struct Arbor {
     R phase = R(((int)P[b + 15]) % 8192) * K(0.618);
     return wind * (K(0.11) * m_sin(time * K(2.1) + phase) + K(0.045) * m_sin(time * K(5.8) + phase * K(1.7)));
   }
+  // When set, leafU/leafN return the stored basis when the breeze is off
+  // (the GPU's leafFrame): the angle wind * (...) is then +-0, cos = 1 and
+  // sin = +-0, so the rotated vectors equal the stored ones in value.
+  // Results must be bit-identical with the flag off (checked on whole frames).
+  static inline bool windShortcut = false;
   static F3 leafU(const R* P, int id, R time, R wind) {
     int b = id * PS;
     F3 u = ld3(P, b + 4), n = ld3(P, b + 12);
+    if (windShortcut && wind == K(0.0)) return u;
     R a = windAngle(P, id, time, wind);
     return u * m_cos(a) + n * m_sin(a);
   }
   static F3 leafN(const R* P, int id, R time, R wind) {
     int b = id * PS;
     F3 u = ld3(P, b + 4), n = ld3(P, b + 12);
+    if (windShortcut && wind == K(0.0)) return n;
     R a = windAngle(P, id, time, wind);
     return n * m_cos(a) - u * m_sin(a);
   }
   // When set, a band-weighted detail term whose band weight is exactly 0 is
   // not evaluated: leafWidth's fine lobes (bw * 0.015 * sin(...) is a signed
-  // zero, and lobes + (+-0) = lobes as lobes >= 0.54), and leafHeight's
-  // midrib, whose value (m * s) * 0 with m > 0 is s * 0 for any positive m.
+  // zero, and lobes + (+-0) = lobes as lobes >= 0.54), leafHeight's midrib,
+  // whose value (m * s) * 0 with m > 0 is s * 0 for any positive m, and
+  // barkCut's plates, splinters and pores (below).
   // Results must be bit-identical with the flag off.
   static inline bool bandSkips = false;
   static R leafWidth(R u, R length, R halfWidth, R footprint) {

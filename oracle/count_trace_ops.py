@@ -4,8 +4,9 @@ Runs the oracle's counting instantiation (oracle/flops.cpp) on the
 full-forest workload bench.py times: seed 7, 324 trees, 1920x1080, shadows
 off, the seeded 120-frame walk. Counts tracePixels with the GPU's traversal
 (lean walk), bounding-sphere rejections, early exits past the caller's best
-the leaf Newton solve's fixed-point/2-cycle exits and the skips of
-band-weighted leaf terms with a zero band weight, sampling every `--frame-step`-th
+the leaf Newton solve's fixed-point/2-cycle exits, the skips of
+band-weighted terms with a zero band weight and the stored leaf basis with
+the breeze off, sampling every `--frame-step`-th
 walk frame and every `--stride`-th pixel in x and y, and prints the mean.
 
 Test / measurement infrastructure: it calls the oracle, never the product.
@@ -55,6 +56,7 @@ def main():
     lib.orc_set_skip.argtypes = [ct.c_int]
     lib.orc_set_newton_exits.argtypes = [ct.c_int]
     lib.orc_set_band_skips.argtypes = [ct.c_int]
+    lib.orc_set_wind_shortcut.argtypes = [ct.c_int]
     W, H, seed = 1920, 1080, 7
     s = o.generate(seed)
     C = o.init_camera(seed)
@@ -71,11 +73,13 @@ def main():
         lib.orc_set_skip(1)
         lib.orc_set_newton_exits(1)
         lib.orc_set_band_skips(1)
+        lib.orc_set_wind_shortcut(1)
         r = o.count_trace_ops(s, C, W, H, args.stride)
         lib.orc_set_cull(0)
         lib.orc_set_skip(0)
         lib.orc_set_newton_exits(0)
         lib.orc_set_band_skips(0)
+        lib.orc_set_wind_shortcut(0)
         per_frame.append(r["ops_per_ray"])
         kinds = r if kinds is None else {k: kinds[k] + r[k] for k in r}
         print("frame %3d: %.1f ops/ray" % (f, r["ops_per_ray"]), flush=True)

@@ -77,6 +77,8 @@ void orc_set_skip(int on) { ACF::skipBeyondBest = on != 0; }
 void orc_set_newton_exits(int on) { ACF::newtonExits = on != 0; }
 // Skips of band-weighted terms whose band weight is 0 on/off for the counting instantiation.
 void orc_set_band_skips(int on) { ACF::bandSkips = on != 0; }
+// The breeze-off shortcut of the leaf basis on/off for the counting instantiation.
+void orc_set_wind_shortcut(int on) { ACF::windShortcut = on != 0; }
 
 // Counts the operations of tracePixels (forest, lean walk) over every
 // stride-th pixel in x and y of a width x height view. out[0] rays, out[1]
@@ -205,12 +207,42 @@ int orc_band_skip_mismatches(const float* P, const float* Nodes, const unsigned*
   return bad;
 }
 
-// The same check on single leaves: n leaf tests with seeded leaves, rays
+// The leaf basis rotated by a zero wind angle vs taken as stored (the GPU's
+// leafFrame with the breeze off), over whole frames (trace, and shade with
+// sun visibility on): pixels whose Hit or Linear differs in any bit (0
+// expected).
+int orc_wind_shortcut_mismatches(const float* P, const float* Nodes, const unsigned* Order, const float* C, int width,
+                               int height) {
+  using A = Arbor<float>;
+  size_t n = (size_t)width * height * 4;
+  std::vector<float> h[2] = {std::vector<float>(n), std::vector<float>(n)};
+  std::vector<float> l[2] = {std::vector<float>(n), std::vector<float>(n)}, g(n);
+  for (int on = 0; on < 2; on++) {
+    A::windShortcut = on != 0;
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = 0; y < height; y++)
+      for (int x = 0; x < width; x++) A::tracePixel(P, Nodes, Order, C, h[on].data(), width, height, x, y);
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = 0; y < height; y++)
+      for (int x = 0; x < width; x++)
+        A::shadePixel(P, Nodes, Order, C, h[on].data(), l[on].data(), g.data(), width, height, x, y);
+  }
+  A::windShortcut = false;
+  int bad = 0;
+  for (size_t i = 0; i < n; i += 4)
+    if (std::memcmp(&h[0][i], &h[1][i], 16) != 0 || std::memcmp(&l[0][i], &l[1][i], 16) != 0) bad++;
+  return bad;
+}
+
+// The same check on single objects: n leaf tests with seeded leaves, rays
 // aimed at the leaf surface from 0.02-0.5 m off it, footprints log-uniform in
 // [1e-5, 1e-2] (across both bands' transitions) and wind on, plus
-// leafHeight/leafWidth at seeded (u, v): the number of results that differ
-// in any bit between skips off and on (0 expected).
-int orc_band_skip_leaf_mismatches(const float* P, int n) {
+// leafHeight/leafWidth at seeded (u, v); and n wood cases: barkCut and
+// woodField at seeded points around a seeded wood segment, and its refined
+// woodT along a ray through them, footprints log-uniform in [1e-5, 1e-1]
+// (across the plate, splinter and pore bands). Returns the number of cases
+// whose results differ in any bit between skips off and on (0 expected).
+int orc_band_skip_object_mismatches(const float* P, int n) {
   using A = Arbor<float>;
   int bad = 0;
   for (int i = 0; i < n; i++) {
@@ -228,6 +260,26 @@ int orc_band_skip_leaf_mismatches(const float* P, int n) {
       res[on][0] = A::leafT(P, id, ro, rd, fp, time, wind);
       res[on][1] = A::leafHeight(uu, vv, P[b + 7], P[b + 11], P[b + 17], P[b + 18], fp);
       res[on][2] = A::leafWidth(uu, P[b + 7], P[b + 11], fp);
+    }
+    A::bandSkips = false;
+    if (std::memcmp(res[0], res[1], sizeof(res[0])) != 0) bad++;
+  }
+  for (int i = 0; i < n; i++) {
+    auto r = [&](int k) { return A::rnd(i * 11 + k, 9191); };
+    const int id = (int)(r(0) * (float)WOOD_COUNT) % WOOD_COUNT, b = id * PS;
+    const float fp = std::pow(10.0f, -5.0f + 4.0f * r(1));
+    V3<float> a = A::ld3(P, b), e = A::ld3(P, b + 4);
+    V3<float> axis = e - a;
+    V3<float> side = A::unit(A::c3(r(2) - 0.5f, r(3) - 0.5f, r(4) - 0.5f));
+    const float rad = std::fmax(P[b + 7], P[b + 11]);
+    V3<float> p = a + axis * r(5) + side * (rad * (0.7f + 0.6f * r(6)));
+    V3<float> ro = p + side * (0.05f + 0.3f * r(7)), rd = A::unit(p - ro);
+    float res[2][3];
+    for (int on = 0; on < 2; on++) {
+      A::bandSkips = on != 0;
+      res[on][0] = A::barkCut(P, id, p, fp);
+      res[on][1] = A::woodField(P, id, p, fp);
+      res[on][2] = A::woodT(P, id, ro, rd, fp, true);
     }
     A::bandSkips = false;
     if (std::memcmp(res[0], res[1], sizeof(res[0])) != 0) bad++;

@@ -179,7 +179,8 @@ class Oracle(_Common):
         self._skip_mismatch = _bind(L, "orc_skip_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
         self._newton_mismatch = _bind(L, "orc_newton_exit_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
         self._band_mismatch = _bind(L, "orc_band_skip_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
-        self._band_leaf_mismatch = _bind(L, "orc_band_skip_leaf_mismatches", _i, [_f32p, _i])
+        self._band_object_mismatch = _bind(L, "orc_band_skip_object_mismatches", _i, [_f32p, _i])
+        self._wind_mismatch = _bind(L, "orc_wind_shortcut_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
         self._generate_cr = _bind(L, "orc_generate_cr", _i, [_i, _f32p, _f32p, _u32p, _u32p, _f32p])
 
     def generate_cr(self, seed):
@@ -215,10 +216,16 @@ class Oracle(_Common):
         with a zero band weight are skipped (0 expected)."""
         return self._band_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h)
 
-    def band_skip_leaf_mismatches(self, s, n):
-        """Single-leaf form of band_skip_mismatches: n seeded leaf tests (and
-        leafHeight/leafWidth evaluations) across the bands' transitions."""
-        return self._band_leaf_mismatch(s.P, n)
+    def band_skip_object_mismatches(self, s, n):
+        """Single-object form of band_skip_mismatches: n seeded leaf tests
+        (with leafHeight/leafWidth) and n wood cases (barkCut, woodField,
+        woodT) across the bands' transitions."""
+        return self._band_object_mismatch(s.P, n)
+
+    def wind_shortcut_mismatches(self, s, C, w, h):
+        """Pixels whose Hit or Linear changes when the leaf basis is taken as
+        stored with the breeze off (0 expected)."""
+        return self._wind_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h)
 
     def skip_mismatches(self, s, C, w, h):
         """Pixels whose Hit or Linear changes when woodT/leafT stop once their

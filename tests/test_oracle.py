@@ -343,10 +343,27 @@ def test_band_skips_change_nothing(oracle, scenes, frame, view, wind, fov):
     assert oracle.band_skip_mismatches(scenes["7"], C, 200, 120) == 0
 
 
-def test_band_skips_change_nothing_per_leaf(oracle, scenes):
-    """The same on 20,000 single leaf tests whose footprints span both bands'
-    transitions (whole frames rarely put leaves there)."""
-    assert oracle.band_skip_leaf_mismatches(scenes["7"], 20000) == 0
+def test_band_skips_change_nothing_per_object(oracle, scenes):
+    """The same on 20,000 single leaf tests and 20,000 wood evaluations whose
+    footprints span the bands' transitions (whole frames rarely put objects
+    there)."""
+    assert oracle.band_skip_object_mismatches(scenes["7"], 20000) == 0
+
+
+@pytest.mark.parametrize("frame,view", [(0, 0), (40, 0), (0, 2), (0, 4)])
+def test_wind_shortcut_changes_nothing(oracle, scenes, frame, view):
+    """With the breeze off, taking the stored leaf basis instead of rotating it
+    by the zero wind angle leaves every Hit and Linear bit unchanged: trace,
+    and shade with sun visibility on."""
+    if view:
+        C = oracle.init_camera(7)
+        I = np.zeros(32, np.float32)
+        I[5] = view
+        oracle.update_camera(C, I, 1.0 / 60, 200, 120)
+    else:
+        C = _walk_camera(oracle, frame, 200, 120, 1)
+    C[22], C[11], C[8] = 1, 0.0, 0.75
+    assert oracle.wind_shortcut_mismatches(scenes["7"], C, 200, 120) == 0
 
 
 def test_operation_count_is_stable(oracle, scenes):
