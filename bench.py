@@ -30,10 +30,10 @@ sys.path.insert(0, os.path.join(HERE, "arbor-cuda_amd"))
 # terms whose band weight is 0, and the stored leaf basis with the breeze off).
 TRACE_FLOP_PER_RAY = 8785.9
 # HBM bytes per scheduled tracePixels launch on this workload (walk frame 0):
-# rocprofv3 --pmc FETCH_SIZE 30,601 KiB x 2.0 + WRITE_SIZE 33,341 KiB x 1.0, the
+# rocprofv3 --pmc FETCH_SIZE 30,960 KiB x 2.0 + WRITE_SIZE 33,345 KiB x 1.0, the
 # factors being what the same counters read for a 256 MiB device copy
 # (131,097 KiB and 262,144 KiB); profiles/tracePixels_forest1080p_pmc.md.
-TRACE_HBM_BYTES = (30601.0 * 2.0 + 33341.0 * 1.0) * 1024
+TRACE_HBM_BYTES = (30960.0 * 2.0 + 33345.0 * 1.0) * 1024
 FP32_PEAK_TFLOPS = 157.3
 # One non-packed FP32 VALU op per lane per clock: 256 CUs x 64 lanes x 2.4 GHz.
 FP32_SINGLE_ISSUE_TOPS = 39.32
