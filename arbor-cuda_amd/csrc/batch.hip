@@ -75,19 +75,76 @@ __global__ __launch_bounds__(kWaves * 64) void k_traceShadeViews(const float* P,
   }
 }
 
+// The split form of the same pass: every view's tiles traced by one launch
+// (the single-view trace kernel's register budget), then shaded by another
+// from Hit (16 B per pixel through HBM). Per pixel the arithmetic is the
+// fused kernel's, so Hit, Linear and Guide are identical.
+template <int kWaves, class W>
+__global__ __launch_bounds__(kWaves * 64) void k_traceViews(const float* P, const float4* Nodes4, const uint32_t* Order,
+                                                            const float* Cs, const ViewDesc* views, int nviews,
+                                                            int totalTiles, float4* Hit, float4*, float4*) {
+  __shared__ TreeInst trees[kTrees];
+  __shared__ ViewDesc sv[kMaxViews];
+  for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, (int)Cs[31]);
+  for (int i = threadIdx.x; i < nviews; i += blockDim.x) sv[i] = views[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x * kWaves + wave; t < totalTiles; t += gridDim.x * kWaves) {
+    const int v = findView(sv, nviews, t);
+    const ViewDesc d = sv[v];
+    const float* C = Cs + 64 * v;
+    const int lt = t - d.tileBase;
+    const int x = (lt % d.tilesX) * 8 + (lane & 7), y = (lt / d.tilesX) * 8 + (lane >> 3);
+    if (x >= d.width || y >= d.height) continue;
+    const View view = makeView(C);
+    v3 ro = view.eye, rd = viewRay(view, x, y, d.width, d.height);
+    float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
+    int tree = -1;
+    v2 h = traceForest<W>(P, Nodes4, nullptr, Order, trees, ro, rd, C[14] / (float)d.height, C[8], C[11],
+                          ground > 0.0f ? ground : kFar, &tree, false);
+    Hit[d.pixelBase + y * d.width + x] = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)d.height);
+  }
+}
+__global__ __launch_bounds__(256) void k_shadeViews(const float* P, const float4* Nodes4, const uint32_t* Order,
+                                                    const float* Cs, const ViewDesc* views, int nviews,
+                                                    int totalTiles, const float4* Hit, float4* Linear, float4* Guide) {
+  __shared__ TreeInst trees[kTrees];
+  __shared__ ViewDesc sv[kMaxViews];
+  for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, (int)Cs[31]);
+  for (int i = threadIdx.x; i < nviews; i += blockDim.x) sv[i] = views[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x * 4 + wave; t < totalTiles; t += gridDim.x * 4) {
+    const int v = findView(sv, nviews, t);
+    const ViewDesc d = sv[v];
+    const int lt = t - d.tileBase;
+    const int x = (lt % d.tilesX) * 8 + (lane & 7), y = (lt / d.tilesX) * 8 + (lane >> 3);
+    if (x >= d.width || y >= d.height) continue;
+    const ShadeCtx sc = makeShadeCtx(Cs + 64 * v, d.width, d.height);
+    const int pixel = d.pixelBase + y * d.width + x;
+    float4 l, g;
+    shadeOne<Walk<true, false, false>>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel], l, g);
+    Linear[pixel] = l;
+    Guide[pixel] = g;
+  }
+}
+
 typedef void (*BatchKernel)(const float*, const float4*, const uint32_t*, const float*, const ViewDesc*, int, int,
                             float4*, float4*, float4*);
 struct BatchVariant {
-  BatchKernel kernel;
+  BatchKernel kernel;  // fused trace + shade, or (split) the trace followed by k_shadeViews
+  bool split;
   int waves;
   const char* name;
 };
 const BatchVariant kBatchVariants[] = {
-    {k_traceShadeViews<1, Walk<true, false, false>>, 1, "1 wave/block, culled walk"},
-    {k_traceShadeViews<2, Walk<true, false, false>>, 2, "2 waves/block, culled walk"},
-    {k_traceShadeViews<4, Walk<true, false, false>>, 4, "4 waves/block, culled walk"},
-    {k_traceShadeViews<1, Walk<false, false, false>>, 1, "1 wave/block, plain walk"},
-    {k_traceShadeViews<4, Walk<false, false, false>>, 4, "4 waves/block, plain walk"},
+    {k_traceShadeViews<1, Walk<true, false, false>>, false, 1, "1 wave/block, culled walk"},
+    {k_traceShadeViews<2, Walk<true, false, false>>, false, 2, "2 waves/block, culled walk"},
+    {k_traceShadeViews<4, Walk<true, false, false>>, false, 4, "4 waves/block, culled walk"},
+    {k_traceShadeViews<1, Walk<false, false, false>>, false, 1, "1 wave/block, plain walk"},
+    {k_traceShadeViews<4, Walk<false, false, false>>, false, 4, "4 waves/block, plain walk"},
+    {k_traceViews<1, Walk<true, false, false>>, true, 1, "split: trace 1 wave/block culled, then shade"},
+    {k_traceViews<2, Walk<true, false, false>>, true, 2, "split: trace 2 waves/block culled, then shade"},
 };
 constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 int batchVariantCount() { return kBatchVariantCount; }
@@ -138,6 +195,13 @@ int issue(arbor_batch* b, const float* P, const float* Nodes, const unsigned* Or
                        Order, b->dC, b->dViews, b->nviews, b->totalTiles, b->Hit, b->Linear, b->Guide);
     int rc = launch_status();
     if (rc != ARBOR_OK) return rc;
+    if (v.split) {
+      int sblocks = (b->totalTiles + 3) / 4;
+      if (sblocks > tuning().grid_cap) sblocks = tuning().grid_cap;
+      hipLaunchKernelGGL(k_shadeViews, dim3(sblocks), dim3(256), 0, s, P, reinterpret_cast<const float4*>(Nodes),
+                         Order, b->dC, b->dViews, b->nviews, b->totalTiles, b->Hit, b->Linear, b->Guide);
+      if ((rc = launch_status()) != ARBOR_OK) return rc;
+    }
   }
   for (int i = 0; i < b->nviews; i++) {
     const ViewDesc& d = b->host[i];

@@ -49,15 +49,27 @@ def _check_batch(px, b, refs, tag):
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (tag, i, name)
 
 
-def test_batch_equals_single_views(px, scene):
-    print("case: batch of 8 uneven views vs single-view renders", flush=True)
+@pytest.mark.parametrize("shadows", [0, 1])
+def test_batch_equals_single_views(px, scene, shadows):
+    """Every batch_variant (fused trace+shade, or split trace then shade), with
+    sun visibility off and on (four views each), against single-view renders."""
+    print("case: batch of 8 uneven views vs single-view renders, shadows", shadows, flush=True)
     cams, sizes = _views(px, 8)
+    cams = cams.copy()
+    cams[:4, 22] = float(shadows)
     refs = [_single(px, scene, cams[i], w, h) for i, (w, h) in enumerate(sizes)]
     b = px.Batch(16, sum(w * h for w, h in sizes) + 1000)
     b.set_views(cams, sizes)
     assert b.pixels() == sum(w * h for w, h in sizes)
-    b.render(scene)
-    _check_batch(px, b, refs, "eager")
+    old = px.tune_get("batch_variant")
+    try:
+        for v in range(7):
+            px.tune_set("batch_variant", v)
+            b.set_views(cams, sizes)
+            b.render(scene)
+            _check_batch(px, b, refs, "variant %d" % v)
+    finally:
+        px.tune_set("batch_variant", old)
     b.close()
 
 
