@@ -100,6 +100,17 @@ AD v2 boxSpanInv(v3 ro, v3 inv, v3 lo, v3 hi) {
   return {fmaxf(mn.x, fmaxf(mn.y, mn.z)), fminf(mx.x, fminf(mx.y, mx.z))};
 }
 AD v2 boxSpan(v3 ro, v3 rd, v3 lo, v3 hi) { return boxSpanInv(ro, invDir(rd), lo, hi); }
+// The BVH walk's slab test: each plane distance (p - o) / d as one fused
+// multiply-add p * (1/d) - o * (1/d), with roi = o * (1/d) computed once per
+// tree entry. Equal to boxSpanInv's value up to rounding (a node box face
+// within ~1e-5 relative of the ray can be entered or missed differently);
+// every walk of the library uses it, so they all test the same nodes.
+AD v2 slabSpan(v3 roi, v3 inv, v3 lo, v3 hi) {
+  v3 a = mk(fmaf(lo.x, inv.x, -roi.x), fmaf(lo.y, inv.y, -roi.y), fmaf(lo.z, inv.z, -roi.z));
+  v3 b = mk(fmaf(hi.x, inv.x, -roi.x), fmaf(hi.y, inv.y, -roi.y), fmaf(hi.z, inv.z, -roi.z));
+  v3 mn = vmin(a, b), mx = vmax(a, b);
+  return {fmaxf(mn.x, fmaxf(mn.y, mn.z)), fminf(mx.x, fminf(mx.y, mx.z))};
+}
 AD v3 basisU(v3 n) { return unit(cross(fabsf(n.y) < 0.91f ? mk(0.0f, 1.0f, 0.0f) : mk(0.0f, 0.0f, 1.0f), n)); }
 AD v3 curveAt(const float* B, int id, float t) {
   int b = id * kCS;

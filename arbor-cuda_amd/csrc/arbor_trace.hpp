@@ -396,8 +396,10 @@ AD float4 nodeQ(const float4* Nodes4, const float4* top, uint32_t q) {
   return Nodes4[q];
 }
 
-// Same visiting order and the same (best, id) as the reference's stack[32]
-// loop. The reference pushes the far child only when both children are hit
+// The reference's stack[32] loop (Arbor.cu:220-230) with its visiting rules.
+// The node slab test is the fused form slabSpan (rounding at box faces may
+// differ from the reference's; object results do not depend on it beyond
+// which boxes a grazing ray enters). The reference pushes the far child only when both children are hit
 // and always takes the nearer one (ties -> right); the stack is therefore
 // always "the far siblings along the current root path", deepest on top. A bit
 // per heap depth records them: popping takes the deepest set bit d and goes
@@ -410,11 +412,11 @@ AD float4 nodeQ(const float4* Nodes4, const float4* top, uint32_t q) {
 template <class W>
 __device__ __forceinline__ v2 traceTree(const float* P, const float4* Nodes4, const float4* top, const uint32_t* Order, v3 ro,
                                v3 rd, float cone, float time, float wind, float best, int ignore, bool shadows) {
-  const v3 inv = invDir(rd);
+  const v3 inv = invDir(rd), roi = ro * inv;
   int found = -1;
   uint32_t node = 1, pending = 0;
   float4 rl = nodeQ<W>(Nodes4, top, 2), rh = nodeQ<W>(Nodes4, top, 3);
-  v2 span = boxSpanInv(ro, inv, mk(rl.x, rl.y, rl.z), mk(rh.x, rh.y, rh.z));
+  v2 span = slabSpan(roi, inv, mk(rl.x, rl.y, rl.z), mk(rh.x, rh.y, rh.z));
   bool live = spanHit(span, rl.w, best);
   float4 k0, k1, k2, k3;  // children boxes of `node` when haveKids
   bool haveKids = false;
@@ -434,8 +436,8 @@ __device__ __forceinline__ v2 traceTree(const float* P, const float4* Nodes4, co
 #pragma unroll
           for (int j = 0; j < 8; j++) g[j] = nodeQ<W>(Nodes4, top, node * 8 + j);
         }
-        v2 ls = boxSpanInv(ro, inv, mk(k0.x, k0.y, k0.z), mk(k1.x, k1.y, k1.z));
-        v2 rs = boxSpanInv(ro, inv, mk(k2.x, k2.y, k2.z), mk(k3.x, k3.y, k3.z));
+        v2 ls = slabSpan(roi, inv, mk(k0.x, k0.y, k0.z), mk(k1.x, k1.y, k1.z));
+        v2 rs = slabSpan(roi, inv, mk(k2.x, k2.y, k2.z), mk(k3.x, k3.y, k3.z));
         bool lh = spanHit(ls, k0.w, best), rhit = spanHit(rs, k2.w, best);
         if (lh || rhit) {
           bool takeLeft = lh && (!rhit || ls.x < rs.x);
@@ -467,7 +469,7 @@ __device__ __forceinline__ v2 traceTree(const float* P, const float4* Nodes4, co
     pending &= ~(1u << d);
     node = (node >> (heapDepth(node) - d)) ^ 1u;
     float4 b0 = nodeQ<W>(Nodes4, top, node * 2), b1 = nodeQ<W>(Nodes4, top, node * 2 + 1);
-    span = boxSpanInv(ro, inv, mk(b0.x, b0.y, b0.z), mk(b1.x, b1.y, b1.z));
+    span = slabSpan(roi, inv, mk(b0.x, b0.y, b0.z), mk(b1.x, b1.y, b1.z));
     live = spanHit(span, b0.w, best);
   }
   return {best, (float)found};

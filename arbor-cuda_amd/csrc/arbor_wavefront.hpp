@@ -141,7 +141,7 @@ AD void laneDda(LaneRay& s, const float4* Nodes4, const TreeInst* trees, const V
   s.node = 1;
   s.pending = 0;
   float4 rl = Nodes4[2], rh = Nodes4[3];
-  s.span = boxSpanInv(s.lr, s.inv, mk(rl.x, rl.y, rl.z), mk(rh.x, rh.y, rh.z));
+  s.span = slabSpan(s.lr * s.inv, s.inv, mk(rl.x, rl.y, rl.z), mk(rh.x, rh.y, rh.z));
   s.live = spanHit(s.span, rl.w, s.lbest);
   s.phase = kPhBvh;
 }
@@ -152,8 +152,8 @@ AD void laneBvh(LaneRay& s, const float4* Nodes4, const uint32_t* Order) {
     if (s.node < (uint32_t)kCapacity) {
       const float4* c = Nodes4 + s.node * 4;
       float4 l0 = c[0], l1 = c[1], r0 = c[2], r1 = c[3];
-      v2 ls = boxSpanInv(s.lr, s.inv, mk(l0.x, l0.y, l0.z), mk(l1.x, l1.y, l1.z));
-      v2 rs = boxSpanInv(s.lr, s.inv, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z));
+      v2 ls = slabSpan(s.lr * s.inv, s.inv, mk(l0.x, l0.y, l0.z), mk(l1.x, l1.y, l1.z));
+      v2 rs = slabSpan(s.lr * s.inv, s.inv, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z));
       bool lh = spanHit(ls, l0.w, s.lbest), rhit = spanHit(rs, r0.w, s.lbest);
       if (lh || rhit) {
         bool takeLeft = lh && (!rhit || ls.x < rs.x);
@@ -176,7 +176,7 @@ AD void laneBvh(LaneRay& s, const float4* Nodes4, const uint32_t* Order) {
     s.pending &= ~(1u << d);
     s.node = (s.node >> (heapDepth(s.node) - d)) ^ 1u;
     float4 b0 = Nodes4[s.node * 2], b1 = Nodes4[s.node * 2 + 1];
-    s.span = boxSpanInv(s.lr, s.inv, mk(b0.x, b0.y, b0.z), mk(b1.x, b1.y, b1.z));
+    s.span = slabSpan(s.lr * s.inv, s.inv, mk(b0.x, b0.y, b0.z), mk(b1.x, b1.y, b1.z));
     s.live = spanHit(s.span, b0.w, s.lbest);
     return;
   }
