@@ -24,6 +24,24 @@ void rows(int w, int h, int threads, F f) {
     for (int x = 0; x < w; x++) f(x, y);
 }
 
+// tracePixels for a list of pixels (x, y) of a w x h view: out[4i..] is pixel
+// i's Hit record (the full view's Hit at that pixel), for parity checks at
+// full view sizes on a sample of the pixels.
+template <class R>
+int trace_pixel_list(const R* P, const R* Nodes, const unsigned* Order, const R* C, const int* xy, int n, int w,
+                     int h, R* out, int threads) {
+  using AR = Arbor<R>;
+  std::vector<R> hit((size_t)w * h * 4);
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; i++) {
+    const int x = xy[2 * i], y = xy[2 * i + 1];
+    if (x < 0 || y < 0 || x >= w || y >= h) continue;
+    AR::tracePixel(P, Nodes, Order, C, hit.data(), w, h, x, y);
+    for (int k = 0; k < 4; k++) out[4 * i + k] = hit[((size_t)y * w + x) * 4 + k];
+  }
+  return 0;
+}
 template <class R>
 void generate_impl(int seed, R* B, R* P, unsigned* Keys, unsigned* Order, R* Nodes, int bitonic) {
   using Ar = Arbor<R>;
@@ -142,6 +160,16 @@ int orc_update_camera(float* C, const float* I, float delta, int width, int heig
 
 ORC_PIXEL_STAGES(f32, float, A32)
 ORC_PIXEL_STAGES(f64, double, A64)
+
+// tracePixels for a list of pixels of a w x h view (trace_pixel_list).
+int orc_trace_pixel_list_f32(const float* P, const float* Nodes, const unsigned* Order, const float* C, const int* xy,
+                             int n, int w, int h, float* out, int threads) {
+  return trace_pixel_list(P, Nodes, Order, C, xy, n, w, h, out, threads);
+}
+int orc_trace_pixel_list_f64(const double* P, const double* Nodes, const unsigned* Order, const double* C,
+                             const int* xy, int n, int w, int h, double* out, int threads) {
+  return trace_pixel_list(P, Nodes, Order, C, xy, n, w, h, out, threads);
+}
 
 // ---- ray-level probes (float) used by the harness-mirroring checks ----
 // traceTree for n rays in tree-local space; out[2i] = t, out[2i+1] = id.

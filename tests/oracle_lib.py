@@ -26,6 +26,7 @@ FAR = np.float32(100000.0)
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i = ct.c_int
 _f = ct.c_float
 
@@ -152,6 +153,10 @@ class Oracle(_Common):
         self._setup_common("orc_")
         L = self.lib
         self._trace64 = _bind(L, "orc_trace_pixels_f64", _i, [_f64p, _f64p, _u32p, _f64p, _f64p, _i, _i, _i])
+        self._trace_list = _bind(L, "orc_trace_pixel_list_f32", _i,
+                                 [_f32p, _f32p, _u32p, _f32p, _i32p, _i, _i, _i, _f32p, _i])
+        self._trace_list64 = _bind(L, "orc_trace_pixel_list_f64", _i,
+                                   [_f64p, _f64p, _u32p, _f64p, _i32p, _i, _i, _i, _f64p, _i])
         self._trace_tree64 = _bind(L, "orc_trace_tree_pixels_f64", _i, [_f64p, _f64p, _u32p, _f64p, _f64p, _i, _i, _i])
         self._shade64 = _bind(L, "orc_shade_pixels_f64", _i,
                               [_f64p, _f64p, _u32p, _f64p, _f64p, _f64p, _f64p, _i, _i, _i])
@@ -237,6 +242,20 @@ class Oracle(_Common):
                                    int(single_tree))
 
     # -- FP64 run: float inputs promoted to double, arithmetic in double --
+    def trace_pixels(self, s, C, xy, w, h, threads=0, fp64=False):
+        """Hit records of the listed pixels (xy: n x 2 ints) of a w x h view,
+        float32 or (fp64) float64."""
+        xy = np.ascontiguousarray(xy, np.int32).reshape(-1)
+        n = xy.size // 2
+        if fp64:
+            out = np.zeros(4 * n, np.float64)
+            self._trace_list64(s.P.astype(np.float64), s.Nodes.astype(np.float64), s.Order, C.astype(np.float64), xy,
+                               n, w, h, out, threads)
+        else:
+            out = np.zeros(4 * n, np.float32)
+            self._trace_list(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), xy, n, w, h, out, threads)
+        return out
+
     def trace64(self, s, C, w, h, threads=0):
         Hit = np.zeros(w * h * 4, np.float64)
         self._trace64(s.P.astype(np.float64), s.Nodes.astype(np.float64), s.Order, C.astype(np.float64), Hit, w, h,

@@ -104,6 +104,45 @@ def test_forest_trace(px, oracle, scene7, view):
     _check_hits("forest v%d" % view, gpu, h32, h64)
 
 
+@pytest.mark.parametrize("frame", [0, 57])
+def test_forest_trace_full_size(px, oracle, scene7, frame):
+    """The benchmark's view size: a 1920x1080 walk frame (seed 7, 324 trees)
+    traced on the GPU. 20,000 seeded pixels are checked against the FP32 and
+    FP64 oracle with the reduced views' bars, and the whole frame against
+    size-independent properties: every t in (0, 1e5], objects nearer than
+    1e5, ids and trees in range and present together, Hit.w = C[14] * t /
+    height."""
+    print("case: 1920x1080 walk frame", frame, flush=True)
+    dev, ref = scene7
+    W, H = 1920, 1080
+    C = oracle.init_camera(7)
+    for f in range(frame + 1):
+        I = px.walk_input(7, f)
+        if f == 0:
+            I[10] = 1.0  # sun visibility off, as the benchmark walk runs
+        oracle.update_camera(C, I, 1.0 / 60.0, W, H)
+    fb = px.FrameBuffers(W, H)
+    px.trace(dev, to_dev(C), fb)
+    hit = to_host(fb.Hit).reshape(-1, 4)
+    rng = np.random.default_rng(1000 + frame)
+    n = 20000
+    xy = np.stack([rng.integers(0, W, n), rng.integers(0, H, n)], axis=1).astype(np.int32)
+    gpu = hit[xy[:, 1] * W + xy[:, 0]].reshape(-1)
+    h32 = oracle.trace_pixels(ref, C, xy, W, H)
+    h64 = oracle.trace_pixels(ref, C, xy, W, H, fp64=True)
+    _check_hits("1080p frame %d (sampled)" % frame, gpu, h32, h64)
+    t, ids, tree = hit[:, 0], hit[:, 1], hit[:, 2]
+    assert (t > 0).all()
+    sky = ids < 0
+    assert ((ids == -1) | ((ids >= 0) & (ids < 29412))).all()
+    assert ((tree == -1) | ((tree >= 0) & (tree < 324))).all()
+    assert np.array_equal(ids >= 0, tree >= 0)
+    assert (t <= np.float32(1e5)).all() and (t[~sky] < np.float32(1e5)).all()  # sky and ground carry no id
+    w_want = (np.float32(C[14]) * t.astype(np.float32)) / np.float32(H)
+    assert np.array_equal(hit[:, 3].view(np.uint32), w_want.astype(np.float32).view(np.uint32))
+    print("objects %.3f ground %.3f sky %.3f" % ((ids >= 0).mean(), ((ids < 0) & (t < 1e5)).mean(), (t == 1e5).mean()))
+
+
 def test_trace_variants_identical(px, oracle, scene7):
     """Every trace_variant / grid_cap setting gives bit-identical Hit (they
     change only how lanes are scheduled)."""
@@ -197,6 +236,36 @@ def test_visibility_pass_identical(px, oracle, scene7):
             v = to_host(vis).reshape(-1, 4)
             assert (v[:, 3] == 1).all() and (v[:, :3] >= 0).all() and (v[:, :3] <= 1).all()
             print("frame %d mean visibility %.4f" % (f, float(v[:, :3].mean())))
+
+
+def test_visibility_pass_full_size(px, oracle, scene7):
+    """The shadowed 4K workload's setting at its full size (3840x2160, seed 7,
+    sun at 35 deg, breeze on, sample 1): the tile-scheduled visibility pass
+    and shading from it give Linear and Guide bit-identical to the fused
+    shadePixels on every pixel."""
+    import torch
+
+    print("case: visibility pass 3840x2160 vs fused shading", flush=True)
+    dev, _ = scene7
+    W, H = 3840, 2160
+    C = oracle.init_camera(7)
+    I = np.zeros(32, np.float32)
+    I[7] = 1.0  # breeze on
+    oracle.update_camera(C, I, 1.0 / 60.0, W, H)
+    C[13], C[9], C[10] = 0.610865, 1.0, 0.0
+    assert C[22] == 1.0 and C[11] > 0.0
+    Cd = to_dev(C)
+    a, b = px.FrameBuffers(W, H), px.FrameBuffers(W, H)
+    px.trace(dev, Cd, a)
+    b.Hit.copy_(a.Hit)
+    px.shade(dev, Cd, a)
+    vis = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+    px.sun_visibility(dev, Cd, b, vis, px.TileSchedule(W, H))
+    px.shade_vis(dev, Cd, b, vis)
+    for name in ("Linear", "Guide"):
+        assert torch.equal(getattr(a, name).view(torch.int32), getattr(b, name).view(torch.int32)), name
+    v = vis.view(-1, 4)[:, :3]
+    print("shadowed (visibility < 1) fraction %.3f" % float((v < 1).any(dim=1).float().mean()))
 
 
 def test_visibility_variants_identical(px, oracle, scene7):
