@@ -171,8 +171,9 @@ __device__ inline v3 sunVisibilityOne(const float* P, const float4* Nodes4, cons
 // (x, y) from its Hit record; with outNormal, also the shading normal (facing
 // the camera; zero for sky). With vis, the sun visibility comes from the
 // separate pass (sunVisibilityOne: same rays, same value) instead of being
-// traced here.
-template <class W>
+// traced here; kTraceVis = false compiles the tracing out (callers then pass
+// vis whenever sun visibility is on).
+template <class W, bool kTraceVis = true>
 __device__ inline void shadeOne(const float* P, const float4* Nodes4, const uint32_t* Order, const TreeInst* trees,
                                 const ShadeCtx& sc, int x, int y, float4 hit, float4& outLinear, float4& outGuide,
                                 v3* outNormal = nullptr, const float4* vis = nullptr) {
@@ -233,7 +234,7 @@ __device__ inline void shadeOne(const float* P, const float4* Nodes4, const uint
     const v3 light = sray.light;
     v3 visibility = mk(1.0f, 1.0f, 1.0f);
     if (sc.shadows) {
-      if (vis) {
+      if (!kTraceVis || vis) {
         const float4 v = vis[pixel];
         visibility = mk(v.x, v.y, v.z);
       } else {

@@ -258,7 +258,10 @@ using WalkAll = Walk<true, true, true>;
 using WalkShade = WalkCull;
 
 // shadePixels (Arbor.cu:277-289).
-__global__ __launch_bounds__(kTileBlock) void k_shadePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
+// kTraceVis = false: Vis is given whenever sun visibility is on (the shadow
+// traversals are compiled out, and with them the call's register cost).
+template <bool kTraceVis>
+__global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTraceVis ? 1 : 3))) void k_shadePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
                                                             const float* C, const float4* Hit, float4* Linear,
                                                             float4* Guide, int width, int height, int tilesX,
                                                             int tiles, float4* Normal, const float4* Vis) {
@@ -273,7 +276,7 @@ __global__ __launch_bounds__(kTileBlock) void k_shadePixels(const float* P, cons
     int pixel = y * width + x;
     float4 l, g;
     v3 n;
-    shadeOne<WalkShade>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel], l, g, Normal ? &n : nullptr, Vis);
+    shadeOne<WalkShade, kTraceVis>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel], l, g, Normal ? &n : nullptr, Vis);
     Linear[pixel] = l;
     Guide[pixel] = g;
     if (Normal) Normal[pixel] = make_float4(n.x, n.y, n.z, 0.0f);
@@ -485,7 +488,7 @@ int arbor_shadePixels(const float* P, const float* Nodes, const unsigned* Order,
   ARBOR_REQUIRE(P && Nodes && Order && C && Hit && Linear && Guide);
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
-  hipLaunchKernelGGL(k_shadePixels, dim3(tileGrid(tiles, kWavesPerBlock)), dim3(kTileBlock), 0, stream, P,
+  hipLaunchKernelGGL(k_shadePixels<true>, dim3(tileGrid(tiles, kWavesPerBlock)), dim3(kTileBlock), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
                      nullptr, nullptr);
@@ -497,7 +500,7 @@ int arbor_shadePixelsVis(const float* P, const float* Nodes, const unsigned* Ord
   ARBOR_REQUIRE(P && Nodes && Order && C && Hit && Vis && Linear && Guide);
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
-  hipLaunchKernelGGL(k_shadePixels, dim3(tileGrid(tiles, kWavesPerBlock)), dim3(kTileBlock), 0, stream, P,
+  hipLaunchKernelGGL(k_shadePixels<false>, dim3(tileGrid(tiles, kWavesPerBlock)), dim3(kTileBlock), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
                      nullptr, reinterpret_cast<const float4*>(Vis));
@@ -559,7 +562,7 @@ int arbor_shadePixelsAOV(const float* P, const float* Nodes, const unsigned* Ord
   ARBOR_REQUIRE(P && Nodes && Order && C && Hit && Linear && Guide && Normal);
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
-  hipLaunchKernelGGL(k_shadePixels, dim3(tileGrid(tiles, kWavesPerBlock)), dim3(kTileBlock), 0, stream, P,
+  hipLaunchKernelGGL(k_shadePixels<true>, dim3(tileGrid(tiles, kWavesPerBlock)), dim3(kTileBlock), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
                      reinterpret_cast<float4*>(Normal), nullptr);
