@@ -76,9 +76,10 @@ __global__ __launch_bounds__(kWaves * 64) void k_traceShadeViews(const float* P,
 }
 
 // The split form of the same pass: every view's tiles traced by one launch
-// (the single-view trace kernel's register budget), then shaded by another
-// from Hit (16 B per pixel through HBM). Per pixel the arithmetic is the
-// fused kernel's, so Hit, Linear and Guide are identical.
+// (the single-view trace kernel's register budget), their sun visibility by
+// a second (views with it on), then shaded by a third from Hit and Vis (16 B
+// per pixel each through HBM). Per pixel the arithmetic is the fused
+// kernel's, so Hit, Linear and Guide are identical.
 template <int kWaves, class W>
 __global__ __launch_bounds__(kWaves * 64) void k_traceViews(const float* P, const float4* Nodes4, const uint32_t* Order,
                                                             const float* Cs, const ViewDesc* views, int nviews,
@@ -105,9 +106,39 @@ __global__ __launch_bounds__(kWaves * 64) void k_traceViews(const float* P, cons
     Hit[d.pixelBase + y * d.width + x] = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)d.height);
   }
 }
-__global__ __launch_bounds__(256) void k_shadeViews(const float* P, const float4* Nodes4, const uint32_t* Order,
-                                                    const float* Cs, const ViewDesc* views, int nviews,
-                                                    int totalTiles, const float4* Hit, float4* Linear, float4* Guide) {
+// Sun visibility of every view's pixels (the visibility pass of frame.hip,
+// per view: a view with C[22] off gets nothing written, and its shading
+// reads nothing).
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5))) void k_sunVisibilityViews(
+    const float* P, const float4* Nodes4, const uint32_t* Order, const float* Cs, const ViewDesc* views, int nviews,
+    int totalTiles, const float4* Hit, float4* Vis) {
+  __shared__ TreeInst trees[kTrees];
+  __shared__ ViewDesc sv[kMaxViews];
+  for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, (int)Cs[31]);
+  for (int i = threadIdx.x; i < nviews; i += blockDim.x) sv[i] = views[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x * 2 + wave; t < totalTiles; t += gridDim.x * 2) {
+    const int v = findView(sv, nviews, t);
+    const ViewDesc d = sv[v];
+    const float* C = Cs + 64 * v;
+    if (!(C[22] > 0.5f)) continue;
+    const int lt = t - d.tileBase;
+    const int x = (lt % d.tilesX) * 8 + (lane & 7), y = (lt / d.tilesX) * 8 + (lane >> 3);
+    if (x >= d.width || y >= d.height) continue;
+    const ShadeCtx sc = makeShadeCtx(C, d.width, d.height);
+    const int pixel = d.pixelBase + y * d.width + x;
+    v3 vis = sunVisibilityOne<Walk<true, false, false>>(P, Nodes4, nullptr, Order, trees, sc, x, y, Hit[pixel]);
+    Vis[pixel] = make_float4(vis.x, vis.y, vis.z, 1.0f);
+  }
+}
+
+// Shading of every view's pixels from Hit; with Vis, the sun visibility comes
+// from k_sunVisibilityViews (the shadow traversals compiled out).
+template <bool kTraceVis>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kTraceVis ? 1 : 3))) void k_shadeViews(
+    const float* P, const float4* Nodes4, const uint32_t* Order, const float* Cs, const ViewDesc* views, int nviews,
+    int totalTiles, const float4* Hit, float4* Linear, float4* Guide, const float4* Vis) {
   __shared__ TreeInst trees[kTrees];
   __shared__ ViewDesc sv[kMaxViews];
   for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, (int)Cs[31]);
@@ -123,7 +154,9 @@ __global__ __launch_bounds__(256) void k_shadeViews(const float* P, const float4
     const ShadeCtx sc = makeShadeCtx(Cs + 64 * v, d.width, d.height);
     const int pixel = d.pixelBase + y * d.width + x;
     float4 l, g;
-    shadeOne<Walk<true, false, false>>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel], l, g);
+    // shadeOne indexes Vis by the pixel within its view
+    shadeOne<Walk<true, false, false>, kTraceVis>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel], l, g, nullptr,
+                                                  Vis ? Vis + d.pixelBase : nullptr);
     Linear[pixel] = l;
     Guide[pixel] = g;
   }
@@ -143,8 +176,8 @@ const BatchVariant kBatchVariants[] = {
     {k_traceShadeViews<4, Walk<true, false, false>>, false, 4, "4 waves/block, culled walk"},
     {k_traceShadeViews<1, Walk<false, false, false>>, false, 1, "1 wave/block, plain walk"},
     {k_traceShadeViews<4, Walk<false, false, false>>, false, 4, "4 waves/block, plain walk"},
-    {k_traceViews<1, Walk<true, false, false>>, true, 1, "split: trace 1 wave/block culled, then shade"},
-    {k_traceViews<2, Walk<true, false, false>>, true, 2, "split: trace 2 waves/block culled, then shade"},
+    {k_traceViews<1, Walk<true, false, false>>, true, 1, "split: trace 1 wave/block culled, visibility, shade"},
+    {k_traceViews<2, Walk<true, false, false>>, true, 2, "split: trace 2 waves/block culled, visibility, shade"},
 };
 constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 int batchVariantCount() { return kBatchVariantCount; }
@@ -163,6 +196,7 @@ struct arbor_batch {
   ViewDesc* dViews = nullptr;
   float* dC = nullptr;
   float4 *Hit = nullptr, *Linear = nullptr, *Guide = nullptr, *Filtered = nullptr, *History = nullptr;
+  float4* Vis = nullptr;  // sun visibility of the split pass
   uint32_t* Pixels = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -196,10 +230,16 @@ int issue(arbor_batch* b, const float* P, const float* Nodes, const unsigned* Or
     int rc = launch_status();
     if (rc != ARBOR_OK) return rc;
     if (v.split) {
-      int sblocks = (b->totalTiles + 3) / 4;
+      int vblocks = (b->totalTiles + 1) / 2, sblocks = (b->totalTiles + 3) / 4;
+      if (vblocks > tuning().grid_cap) vblocks = tuning().grid_cap;
       if (sblocks > tuning().grid_cap) sblocks = tuning().grid_cap;
-      hipLaunchKernelGGL(k_shadeViews, dim3(sblocks), dim3(256), 0, s, P, reinterpret_cast<const float4*>(Nodes),
-                         Order, b->dC, b->dViews, b->nviews, b->totalTiles, b->Hit, b->Linear, b->Guide);
+      hipLaunchKernelGGL(k_sunVisibilityViews, dim3(vblocks), dim3(128), 0, s, P,
+                         reinterpret_cast<const float4*>(Nodes), Order, b->dC, b->dViews, b->nviews, b->totalTiles,
+                         b->Hit, b->Vis);
+      if ((rc = launch_status()) != ARBOR_OK) return rc;
+      hipLaunchKernelGGL(k_shadeViews<false>, dim3(sblocks), dim3(256), 0, s, P,
+                         reinterpret_cast<const float4*>(Nodes), Order, b->dC, b->dViews, b->nviews, b->totalTiles,
+                         b->Hit, b->Linear, b->Guide, b->Vis);
       if ((rc = launch_status()) != ARBOR_OK) return rc;
     }
   }
@@ -243,6 +283,7 @@ int arbor_batch_create(arbor_batch** out, int max_views, long long max_pixels) {
   rc |= dalloc(&b->Guide, 16 * n);
   rc |= dalloc(&b->Filtered, 16 * n);
   rc |= dalloc(&b->History, 16 * n);
+  rc |= dalloc(&b->Vis, 16 * n);
   rc |= dalloc(&b->Pixels, 4 * n);
   if (rc != ARBOR_OK || hipMemset(b->History, 0, 16 * n) != hipSuccess) {
     arbor_batch_destroy(b);
@@ -257,7 +298,7 @@ int arbor_batch_destroy(arbor_batch* b) {
   (void)hipDeviceSynchronize();
   dropGraph(b);
   dfree(b->dViews); dfree(b->dC); dfree(b->Hit); dfree(b->Linear); dfree(b->Guide); dfree(b->Filtered);
-  dfree(b->History); dfree(b->Pixels);
+  dfree(b->History); dfree(b->Pixels); dfree(b->Vis);
   delete b;
   return ARBOR_OK;
 }
