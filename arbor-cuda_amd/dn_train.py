@@ -20,8 +20,9 @@ by PSNR and SSIM:
 The times of filterLighting and of the network (features, forward, output) at
 1080p are measured with HIP events.
 
-Usage: python arbor-cuda_amd/dn_train.py --crops 2048 --size 256 --steps 3000
+Usage: python arbor-cuda_amd/dn_train.py --crops 4096 --size 512 --spp 64 --steps 6000
        --out denoiser.bin --report report.json
+(the committed checkpoint's configuration; about 12 minutes on one MI355X)
 """
 import argparse
 import ctypes as ct
@@ -82,6 +83,8 @@ def render_targets_batched(px, torch, scene, cams, w, h, spp):
                                                  w, rk * h, px._stream()), "arbor_resolvePixels")
         torch.cuda.synchronize()
         b.close()
+        if n > 1:
+            _log("targets: views %d-%d of %d done" % (v0, v0 + k - 1, n))
     return out
 
 
@@ -134,10 +137,10 @@ def event_ms(torch, fn, reps=20):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--crops", type=int, default=2048)
-    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--crops", type=int, default=4096)
+    ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--spp", type=int, default=64)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=7)
@@ -180,6 +183,8 @@ def main():
             G0[i].copy_(fb.Guide)
             N0[i].copy_(normal)
             H0[i].copy_(fb.Hit)
+            if (i + 1) % 512 == 0:
+                _log("inputs: %d of %d crops" % (i + 1, n))
         torch.cuda.synchronize()
         t1 = time.time()
         _log("inputs: %d crops %dx%d in %.1f s" % (n, w, h, t1 - t0))
