@@ -90,19 +90,34 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __bf16* __restrict__ X, c
     __syncthreads();
     const int ci = c0 + g * 8;
     const bool chan = ci < Cin;
-    for (int tap = 0; tap < 9; tap++) {
-      const int dy = tap / 3, dx = tap - dy * 3;
-      bf16x8 w8[NSUB];
+    // The weight fragments of a tap come from global memory (L2): the next
+    // tap's are loaded while the current one computes (every MFMA sees the
+    // same operands in the same order).
+    auto loadW = [&](int tap, bf16x8* w8) {
 #pragma unroll
       for (int j = 0; j < NSUB; j++)
         w8[j] = chan ? *reinterpret_cast<const bf16x8*>(Wt + ((size_t)tap * CoutTotal + co0 + j * 16 + row) * Cin + ci)
                      : zero8();
+    };
+    auto taps = [&](int tap, const bf16x8* w8) {
+      const int dy = tap / 3, dx = tap - dy * 3;
 #pragma unroll
       for (int m = 0; m < MSUB; m++) {
         const bf16x8 a =
             *reinterpret_cast<const bf16x8*>(&tile[((MSUB * wave + m + dy) * kHaloW + row + dx) * kLdsStride + g * 8]);
 #pragma unroll
         for (int j = 0; j < NSUB; j++) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w8[j], acc[m][j], 0, 0, 0);
+      }
+    };
+    {
+      bf16x8 wCur[NSUB], wNext[NSUB];
+      loadW(0, wCur);
+#pragma unroll 1
+      for (int tap = 0; tap < 9; tap++) {
+        loadW(tap < 8 ? tap + 1 : tap, wNext);
+        taps(tap, wCur);
+#pragma unroll
+        for (int j = 0; j < NSUB; j++) wCur[j] = wNext[j];
       }
     }
     __syncthreads();
