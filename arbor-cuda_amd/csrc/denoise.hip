@@ -25,6 +25,7 @@ namespace arbor {
 namespace dn {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
@@ -40,25 +41,39 @@ __device__ __forceinline__ bf16x8 zero8() {
 // A block computes a (4*MSUB)x16-pixel output tile of one image for the
 // output channels [co0, co0 + 16*NSUB) of a CoutTotal-channel output. The
 // input tile with its 1-pixel halo is staged in LDS 32 channels at a time, so
-// the nine taps read their A fragments from LDS: each input value leaves
+// the nine taps read their pixel fragments from LDS: each input value leaves
 // HBM/L2 about 1.3 times instead of nine. Wave w owns output rows
-// MSUB*w .. MSUB*w + MSUB-1 (one 16-pixel M-subtile each); B fragments
-// (weights) come from global memory, shared by every block through the
-// caches, and each feeds MSUB matrix instructions.
+// MSUB*w .. MSUB*w + MSUB-1 (one 16-pixel subtile each); weight fragments
+// come from global memory, shared by every block through the caches, and
+// each feeds MSUB matrix instructions.
 //
 // With Xc non-NULL the input is the channel concatenation [up2(Xc), X] of a
 // half-resolution tensor Xc (Cc channels, read at (y/2, x/2)) and X (Cin - Cc
 // channels): the decoder's upsampling and skip concatenation happen in the
 // staging, never in memory.
 constexpr int kTileW = 16, kHaloW = kTileW + 2, kChunk = 32;
-constexpr int kLdsStride = kChunk + 8;  // bf16 per staged pixel (80 bytes: 16-byte aligned, staggered banks)
+// LDS index (in bf16) of the 16-byte channel group grp (0..3) of staged pixel
+// pix: 64 bytes per pixel, the group XORed with bits 1-2 of the pixel index.
+// Both access patterns are then free of bank conflicts on gfx950 (measured
+// with SQ_LDS_BANK_CONFLICT): the staging stores (four lanes per pixel) and
+// the tap reads (16 consecutive pixels per quarter-wave, one group each). A
+// plain 80-byte pixel stride costs 7.7 conflict cycles per store and 4 per
+// read.
+__device__ __forceinline__ int ldsSlot(int pix, int grp) { return pix * kChunk + ((grp ^ (pix >> 1)) & 3) * 8; }
+// Waves per SIMD the register allocation targets, per output width: the
+// highest each width reaches without scratch (5 for 32 channels, 4 for 48, 3
+// for 128; 1 leaves the allocation unforced). Over the network's layer shapes
+// at 1088x1920 these targets give the lowest total time.
+constexpr int convWaves(int nsub) { return nsub == 2 ? 5 : (nsub == 3 ? 4 : (nsub == 8 ? 3 : 1)); }
 template <int NSUB, int MSUB>
-__global__ __launch_bounds__(256) void k_conv3x3(const __bf16* __restrict__ X, const __bf16* __restrict__ Xc, int Cc,
-                                                 int N, int H, int W, int Cin, const __bf16* __restrict__ Wt,
-                                                 const float* __restrict__ bias, int relu, __bf16* __restrict__ Y,
-                                                 int CoutTotal, int co0) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(convWaves(NSUB)))) void k_conv3x3(
+    const __bf16* __restrict__ X, const __bf16* __restrict__ Xc, int Cc, int N, int H, int W, int Cin,
+    const __bf16* __restrict__ Wt, const float* __restrict__ bias, int relu, __bf16* __restrict__ Y, int CoutTotal,
+    int co0) {
   constexpr int kTileH = 4 * MSUB, kHaloH = kTileH + 2;
-  __shared__ __attribute__((aligned(16))) __bf16 tile[kHaloH * kHaloW * kLdsStride];
+  constexpr int kGroups = kHaloH * kHaloW * 4;   // 16-byte groups staged per chunk
+  constexpr int kStage = (kGroups + 255) / 256;  // groups per thread
+  __shared__ __attribute__((aligned(16))) __bf16 tile[kHaloH * kHaloW * kChunk];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = lane & 15, g = lane >> 4;
   const int tilesX = (W + kTileW - 1) / kTileW, tilesY = (H + kTileH - 1) / kTileH;
@@ -68,76 +83,103 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __bf16* __restrict__ X, c
   const int ty = b % tilesY, n = b / tilesY;
   const int x0 = tx * kTileW, y0 = ty * kTileH;
   const int Cs = Cin - Cc;  // channels read from X
+  // The weights are the A operand and the pixels the B operand, so lane
+  // (row, g) accumulates output pixel x0 + row, channels co0 + 16j + 4g + i:
+  // four consecutive channels of one pixel, stored as one 8-byte write.
   f32x4 acc[MSUB][NSUB];
 #pragma unroll
   for (int m = 0; m < MSUB; m++)
 #pragma unroll
     for (int j = 0; j < NSUB; j++) acc[m][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   for (int c0 = 0; c0 < Cin; c0 += kChunk) {
-    // stage (kHaloH x 18 pixels) x 32 channels as 16-byte groups
-    for (int i = threadIdx.x; i < kHaloH * kHaloW * 4; i += 256) {
+    // Stage (kHaloH x 18 pixels) x 32 channels as 16-byte groups. Every load
+    // is issued before the first store, from an address clamped into the
+    // tensor; groups outside the image or past Cin are then stored as zeros.
+    bf16x8 sv[kStage];
+    bool ok[kStage];
+    // the staging addresses are recomputed per chunk rather than held in
+    // registers through the matrix loop
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+#pragma unroll
+    for (int s = 0; s < kStage; s++) {
+      const int i = tid + 256 * s;
       const int pix = i >> 2, grp = i & 3;
       const int yy = y0 - 1 + pix / kHaloW, xx = x0 - 1 + pix % kHaloW, ci = c0 + grp * 8;
-      bf16x8 v = zero8();
-      if (yy >= 0 && yy < H && xx >= 0 && xx < W && ci < Cin) {
-        if (ci < Cc)
-          v = *reinterpret_cast<const bf16x8*>(Xc + ((size_t)(n * (H >> 1) + (yy >> 1)) * (W >> 1) + (xx >> 1)) * Cc + ci);
-        else
-          v = *reinterpret_cast<const bf16x8*>(X + ((size_t)(n * H + yy) * W + xx) * Cs + ci - Cc);
-      }
-      *reinterpret_cast<bf16x8*>(&tile[pix * kLdsStride + grp * 8]) = v;
+      ok[s] = i < kGroups && yy >= 0 && yy < H && xx >= 0 && xx < W && ci < Cin;
+      const int yc = min(max(yy, 0), H - 1), xc = min(max(xx, 0), W - 1), cc = min(ci, Cin - 8);
+      const __bf16* src =
+          cc < Cc ? Xc + ((size_t)(n * (H >> 1) + (yc >> 1)) * (W >> 1) + (xc >> 1)) * Cc + cc
+                  : X + ((size_t)(n * H + yc) * W + xc) * Cs + cc - Cc;
+      sv[s] = *reinterpret_cast<const bf16x8*>(src);
+    }
+#pragma unroll
+    for (int s = 0; s < kStage; s++) {
+      const int i = tid + 256 * s;
+      if (i < kGroups) *reinterpret_cast<bf16x8*>(&tile[ldsSlot(i >> 2, i & 3)]) = ok[s] ? sv[s] : zero8();
     }
     __syncthreads();
-    const int ci = c0 + g * 8;
-    const bool chan = ci < Cin;
-    // The weight fragments of a tap come from global memory (L2): the next
-    // tap's are loaded while the current one computes (every MFMA sees the
-    // same operands in the same order).
+    // Weight fragments come from global memory (L2); the next tap's are
+    // loaded while the current one computes. A lane whose channel group lies
+    // past Cin reads group 0's weights instead: its staged pixels are zero,
+    // so those products are zero either way.
+    const int ci = min(c0 + g * 8, Cin - 8);
     auto loadW = [&](int tap, bf16x8* w8) {
 #pragma unroll
       for (int j = 0; j < NSUB; j++)
-        w8[j] = chan ? *reinterpret_cast<const bf16x8*>(Wt + ((size_t)tap * CoutTotal + co0 + j * 16 + row) * Cin + ci)
-                     : zero8();
+        w8[j] = *reinterpret_cast<const bf16x8*>(Wt + ((size_t)tap * CoutTotal + co0 + j * 16 + row) * Cin + ci);
     };
     auto taps = [&](int tap, const bf16x8* w8) {
       const int dy = tap / 3, dx = tap - dy * 3;
 #pragma unroll
       for (int m = 0; m < MSUB; m++) {
         const bf16x8 a =
-            *reinterpret_cast<const bf16x8*>(&tile[((MSUB * wave + m + dy) * kHaloW + row + dx) * kLdsStride + g * 8]);
+            *reinterpret_cast<const bf16x8*>(&tile[ldsSlot((MSUB * wave + m + dy) * kHaloW + row + dx, g)]);
 #pragma unroll
-        for (int j = 0; j < NSUB; j++) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w8[j], acc[m][j], 0, 0, 0);
+        for (int j = 0; j < NSUB; j++) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w8[j], a, acc[m][j], 0, 0, 0);
       }
     };
     {
-      bf16x8 wCur[NSUB], wNext[NSUB];
-      loadW(0, wCur);
+      // two weight buffers used in turn, two taps per iteration; each empty
+      // asm statement keeps the loads above it from sinking to their use
+      bf16x8 w0[NSUB], w1[NSUB];
+      loadW(0, w0);
 #pragma unroll 1
-      for (int tap = 0; tap < 9; tap++) {
-        loadW(tap < 8 ? tap + 1 : tap, wNext);
-        taps(tap, wCur);
-#pragma unroll
-        for (int j = 0; j < NSUB; j++) wCur[j] = wNext[j];
+      for (int tap = 0; tap < 8; tap += 2) {
+        loadW(tap + 1, w1);
+        asm volatile("" ::: "memory");
+        taps(tap, w0);
+        loadW(tap + 2, w0);
+        asm volatile("" ::: "memory");
+        taps(tap + 1, w1);
       }
+      taps(8, w0);
     }
     __syncthreads();
   }
+  const int x = x0 + row;
+  if (x >= W) return;
+  // every bias load is issued before the first store
+  float bv[NSUB][4];
+#pragma unroll
+  for (int j = 0; j < NSUB; j++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) bv[j][i] = bias ? bias[co0 + 16 * j + 4 * g + i] : 0.0f;
 #pragma unroll
   for (int m = 0; m < MSUB; m++) {
     const int y = y0 + MSUB * wave + m;
     if (y >= H) continue;
+    const size_t p = (size_t)(n * H + y) * W + x;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int x = x0 + 4 * g + i;
-      if (x >= W) continue;
-      const size_t p = (size_t)(n * H + y) * W + x;
+    for (int j = 0; j < NSUB; j++) {
+      bf16x4 o;
 #pragma unroll
-      for (int j = 0; j < NSUB; j++) {
-        const int co = co0 + j * 16 + row;
-        float v = acc[m][j][i] + (bias ? bias[co] : 0.0f);
+      for (int i = 0; i < 4; i++) {
+        float v = acc[m][j][i] + bv[j][i];
         if (relu) v = fmaxf(v, 0.0f);
-        Y[p * CoutTotal + co] = f2bf(v);
+        o[i] = f2bf(v);
       }
+      *reinterpret_cast<bf16x4*>(Y + p * CoutTotal + co0 + 16 * j + 4 * g) = o;
     }
   }
 }
