@@ -22,6 +22,7 @@ struct Buffers {
   void* Schedule = nullptr;     // tile schedule of the trace (arbor_tracePixelsScheduled)
   void* VisSchedule = nullptr;  // tile schedule of the sun-visibility pass
   float* Vis = nullptr;         // sun visibility per pixel
+  void* Accel = nullptr;        // SAH trace hierarchy (arbor_buildTraceAccel)
 };
 
 template <class T>
@@ -48,6 +49,7 @@ struct arbor_engine {
   float stage_ms[6] = {0, 0, 0, 0, 0, 0};  // update, trace, shade, filter, resolve, whole frame
   float input[ARBOR_I_FLOATS] = {};
   FILE* rec = nullptr;  // input recording (arbor_engine_record)
+  bool accel = true;    // trace and sun visibility over the SAH hierarchy (arbor_engine_set_accel)
 };
 
 namespace {
@@ -90,11 +92,22 @@ int recordFrame(arbor_engine* e, float dt, const float* device_input, int sample
   return ARBOR_OK;
 }
 
+// tracePixels for the engine: tile-scheduled, over the SAH hierarchy unless
+// arbor_engine_set_accel turned it off.
+int traceEngine(arbor_engine* e, int w, int h, hipStream_t stream) {
+  if (e->accel)
+    return arbor_tracePixelsAccel(e->b.P, e->b.Nodes, e->b.Order, e->b.Accel, e->b.C, e->b.Hit, w, h, e->b.Schedule,
+                                  stream);
+  return arbor_tracePixelsScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, w, h, e->b.Schedule, stream);
+}
+
 // shadePixels for the engine: the sun-visibility pass (tile-scheduled) and
 // shading from it; Linear and Guide equal arbor_shadePixels'.
 int shadeEngine(arbor_engine* e, int w, int h, hipStream_t stream) {
-  int rc = arbor_sunVisibilityScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, w, h,
-                                        e->b.VisSchedule, stream);
+  int rc = e->accel ? arbor_sunVisibilityAccel(e->b.P, e->b.Nodes, e->b.Order, e->b.Accel, e->b.C, e->b.Hit,
+                                               e->b.Vis, w, h, e->b.VisSchedule, stream)
+                    : arbor_sunVisibilityScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, w, h,
+                                                   e->b.VisSchedule, stream);
   if (rc != ARBOR_OK) return rc;
   return arbor_shadePixelsVis(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, e->b.Linear, e->b.Guide, w,
                               h, stream);
@@ -127,6 +140,7 @@ int arbor_engine_create(arbor_engine** out, int width, int height, int seed, int
   rc |= alloc(&e->b.Nodes, (size_t)ARBOR_NODE_COUNT * 8 * 4);
   rc |= alloc(&e->b.C, ARBOR_C_FLOATS * 4);
   rc |= alloc(&e->b.I, ARBOR_I_FLOATS * 4);
+  rc |= alloc(&e->b.Accel, (size_t)arbor_trace_accel_bytes());
   for (int i = 0; i < 6 && rc == ARBOR_OK; i++)
     if (hipEventCreate(&e->ev[i]) != hipSuccess) rc = ARBOR_ERR_DEVICE;
   if (rc == ARBOR_OK) {
@@ -151,7 +165,7 @@ int arbor_engine_destroy(arbor_engine* e) {
   if (e->rec) fclose(e->rec);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   free_pixels(e);
-  release(e->b.B); release(e->b.P); release(e->b.Keys); release(e->b.Order); release(e->b.Nodes);
+  release(e->b.B); release(e->b.P); release(e->b.Keys); release(e->b.Order); release(e->b.Nodes); release(e->b.Accel);
   release(e->b.C); release(e->b.I);
   for (auto& v : e->ev)
     if (v) (void)hipEventDestroy(v);
@@ -165,6 +179,7 @@ int arbor_engine_generate(arbor_engine* e, int seed) {
   if (seed < 0 || seed > 1000000) return ARBOR_ERR_ARG;  // engine.js:28
   int rc = arbor_generate(e->b.B, e->b.P, e->b.Keys, e->b.Order, e->b.Nodes, e->b.C, seed, e->b.History, e->width,
                           e->height, e->stream);
+  if (rc == ARBOR_OK) rc = arbor_buildTraceAccel(e->b.Nodes, e->b.Order, e->b.Accel, e->stream);
   if (rc == ARBOR_OK) {
     e->seed = seed;
     e->generated = true;
@@ -223,6 +238,12 @@ int arbor_engine_set_timing(arbor_engine* e, int on) {
   return ARBOR_OK;
 }
 
+int arbor_engine_set_accel(arbor_engine* e, int on) {
+  if (!e) return ARBOR_ERR_NULL;
+  e->accel = on != 0;
+  return ARBOR_OK;
+}
+
 // One frame: updateCamera -> tracePixels -> shadePixels -> filterLighting ->
 // resolvePixels, five dispatches (engine.js:45-53). With device_input NULL
 // the host input set by arbor_engine_set_input is uploaded first.
@@ -243,9 +264,7 @@ int arbor_engine_frame(arbor_engine* e, float dt, const float* device_input) {
   if (e->timing) (void)hipEventRecord(e->ev[0], e->stream);
   if ((rc = arbor_updateCamera(e->b.C, I, e->b.P, delta, w, h, e->stream)) != ARBOR_OK) return rc;
   if (e->timing) (void)hipEventRecord(e->ev[1], e->stream);
-  if ((rc = arbor_tracePixelsScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, w, h, e->b.Schedule,
-                                       e->stream)) != ARBOR_OK)
-    return rc;
+  if ((rc = traceEngine(e, w, h, e->stream)) != ARBOR_OK) return rc;
   if (e->timing) (void)hipEventRecord(e->ev[2], e->stream);
   if ((rc = shadeEngine(e, w, h, e->stream)) != ARBOR_OK)
     return rc;
@@ -399,7 +418,7 @@ int arbor_engine_render(arbor_engine* e, int view, int samples, int light) {
   for (int i = 0; i < samples && rc == ARBOR_OK; i++) {
     hipLaunchKernelGGL(k_setSample, dim3(1), dim3(64), 0, e->stream, e->b.C, i, light);
     if ((rc = arbor::launch_status()) != ARBOR_OK) break;
-    rc = arbor_tracePixelsScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, w, h, e->b.Schedule, e->stream);
+    rc = traceEngine(e, w, h, e->stream);
     if (rc == ARBOR_OK)
       rc = shadeEngine(e, w, h, e->stream);
     if (rc == ARBOR_OK) rc = arbor_filterLighting(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.C, w, h, e->stream);
@@ -427,7 +446,7 @@ int arbor_engine_frame_samples(arbor_engine* e, float dt, const float* device_in
   for (int i = 0; i < samples && rc == ARBOR_OK; i++) {
     hipLaunchKernelGGL(k_setSample, dim3(1), dim3(64), 0, e->stream, e->b.C, i, -1);
     if ((rc = arbor::launch_status()) != ARBOR_OK) break;
-    rc = arbor_tracePixelsScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, w, h, e->b.Schedule, e->stream);
+    rc = traceEngine(e, w, h, e->stream);
     if (rc == ARBOR_OK)
       rc = shadeEngine(e, w, h, e->stream);
     if (rc == ARBOR_OK) rc = arbor_filterLighting(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.C, w, h, e->stream);

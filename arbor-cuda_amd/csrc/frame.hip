@@ -35,11 +35,12 @@ __device__ __forceinline__ void fillTop(float4* top, const float4* Nodes4) {
 // schedule, work item i is tile tileOrder[i] and each wave records how long its
 // tile took (wall clock ticks, >= 1) in tileCost; the order changes only which
 // wave traces which tile, never a pixel's result.
+// With a SAH walk policy, accel is the hierarchy of arbor_buildTraceAccel.
 template <int kWaves, class W>
 __global__ __launch_bounds__(kWaves * 64) void k_tracePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
                                                              const float* C, float4* Hit, int width, int height,
                                                              int tilesX, int tiles, const uint32_t* tileOrder,
-                                                             uint32_t* tileCost) {
+                                                             uint32_t* tileCost, const float4* accel) {
   __shared__ TreeInst trees[kTrees];
   __shared__ float4 top[W::ldsTop ? 2 * kTopNodes : 1];
   fillTrees(trees, (int)C[31]);
@@ -56,8 +57,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_tracePixels(const float* P, con
       v3 ro = view.eye, rd = viewRay(view, x, y, width, height);
       float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
       int tree = -1;
-      v2 h = traceForest<W>(P, Nodes4, top, Order, trees, ro, rd, cone, time, wind, ground > 0.0f ? ground : kFar,
-                            &tree, false);
+      v2 h = traceForest<W>(P, Nodes4, W::sah ? accel : top, Order, trees, ro, rd, cone, time, wind,
+                            ground > 0.0f ? ground : kFar, &tree, false);
       Hit[y * width + x] = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)height);
     }
     if (tileCost && lane == 0) {
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(kMi
                                                                const uint32_t* Order, const float* C, const float4* Hit,
                                                                float4* Vis, int width, int height, int tilesX,
                                                                int tiles, const uint32_t* tileOrder,
-                                                               uint32_t* tileCost) {
+                                                               uint32_t* tileCost, const float4* accel) {
   __shared__ TreeInst trees[kTrees];
   __shared__ float4 top[W::ldsTop ? 2 * kTopNodes : 1];
   if (!(C[22] > 0.5f)) return;  // visibility off: shading never reads Vis
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(kMi
       const int pixel = y * width + x;
       // per tile from C: the context is dead during the traversal instead of live across the loop
       const ShadeCtx sc = makeShadeCtx(C, width, height);
-      v3 v = sunVisibilityOne<W>(P, Nodes4, top, Order, trees, sc, x, y, Hit[pixel]);
+      v3 v = sunVisibilityOne<W>(P, Nodes4, W::sah ? accel : top, Order, trees, sc, x, y, Hit[pixel]);
       Vis[pixel] = make_float4(v.x, v.y, v.z, 1.0f);
     }
     if (tileCost && lane == 0) {
@@ -206,7 +207,7 @@ template <int kWaves, class W>
 __global__ __launch_bounds__(kWaves * 64) void k_traceTreePixels(const float* P, const float4* Nodes4,
                                                                  const uint32_t* Order, const float* C, float4* Hit,
                                                                  int width, int height, int tilesX, int tiles,
-                                                                 const uint32_t*, uint32_t*) {
+                                                                 const uint32_t*, uint32_t*, const float4* accel) {
   __shared__ float4 top[W::ldsTop ? 2 * kTopNodes : 1];
   fillTop<W>(top, Nodes4);
   __syncthreads();
@@ -218,7 +219,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_traceTreePixels(const float* P,
     if (x >= width || y >= height) continue;
     v3 ro = view.eye, rd = viewRay(view, x, y, width, height);
     float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
-    v2 h = traceTree<W>(P, Nodes4, top, Order, ro, rd, cone, time, wind, ground > 0.0f ? ground : kFar, -1, false);
+    v2 h = traceTree<W>(P, Nodes4, W::sah ? accel : top, Order, ro, rd, cone, time, wind,
+                        ground > 0.0f ? ground : kFar, -1, false);
     Hit[y * width + x] = make_float4(h.x, h.y, -1.0f, C[14] * h.x / (float)height);
   }
 }
@@ -256,6 +258,7 @@ using WalkCullPrefetch = Walk<true, true, false>;
 using WalkCullTop = Walk<true, false, true>;
 using WalkAll = Walk<true, true, true>;
 using WalkShade = WalkCull;
+using WalkSah = Walk<true, false, false, true>;  // the accel entry points
 
 // shadePixels (Arbor.cu:277-289).
 // kTraceVis = false: Vis is given whenever sun visibility is on (the shadow
@@ -386,7 +389,7 @@ inline int tileGrid(int tiles, int wavesPerBlock) {
 }
 
 typedef void (*TraceKernel)(const float*, const float4*, const uint32_t*, const float*, float4*, int, int, int, int,
-                            const uint32_t*, uint32_t*);
+                            const uint32_t*, uint32_t*, const float4*);
 struct TraceVariant {
   TraceKernel forest, tree;
   int waves;
@@ -423,7 +426,26 @@ int launchTrace(bool forest, const float* P, const float* Nodes, const unsigned*
   const TraceVariant& v = kTraceVariants[tu.trace_variant < kTraceVariantCount ? tu.trace_variant : 5];
   hipLaunchKernelGGL(forest ? v.forest : v.tree, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
-                     tx, tiles, tileOrder, tileCost);
+                     tx, tiles, tileOrder, tileCost, nullptr);
+  return launch_status();
+}
+
+// accel_variant index -> primary-ray kernel over the SAH hierarchy.
+const TraceVariant kAccelTraceVariants[] = {
+    {k_tracePixels<1, WalkSah>, k_traceTreePixels<1, WalkSah>, 1},
+    {k_tracePixels<2, WalkSah>, k_traceTreePixels<2, WalkSah>, 2},
+    {k_tracePixels<4, WalkSah>, k_traceTreePixels<4, WalkSah>, 4},
+};
+constexpr int kAccelTraceVariantCount = sizeof(kAccelTraceVariants) / sizeof(kAccelTraceVariants[0]);
+
+int launchTraceAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel, const float* C,
+                     float* Hit, int width, int height, hipStream_t stream, const uint32_t* tileOrder,
+                     uint32_t* tileCost) {
+  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  const TraceVariant& v = kAccelTraceVariants[tuning().accel_variant];
+  hipLaunchKernelGGL(v.forest, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), 0, stream, P,
+                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
+                     tx, tiles, tileOrder, tileCost, static_cast<const float4*>(accel));
   return launch_status();
 }
 }  // namespace
@@ -509,7 +531,7 @@ int arbor_shadePixelsVis(const float* P, const float* Nodes, const unsigned* Ord
 
 namespace {
 typedef void (*VisKernel)(const float*, const float4*, const uint32_t*, const float*, const float4*, float4*, int, int,
-                          int, int, const uint32_t*, uint32_t*);
+                          int, int, const uint32_t*, uint32_t*, const float4*);
 struct VisVariant {
   VisKernel kernel;
   int waves;
@@ -533,7 +555,26 @@ int launchVisibility(const float* P, const float* Nodes, const unsigned* Order, 
   const VisVariant& v = kVisVariants[tuning().vis_variant];
   hipLaunchKernelGGL(v.kernel, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
-                     reinterpret_cast<float4*>(Vis), width, height, tx, tiles, tileOrder, tileCost);
+                     reinterpret_cast<float4*>(Vis), width, height, tx, tiles, tileOrder, tileCost, nullptr);
+  return launch_status();
+}
+
+// accel_vis_variant index -> sun-visibility kernel over the SAH hierarchy.
+const VisVariant kAccelVisVariants[] = {
+    {k_sunVisibility<2, WalkSah, 5>, 2}, {k_sunVisibility<1, WalkSah>, 1}, {k_sunVisibility<2, WalkSah>, 2},
+    {k_sunVisibility<4, WalkSah, 5>, 4}, {k_sunVisibility<2, WalkSah, 4>, 2},
+};
+constexpr int kAccelVisVariantCount = sizeof(kAccelVisVariants) / sizeof(kAccelVisVariants[0]);
+
+int launchVisibilityAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,
+                          const float* C, const float* Hit, float* Vis, int width, int height,
+                          const uint32_t* tileOrder, uint32_t* tileCost, hipStream_t stream) {
+  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  const VisVariant& v = kAccelVisVariants[tuning().accel_vis_variant];
+  hipLaunchKernelGGL(v.kernel, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), 0, stream, P,
+                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
+                     reinterpret_cast<float4*>(Vis), width, height, tx, tiles, tileOrder, tileCost,
+                     static_cast<const float4*>(accel));
   return launch_status();
 }
 }  // namespace
@@ -555,6 +596,33 @@ int arbor_sunVisibilityScheduled(const float* P, const float* Nodes, const unsig
   int rc = orderTiles(C, schedule, width, height, stream, &order, &cost);
   if (rc != ARBOR_OK) return rc;
   return launchVisibility(P, Nodes, Order, C, Hit, Vis, width, height, order, cost, stream);
+}
+
+int arbor_tracePixelsAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,
+                           const float* C, float* Hit, int width, int height, void* schedule, hipStream_t stream) {
+  ARBOR_REQUIRE(P && Nodes && Order && accel && C && Hit);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  uint32_t* cost = nullptr;
+  const uint32_t* order = nullptr;
+  if (schedule) {
+    int rc = orderTiles(C, schedule, width, height, stream, &order, &cost);
+    if (rc != ARBOR_OK) return rc;
+  }
+  return launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, order, cost);
+}
+
+int arbor_sunVisibilityAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,
+                             const float* C, const float* Hit, float* Vis, int width, int height, void* schedule,
+                             hipStream_t stream) {
+  ARBOR_REQUIRE(P && Nodes && Order && accel && C && Hit && Vis);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  uint32_t* cost = nullptr;
+  const uint32_t* order = nullptr;
+  if (schedule) {
+    int rc = orderTiles(C, schedule, width, height, stream, &order, &cost);
+    if (rc != ARBOR_OK) return rc;
+  }
+  return launchVisibilityAccel(P, Nodes, Order, accel, C, Hit, Vis, width, height, order, cost, stream);
 }
 
 int arbor_shadePixelsAOV(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
@@ -604,7 +672,9 @@ int arbor_tune_set(const char* key, int value) {
       {"sched_leaf_min", &t.sched_leaf_min, 1, 64},
       {"sched_newton_min", &t.sched_newton_min, 1, 64},
       {"batch_variant", &t.batch_variant, 0, batchVariantCount() - 1},
-      {"vis_variant", &t.vis_variant, 0, kVisVariantCount - 1}};
+      {"vis_variant", &t.vis_variant, 0, kVisVariantCount - 1},
+      {"accel_variant", &t.accel_variant, 0, kAccelTraceVariantCount - 1},
+      {"accel_vis_variant", &t.accel_vis_variant, 0, kAccelVisVariantCount - 1}};
   for (auto& k : keys)
     if (!strcmp(key, k.name)) {
       if (value < k.lo || value > k.hi) return ARBOR_ERR_ARG;
@@ -624,11 +694,14 @@ int arbor_tune_get(const char* key) {
   if (!strcmp(key, "sched_newton_min")) return t.sched_newton_min;
   if (!strcmp(key, "batch_variant")) return t.batch_variant;
   if (!strcmp(key, "vis_variant")) return t.vis_variant;
+  if (!strcmp(key, "accel_variant")) return t.accel_variant;
+  if (!strcmp(key, "accel_vis_variant")) return t.accel_vis_variant;
   return ARBOR_ERR_ARG;
 }
 
-static const char* const kTuneKeys[] = {"trace_variant", "grid_cap", "sched_tiles", "sched_leaf_min",
-                                        "sched_newton_min", "batch_variant", "vis_variant"};
+static const char* const kTuneKeys[] = {"trace_variant",    "grid_cap",      "sched_tiles",
+                                        "sched_leaf_min",   "sched_newton_min", "batch_variant",
+                                        "vis_variant",      "accel_variant", "accel_vis_variant"};
 
 // Tuning cache: one "key value" line per key; unknown keys and out-of-range
 // values in a file are rejected as a whole.

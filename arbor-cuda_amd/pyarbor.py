@@ -73,6 +73,10 @@ SIGNATURES = {
     "arbor_tracePixelsScheduled": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "arbor_sunVisibility": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_sunVisibilityScheduled": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
+    "arbor_trace_accel_bytes": [],
+    "arbor_buildTraceAccel": [_vp, _vp, _vp, _vp],
+    "arbor_tracePixelsAccel": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
+    "arbor_sunVisibilityAccel": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "arbor_shadePixelsVis": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_shadePixels": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_filterLighting": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -107,6 +111,7 @@ SIGNATURES = {
     "arbor_engine_reset_camera": [_vp],
     "arbor_engine_set_input": [_vp, ct.POINTER(_f)],
     "arbor_engine_set_timing": [_vp, _i],
+    "arbor_engine_set_accel": [_vp, _i],
     "arbor_engine_frame": [_vp, _f, _vp],
     "arbor_engine_stage_ms": [_vp, ct.POINTER(_f)],
     "arbor_engine_sync": [_vp],
@@ -119,6 +124,7 @@ SIGNATURES = {
 _RESTYPES = {"arbor_version": ct.c_char_p, "arbor_engine_round_size": None, "arbor_engine_buffer": _vp,
              "arbor_engine_stream": _vp, "arbor_batch_buffer": _vp, "arbor_batch_pixels": ct.c_longlong,
              "arbor_tile_schedule_bytes": ct.c_longlong, "arbor_dn_wgrad_partial_floats": ct.c_longlong,
+             "arbor_trace_accel_bytes": ct.c_longlong,
              "arbor_denoiser_parameters": ct.c_longlong, "arbor_denoiser_tensor": _vp}
 BATCH_TRACE_SHADE, BATCH_FILTER, BATCH_RESOLVE, BATCH_ALL = 1, 2, 4, 7
 
@@ -258,6 +264,27 @@ class TileSchedule:
 def trace_scheduled(scene, C, fb, schedule):
     call("arbor_tracePixelsScheduled", scene.P, scene.Nodes, scene.Order, C, fb.Hit, fb.width, fb.height,
          schedule.buf)
+
+
+class TraceAccel:
+    """Device SAH hierarchy of one scene (arbor_buildTraceAccel)."""
+
+    def __init__(self, scene, device="cuda"):
+        import torch
+
+        n = lib().arbor_trace_accel_bytes()
+        self.buf = torch.zeros(n, dtype=torch.uint8, device=device)
+        call("arbor_buildTraceAccel", scene.Nodes, scene.Order, self.buf)
+
+
+def trace_accel(scene, accel, C, fb, schedule=None):
+    call("arbor_tracePixelsAccel", scene.P, scene.Nodes, scene.Order, accel.buf, C, fb.Hit, fb.width, fb.height,
+         None if schedule is None else schedule.buf)
+
+
+def sun_visibility_accel(scene, accel, C, fb, vis, schedule=None):
+    call("arbor_sunVisibilityAccel", scene.P, scene.Nodes, scene.Order, accel.buf, C, fb.Hit, vis, fb.width,
+         fb.height, None if schedule is None else schedule.buf)
 
 
 def trace_tree(scene, C, fb):
@@ -520,6 +547,10 @@ class Engine:
 
     def set_timing(self, on):
         check(self._lib.arbor_engine_set_timing(self.h, int(on)), "arbor_engine_set_timing")
+
+    def set_accel(self, on):
+        """Trace over the SAH hierarchy (default) or over the heap BVH."""
+        check(self._lib.arbor_engine_set_accel(self.h, int(on)), "arbor_engine_set_accel")
 
     def frame(self, dt=1.0 / 60.0, device_input=None):
         dev = None if device_input is None else ct.c_void_p(device_input.data_ptr())

@@ -31,6 +31,10 @@ int arbor_engine_reset_camera(arbor_engine* e);
 int arbor_engine_set_input(arbor_engine* e, const float* input32);
 /* Per-stage HIP-event timing of every frame (synchronises each frame). */
 int arbor_engine_set_timing(arbor_engine* e, int on);
+/* Trace and sun visibility over the SAH hierarchy built at generate
+ * (arbor_tracePixelsAccel / arbor_sunVisibilityAccel; on by default) or over
+ * the heap BVH (arbor_tracePixelsScheduled / arbor_sunVisibilityScheduled). */
+int arbor_engine_set_accel(arbor_engine* e, int on);
 /* engine.js:45-53: updateCamera, tracePixels, shadePixels, filterLighting,
  * resolvePixels. device_input, if not NULL, is a device array of 32 floats
  * used instead of the host input. */

@@ -150,6 +150,25 @@ def test_tune_cache_roundtrip(L, tmp_path):
     assert L.arbor_tune_set(b"sched_tiles", old) == 0
 
 
+def test_trace_accel_size_and_checks(L):
+    # header + 4 float4 per interior node of a binary tree over CAPACITY
+    # leaves, then a u32 heap slot per primitive id
+    assert L.arbor_trace_accel_bytes() == (4 + 4 * (32768 - 1)) * 16 + 4 * 32768
+    E = -1
+    assert L.arbor_buildTraceAccel(NULL, DUMMY, DUMMY, NULL) == E
+    assert L.arbor_buildTraceAccel(DUMMY, DUMMY, NULL, NULL) == E
+    assert L.arbor_tracePixelsAccel(DUMMY, DUMMY, DUMMY, NULL, DUMMY, DUMMY, 64, 64, NULL, NULL) == E
+    assert L.arbor_tracePixelsAccel(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 0, 64, NULL, NULL) == -2
+    assert L.arbor_sunVisibilityAccel(DUMMY, DUMMY, DUMMY, NULL, DUMMY, DUMMY, DUMMY, 64, 64, NULL, NULL) == E
+    assert L.arbor_sunVisibilityAccel(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 64, 9000, NULL, NULL) == -2
+    assert L.arbor_engine_set_accel(NULL, 1) == E
+    for key, n in ((b"accel_variant", 3), (b"accel_vis_variant", 5)):
+        old = L.arbor_tune_get(key)
+        assert L.arbor_tune_set(key, n) == -2 and L.arbor_tune_set(key, -1) == -2
+        assert L.arbor_tune_set(key, n - 1) == 0 and L.arbor_tune_get(key) == n - 1
+        assert L.arbor_tune_set(key, old) == 0
+
+
 def test_tile_schedule_size_and_checks(L):
     # 2048-byte header (previous camera, valid flag, bucket histogram and starts)
     # + cost, order and key per 8x8 tile

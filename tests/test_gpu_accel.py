@@ -1,0 +1,197 @@
+"""The SAH trace hierarchy (arbor_buildTraceAccel) and the trace and
+sun-visibility passes over it.
+
+* Structure: deterministic bytes; every primitive of the heap BVH appears once
+  with its heap leaf box; every interior child box is the exact union of its
+  children's; depth within the walk's 32-entry stack; the slot table maps
+  each primitive to its heap leaf (the tie order).
+* Results: the Accel passes equal the heap passes (arbor_tracePixels,
+  arbor_sunVisibility) bit for bit on whole 1920x1080 walk frames, except for
+  rays whose reported hit depends on the walk order in the reference itself:
+  a swaying leaf hit before its rest-pose leaf box is entered. Such pixels
+  carry two valid object hits of the reference's tests at nearly the same t;
+  at most 4 per frame are allowed (measured: 0 or 1 per 2 million rays).
+* Launch shapes (accel_variant, accel_vis_variant) and the tile schedule
+  change nothing.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import require_gpu, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+CAPACITY = 32768
+
+
+@pytest.fixture(scope="module")
+def px():
+    require_gpu()
+    import pyarbor
+
+    return pyarbor
+
+
+@pytest.fixture(scope="module")
+def scene(px):
+    dev = px.DeviceScene().generate(7)
+    return dev, px.TraceAccel(dev)
+
+
+def _walk_camera(px, oracle, frame, w, h, shadows):
+    C = oracle.init_camera(7)
+    for f in range(frame + 1):
+        I = px.walk_input(7, f)
+        if f == 0:
+            I[10] = 1.0
+        oracle.update_camera(C, I, 1.0 / 60.0, w, h)
+    C[22] = shadows
+    return C
+
+
+def _parse(buf):
+    f = buf.view(np.float32)
+    i = buf.view(np.int32)
+    n_nodes, n_prims, depth = int(f[8]), int(f[9]), int(f[7])
+    return f, i, n_nodes, n_prims, depth
+
+
+def test_accel_structure(px, scene):
+    print("case: accel structure, seed 7", flush=True)
+    dev, acc = scene
+    buf = acc.buf.cpu().numpy()
+    again = px.TraceAccel(dev).buf.cpu().numpy()
+    assert np.array_equal(buf, again), "the build is not deterministic"
+    f, i, n_nodes, n_prims, depth = _parse(buf)
+    nodes = to_host(dev.Nodes).reshape(-1, 8)
+    order = to_host(dev.Order).view(np.int32)
+    leaves = nodes[CAPACITY:]
+    used = leaves[:, 3] > 0
+    assert n_prims == int(used.sum()) and n_nodes == n_prims - 1
+    assert 1 <= depth <= 32
+    heap_box = {int(order[s]): leaves[s] for s in np.nonzero(used)[0]}
+    seen_prims, seen_nodes = set(), set()
+
+    def child(q, k):  # box (lo, hi) and ref of child k of interior node q
+        base = 16 + 16 * q + 8 * k
+        return f[base:base + 3], f[base + 4:base + 7], int(i[base + 3])
+
+    # iterative walk: (ref, lo, hi, level)
+    stack = [(int(i[3]), f[0:3], f[4:7], 0)]
+    max_level = 0
+    while stack:
+        ref, lo, hi, level = stack.pop()
+        max_level = max(max_level, level)
+        if ref < 0:
+            pid = ~ref
+            assert pid not in seen_prims and pid in heap_box
+            seen_prims.add(pid)
+            hb = heap_box[pid]
+            assert np.array_equal(lo, hb[0:3]) and np.array_equal(hi, hb[4:7])
+            continue
+        assert 0 <= ref < n_nodes and ref not in seen_nodes
+        seen_nodes.add(ref)
+        l_lo, l_hi, l_ref = child(ref, 0)
+        r_lo, r_hi, r_ref = child(ref, 1)
+        assert np.array_equal(lo, np.minimum(l_lo, r_lo)) and np.array_equal(hi, np.maximum(l_hi, r_hi))
+        stack.append((l_ref, l_lo, l_hi, level + 1))
+        stack.append((r_ref, r_lo, r_hi, level + 1))
+    assert len(seen_prims) == n_prims and len(seen_nodes) == n_nodes
+    assert max_level == depth
+    slots = i[(4 + 4 * (CAPACITY - 1)) * 4:][:CAPACITY]
+    for s in np.nonzero(used)[0]:
+        assert slots[order[s]] == s
+
+
+def _order_dependent(a, b):
+    """Pixels where two Hit buffers differ; each must be two object hits at
+    nearly the same t (the walk-order-dependent case)."""
+    diff = np.any(a.view(np.uint32) != b.view(np.uint32), axis=1)
+    for k in np.nonzero(diff)[0]:
+        assert a[k, 1] >= 0 and b[k, 1] >= 0, (k, a[k], b[k])
+        assert abs(a[k, 0] - b[k, 0]) <= 1e-3 * a[k, 0], (k, a[k], b[k])
+    return int(diff.sum())
+
+
+@pytest.mark.parametrize("frame", [0, 57])
+def test_accel_trace_equals_heap_trace(px, oracle, scene, frame):
+    print("case: accel trace 1920x1080 walk frame", frame, flush=True)
+    dev, acc = scene
+    W, H = 1920, 1080
+    C = to_dev(_walk_camera(px, oracle, frame, W, H, 0.0))
+    fa, fb = px.FrameBuffers(W, H), px.FrameBuffers(W, H)
+    px.trace(dev, C, fa)
+    px.trace_accel(dev, acc, C, fb)
+    a, b = to_host(fa.Hit).reshape(-1, 4), to_host(fb.Hit).reshape(-1, 4)
+    n = _order_dependent(a, b)
+    print("pixels differing:", n)
+    assert n <= 4
+
+
+def test_accel_launch_shapes_and_schedule_change_nothing(px, oracle, scene):
+    print("case: accel variants and schedule, 640x360", flush=True)
+    dev, acc = scene
+    W, H = 640, 360
+    C = to_dev(_walk_camera(px, oracle, 3, W, H, 1.0))
+    lib = px.lib()
+    old_t, old_v = lib.arbor_tune_get(b"accel_variant"), lib.arbor_tune_get(b"accel_vis_variant")
+    try:
+        ref_fb = px.FrameBuffers(W, H)
+        px.trace_accel(dev, acc, C, ref_fb)
+        want = to_host(ref_fb.Hit)
+        sched = px.TileSchedule(W, H)
+        for v in range(3):
+            assert lib.arbor_tune_set(b"accel_variant", v) == 0
+            for s in (None, sched, sched):  # unscheduled, first and second scheduled call
+                fb = px.FrameBuffers(W, H)
+                px.trace_accel(dev, acc, C, fb, s)
+                assert np.array_equal(to_host(fb.Hit).view(np.uint32), want.view(np.uint32)), (v, s is None)
+        import torch
+
+        vis = [torch.zeros(W * H * 4, dtype=torch.float32, device="cuda") for _ in range(5)]
+        for v in range(5):
+            assert lib.arbor_tune_set(b"accel_vis_variant", v) == 0
+            px.sun_visibility_accel(dev, acc, C, ref_fb, vis[v], sched if v % 2 else None)
+        for v in range(1, 5):
+            assert np.array_equal(to_host(vis[v]).view(np.uint32), to_host(vis[0]).view(np.uint32)), v
+    finally:
+        lib.arbor_tune_set(b"accel_variant", old_t)
+        lib.arbor_tune_set(b"accel_vis_variant", old_v)
+
+
+def test_accel_visibility_equals_heap_visibility(px, oracle, scene):
+    print("case: accel sun visibility 1920x1080 walk frame 0, shadows on", flush=True)
+    import torch
+
+    dev, acc = scene
+    W, H = 1920, 1080
+    C = to_dev(_walk_camera(px, oracle, 0, W, H, 1.0))
+    fb = px.FrameBuffers(W, H)
+    px.trace(dev, C, fb)
+    va = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+    vb = torch.zeros_like(va)
+    px.sun_visibility(dev, C, fb, va)
+    px.sun_visibility_accel(dev, acc, C, fb, vb)
+    a, b = to_host(va).reshape(-1, 4), to_host(vb).reshape(-1, 4)
+    diff = np.any(a.view(np.uint32) != b.view(np.uint32), axis=1)
+    print("pixels differing:", int(diff.sum()), "shadowed fraction %.3f" % (a[:, 0] < 1).mean())
+    assert diff.sum() <= 4
+
+
+def test_engine_accel_on_and_off(px):
+    print("case: engine 320x192 seed 7, accel on vs off, 3 frames with shadows", flush=True)
+    w, h = 320, 192
+    out = []
+    for on in (True, False):
+        eng = px.Engine(w, h, 7)
+        eng.set_accel(on)
+        inp = np.zeros(32, np.float32)
+        for f in range(3):
+            inp[0] = 0.3 * f
+            eng.set_input(inp)
+            eng.frame()
+        eng.sync()
+        out.append(eng.download("Hit", w * h * 4).reshape(-1, 4))
+    n = _order_dependent(out[0], out[1])
+    print("pixels differing:", n)
+    assert n <= 2
