@@ -45,7 +45,8 @@ template <int kWaves, class W>
 __global__ __launch_bounds__(kWaves * 64) void k_traceShadeViews(const float* P, const float4* Nodes4,
                                                                  const uint32_t* Order, const float* Cs,
                                                                  const ViewDesc* views, int nviews, int totalTiles,
-                                                                 float4* Hit, float4* Linear, float4* Guide) {
+                                                                 float4* Hit, float4* Linear, float4* Guide,
+                                                                 const float4* accel) {
   __shared__ TreeInst trees[kTrees];
   __shared__ ViewDesc sv[kMaxViews];
   for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, (int)Cs[31]);
@@ -63,8 +64,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_traceShadeViews(const float* P,
     v3 ro = sc.view.eye, rd = viewRay(sc.view, x, y, d.width, d.height);
     float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
     int tree = -1;
-    v2 h = traceForest<W>(P, Nodes4, nullptr, Order, trees, ro, rd, C[14] / (float)d.height, C[8], C[11],
-                          ground > 0.0f ? ground : kFar, &tree, false);
+    v2 h = traceForest<W>(P, Nodes4, W::sah ? accel : nullptr, Order, trees, ro, rd, C[14] / (float)d.height, C[8],
+                          C[11], ground > 0.0f ? ground : kFar, &tree, false);
     const int pixel = d.pixelBase + y * d.width + x;
     const float4 hit = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)d.height);
     Hit[pixel] = hit;
@@ -83,7 +84,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_traceShadeViews(const float* P,
 template <int kWaves, class W>
 __global__ __launch_bounds__(kWaves * 64) void k_traceViews(const float* P, const float4* Nodes4, const uint32_t* Order,
                                                             const float* Cs, const ViewDesc* views, int nviews,
-                                                            int totalTiles, float4* Hit, float4*, float4*) {
+                                                            int totalTiles, float4* Hit, float4*, float4*,
+                                                            const float4* accel) {
   __shared__ TreeInst trees[kTrees];
   __shared__ ViewDesc sv[kMaxViews];
   for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, (int)Cs[31]);
@@ -101,17 +103,18 @@ __global__ __launch_bounds__(kWaves * 64) void k_traceViews(const float* P, cons
     v3 ro = view.eye, rd = viewRay(view, x, y, d.width, d.height);
     float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
     int tree = -1;
-    v2 h = traceForest<W>(P, Nodes4, nullptr, Order, trees, ro, rd, C[14] / (float)d.height, C[8], C[11],
-                          ground > 0.0f ? ground : kFar, &tree, false);
+    v2 h = traceForest<W>(P, Nodes4, W::sah ? accel : nullptr, Order, trees, ro, rd, C[14] / (float)d.height, C[8],
+                          C[11], ground > 0.0f ? ground : kFar, &tree, false);
     Hit[d.pixelBase + y * d.width + x] = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)d.height);
   }
 }
 // Sun visibility of every view's pixels (the visibility pass of frame.hip,
 // per view: a view with C[22] off gets nothing written, and its shading
 // reads nothing).
+template <class W>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5))) void k_sunVisibilityViews(
     const float* P, const float4* Nodes4, const uint32_t* Order, const float* Cs, const ViewDesc* views, int nviews,
-    int totalTiles, const float4* Hit, float4* Vis) {
+    int totalTiles, const float4* Hit, float4* Vis, const float4* accel) {
   __shared__ TreeInst trees[kTrees];
   __shared__ ViewDesc sv[kMaxViews];
   for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, (int)Cs[31]);
@@ -128,7 +131,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5))) void k
     if (x >= d.width || y >= d.height) continue;
     const ShadeCtx sc = makeShadeCtx(C, d.width, d.height);
     const int pixel = d.pixelBase + y * d.width + x;
-    v3 vis = sunVisibilityOne<Walk<true, false, false>>(P, Nodes4, nullptr, Order, trees, sc, x, y, Hit[pixel]);
+    v3 vis = sunVisibilityOne<W>(P, Nodes4, W::sah ? accel : nullptr, Order, trees, sc, x, y, Hit[pixel]);
     Vis[pixel] = make_float4(vis.x, vis.y, vis.z, 1.0f);
   }
 }
@@ -163,7 +166,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kTraceVis ?
 }
 
 typedef void (*BatchKernel)(const float*, const float4*, const uint32_t*, const float*, const ViewDesc*, int, int,
-                            float4*, float4*, float4*);
+                            float4*, float4*, float4*, const float4*);
 struct BatchVariant {
   BatchKernel kernel;  // fused trace + shade, or (split) the trace followed by k_shadeViews
   bool split;
@@ -181,6 +184,15 @@ const BatchVariant kBatchVariants[] = {
 };
 constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 int batchVariantCount() { return kBatchVariantCount; }
+// The same launch shapes over the SAH hierarchy (arbor_batch_set_accel).
+using WalkSahB = Walk<true, false, false, true>;
+const BatchVariant kBatchVariantsSah[] = {
+    {k_traceShadeViews<1, WalkSahB>, false, 1, ""}, {k_traceShadeViews<2, WalkSahB>, false, 2, ""},
+    {k_traceShadeViews<4, WalkSahB>, false, 4, ""}, {k_traceShadeViews<1, WalkSahB>, false, 1, ""},
+    {k_traceShadeViews<4, WalkSahB>, false, 4, ""}, {k_traceViews<1, WalkSahB>, true, 1, ""},
+    {k_traceViews<2, WalkSahB>, true, 2, ""},
+};
+static_assert(sizeof(kBatchVariantsSah) == sizeof(kBatchVariants), "one SAH entry per batch variant");
 
 }  // namespace arbor
 
@@ -200,6 +212,7 @@ struct arbor_batch {
   uint32_t* Pixels = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  const float4* accel = nullptr;  // SAH hierarchy (arbor_batch_set_accel), or NULL for the heap walk
 };
 
 namespace {
@@ -222,20 +235,22 @@ void dropGraph(arbor_batch* b) {
 // The launches of one batched render on `stream` (also what gets captured).
 int issue(arbor_batch* b, const float* P, const float* Nodes, const unsigned* Order, int stages, hipStream_t s) {
   if (stages & ARBOR_BATCH_TRACE_SHADE) {
-    const BatchVariant& v = kBatchVariants[tuning().batch_variant];
+    const BatchVariant& v = (b->accel ? kBatchVariantsSah : kBatchVariants)[tuning().batch_variant];
     int blocks = (b->totalTiles + v.waves - 1) / v.waves;
     if (blocks > tuning().grid_cap) blocks = tuning().grid_cap;
     hipLaunchKernelGGL(v.kernel, dim3(blocks), dim3(v.waves * 64), 0, s, P, reinterpret_cast<const float4*>(Nodes),
-                       Order, b->dC, b->dViews, b->nviews, b->totalTiles, b->Hit, b->Linear, b->Guide);
+                       Order, b->dC, b->dViews, b->nviews, b->totalTiles, b->Hit, b->Linear, b->Guide, b->accel);
     int rc = launch_status();
     if (rc != ARBOR_OK) return rc;
     if (v.split) {
       int vblocks = (b->totalTiles + 1) / 2, sblocks = (b->totalTiles + 3) / 4;
       if (vblocks > tuning().grid_cap) vblocks = tuning().grid_cap;
       if (sblocks > tuning().grid_cap) sblocks = tuning().grid_cap;
-      hipLaunchKernelGGL(k_sunVisibilityViews, dim3(vblocks), dim3(128), 0, s, P,
-                         reinterpret_cast<const float4*>(Nodes), Order, b->dC, b->dViews, b->nviews, b->totalTiles,
-                         b->Hit, b->Vis);
+      typedef void (*VisViewsKernel)(const float*, const float4*, const uint32_t*, const float*, const ViewDesc*, int,
+                                     int, const float4*, float4*, const float4*);
+      const VisViewsKernel vk = b->accel ? k_sunVisibilityViews<WalkSahB> : k_sunVisibilityViews<Walk<true, false, false>>;
+      hipLaunchKernelGGL(vk, dim3(vblocks), dim3(128), 0, s, P, reinterpret_cast<const float4*>(Nodes), Order, b->dC,
+                         b->dViews, b->nviews, b->totalTiles, b->Hit, b->Vis, b->accel);
       if ((rc = launch_status()) != ARBOR_OK) return rc;
       hipLaunchKernelGGL(k_shadeViews<false>, dim3(sblocks), dim3(256), 0, s, P,
                          reinterpret_cast<const float4*>(Nodes), Order, b->dC, b->dViews, b->nviews, b->totalTiles,
@@ -290,6 +305,13 @@ int arbor_batch_create(arbor_batch** out, int max_views, long long max_pixels) {
     return ARBOR_ERR_DEVICE;
   }
   *out = b;
+  return ARBOR_OK;
+}
+
+int arbor_batch_set_accel(arbor_batch* b, const void* accel) {
+  if (!b) return ARBOR_ERR_NULL;
+  b->accel = static_cast<const float4*>(accel);
+  dropGraph(b);  // a captured graph holds the previous walk's launches
   return ARBOR_OK;
 }
 

@@ -302,7 +302,8 @@ void* arbor_engine_buffer(arbor_engine* e, const char* name) {
   struct { const char* n; void* p; } table[] = {
       {"B", b.B}, {"P", b.P}, {"Keys", b.Keys}, {"Order", b.Order}, {"Nodes", b.Nodes}, {"C", b.C}, {"I", b.I},
       {"Hit", b.Hit}, {"Linear", b.Linear}, {"Guide", b.Guide}, {"Filtered", b.Filtered}, {"History", b.History},
-      {"Pixels", b.Pixels}, {"Schedule", b.Schedule}};
+      {"Pixels", b.Pixels}, {"Schedule", b.Schedule}, {"VisSchedule", b.VisSchedule}, {"Vis", b.Vis},
+      {"Accel", b.Accel}};
   for (auto& t : table)
     if (strcmp(t.n, name) == 0) return t.p;
   return nullptr;

@@ -97,6 +97,7 @@ SIGNATURES = {
     "arbor_batch_render": [_vp, _vp, _vp, _vp, _i, _vp],
     "arbor_batch_capture": [_vp, _vp, _vp, _vp, _i, _vp],
     "arbor_batch_launch": [_vp, _vp],
+    "arbor_batch_set_accel": [_vp, _vp],
     "arbor_batch_buffer": [_vp, _i, ct.c_char_p],
     "arbor_batch_pixels": [_vp],
     "arbor_batch_tiles": [_vp],
@@ -381,6 +382,12 @@ class Batch:
 
     def launch(self, stream=None):
         check(lib().arbor_batch_launch(self.h, _stream(stream)), "arbor_batch_launch")
+
+    def set_accel(self, accel):
+        """Trace over a TraceAccel's SAH hierarchy (None: the heap BVH)."""
+        self.accel = accel  # keeps the device buffer alive while the batch uses it
+        check(lib().arbor_batch_set_accel(self.h, None if accel is None else _ptr(accel.buf)),
+              "arbor_batch_set_accel")
 
     def autotune(self, scene, reps=2):
         import json

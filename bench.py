@@ -21,19 +21,23 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "arbor-cuda_amd"))
 
-# Algorithmic FP32 operations per primary ray of tracePixels on this workload,
-# averaged over every 8th frame of the 120-frame walk at every 8th pixel:
-# `python oracle/count_trace_ops.py` (the oracle's counting instantiation,
-# oracle/flops.cpp: each +,-,*,/,sqrt,min,max,abs,floor and libm call counts 1,
-# with the GPU's traversal, rejection tests and early exits, including the leaf
-# Newton solve's fixed-point and 2-cycle exits, the skipped band-weighted
-# terms whose band weight is 0, and the stored leaf basis with the breeze off).
-TRACE_FLOP_PER_RAY = 8785.9
-# HBM bytes per scheduled tracePixels launch on this workload (walk frame 0):
-# rocprofv3 --pmc FETCH_SIZE 30,960 KiB x 2.0 + WRITE_SIZE 33,345 KiB x 1.0, the
-# factors being what the same counters read for a 256 MiB device copy
-# (131,097 KiB and 262,144 KiB); profiles/tracePixels_forest1080p_pmc.md.
-TRACE_HBM_BYTES = (30960.0 * 2.0 + 33345.0 * 1.0) * 1024
+# Algorithmic FP32 operations per primary ray of the engine's trace on this
+# workload, averaged over every 8th frame of the 120-frame walk at every 8th
+# pixel: `python oracle/count_trace_ops.py` (the oracle's counting
+# instantiation, oracle/flops.cpp: each +,-,*,/,sqrt,min,max,abs,floor and libm
+# call counts 1, with the GPU's walk over the SAH hierarchy and its tie-order
+# box tests, rejection tests and early exits, including the leaf Newton
+# solve's fixed-point and 2-cycle exits, the skipped band-weighted terms whose
+# band weight is 0, and the stored leaf basis with the breeze off). The heap
+# BVH walk of arbor_tracePixels does 8785.9 (`--heap`) for the same Hit.
+TRACE_FLOP_PER_RAY = 5030.3
+TRACE_FLOP_PER_RAY_HEAP = 8785.9
+# HBM bytes per scheduled SAH-walk trace launch on this workload (walk frame
+# 0): rocprofv3 --pmc FETCH_SIZE 124,000 KiB x 2.0 + WRITE_SIZE 490,050 KiB x
+# 1.0, the factors being what the same counters read for a 256 MiB device copy
+# (131,097 KiB and 262,144 KiB); profiles/tracePixels_forest1080p_pmc.md. Most
+# of it is the walk's per-lane scratch stack leaving L2.
+TRACE_HBM_BYTES = (124000.0 * 2.0 + 490050.0 * 1.0) * 1024
 FP32_PEAK_TFLOPS = 157.3
 # One non-packed FP32 VALU op per lane per clock: 256 CUs x 64 lanes x 2.4 GHz.
 FP32_SINGLE_ISSUE_TOPS = 39.32
@@ -51,8 +55,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-trace", type=int, default=0, metavar="N",
-                    help="profiling driver: N scheduled tracePixels launches of walk frame 0 (the engine's trace "
-                         "call: tile-order kernel + trace kernel), then a 256 MiB device copy "
+                    help="profiling driver: N scheduled SAH-walk tracePixels launches of walk frame 0 (the "
+                         "engine's trace call: tile-order kernels + trace kernel), then a 256 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE/WRITE_SIZE); prints no JSON")
     ap.add_argument("--views", type=int, default=0, metavar="N",
                     help="batched-views workload instead: N seeded walk views of uneven resolution rendered by one "
@@ -234,8 +238,10 @@ def bench_views(args):
     scene = px.DeviceScene().generate(args.seed)
     cams, sizes = px.batch_walk_views(args.views_seed, args.seed, n)
     rays = sum(w * h for w, h in sizes)
+    accel = px.TraceAccel(scene)
     b = px.Batch(n, rays)
     b.set_views(cams, sizes)
+    b.set_accel(accel)
     tuned = b.autotune(scene, reps=1)
     stream = torch.cuda.Stream()
     b.capture(scene, stream)
@@ -263,7 +269,7 @@ def bench_views(args):
         t1 = time.perf_counter()
         for i, (w, h) in enumerate(sizes):
             fb = fbs[(w, h)]
-            px.trace(scene, cds[i], fb)
+            px.trace_accel(scene, accel, cds[i], fb)
             px.shade(scene, cds[i], fb)
             px.filter_lighting(cds[i], fb)
             px.resolve(cds[i], fb)
@@ -285,7 +291,8 @@ def bench_views(args):
         "data": "synthetic (procedural forest from seed; seeded view positions and resolutions)",
         "config": {"workload": "batched views: %d cameras inside the 324-tree forest, resolutions from "
                                "{256x256, 640x360, 1280x720, 1920x1080, 2560x1440}, 1 spp primary+shade+filter+"
-                               "resolve, shadows off, one captured hipGraph per step" % n,
+                               "resolve, shadows off, one captured hipGraph per step, trace over the SAH "
+                               "hierarchy" % n,
                    "views": n, "pixels": rays, "seed": args.seed, "views_seed": args.views_seed},
         "graph_ms_per_step": round(graph_ms, 4),
         "autotune": tuned,
@@ -307,10 +314,10 @@ def profile_trace(args):
     eng.sync()
     lib = px.lib()
     s = lib.arbor_engine_stream(eng.h)
-    bufs = [eng.buffer_ptr(n) for n in ("P", "Nodes", "Order", "C", "Hit")]
-    sched = eng.buffer_ptr("Schedule")
-    for _ in range(args.profile_trace):  # the engine's own trace call: scheduled, same camera
-        px.check(lib.arbor_tracePixelsScheduled(*bufs, W, H, sched, s), "arbor_tracePixelsScheduled")
+    P, Nodes, Order, C, Hit = [eng.buffer_ptr(n) for n in ("P", "Nodes", "Order", "C", "Hit")]
+    accel, sched = eng.buffer_ptr("Accel"), eng.buffer_ptr("Schedule")
+    for _ in range(args.profile_trace):  # the engine's own trace call: SAH walk, scheduled, same camera
+        px.check(lib.arbor_tracePixelsAccel(P, Nodes, Order, accel, C, Hit, W, H, sched, s), "arbor_tracePixelsAccel")
     eng.sync()
     a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
@@ -434,7 +441,8 @@ def main():
     mean = st.mean(axis=0)
     trace_ms = float(mean[1])
 
-    roofline = {"bound": "mfma", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS, "kernel": "tracePixels",
+    roofline = {"bound": "mfma", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
+                "kernel": "tracePixels (SAH walk, arbor_tracePixelsAccel)",
                 "compute": "FP32 vector ALU (ray traversal has no MFMA shape); peak is MI355X's FP32 157.3 TFLOP/s",
                 "traffic": None}
     std_res = (W, H) == (1920, 1080) and args.seed == 7
@@ -442,6 +450,7 @@ def main():
         achieved = TRACE_FLOP_PER_RAY * W * H / (trace_ms * 1e-3) / 1e12
         roofline.update({"achieved": round(achieved, 3), "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                          "flop_per_ray": TRACE_FLOP_PER_RAY,
+                         "flop_per_ray_heap_walk": TRACE_FLOP_PER_RAY_HEAP,
                          "frac_of_single_issue_rate": round(achieved / FP32_SINGLE_ISSUE_TOPS, 4),
                          "traffic": int(TRACE_HBM_BYTES),
                          "traffic_algorithmic": int(16 * W * H + 5373952)})

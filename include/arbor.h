@@ -162,13 +162,13 @@ int arbor_shadePixelsVis(const float* P, const float* Nodes, const unsigned* Ord
  * of device memory; it synchronises the stream, and the same Nodes/Order give
  * the same bytes. The Accel trace and visibility entry points walk it in place
  * of the heap (about a third of the box tests and 30% fewer object tests).
- * They take the heap too: a tree whose nearest hit is an exact tie in t is
- * walked again over the heap, so ties resolve as the reference's walk does.
- * Results equal arbor_tracePixels' / arbor_sunVisibility's except for rays
- * that hit a swaying leaf outside its rest-pose leaf box, where which of two
- * valid hits the reference reports depends on its walk order (about 1 pixel
- * in 2 million at 1920x1080). `schedule` (may be NULL) is a tile schedule
- * as for the Scheduled entry points. */
+ * They take the heap too: objects hit at exactly the same t resolve in the
+ * heap walk's order, decided at their common heap ancestor. Results equal
+ * arbor_tracePixels' / arbor_sunVisibility's except for rays that hit a
+ * swaying leaf outside its rest-pose leaf box, where which of two valid hits
+ * the reference reports depends on its walk order (about 1 pixel in 2
+ * million at 1920x1080). `schedule` (may be NULL) is a tile schedule as for
+ * the Scheduled entry points. */
 long long arbor_trace_accel_bytes(void);
 int arbor_buildTraceAccel(const float* Nodes, const unsigned* Order, void* accel, hipStream_t stream);
 int arbor_tracePixelsAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,

@@ -40,6 +40,11 @@ int arbor_batch_render(arbor_batch* b, const float* P, const float* Nodes, const
 int arbor_batch_capture(arbor_batch* b, const float* P, const float* Nodes, const unsigned* Order, int stages,
                         hipStream_t stream);
 int arbor_batch_launch(arbor_batch* b, hipStream_t stream);
+/* Trace and sun visibility over a SAH hierarchy of arbor_buildTraceAccel
+ * (arbor.h) for the scene passed to render/capture, or over the heap BVH
+ * with accel NULL (the default). batch_variant keeps selecting the launch
+ * shape. Invalidates a captured graph. No counterpart in the reference. */
+int arbor_batch_set_accel(arbor_batch* b, const void* accel);
 /* Device pointer of view `view`'s slice of a buffer: Hit Linear Guide
  * Filtered History Pixels C; NULL for an unknown name or view. */
 void* arbor_batch_buffer(arbor_batch* b, int view, const char* name);

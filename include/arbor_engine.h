@@ -44,7 +44,8 @@ int arbor_engine_frame(arbor_engine* e, float dt, const float* device_input);
 int arbor_engine_stage_ms(const arbor_engine* e, float* out6);
 int arbor_engine_sync(arbor_engine* e);
 /* Device pointer of a named buffer: B P Keys Order Nodes C I Hit Linear Guide
- * Filtered History Pixels; NULL for an unknown name. */
+ * Filtered History Pixels Schedule VisSchedule Vis Accel; NULL for an unknown
+ * name. */
 void* arbor_engine_buffer(arbor_engine* e, const char* name);
 hipStream_t arbor_engine_stream(arbor_engine* e);
 int arbor_engine_size(const arbor_engine* e, int* width, int* height);

@@ -832,9 +832,111 @@ template <class R> struct Arbor {
     return {best, R(found)};
   }
 
+  // ---- walk over a SAH hierarchy (no counterpart in the reference) ----
+  // The GPU's accel walk, restated in this file's arithmetic: a binary
+  // hierarchy over the heap's leaf boxes in the device layout of
+  // arbor-cuda_amd/csrc/sah_build.hpp (sahAccel, set by the caller), walked
+  // near-first with a stack of the far siblings and their entry t. The object
+  // tests run with their early exits made strict (best raised by one ulp), so
+  // an object with t equal to the best reports it; of two such, the one the
+  // heap walk tests first is kept: their order is decided at their lowest
+  // common heap ancestor, by the entry t of its two children (nearer first,
+  // the right one on equal t). Used to check that walk against the heap walk
+  // on whole frames and to count its operations.
+  static inline const uint32_t* sahAccel = nullptr;
+  static inline bool sahTieReversed = false;  // planted mistake for the tests: the tie order inverted
+  static R nextUpR(R x) {  // x > 0 and finite: the next representable value
+    if constexpr (sizeof(R) == 8) {
+      uint64_t u;
+      std::memcpy(&u, &x, 8);
+      u++;
+      std::memcpy(&x, &u, 8);
+    } else {
+      uint32_t u;
+      std::memcpy(&u, &x, 4);
+      u++;
+      std::memcpy(&x, &u, 4);
+    }
+    return x;
+  }
+  static F3 sahV(const uint32_t* w) {
+    float f[3];
+    std::memcpy(f, w, sizeof f);
+    return c3(R(f[0]), R(f[1]), R(f[2]));
+  }
+  static F2 spanInv(F3 ro, F3 inv, F3 lo, F3 hi) {
+    F3 a = (lo - ro) * inv, b = (hi - ro) * inv;
+    F3 mn = vmin(a, b), mx = vmax(a, b);
+    return F2{m_max(mn.x, m_max(mn.y, mn.z)), m_min(mx.x, m_min(mx.y, mx.z))};
+  }
+  static bool heapLeafFirst(const R* Nodes, F3 ro, F3 inv, unsigned sa, unsigned sb) {
+    const unsigned na = CAPACITY + sa, nb = CAPACITY + sb;
+    const int k = 32 - __builtin_clz(na ^ nb);
+    const unsigned ca = na >> (k - 1), cb = nb >> (k - 1);
+    const unsigned l = std::min(ca, cb), r = l ^ 1u;
+    F2 ls = spanInv(ro, inv, ld3(Nodes, l * 8), ld3(Nodes, l * 8 + 4));
+    F2 rs = spanInv(ro, inv, ld3(Nodes, r * 8), ld3(Nodes, r * 8 + 4));
+    return ((ls.x < rs.x ? l : r) == ca) != sahTieReversed;
+  }
+  static F2 traceTreeSah(const R* P, const R* Nodes, F3 ro, F3 rd, R cone, R time, R wind, R best, int ignore,
+                         bool shadows) {
+    const uint32_t* A = sahAccel;
+    const uint32_t* slotOf = A + 16 + 16 * (CAPACITY - 1);
+    const F3 inv = c3(invSafe(rd.x), invSafe(rd.y), invSafe(rd.z));
+    auto live = [&](F2 sp) { return sp.y >= m_max(K(0.0001), sp.x) && sp.x < best; };
+    int found = -1;
+    F2 span = spanInv(ro, inv, sahV(A), sahV(A + 4));
+    int ref = (int)A[3];
+    bool hit = live(span);
+    std::vector<std::pair<int, R>> stack;
+    while (true) {
+      if (hit) {
+        if (ref >= 0) {
+          const uint32_t* nd = A + 16 + 16 * (size_t)ref;
+          F2 ls = spanInv(ro, inv, sahV(nd), sahV(nd + 4)), rs = spanInv(ro, inv, sahV(nd + 8), sahV(nd + 12));
+          bool lh = live(ls), rh = live(rs);
+          if (lh || rh) {
+            bool takeLeft = lh && (!rh || ls.x < rs.x);
+            int lref = (int)nd[3], rref = (int)nd[11];
+            if (lh && rh) stack.push_back({takeLeft ? rref : lref, takeLeft ? rs.x : ls.x});
+            ref = takeLeft ? lref : rref;
+            span = takeLeft ? ls : rs;
+            continue;
+          }
+        } else {
+          const int id = ~ref;
+          if (id != ignore) {
+            R footprint = m_max(K(0.000005), cone * m_max(K(0.05), span.x));
+            R t = objectT(P, id, ro, rd, footprint, time, wind, !shadows, nextUpR(best));
+            if (t < best) {
+              best = t;
+              found = id;
+            } else if (t == best && found >= 0 && heapLeafFirst(Nodes, ro, inv, slotOf[id], slotOf[found])) {
+              found = id;
+            }
+          }
+        }
+      }
+      hit = false;
+      while (!stack.empty()) {
+        auto e = stack.back();
+        stack.pop_back();
+        if (e.second < best) {
+          ref = e.first;
+          span.x = e.second;
+          hit = true;
+          break;
+        }
+      }
+      if (!hit) break;
+    }
+    return {best, R(found)};
+  }
+
   // Heap BVH walk, explicit stack (Arbor.cu:220-230).
   static F2 traceTree(const R* P, const R* Nodes, const unsigned* Order, F3 ro, F3 rd, R cone, R time, R wind,
                       R best, int ignore, bool shadows) {
+    if (sahAccel) return traceTreeSah(P, Nodes, ro, rd, cone, time, wind, best, ignore, shadows);
     if (leanWalk) return traceTreeLean(P, Nodes, Order, ro, rd, cone, time, wind, best, ignore, shadows);
     int stack[32], top = 0, node = 1, found = -1;
     for (int visit = 0; visit < NODE_COUNT; visit++) {

@@ -3,7 +3,9 @@
 Runs the oracle's counting instantiation (oracle/flops.cpp) on the
 full-forest workload bench.py times: seed 7, 324 trees, 1920x1080, shadows
 off, the seeded 120-frame walk. Counts tracePixels with the GPU's traversal
-(lean walk), bounding-sphere rejections, early exits past the caller's best
+(the engine's walk over the SAH hierarchy, including its tie-order box tests;
+with --heap the lean heap walk), bounding-sphere rejections, early exits past
+the caller's best
 the leaf Newton solve's fixed-point/2-cycle exits, the skips of
 band-weighted terms with a zero band weight and the stored leaf basis with
 the breeze off, sampling every `--frame-step`-th
@@ -49,6 +51,8 @@ def main():
     ap.add_argument("--frame-step", type=int, default=8)
     ap.add_argument("--stride", type=int, default=8)
     ap.add_argument("--frames", type=int, default=120)
+    ap.add_argument("--heap", action="store_true",
+                    help="count the heap BVH walk (arbor_tracePixels) instead of the SAH walk the engine runs")
     args = ap.parse_args()
     o = Oracle()
     lib = o.lib
@@ -60,6 +64,10 @@ def main():
     W, H, seed = 1920, 1080, 7
     s = o.generate(seed)
     C = o.init_camera(seed)
+    lib.orc_set_sah.argtypes = [ct.c_void_p]
+    accel = None if args.heap else o.build_sah(s)
+    lib.orc_set_sah(None if accel is None else accel.ctypes.data)
+    print("walk:", "heap BVH" if accel is None else "SAH hierarchy", flush=True)
     per_frame = []
     kinds = None
     for f in range(args.frames):

@@ -11,6 +11,7 @@
 #include <cstdint>
 
 #include "arbor_oracle.hpp"
+#include "../arbor-cuda_amd/csrc/sah_build.hpp"
 
 namespace orc {
 
@@ -316,6 +317,63 @@ int orc_cull_mismatches(const float* P, const float* Nodes, const unsigned* Orde
 
 // Reference walk vs lean walk over whole frames: returns the number of
 // pixels whose Hit differs in any bit (0 expected).
+// The SAH hierarchy of the GPU's accel walk over this heap BVH (the
+// library's own builder, sah_build.hpp) into out[words]: 16 + 16 * (CAPACITY -
+// 1) + CAPACITY words. Returns 0, or -1 if out is too small or the depth
+// limit (32) failed.
+int orc_build_sah(const float* Nodes, const unsigned* Order, uint32_t* out, long long words) {
+  const long long need = 16 + 16LL * (CAPACITY - 1) + CAPACITY;
+  if (words < need) return -1;
+  std::vector<uint32_t> w = arbor::sah::buildAccel(Nodes + (size_t)CAPACITY * 8, Order, CAPACITY, 32, 100000.0f);
+  if (w.empty() || (long long)w.size() != need) return -1;
+  std::memcpy(out, w.data(), w.size() * 4);
+  return 0;
+}
+// Operation counts walk the SAH hierarchy `accel` (NULL: the heap).
+void orc_set_sah(const uint32_t* accel) { ACF::sahAccel = accel; }
+// Test hook: invert the SAH walk's tie order (a planted mistake).
+void orc_set_sah_tie_reversed(int on) { Arbor<float>::sahTieReversed = on != 0; }
+
+// The SAH walk against the heap walk over whole frames, with the GPU's
+// intersection shortcuts: mode 0 compares tracePixels' Hit, mode 1
+// shadePixels' Linear with the same Hit (its sun visibility walks the
+// hierarchy too). out[0] pixels differing, out[1] of them where both record
+// object hits within 1e-3 relative in t (mode 0 only; the walk-order-dependent
+// hits of swaying leaves), out[2] index of the first differing pixel or -1.
+int orc_sah_walk_mismatches(const float* P, const float* Nodes, const unsigned* Order, const float* C, int width,
+                            int height, const uint32_t* accel, int mode, long long* out) {
+  using A = Arbor<float>;
+  A::cullSpheres = A::skipBeyondBest = A::newtonExits = A::bandSkips = A::windShortcut = true;
+  A::leanWalk = true;
+  const size_t n = (size_t)width * height * 4;
+  std::vector<float> hit(n), a(n), b(n), guide(n);
+#pragma omp parallel for schedule(dynamic, 2)
+  for (int y = 0; y < height; y++)
+    for (int x = 0; x < width; x++) A::tracePixel(P, Nodes, Order, C, hit.data(), width, height, x, y);
+  for (int sah = 0; sah < 2; sah++) {
+    A::sahAccel = sah ? accel : nullptr;
+    float* H = sah ? b.data() : a.data();
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = 0; y < height; y++)
+      for (int x = 0; x < width; x++) {
+        if (mode == 0) A::tracePixel(P, Nodes, Order, C, H, width, height, x, y);
+        else A::shadePixel(P, Nodes, Order, C, hit.data(), H, guide.data(), width, height, x, y);
+      }
+  }
+  A::sahAccel = nullptr;
+  A::leanWalk = false;
+  A::cullSpheres = A::skipBeyondBest = A::newtonExits = A::bandSkips = A::windShortcut = false;
+  out[0] = out[1] = 0;
+  out[2] = -1;
+  for (size_t i = 0; i < n; i += 4) {
+    if (std::memcmp(&a[i], &b[i], 16) == 0) continue;
+    if (out[2] < 0) out[2] = (long long)(i / 4);
+    out[0]++;
+    if (mode == 0 && a[i + 1] >= 0 && b[i + 1] >= 0 && std::fabs(a[i] - b[i]) <= 1e-3f * a[i]) out[1]++;
+  }
+  return 0;
+}
+
 int orc_lean_walk_mismatches(const float* P, const float* Nodes, const unsigned* Order, const float* C, int width,
                              int height, int single_tree) {
   using A = Arbor<float>;

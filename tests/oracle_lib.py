@@ -180,6 +180,11 @@ class Oracle(_Common):
                                 [_f32p, _f32p, _u32p, _f32p, _i, _i, _i, _i,
                                  np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")])
         self._lean_mismatch = _bind(L, "orc_lean_walk_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i, _i])
+        _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        self._build_sah = _bind(L, "orc_build_sah", _i, [_f32p, _u32p, _u32p, ct.c_longlong])
+        self._sah_mismatch = _bind(L, "orc_sah_walk_mismatches", _i,
+                                   [_f32p, _f32p, _u32p, _f32p, _i, _i, _u32p, _i, _i64p])
+        self._set_sah_tie_reversed = _bind(L, "orc_set_sah_tie_reversed", None, [_i])
         self._cull_mismatch = _bind(L, "orc_cull_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
         self._skip_mismatch = _bind(L, "orc_skip_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
         self._newton_mismatch = _bind(L, "orc_newton_exit_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
@@ -236,6 +241,26 @@ class Oracle(_Common):
         """Pixels whose Hit or Linear changes when woodT/leafT stop once their
         answer is known to be >= the caller's best (the GPU's early exits)."""
         return self._skip_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h)
+
+    SAH_WORDS = 16 + 16 * (32768 - 1) + 32768
+
+    def build_sah(self, s):
+        """The GPU accel walk's SAH hierarchy of this scene (u32 words)."""
+        out = np.zeros(self.SAH_WORDS, np.uint32)
+        assert self._build_sah(s.Nodes, s.Order, out, out.size) == 0
+        return out
+
+    def sah_walk_mismatches(self, s, C, w, h, accel, mode=0, tie_reversed=False):
+        """(pixels differing, of them walk-order-dependent object hits, first
+        differing pixel) between the SAH walk and the heap walk: mode 0 Hit,
+        mode 1 Linear with sun visibility as C sets it."""
+        out = np.zeros(3, np.int64)
+        self._set_sah_tie_reversed(int(tie_reversed))
+        try:
+            self._sah_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h, accel, mode, out)
+        finally:
+            self._set_sah_tie_reversed(0)
+        return tuple(int(v) for v in out)
 
     def lean_walk_mismatches(self, s, C, w, h, single_tree=False):
         return self._lean_mismatch(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), w, h,

@@ -17,11 +17,11 @@ sun-visibility passes over it.
 import numpy as np
 import pytest
 
+from accel_check import check_accel_structure
 from gpu_util import require_gpu, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-CAPACITY = 32768
 
 
 @pytest.fixture(scope="module")
@@ -49,58 +49,13 @@ def _walk_camera(px, oracle, frame, w, h, shadows):
     return C
 
 
-def _parse(buf):
-    f = buf.view(np.float32)
-    i = buf.view(np.int32)
-    n_nodes, n_prims, depth = int(f[8]), int(f[9]), int(f[7])
-    return f, i, n_nodes, n_prims, depth
-
-
 def test_accel_structure(px, scene):
     print("case: accel structure, seed 7", flush=True)
     dev, acc = scene
     buf = acc.buf.cpu().numpy()
     again = px.TraceAccel(dev).buf.cpu().numpy()
     assert np.array_equal(buf, again), "the build is not deterministic"
-    f, i, n_nodes, n_prims, depth = _parse(buf)
-    nodes = to_host(dev.Nodes).reshape(-1, 8)
-    order = to_host(dev.Order).view(np.int32)
-    leaves = nodes[CAPACITY:]
-    used = leaves[:, 3] > 0
-    assert n_prims == int(used.sum()) and n_nodes == n_prims - 1
-    assert 1 <= depth <= 32
-    heap_box = {int(order[s]): leaves[s] for s in np.nonzero(used)[0]}
-    seen_prims, seen_nodes = set(), set()
-
-    def child(q, k):  # box (lo, hi) and ref of child k of interior node q
-        base = 16 + 16 * q + 8 * k
-        return f[base:base + 3], f[base + 4:base + 7], int(i[base + 3])
-
-    # iterative walk: (ref, lo, hi, level)
-    stack = [(int(i[3]), f[0:3], f[4:7], 0)]
-    max_level = 0
-    while stack:
-        ref, lo, hi, level = stack.pop()
-        max_level = max(max_level, level)
-        if ref < 0:
-            pid = ~ref
-            assert pid not in seen_prims and pid in heap_box
-            seen_prims.add(pid)
-            hb = heap_box[pid]
-            assert np.array_equal(lo, hb[0:3]) and np.array_equal(hi, hb[4:7])
-            continue
-        assert 0 <= ref < n_nodes and ref not in seen_nodes
-        seen_nodes.add(ref)
-        l_lo, l_hi, l_ref = child(ref, 0)
-        r_lo, r_hi, r_ref = child(ref, 1)
-        assert np.array_equal(lo, np.minimum(l_lo, r_lo)) and np.array_equal(hi, np.maximum(l_hi, r_hi))
-        stack.append((l_ref, l_lo, l_hi, level + 1))
-        stack.append((r_ref, r_lo, r_hi, level + 1))
-    assert len(seen_prims) == n_prims and len(seen_nodes) == n_nodes
-    assert max_level == depth
-    slots = i[(4 + 4 * (CAPACITY - 1)) * 4:][:CAPACITY]
-    for s in np.nonzero(used)[0]:
-        assert slots[order[s]] == s
+    check_accel_structure(buf.view(np.uint32), to_host(dev.Nodes), to_host(dev.Order))
 
 
 def _order_dependent(a, b):
@@ -195,3 +150,36 @@ def test_engine_accel_on_and_off(px):
     n = _order_dependent(out[0], out[1])
     print("pixels differing:", n)
     assert n <= 2
+
+
+def test_batch_accel_equals_batch_heap(px, scene):
+    """The batched views over the SAH hierarchy (every batch_variant, shadows
+    on) give the heap batch's Hit, Linear and Guide, up to the
+    walk-order-dependent pixels."""
+    print("case: batch of 6 walk views, accel vs heap, batch_variant 0..6, shadows on", flush=True)
+    dev, acc = scene
+    cams, sizes = px.batch_walk_views(3, 7, 6)
+    sizes = [(min(w, 640), min(h, 360)) for w, h in sizes]
+    cams = np.array(cams, np.float32)
+    cams[:, 22] = 1.0
+    lib = px.lib()
+    old = lib.arbor_tune_get(b"batch_variant")
+    try:
+        for v in range(7):
+            assert lib.arbor_tune_set(b"batch_variant", v) == 0
+            out = []
+            for use in (None, acc):
+                b = px.Batch(len(sizes), sum(w * h for w, h in sizes))
+                b.set_views(cams, sizes)
+                b.set_accel(use)
+                b.render(dev, px.BATCH_TRACE_SHADE)
+                out.append([b.download(i, "Hit") for i in range(len(sizes))] +
+                           [b.download(i, "Linear") for i in range(len(sizes))])
+                b.close()
+            n_hit = sum(_order_dependent(a.reshape(-1, 4), c.reshape(-1, 4)) for a, c in zip(out[0][:6], out[1][:6]))
+            lin = [np.any(a.reshape(-1, 4).view(np.uint32) != c.reshape(-1, 4).view(np.uint32), axis=1)
+                   for a, c in zip(out[0][6:], out[1][6:])]
+            print("batch_variant", v, "Hit pixels differing", n_hit, "Linear", int(sum(x.sum() for x in lin)))
+            assert n_hit <= 2 and sum(x.sum() for x in lin) <= 2 + 2 * n_hit
+    finally:
+        lib.arbor_tune_set(b"batch_variant", old)

@@ -265,6 +265,50 @@ def test_lean_walk_is_the_reference_walk(oracle, scenes, view, single):
     assert oracle.lean_walk_mismatches(scenes["7"], C, 200, 120, single) == 0
 
 
+def test_sah_hierarchy_structure(oracle, scenes):
+    """The GPU accel walk's SAH hierarchy (the library's host builder) over
+    the seed-7 heap BVH: deterministic, and a valid hierarchy of exactly the
+    heap's leaf boxes."""
+    from accel_check import check_accel_structure
+
+    s = scenes["7"]
+    a = oracle.build_sah(s)
+    assert np.array_equal(a, oracle.build_sah(s))
+    depth = check_accel_structure(a, s.Nodes, s.Order)
+    print("SAH depth", depth, "interior nodes", int(a.view(np.float32)[8]))
+
+
+@pytest.mark.parametrize("frame,w,h,mode", [(0, 480, 270, 0), (57, 480, 270, 0), (0, 240, 136, 1)])
+def test_sah_walk_is_the_heap_walk(oracle, scenes, frame, w, h, mode):
+    """The SAH walk (near-first over the hierarchy, strict early exits, exact
+    ties kept in the heap walk's order) gives the heap walk's Hit (mode 0)
+    and, with sun visibility on, shadePixels' Linear (mode 1) bit for bit,
+    except for walk-order-dependent hits of swaying leaves outside their
+    rest-pose boxes (both object hits, t within 1e-3; at most 2 here)."""
+    s = scenes["7"]
+    a = oracle.build_sah(s)
+    C = _walk_camera(oracle, frame, w, h, 1.0 if mode == 1 else 0.0)
+    diff, near, first = oracle.sah_walk_mismatches(s, C, w, h, a, mode)
+    print("frame", frame, "mode", mode, "differing", diff, "order-dependent", near, "first", first)
+    if mode == 0:
+        assert diff == near and near <= 2
+    else:
+        assert diff <= 2
+
+
+def test_sah_walk_tie_order_is_checked(oracle, scenes):
+    """Inverting the tie order (a planted mistake in the oracle's SAH walk)
+    changes the Hit of many pixels of the same frame: exact ties are common
+    (a ray that hits nothing in a tree scaled below 1 ties at the no-hit t
+    with every object it tests), so the comparison above pins the rule."""
+    s = scenes["7"]
+    a = oracle.build_sah(s)
+    C = _walk_camera(oracle, 0, 480, 270, 0.0)
+    diff, near, _ = oracle.sah_walk_mismatches(s, C, 480, 270, a, 0, tie_reversed=True)
+    print("with the tie order inverted: differing", diff)
+    assert diff > 100  # far beyond the 2 order-dependent pixels the comparison allows
+
+
 def _walk_camera(oracle, frame, w, h, shadows):
     import pyarbor
 
