@@ -387,10 +387,16 @@ AD int heapDepth(uint32_t node) { return 31 - __builtin_clz(node); }
 //   kSah:      the tree is walked over the SAH hierarchy of traceTreeSah
 //              below instead of the heap; the walk's `top` pointer is then
 //              that hierarchy (device memory).
-template <bool kCull, bool kPrefetch, bool kLdsTop, bool kSah = false>
+//   kLdsStack: with kSah, the first kLdsStack entries of each lane's stack
+//              live in the block's dynamic LDS (the launch provides
+//              waves x kLdsStack x 64 x 8 bytes), the rest in scratch.
+template <bool kCull, bool kPrefetch, bool kLdsTop, bool kSah = false, int kLdsStack = 0>
 struct Walk {
   static constexpr bool cull = kCull, prefetch = kPrefetch, ldsTop = kLdsTop, sah = kSah;
+  static constexpr int ldsStack = kLdsStack;
 };
+// Dynamic LDS of a kernel launch (the SAH walk's stack entries, kLdsStack).
+extern __shared__ uint64_t sahLdsStack[];
 constexpr int kTopNodes = 256;
 
 template <class W>
@@ -555,6 +561,9 @@ __device__ __forceinline__ v2 traceTreeSah(const float* P, const float4* Nodes4,
   uint64_t stack[kSahDepth];
   asm volatile("" : : "v"(stack) : "memory");
   int top = 0;
+  // entries [0, kLdsStack) in LDS: lane-interleaved per wave
+  constexpr int kL = W::ldsStack;
+  uint64_t* lds = kL > 0 ? sahLdsStack + (size_t)(threadIdx.x >> 6) * kL * 64 + (threadIdx.x & 63) : nullptr;
   // the most recent push stays in registers until the next push or its pop
   uint64_t cached = 0;
   bool haveCached = false;
@@ -572,7 +581,11 @@ __device__ __forceinline__ v2 traceTreeSah(const float* P, const float4* Nodes4,
           if (lh && rhit) {
             const uint32_t farRef = (uint32_t)(takeLeft ? rref : lref);
             const uint32_t farT = __float_as_uint(takeLeft ? rs.x : ls.x);
-            if (haveCached) stack[top++] = cached;
+            if (haveCached) {
+              if (kL > 0 && top < kL) lds[top * 64] = cached;
+              else stack[top] = cached;
+              top++;
+            }
             cached = ((uint64_t)farRef << 32) | farT;
             haveCached = true;
           }
@@ -599,7 +612,11 @@ __device__ __forceinline__ v2 traceTreeSah(const float* P, const float4* Nodes4,
     // other two liveness conditions held when it was pushed)
     live = false;
     while (haveCached || top > 0) {
-      const uint64_t e = haveCached ? cached : stack[--top];
+      uint64_t e = cached;
+      if (!haveCached) {
+        --top;
+        e = (kL > 0 && top < kL) ? lds[top * 64] : stack[top];
+      }
       haveCached = false;
       const float t = __uint_as_float((uint32_t)e);
       if (t < best) {

@@ -259,6 +259,9 @@ using WalkCullTop = Walk<true, false, true>;
 using WalkAll = Walk<true, true, true>;
 using WalkShade = WalkCull;
 using WalkSah = Walk<true, false, false, true>;  // the accel entry points
+template <int K>
+using WalkSahL = Walk<true, false, false, true, K>;  // stack entries [0, K) in LDS
+constexpr int sahLdsPerWave(int k) { return k * 64 * 8; }
 
 // shadePixels (Arbor.cu:277-289).
 // kTraceVis = false: Vis is given whenever sun visibility is on (the shadow
@@ -393,6 +396,7 @@ typedef void (*TraceKernel)(const float*, const float4*, const uint32_t*, const 
 struct TraceVariant {
   TraceKernel forest, tree;
   int waves;
+  int ldsPerWave = 0;  // dynamic LDS bytes per wave (the SAH walk's stack entries)
 };
 // trace_variant index -> kernels. Identical results for every entry.
 const TraceVariant kTraceVariants[] = {
@@ -435,6 +439,10 @@ const TraceVariant kAccelTraceVariants[] = {
     {k_tracePixels<1, WalkSah>, k_traceTreePixels<1, WalkSah>, 1},
     {k_tracePixels<2, WalkSah>, k_traceTreePixels<2, WalkSah>, 2},
     {k_tracePixels<4, WalkSah>, k_traceTreePixels<4, WalkSah>, 4},
+    {k_tracePixels<2, WalkSahL<8>>, k_traceTreePixels<2, WalkSahL<8>>, 2, sahLdsPerWave(8)},
+    {k_tracePixels<1, WalkSahL<4>>, k_traceTreePixels<1, WalkSahL<4>>, 1, sahLdsPerWave(4)},
+    {k_tracePixels<2, WalkSahL<12>>, k_traceTreePixels<2, WalkSahL<12>>, 2, sahLdsPerWave(12)},
+    {k_tracePixels<4, WalkSahL<6>>, k_traceTreePixels<4, WalkSahL<6>>, 4, sahLdsPerWave(6)},
 };
 constexpr int kAccelTraceVariantCount = sizeof(kAccelTraceVariants) / sizeof(kAccelTraceVariants[0]);
 
@@ -443,7 +451,7 @@ int launchTraceAccel(const float* P, const float* Nodes, const unsigned* Order, 
                      uint32_t* tileCost) {
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
   const TraceVariant& v = kAccelTraceVariants[tuning().accel_variant];
-  hipLaunchKernelGGL(v.forest, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), 0, stream, P,
+  hipLaunchKernelGGL(v.forest, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), v.ldsPerWave * v.waves, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
                      tx, tiles, tileOrder, tileCost, static_cast<const float4*>(accel));
   return launch_status();
@@ -535,6 +543,7 @@ typedef void (*VisKernel)(const float*, const float4*, const uint32_t*, const fl
 struct VisVariant {
   VisKernel kernel;
   int waves;
+  int ldsPerWave = 0;  // dynamic LDS bytes per wave (the SAH walk's stack entries)
 };
 // vis_variant index -> kernel. Identical results for every entry. The
 // kernel keeps the 324-entry tree table (11.7 KB) in LDS per block, so
@@ -561,8 +570,14 @@ int launchVisibility(const float* P, const float* Nodes, const unsigned* Order, 
 
 // accel_vis_variant index -> sun-visibility kernel over the SAH hierarchy.
 const VisVariant kAccelVisVariants[] = {
-    {k_sunVisibility<2, WalkSah, 5>, 2}, {k_sunVisibility<1, WalkSah>, 1}, {k_sunVisibility<2, WalkSah>, 2},
-    {k_sunVisibility<4, WalkSah, 5>, 4}, {k_sunVisibility<2, WalkSah, 4>, 2},
+    {k_sunVisibility<2, WalkSah, 5>, 2},
+    {k_sunVisibility<1, WalkSah>, 1},
+    {k_sunVisibility<2, WalkSah>, 2},
+    {k_sunVisibility<4, WalkSah, 5>, 4},
+    {k_sunVisibility<2, WalkSah, 4>, 2},
+    {k_sunVisibility<2, WalkSahL<4>, 5>, 2, sahLdsPerWave(4)},
+    {k_sunVisibility<2, WalkSahL<8>, 4>, 2, sahLdsPerWave(8)},
+    {k_sunVisibility<4, WalkSahL<4>, 5>, 4, sahLdsPerWave(4)},
 };
 constexpr int kAccelVisVariantCount = sizeof(kAccelVisVariants) / sizeof(kAccelVisVariants[0]);
 
@@ -571,7 +586,7 @@ int launchVisibilityAccel(const float* P, const float* Nodes, const unsigned* Or
                           const uint32_t* tileOrder, uint32_t* tileCost, hipStream_t stream) {
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
   const VisVariant& v = kAccelVisVariants[tuning().accel_vis_variant];
-  hipLaunchKernelGGL(v.kernel, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), 0, stream, P,
+  hipLaunchKernelGGL(v.kernel, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), v.ldsPerWave * v.waves, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Vis), width, height, tx, tiles, tileOrder, tileCost,
                      static_cast<const float4*>(accel));

@@ -95,7 +95,7 @@ def test_accel_launch_shapes_and_schedule_change_nothing(px, oracle, scene):
         px.trace_accel(dev, acc, C, ref_fb)
         want = to_host(ref_fb.Hit)
         sched = px.TileSchedule(W, H)
-        for v in range(3):
+        for v in range(7):
             assert lib.arbor_tune_set(b"accel_variant", v) == 0
             for s in (None, sched, sched):  # unscheduled, first and second scheduled call
                 fb = px.FrameBuffers(W, H)
@@ -103,11 +103,11 @@ def test_accel_launch_shapes_and_schedule_change_nothing(px, oracle, scene):
                 assert np.array_equal(to_host(fb.Hit).view(np.uint32), want.view(np.uint32)), (v, s is None)
         import torch
 
-        vis = [torch.zeros(W * H * 4, dtype=torch.float32, device="cuda") for _ in range(5)]
-        for v in range(5):
+        vis = [torch.zeros(W * H * 4, dtype=torch.float32, device="cuda") for _ in range(8)]
+        for v in range(8):
             assert lib.arbor_tune_set(b"accel_vis_variant", v) == 0
             px.sun_visibility_accel(dev, acc, C, ref_fb, vis[v], sched if v % 2 else None)
-        for v in range(1, 5):
+        for v in range(1, 8):
             assert np.array_equal(to_host(vis[v]).view(np.uint32), to_host(vis[0]).view(np.uint32)), v
     finally:
         lib.arbor_tune_set(b"accel_variant", old_t)
