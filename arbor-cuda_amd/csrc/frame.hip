@@ -307,27 +307,44 @@ __global__ __launch_bounds__(kFilterW * kFilterH) void k_filterLighting(const fl
   __shared__ float sdr[kTileH][kTileW], sdg[kTileH][kTileW], sdb[kTileH][kTileW];
   const int x0 = blockIdx.x * kFilterW - kHalo, y0 = blockIdx.y * kFilterH - kHalo;
   const int tid = threadIdx.y * kFilterW + threadIdx.x;
-  for (int i = tid; i < kTileW * kTileH; i += kFilterW * kFilterH) {
-    int ty = i / kTileW, tx = i % kTileW, gx = x0 + tx, gy = y0 + ty;
-    if (gx < 0 || gy < 0 || gx >= width || gy >= height) {
+  // Every load of the block is issued before the first is used: the tile's
+  // halo pixels (pixel 0 stands in outside the view) and the thread's own
+  // pixel, whose raw Linear and Guide the blend needs.
+  constexpr int kItems = kTileW * kTileH, kThreads = kFilterW * kFilterH;
+  constexpr int kIter = (kItems + kThreads - 1) / kThreads;
+  float4 hq[kIter], lq[kIter], gq[kIter];
+  bool inView[kIter];
+#pragma unroll
+  for (int s = 0; s < kIter; s++) {
+    const int i = tid + s * kThreads, ty = i / kTileW, tx = i % kTileW, gx = x0 + tx, gy = y0 + ty;
+    inView[s] = i < kItems && gx >= 0 && gy >= 0 && gx < width && gy < height;
+    const int q = inView[s] ? gy * width + gx : 0;
+    hq[s] = Hit[q];
+    lq[s] = Linear[q];
+    gq[s] = Guide[q];
+  }
+  const int x = blockIdx.x * kFilterW + threadIdx.x, y = blockIdx.y * kFilterH + threadIdx.y;
+  const bool mine = x < width && y < height;
+  const int b = mine ? y * width + x : 0;
+  const float4 hb = Hit[b], lb = Linear[b], gb = Guide[b];
+#pragma unroll
+  for (int s = 0; s < kIter; s++) {
+    const int i = tid + s * kThreads, ty = i / kTileW, tx = i % kTileW;
+    if (i >= kItems) continue;
+    if (!inView[s]) {
       sid[ty][tx] = INT_MIN;  // never equals a pixel's id (ids are >= -1)
       continue;
     }
-    int q = gy * width + gx;
-    float4 hq = Hit[q], lq = Linear[q], gq = Guide[q];
-    st[ty][tx] = hq.x;
-    sid[ty][tx] = (int)hq.y;
-    stree[ty][tx] = (int)hq.z;
-    v3 g = mk(gq.x, gq.y, gq.z) + mk(0.02f, 0.02f, 0.02f);
-    sdr[ty][tx] = lq.x / g.x;
-    sdg[ty][tx] = lq.y / g.y;
-    sdb[ty][tx] = lq.z / g.z;
+    st[ty][tx] = hq[s].x;
+    sid[ty][tx] = (int)hq[s].y;
+    stree[ty][tx] = (int)hq[s].z;
+    v3 g = mk(gq[s].x, gq[s].y, gq[s].z) + mk(0.02f, 0.02f, 0.02f);
+    sdr[ty][tx] = lq[s].x / g.x;
+    sdg[ty][tx] = lq[s].y / g.y;
+    sdb[ty][tx] = lq[s].z / g.z;
   }
   __syncthreads();
-  int x = blockIdx.x * kFilterW + threadIdx.x, y = blockIdx.y * kFilterH + threadIdx.y;
-  if (x >= width || y >= height) return;
-  int b = y * width + x;
-  float4 hb = Hit[b], lb = Linear[b], gb = Guide[b];
+  if (!mine) return;
   v3 col = mk(lb.x, lb.y, lb.z);
   v3 base = mk(gb.x, gb.y, gb.z) + mk(0.02f, 0.02f, 0.02f);
   v3 sum = mk(0.0f, 0.0f, 0.0f);
