@@ -58,6 +58,37 @@ def test_accel_structure(px, scene):
     check_accel_structure(buf.view(np.uint32), to_host(dev.Nodes), to_host(dev.Order))
 
 
+@pytest.mark.parametrize("seed", [1, 42, 1788])
+def test_accel_other_scenes(px, oracle, seed):
+    """Other scenes: the hierarchy is valid and the Accel trace and sun
+    visibility equal the heap passes on a 640x360 view of the scene's own
+    forest (walk frame 0 of its seed; shadows on for the visibility pass)."""
+    print("case: accel seed", seed, "640x360", flush=True)
+    import torch
+
+    dev = px.DeviceScene().generate(seed)
+    acc = px.TraceAccel(dev)
+    check_accel_structure(acc.buf.cpu().numpy().view(np.uint32), to_host(dev.Nodes), to_host(dev.Order))
+    W, H = 640, 360
+    C = oracle.init_camera(seed)
+    I = px.walk_input(seed, 0)
+    oracle.update_camera(C, I, 1.0 / 60.0, W, H)
+    C[22] = 1.0
+    Cd = to_dev(C)
+    fa, fb = px.FrameBuffers(W, H), px.FrameBuffers(W, H)
+    px.trace(dev, Cd, fa)
+    px.trace_accel(dev, acc, Cd, fb)
+    n = _order_dependent(to_host(fa.Hit).reshape(-1, 4), to_host(fb.Hit).reshape(-1, 4))
+    va = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+    vb = torch.zeros_like(va)
+    px.sun_visibility(dev, Cd, fa, va)
+    px.sun_visibility_accel(dev, acc, Cd, fa, vb)
+    nv = int(np.any(to_host(va).reshape(-1, 4).view(np.uint32) != to_host(vb).reshape(-1, 4).view(np.uint32),
+                    axis=1).sum())
+    print("Hit pixels differing", n, "Vis pixels differing", nv)
+    assert n <= 2 and nv <= 2
+
+
 def _order_dependent(a, b):
     """Pixels where two Hit buffers differ; each must be two object hits at
     nearly the same t (the walk-order-dependent case)."""

@@ -296,6 +296,26 @@ def test_sah_walk_is_the_heap_walk(oracle, scenes, frame, w, h, mode):
         assert diff <= 2
 
 
+@pytest.mark.parametrize("seed", SEEDS)
+def test_sah_walk_other_scenes(oracle, scenes, seed):
+    """The hierarchy and its walk on every golden scene: a valid hierarchy of
+    the heap's leaf boxes, and the heap walk's Hit on a 240x136 view of that
+    scene's forest (walk frame 0 of its own seed)."""
+    from accel_check import check_accel_structure
+    import pyarbor
+
+    s = scenes[seed]
+    a = oracle.build_sah(s)
+    check_accel_structure(a, s.Nodes, s.Order)
+    C = oracle.init_camera(int(seed))
+    I = pyarbor.walk_input(int(seed), 0)
+    I[10] = 1.0
+    oracle.update_camera(C, I, 1.0 / 60, 240, 136)
+    diff, near, _ = oracle.sah_walk_mismatches(s, C, 240, 136, a, 0)
+    print("seed", seed, "differing", diff, "order-dependent", near)
+    assert diff == near and near <= 2
+
+
 def test_sah_walk_tie_order_is_checked(oracle, scenes):
     """Inverting the tie order (a planted mistake in the oracle's SAH walk)
     changes the Hit of many pixels of the same frame: exact ties are common
