@@ -36,8 +36,9 @@ __device__ __forceinline__ void fillTop(float4* top, const float4* Nodes4) {
 // tile took (wall clock ticks, >= 1) in tileCost; the order changes only which
 // wave traces which tile, never a pixel's result.
 // With a SAH walk policy, accel is the hierarchy of arbor_buildTraceAccel.
-template <int kWaves, class W>
-__global__ __launch_bounds__(kWaves * 64) void k_tracePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
+// kMinWaves > 1 caps the registers at that many waves per SIMD.
+template <int kWaves, class W, int kMinWaves = 1>
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(kMinWaves))) void k_tracePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
                                                              const float* C, float4* Hit, int width, int height,
                                                              int tilesX, int tiles, const uint32_t* tileOrder,
                                                              uint32_t* tileCost, const float4* accel) {
@@ -460,6 +461,7 @@ const TraceVariant kAccelTraceVariants[] = {
     {k_tracePixels<1, WalkSahL<4>>, k_traceTreePixels<1, WalkSahL<4>>, 1, sahLdsPerWave(4)},
     {k_tracePixels<2, WalkSahL<12>>, k_traceTreePixels<2, WalkSahL<12>>, 2, sahLdsPerWave(12)},
     {k_tracePixels<4, WalkSahL<6>>, k_traceTreePixels<4, WalkSahL<6>>, 4, sahLdsPerWave(6)},
+    {k_tracePixels<2, WalkSahL<8>, 4>, k_traceTreePixels<2, WalkSahL<8>>, 2, sahLdsPerWave(8)},
 };
 constexpr int kAccelTraceVariantCount = sizeof(kAccelTraceVariants) / sizeof(kAccelTraceVariants[0]);
 

@@ -196,7 +196,7 @@ int arbor_tune_set(const char* key, int value);
  * "sched_newton_min" (1..64) for the wave-scheduled trace kernel,
  * "batch_variant" (arbor_batch.h), "vis_variant" (0..7, the
  * sun-visibility kernel's launch shape and register budget), and for the
- * SAH walk "accel_variant" (0..6) and "accel_vis_variant" (0..7): block size,
+ * SAH walk "accel_variant" (0..7) and "accel_vis_variant" (0..7): block size,
  * register budget and how many of each lane's stack entries live in LDS. */
 int arbor_tune_get(const char* key);
 /* Persisted tuning cache: writes / reads "key value" lines. A file with an

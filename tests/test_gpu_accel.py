@@ -126,7 +126,7 @@ def test_accel_launch_shapes_and_schedule_change_nothing(px, oracle, scene):
         px.trace_accel(dev, acc, C, ref_fb)
         want = to_host(ref_fb.Hit)
         sched = px.TileSchedule(W, H)
-        for v in range(7):
+        for v in range(8):
             assert lib.arbor_tune_set(b"accel_variant", v) == 0
             for s in (None, sched, sched):  # unscheduled, first and second scheduled call
                 fb = px.FrameBuffers(W, H)
