@@ -807,4 +807,45 @@ int arbor_autotune_trace(const float* P, const float* Nodes, const unsigned* Ord
   return ARBOR_OK;
 }
 
+int arbor_autotune_trace_accel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,
+                               const float* C, float* Hit, int width, int height, int reps, hipStream_t stream,
+                               char* json, int cap) {
+  ARBOR_REQUIRE(P && Nodes && Order && accel && C && Hit);
+  if (!dims_ok(width, height) || reps < 1 || reps > 1000) return ARBOR_ERR_ARG;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return ARBOR_ERR_DEVICE;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return ARBOR_ERR_DEVICE;
+  }
+  Tuning& t = tuning();
+  const int saved = t.accel_variant;
+  float ms[kAccelTraceVariantCount] = {};
+  int best = 0, rc = ARBOR_OK;
+  for (int v = 0; v < kAccelTraceVariantCount && rc == ARBOR_OK; v++) {
+    t.accel_variant = v;
+    rc = launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, nullptr, nullptr);
+    (void)hipEventRecord(e0, stream);
+    for (int r = 0; r < reps && rc == ARBOR_OK; r++)
+      rc = launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, nullptr, nullptr);
+    (void)hipEventRecord(e1, stream);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = ARBOR_ERR_DEVICE;
+    (void)hipEventElapsedTime(&ms[v], e0, e1);
+    ms[v] /= (float)reps;
+    if (ms[v] < ms[best]) best = v;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  t.accel_variant = rc == ARBOR_OK ? best : saved;
+  if (rc != ARBOR_OK) return rc;
+  if (json && cap > 0) {
+    int used = snprintf(json, (size_t)cap, "{\"best\":%d,\"ms\":[", best);
+    for (int v = 0; v < kAccelTraceVariantCount && used < cap; v++)
+      used += snprintf(json + used, (size_t)(cap - used), "%s%.4f", v ? "," : "", ms[v]);
+    if (used < cap) used += snprintf(json + used, (size_t)(cap - used), "]}");
+    if (used >= cap) return ARBOR_ERR_ARG;
+  }
+  return ARBOR_OK;
+}
+
 }  // extern "C"

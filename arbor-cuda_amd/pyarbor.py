@@ -77,6 +77,7 @@ SIGNATURES = {
     "arbor_buildTraceAccel": [_vp, _vp, _vp, _vp],
     "arbor_tracePixelsAccel": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "arbor_sunVisibilityAccel": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
+    "arbor_autotune_trace_accel": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, ct.c_char_p, _i],
     "arbor_shadePixelsVis": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_shadePixels": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_filterLighting": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -281,6 +282,17 @@ class TraceAccel:
 def trace_accel(scene, accel, C, fb, schedule=None):
     call("arbor_tracePixelsAccel", scene.P, scene.Nodes, scene.Order, accel.buf, C, fb.Hit, fb.width, fb.height,
          None if schedule is None else schedule.buf)
+
+
+def autotune_trace_accel(scene, accel, C, fb, reps=2):
+    """Times every accel_variant on this view and keeps the fastest."""
+    import json
+
+    buf = ct.create_string_buffer(4096)
+    check(lib().arbor_autotune_trace_accel(_ptr(scene.P), _ptr(scene.Nodes), _ptr(scene.Order), _ptr(accel.buf),
+                                           _ptr(C), _ptr(fb.Hit), fb.width, fb.height, reps, _stream(), buf, 4096),
+          "arbor_autotune_trace_accel")
+    return json.loads(buf.value.decode())
 
 
 def sun_visibility_accel(scene, accel, C, fb, vis, schedule=None):

@@ -208,6 +208,11 @@ int arbor_tune_load(const char* path);
  * "trace_variant", writes {"best": i, "ms": [...]} to json. Synchronises. */
 int arbor_autotune_trace(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
                          int width, int height, int reps, hipStream_t stream, char* json, int cap);
+/* The same over the SAH hierarchy: times every "accel_variant" and keeps the
+ * fastest (no counterpart in the reference). */
+int arbor_autotune_trace_accel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,
+                               const float* C, float* Hit, int width, int height, int reps, hipStream_t stream,
+                               char* json, int cap);
 
 #ifdef __cplusplus
 }

@@ -214,3 +214,26 @@ def test_batch_accel_equals_batch_heap(px, scene):
             assert n_hit <= 2 and sum(x.sum() for x in lin) <= 2 + 2 * n_hit
     finally:
         lib.arbor_tune_set(b"batch_variant", old)
+
+
+def test_accel_autotune(px, oracle, scene):
+    """The SAH-walk autotuner times every accel_variant, keeps the fastest,
+    and the trace it selects gives the same Hit."""
+    print("case: accel autotune 640x360", flush=True)
+    dev, acc = scene
+    W, H = 640, 360
+    C = to_dev(_walk_camera(px, oracle, 3, W, H, 0.0))
+    lib = px.lib()
+    old = lib.arbor_tune_get(b"accel_variant")
+    try:
+        ref = px.FrameBuffers(W, H)
+        px.trace_accel(dev, acc, C, ref)
+        fb = px.FrameBuffers(W, H)
+        r = px.autotune_trace_accel(dev, acc, C, fb, reps=1)
+        assert len(r["ms"]) == 8 and r["best"] == int(np.argmin(r["ms"]))
+        assert lib.arbor_tune_get(b"accel_variant") == r["best"]
+        px.trace_accel(dev, acc, C, fb)
+        assert np.array_equal(to_host(fb.Hit).view(np.uint32), to_host(ref.Hit).view(np.uint32))
+        print("autotune:", r)
+    finally:
+        lib.arbor_tune_set(b"accel_variant", old)
