@@ -168,7 +168,8 @@ int arbor_shadePixelsVis(const float* P, const float* Nodes, const unsigned* Ord
  * swaying leaf outside its rest-pose leaf box, where which of two valid hits
  * the reference reports depends on its walk order (about 1 pixel in 2
  * million at 1920x1080). `schedule` (may be NULL) is a tile schedule as for
- * the Scheduled entry points. */
+ * the Scheduled entry points. `accel` must be the one built from the Nodes
+ * and Order passed with it (it is not checked). */
 long long arbor_trace_accel_bytes(void);
 int arbor_buildTraceAccel(const float* Nodes, const unsigned* Order, void* accel, hipStream_t stream);
 int arbor_tracePixelsAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,

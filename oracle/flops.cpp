@@ -320,14 +320,19 @@ int orc_cull_mismatches(const float* P, const float* Nodes, const unsigned* Orde
 // The SAH hierarchy of the GPU's accel walk over this heap BVH (the
 // library's own builder, sah_build.hpp) into out[words]: 16 + 16 * (CAPACITY -
 // 1) + CAPACITY words. Returns 0, or -1 if out is too small or the depth
-// limit (32) failed.
-int orc_build_sah(const float* Nodes, const unsigned* Order, uint32_t* out, long long words) {
+// limit (32 as the walk's stack; lower in the limit form, to test the
+// builder's bound) failed.
+int orc_build_sah_limit(const float* Nodes, const unsigned* Order, uint32_t* out, long long words, int depthLimit) {
   const long long need = 16 + 16LL * (CAPACITY - 1) + CAPACITY;
-  if (words < need) return -1;
-  std::vector<uint32_t> w = arbor::sah::buildAccel(Nodes + (size_t)CAPACITY * 8, Order, CAPACITY, 32, 100000.0f);
+  if (words < need || depthLimit < 15) return -1;
+  std::vector<uint32_t> w =
+      arbor::sah::buildAccel(Nodes + (size_t)CAPACITY * 8, Order, CAPACITY, depthLimit, 100000.0f);
   if (w.empty() || (long long)w.size() != need) return -1;
   std::memcpy(out, w.data(), w.size() * 4);
   return 0;
+}
+int orc_build_sah(const float* Nodes, const unsigned* Order, uint32_t* out, long long words) {
+  return orc_build_sah_limit(Nodes, Order, out, words, 32);
 }
 // Operation counts walk the SAH hierarchy `accel` (NULL: the heap).
 void orc_set_sah(const uint32_t* accel) { ACF::sahAccel = accel; }

@@ -182,6 +182,7 @@ class Oracle(_Common):
         self._lean_mismatch = _bind(L, "orc_lean_walk_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i, _i])
         _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
         self._build_sah = _bind(L, "orc_build_sah", _i, [_f32p, _u32p, _u32p, ct.c_longlong])
+        self._build_sah_limit = _bind(L, "orc_build_sah_limit", _i, [_f32p, _u32p, _u32p, ct.c_longlong, _i])
         self._sah_mismatch = _bind(L, "orc_sah_walk_mismatches", _i,
                                    [_f32p, _f32p, _u32p, _f32p, _i, _i, _u32p, _i, _i64p])
         self._set_sah_tie_reversed = _bind(L, "orc_set_sah_tie_reversed", None, [_i])
@@ -244,10 +245,14 @@ class Oracle(_Common):
 
     SAH_WORDS = 16 + 16 * (32768 - 1) + 32768
 
-    def build_sah(self, s):
-        """The GPU accel walk's SAH hierarchy of this scene (u32 words)."""
+    def build_sah(self, s, depth_limit=None):
+        """The GPU accel walk's SAH hierarchy of this scene (u32 words); with
+        depth_limit, the builder's bound set lower than the walk's 32."""
         out = np.zeros(self.SAH_WORDS, np.uint32)
-        assert self._build_sah(s.Nodes, s.Order, out, out.size) == 0
+        if depth_limit is None:
+            assert self._build_sah(s.Nodes, s.Order, out, out.size) == 0
+        else:
+            assert self._build_sah_limit(s.Nodes, s.Order, out, out.size, depth_limit) == 0
         return out
 
     def sah_walk_mismatches(self, s, C, w, h, accel, mode=0, tie_reversed=False):

@@ -316,6 +316,58 @@ def test_sah_walk_other_scenes(oracle, scenes, seed):
     assert diff == near and near <= 2
 
 
+@pytest.mark.parametrize("shape", ["identical", "exponential", "nested", "full"])
+def test_sah_depth_bound_on_skewed_leaf_sets(oracle, shape):
+    """The builder keeps every leaf within 32 levels (the walk's stack) on
+    leaf sets built to defeat the surface-area heuristic: every box the same
+    (no split separates them), centres spaced exponentially along one axis
+    (each SAH split would peel off one leaf), nested boxes growing
+    geometrically, and all 32,768 slots used. The
+    hierarchy stays a valid one of exactly those leaf boxes."""
+    from accel_check import check_accel_structure
+
+    cap = 32768
+    rng = np.random.default_rng(5)
+    n = {"identical": 20000, "exponential": 30000, "nested": 4000, "full": cap}[shape]
+    nodes = np.zeros((2 * cap, 8), np.float32)
+    leaves = nodes[cap:]
+    if shape == "identical":
+        c = np.zeros((n, 3), np.float32)
+    elif shape == "exponential":
+        c = np.zeros((n, 3), np.float32)
+        c[:, 0] = np.float32(1.0009) ** np.arange(n, dtype=np.float32)
+    else:
+        c = rng.uniform(-10, 10, (n, 3)).astype(np.float32)
+    half = np.float32(0.01)
+    if shape == "nested":  # each box holds every smaller one: the heuristic would peel one per level
+        g = np.float32(1.004) ** np.arange(n, dtype=np.float32)
+        c = np.zeros((n, 3), np.float32)
+        c[:, 0] = g
+        half = (g * np.float32(2.0))[:, None]
+    leaves[:n, 0:3] = c - half
+    leaves[:n, 4:7] = c + half
+    leaves[:n, 3] = 1.0
+    order = np.arange(cap, dtype=np.uint32)
+    perm = rng.permutation(n).astype(np.uint32)
+    order[:n] = perm
+    leaves[:n, 7] = perm.astype(np.float32)
+
+    class S:
+        pass
+
+    s = S()
+    s.Nodes = np.ascontiguousarray(nodes.reshape(-1))
+    s.Order = order
+    a = oracle.build_sah(s)
+    depth = check_accel_structure(a, s.Nodes, s.Order)
+    # the same leaves with the bound at 16: the median splits near the limit
+    # must take over wherever the heuristic alone would go deeper
+    b = oracle.build_sah(s, depth_limit=16)
+    depth16 = check_accel_structure(b, s.Nodes, s.Order)
+    print(shape, "depth", depth, "with the bound at 16:", depth16)
+    assert depth <= 32 and depth16 <= 16
+
+
 def test_sah_walk_tie_order_is_checked(oracle, scenes):
     """Inverting the tie order (a planted mistake in the oracle's SAH walk)
     changes the Hit of many pixels of the same frame: exact ties are common
