@@ -323,15 +323,18 @@ int arbor_engine_inspect(const arbor_engine* e, char* out, int cap) {
   if (!e || !out || cap <= 0) return ARBOR_ERR_NULL;
   size_t scene = (size_t)ARBOR_CURVES * ARBOR_CS * 4 + (size_t)ARBOR_CAPACITY * ARBOR_PS * 4 +
                  (size_t)ARBOR_CAPACITY * 8 + (size_t)ARBOR_NODE_COUNT * 32;
-  size_t allocated = scene + ARBOR_C_FLOATS * 4 + ARBOR_I_FLOATS * 4 + 5 * pixel_bytes(e->width, e->height) +
-                     (size_t)e->width * e->height * 4;
+  // Hit, Linear, Guide, Filtered, History and Vis at 16 B a pixel, Pixels at
+  // 4 B, the two tile schedules, and the SAH hierarchy beside the scene.
+  const size_t sched = e->width > 0 ? (size_t)arbor_tile_schedule_bytes(e->width, e->height) : 0;
+  size_t allocated = scene + ARBOR_C_FLOATS * 4 + ARBOR_I_FLOATS * 4 + (size_t)arbor_trace_accel_bytes() +
+                     6 * pixel_bytes(e->width, e->height) + (size_t)e->width * e->height * 4 + 2 * sched;
   int n = snprintf(out, (size_t)cap,
-                   "{\"width\":%d,\"height\":%d,\"seed\":%d,\"frames\":%ld,\"sceneBytes\":%zu,"
+                   "{\"width\":%d,\"height\":%d,\"seed\":%d,\"frames\":%ld,\"accel\":%s,\"sceneBytes\":%zu,"
                    "\"allocatedBytes\":%zu,\"timings_ms\":{\"camera\":%.4f,\"visibility\":%.4f,\"lighting\":%.4f,"
                    "\"filter\":%.4f,\"resolve\":%.4f,\"frame\":%.4f},\"abi\":{\"CAPACITY\":%d,\"NODE_COUNT\":%d,"
                    "\"PS\":%d,\"CS\":%d,\"CURVES\":%d,\"WOOD_COUNT\":%d,\"LEAF_COUNT\":%d,\"ROCK_COUNT\":%d,"
                    "\"OBJECT_COUNT\":%d}}",
-                   e->width, e->height, e->seed, e->frames, scene, allocated, e->stage_ms[0], e->stage_ms[1],
+                   e->width, e->height, e->seed, e->frames, e->accel ? "true" : "false", scene, allocated, e->stage_ms[0], e->stage_ms[1],
                    e->stage_ms[2], e->stage_ms[3], e->stage_ms[4], e->stage_ms[5], ARBOR_CAPACITY, ARBOR_NODE_COUNT, ARBOR_PS,
                    ARBOR_CS, ARBOR_CURVES, ARBOR_WOOD_COUNT, ARBOR_LEAF_COUNT, ARBOR_ROCK_COUNT, ARBOR_OBJECT_COUNT);
   return (n >= 0 && n < cap) ? ARBOR_OK : ARBOR_ERR_ARG;

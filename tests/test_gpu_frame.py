@@ -491,4 +491,7 @@ def test_engine_frame_pipeline(px, oracle):
     info = eng.inspect()
     assert info["sceneBytes"] < 7 * 1048576  # source.test.mjs:6
     assert info["frames"] == 3
+    assert info["accel"] is True
+    px_bytes = w * h * (6 * 16 + 4)
+    assert info["allocatedBytes"] > info["sceneBytes"] + px_bytes + px.lib().arbor_trace_accel_bytes()
     eng.close()
