@@ -390,10 +390,16 @@ AD int heapDepth(uint32_t node) { return 31 - __builtin_clz(node); }
 //   kLdsStack: with kSah, the first kLdsStack entries of each lane's stack
 //              live in the block's dynamic LDS (the launch provides
 //              waves x kLdsStack x 64 x 8 bytes), the rest in scratch.
-template <bool kCull, bool kPrefetch, bool kLdsTop, bool kSah = false, int kLdsStack = 0>
+//   kGather:   with kSah and kGather = g > 0, lanes keep stepping through
+//              interior nodes until at least g/8 of the wave's live lanes
+//              wait at a leaf (or none has an interior step left); the object
+//              tests of the waiting lanes then run together. Each lane
+//              performs the same steps in the same order as with g = 0; only
+//              their interleaving within the wave changes.
+template <bool kCull, bool kPrefetch, bool kLdsTop, bool kSah = false, int kLdsStack = 0, int kGather = 0>
 struct Walk {
   static constexpr bool cull = kCull, prefetch = kPrefetch, ldsTop = kLdsTop, sah = kSah;
-  static constexpr int ldsStack = kLdsStack;
+  static constexpr int ldsStack = kLdsStack, gather = kGather;
 };
 // Dynamic LDS of a kernel launch (the SAH walk's stack entries, kLdsStack).
 extern __shared__ uint64_t sahLdsStack[];
@@ -567,50 +573,9 @@ __device__ __forceinline__ v2 traceTreeSah(const float* P, const float4* Nodes4,
   // the most recent push stays in registers until the next push or its pop
   uint64_t cached = 0;
   bool haveCached = false;
-  while (true) {
-    if (live) {
-      if (ref >= 0) {
-        const float4* nd = accel + kAccelHeader + 4 * ref;
-        const float4 k0 = nd[0], k1 = nd[1], k2 = nd[2], k3 = nd[3];
-        const v2 ls = slabSpan(roi, inv, mk(k0.x, k0.y, k0.z), mk(k1.x, k1.y, k1.z));
-        const v2 rs = slabSpan(roi, inv, mk(k2.x, k2.y, k2.z), mk(k3.x, k3.y, k3.z));
-        const bool lh = spanHit(ls, 1.0f, best), rhit = spanHit(rs, 1.0f, best);
-        if (lh || rhit) {
-          const bool takeLeft = lh && (!rhit || ls.x < rs.x);
-          const int lref = __float_as_int(k0.w), rref = __float_as_int(k2.w);
-          if (lh && rhit) {
-            const uint32_t farRef = (uint32_t)(takeLeft ? rref : lref);
-            const uint32_t farT = __float_as_uint(takeLeft ? rs.x : ls.x);
-            if (haveCached) {
-              if (kL > 0 && top < kL) lds[top * 64] = cached;
-              else stack[top] = cached;
-              top++;
-            }
-            cached = ((uint64_t)farRef << 32) | farT;
-            haveCached = true;
-          }
-          ref = takeLeft ? lref : rref;
-          span = takeLeft ? ls : rs;
-          continue;
-        }
-      } else {
-        const int id = ~ref;
-        if (id != ignore) {
-          const float footprint = fmaxf(0.000005f, cone * fmaxf(0.05f, span.x));
-          const Prim pr = loadPrim(P, id);
-          const float t = objectT(pr, ro, rd, footprint, time, wind, !shadows, W::cull, nextUp(best));
-          if (t < best) {
-            best = t;
-            found = id;
-          } else if (t == best && found >= 0 && heapLeafFirst(Nodes4, roi, inv, slotOf[id], slotOf[found])) {
-            found = id;
-          }
-        }
-      }
-    }
-    // pop the deepest pending sibling still entered before the best (the
-    // other two liveness conditions held when it was pushed)
-    live = false;
+  // pop the deepest pending sibling still entered before the best (the
+  // other two liveness conditions held when it was pushed)
+  auto popNext = [&]() -> bool {
     while (haveCached || top > 0) {
       uint64_t e = cached;
       if (!haveCached) {
@@ -622,10 +587,77 @@ __device__ __forceinline__ v2 traceTreeSah(const float* P, const float4* Nodes4,
       if (t < best) {
         ref = (int)(uint32_t)(e >> 32);
         span.x = t;
-        live = true;
-        break;
+        return true;
       }
     }
+    return false;
+  };
+  // one interior step: false when neither child is entered
+  auto stepNode = [&]() -> bool {
+    const float4* nd = accel + kAccelHeader + 4 * ref;
+    const float4 k0 = nd[0], k1 = nd[1], k2 = nd[2], k3 = nd[3];
+    const v2 ls = slabSpan(roi, inv, mk(k0.x, k0.y, k0.z), mk(k1.x, k1.y, k1.z));
+    const v2 rs = slabSpan(roi, inv, mk(k2.x, k2.y, k2.z), mk(k3.x, k3.y, k3.z));
+    const bool lh = spanHit(ls, 1.0f, best), rhit = spanHit(rs, 1.0f, best);
+    if (!(lh || rhit)) return false;
+    const bool takeLeft = lh && (!rhit || ls.x < rs.x);
+    const int lref = __float_as_int(k0.w), rref = __float_as_int(k2.w);
+    if (lh && rhit) {
+      const uint32_t farRef = (uint32_t)(takeLeft ? rref : lref);
+      const uint32_t farT = __float_as_uint(takeLeft ? rs.x : ls.x);
+      if (haveCached) {
+        if (kL > 0 && top < kL) lds[top * 64] = cached;
+        else stack[top] = cached;
+        top++;
+      }
+      cached = ((uint64_t)farRef << 32) | farT;
+      haveCached = true;
+    }
+    ref = takeLeft ? lref : rref;
+    span = takeLeft ? ls : rs;
+    return true;
+  };
+  auto testLeaf = [&]() {
+    const int id = ~ref;
+    if (id != ignore) {
+      const float footprint = fmaxf(0.000005f, cone * fmaxf(0.05f, span.x));
+      const Prim pr = loadPrim(P, id);
+      const float t = objectT(pr, ro, rd, footprint, time, wind, !shadows, W::cull, nextUp(best));
+      if (t < best) {
+        best = t;
+        found = id;
+      } else if (t == best && found >= 0 && heapLeafFirst(Nodes4, roi, inv, slotOf[id], slotOf[found])) {
+        found = id;
+      }
+    }
+  };
+  if constexpr (W::gather > 0) {
+    while (true) {
+      // interior steps (and pops after a missed node) until enough of the
+      // live lanes wait at a leaf; the exit test is uniform over the wave
+      while (true) {
+        const bool step = live && ref >= 0;
+        const uint64_t stepping = __ballot(step);
+        if (stepping == 0) break;
+        const uint64_t waiting = __ballot(live && ref < 0);
+        if (8 * __popcll(waiting) >= W::gather * __popcll(waiting | stepping)) break;
+        if (step && !stepNode()) live = popNext();
+      }
+      if (live && ref < 0) {
+        testLeaf();
+        live = popNext();
+      }
+      if (!live) break;
+    }
+    return {best, (float)found};
+  }
+  while (true) {
+    if (live && ref >= 0) {
+      if (stepNode()) continue;
+      live = false;  // fall through to the pop
+    }
+    if (live) testLeaf();  // at a leaf
+    live = popNext();
     if (!live) break;
   }
   return {best, (float)found};

@@ -185,7 +185,12 @@ const BatchVariant kBatchVariants[] = {
 constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 int batchVariantCount() { return kBatchVariantCount; }
 // The same launch shapes over the SAH hierarchy (arbor_batch_set_accel).
-using WalkSahB = Walk<true, false, false, true>;
+// SAH walk, whole stack in scratch (LDS holds the view and tree tables);
+// object tests gathered at 6/8 of a wave's live lanes for the primary rays
+// and 5/8 for the sun rays, the thresholds that timed best for the frame
+// kernels.
+using WalkSahB = Walk<true, false, false, true, 0, 6>;
+using WalkSahBVis = Walk<true, false, false, true, 0, 5>;
 const BatchVariant kBatchVariantsSah[] = {
     {k_traceShadeViews<1, WalkSahB>, false, 1, ""}, {k_traceShadeViews<2, WalkSahB>, false, 2, ""},
     {k_traceShadeViews<4, WalkSahB>, false, 4, ""}, {k_traceShadeViews<1, WalkSahB>, false, 1, ""},
@@ -248,7 +253,7 @@ int issue(arbor_batch* b, const float* P, const float* Nodes, const unsigned* Or
       if (sblocks > tuning().grid_cap) sblocks = tuning().grid_cap;
       typedef void (*VisViewsKernel)(const float*, const float4*, const uint32_t*, const float*, const ViewDesc*, int,
                                      int, const float4*, float4*, const float4*);
-      const VisViewsKernel vk = b->accel ? k_sunVisibilityViews<WalkSahB> : k_sunVisibilityViews<Walk<true, false, false>>;
+      const VisViewsKernel vk = b->accel ? k_sunVisibilityViews<WalkSahBVis> : k_sunVisibilityViews<Walk<true, false, false>>;
       hipLaunchKernelGGL(vk, dim3(vblocks), dim3(128), 0, s, P, reinterpret_cast<const float4*>(Nodes), Order, b->dC,
                          b->dViews, b->nviews, b->totalTiles, b->Hit, b->Vis, b->accel);
       if ((rc = launch_status()) != ARBOR_OK) return rc;

@@ -126,7 +126,7 @@ def test_accel_launch_shapes_and_schedule_change_nothing(px, oracle, scene):
         px.trace_accel(dev, acc, C, ref_fb)
         want = to_host(ref_fb.Hit)
         sched = px.TileSchedule(W, H)
-        for v in range(8):
+        for v in range(10):
             assert lib.arbor_tune_set(b"accel_variant", v) == 0
             for s in (None, sched, sched):  # unscheduled, first and second scheduled call
                 fb = px.FrameBuffers(W, H)
@@ -134,11 +134,11 @@ def test_accel_launch_shapes_and_schedule_change_nothing(px, oracle, scene):
                 assert np.array_equal(to_host(fb.Hit).view(np.uint32), want.view(np.uint32)), (v, s is None)
         import torch
 
-        vis = [torch.zeros(W * H * 4, dtype=torch.float32, device="cuda") for _ in range(8)]
-        for v in range(8):
+        vis = [torch.zeros(W * H * 4, dtype=torch.float32, device="cuda") for _ in range(10)]
+        for v in range(10):
             assert lib.arbor_tune_set(b"accel_vis_variant", v) == 0
             px.sun_visibility_accel(dev, acc, C, ref_fb, vis[v], sched if v % 2 else None)
-        for v in range(1, 8):
+        for v in range(1, 10):
             assert np.array_equal(to_host(vis[v]).view(np.uint32), to_host(vis[0]).view(np.uint32)), v
     finally:
         lib.arbor_tune_set(b"accel_variant", old_t)
@@ -230,7 +230,7 @@ def test_accel_autotune(px, oracle, scene):
         px.trace_accel(dev, acc, C, ref)
         fb = px.FrameBuffers(W, H)
         r = px.autotune_trace_accel(dev, acc, C, fb, reps=1)
-        assert len(r["ms"]) == 8 and r["best"] == int(np.argmin(r["ms"]))
+        assert len(r["ms"]) == 10 and r["best"] == int(np.argmin(r["ms"]))
         assert lib.arbor_tune_get(b"accel_variant") == r["best"]
         px.trace_accel(dev, acc, C, fb)
         assert np.array_equal(to_host(fb.Hit).view(np.uint32), to_host(ref.Hit).view(np.uint32))
