@@ -33,10 +33,10 @@ sys.path.insert(0, os.path.join(HERE, "arbor-cuda_amd"))
 TRACE_FLOP_PER_RAY = 5030.3
 TRACE_FLOP_PER_RAY_HEAP = 8785.9
 # HBM bytes per scheduled SAH-walk trace launch on this workload (walk frame
-# 0): rocprofv3 --pmc FETCH_SIZE 40,080 KiB x 2.0 + WRITE_SIZE 52,550 KiB x
+# 0): rocprofv3 --pmc FETCH_SIZE 39,315 KiB x 2.0 + WRITE_SIZE 53,110 KiB x
 # 1.0, the factors being what the same counters read for a 256 MiB device copy
 # (131,097 KiB and 262,144 KiB); profiles/tracePixels_forest1080p_pmc.md.
-TRACE_HBM_BYTES = (40080.0 * 2.0 + 52550.0 * 1.0) * 1024
+TRACE_HBM_BYTES = (39315.0 * 2.0 + 53110.0 * 1.0) * 1024
 FP32_PEAK_TFLOPS = 157.3
 # One non-packed FP32 VALU op per lane per clock: 256 CUs x 64 lanes x 2.4 GHz.
 FP32_SINGLE_ISSUE_TOPS = 39.32
