@@ -188,8 +188,9 @@ int arbor_traceTreePixels(const float* P, const float* Nodes, const unsigned* Or
 const char* arbor_version(void);
 
 /* Launch tuning (no counterpart in the reference, whose launch shapes are
- * fixed by its WebGPU host). Keys: "trace_variant" (0 per-lane walk, 1
- * wave-batched leaf tests), "grid_cap" (max blocks of a tile kernel, 1..1048576).
+ * fixed by its WebGPU host). Keys: "trace_variant" (0..9: 0-7 the heap
+ * walk's launch shape and walk options, 8 and 9 the wave-scheduled kernels),
+ * "grid_cap" (max blocks of a tile kernel, 1..1048576).
  * Every setting gives identical results. Process-wide; set before launching. */
 int arbor_tune_set(const char* key, int value);
 /* Current value of a tuning key, or a negative status for an unknown key.
