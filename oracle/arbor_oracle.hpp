@@ -198,6 +198,18 @@ template <class R> struct Arbor {
     F3 f = forwardC(C), r = rightC(C), up = cross(r, f);
     return unit(f + r * (u * C[14]) + up * (v * C[14]));
   }
+  // cameraRay through a point (dx, dy) pixels away from the sample position.
+  // Test infrastructure: defines "a ray within eps of the pixel's ray". With
+  // dx = dy = 0 it is cameraRay bit for bit (adding +0 changes no value).
+  static F3 cameraRayOffset(const R* C, int x, int y, int width, int height, R dx, R dy) {
+    int sample = (int)C[9];
+    R jx = halton(sample + 1, 2) - K(0.5) + dx;
+    R jy = halton(sample + 1, 3) - K(0.5) + dy;
+    R u = (R(x) + K(0.5) + jx - R(width) * K(0.5)) / R(height);
+    R v = (R(height) * K(0.5) - R(y) - K(0.5) - jy) / R(height);
+    F3 f = forwardC(C), r = rightC(C), up = cross(r, f);
+    return unit(f + r * (u * C[14]) + up * (v * C[14]));
+  }
 
   // -- forest instancing (Arbor.cu:57-69) --
   static int forestSeed(int tree, int ws) {
@@ -1050,6 +1062,19 @@ template <class R> struct Arbor {
     F2 h = traceForest(P, Nodes, Order, ro, rd, C[14] / R(height), C[8], C[11], ground > K(0.0) ? ground : FAR(),
                        (int)C[31], &tree, false);
     Hit[b] = h.x; Hit[b + 1] = h.y; Hit[b + 2] = R(tree); Hit[b + 3] = C[14] * h.x / R(height);
+  }
+  // tracePixel along cameraRayOffset: the primary ray through a point (dx, dy)
+  // pixels from the pixel's sample position; out = the Hit record.
+  static void tracePixelOffset(const R* P, const R* Nodes, const unsigned* Order, const R* C, int width, int height,
+                               int x, int y, R dx, R dy, R* out, bool single = false) {
+    F3 ro = cameraPos(C), rd = cameraRayOffset(C, x, y, width, height, dx, dy);
+    R ground = rd.y < K(-0.00001) ? -ro.y / rd.y : FAR();
+    int tree = -1;
+    F2 h = single ? traceTree(P, Nodes, Order, ro, rd, C[14] / R(height), C[8], C[11],
+                              ground > K(0.0) ? ground : FAR(), -1, false)
+                  : traceForest(P, Nodes, Order, ro, rd, C[14] / R(height), C[8], C[11],
+                                ground > K(0.0) ? ground : FAR(), (int)C[31], &tree, false);
+    out[0] = h.x; out[1] = h.y; out[2] = R(tree); out[3] = C[14] * h.x / R(height);
   }
   // Single-tree primary ray in tree-local space: traceTree with the ground plane
   // as the initial best and no forest transform (the single-specimen workload

@@ -42,6 +42,22 @@ int trace_pixel_list(const R* P, const R* Nodes, const unsigned* Order, const R*
   }
   return 0;
 }
+// Primary rays through points offset from the pixels' sample positions: ray i
+// is pixel (xy[2i], xy[2i+1])'s ray moved by (dxy[2i], dxy[2i+1]) pixels;
+// single != 0 traces the single specimen in tree-local space (traceTreePixels).
+template <class R>
+int trace_pixel_offsets(const R* P, const R* Nodes, const unsigned* Order, const R* C, const int* xy,
+                        const float* dxy, int n, int w, int h, R* out, int threads, int single) {
+  using AR = Arbor<R>;
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; i++) {
+    const int x = xy[2 * i], y = xy[2 * i + 1];
+    if (x < 0 || y < 0 || x >= w || y >= h) continue;
+    AR::tracePixelOffset(P, Nodes, Order, C, w, h, x, y, R(dxy[2 * i]), R(dxy[2 * i + 1]), out + 4 * i, single != 0);
+  }
+  return 0;
+}
 template <class R>
 void generate_impl(int seed, R* B, R* P, unsigned* Keys, unsigned* Order, R* Nodes, int bitonic) {
   using Ar = Arbor<R>;
@@ -165,6 +181,16 @@ ORC_PIXEL_STAGES(f64, double, A64)
 int orc_trace_pixel_list_f32(const float* P, const float* Nodes, const unsigned* Order, const float* C, const int* xy,
                              int n, int w, int h, float* out, int threads) {
   return trace_pixel_list(P, Nodes, Order, C, xy, n, w, h, out, threads);
+}
+int orc_trace_pixel_offsets_f32(const float* P, const float* Nodes, const unsigned* Order, const float* C,
+                                const int* xy, const float* dxy, int n, int w, int h, float* out, int threads,
+                                int single) {
+  return trace_pixel_offsets(P, Nodes, Order, C, xy, dxy, n, w, h, out, threads, single);
+}
+int orc_trace_pixel_offsets_f64(const double* P, const double* Nodes, const unsigned* Order, const double* C,
+                                const int* xy, const float* dxy, int n, int w, int h, double* out, int threads,
+                                int single) {
+  return trace_pixel_offsets(P, Nodes, Order, C, xy, dxy, n, w, h, out, threads, single);
 }
 int orc_trace_pixel_list_f64(const double* P, const double* Nodes, const unsigned* Order, const double* C,
                              const int* xy, int n, int w, int h, double* out, int threads) {

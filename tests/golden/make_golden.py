@@ -10,7 +10,13 @@ from the reference checkout by oracle/Makefile) and records:
   * the bark footprint sweep maximum increment (native_harness.cpp:89-91);
   * the 50 leaf near/far intersections (native_harness.cpp:92-93);
   * small frames (trace, shade with and without sun visibility, filter,
-    resolve) as digests plus raw 16x16 crops, and a single-tree 64x64 frame.
+    resolve) as digests plus raw 16x16 crops, and a single-tree 64x64 frame;
+  * frames of the shadowed, windy forest setting: breeze on (C[11]=0.6) at two
+    wind times, a progressive 4-sample sequence (C[9]=0..3, reset only on the
+    first) whose jittered rays and sun-disk samples change every sample and
+    whose History accumulates, the sun at 35 degrees with visibility on and
+    off, one resolve over a seeded non-zero History, sample numbers past 3,
+    and the three debug views (Arbor.cu:50-51,171-172,256-304).
 
 Usage: make -C oracle && python tests/golden/make_golden.py
 """
@@ -22,7 +28,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
-from oracle_lib import RefHarness, digest, OBJECT_COUNT, PS  # noqa: E402
+from oracle_lib import RefHarness, digest, seeded_history, c3_camera, C3_CASES, OBJECT_COUNT, PS  # noqa: E402
 
 SEEDS = (1, 7, 42, 1788)
 FRAME_W, FRAME_H = 96, 64
@@ -124,6 +130,28 @@ def main():
         crops["%s_Pixels" % key] = px.reshape(FRAME_H, FRAME_W)[CROP].copy()
     out["frames"] = {"seed": 7, "width": FRAME_W, "height": FRAME_H, "views": frames,
                      "crop_rows": [CROP[0].start, CROP[0].stop], "crop_cols": [CROP[1].start, CROP[1].stop]}
+
+    # shadowed, windy forest frames; a case with the same sequence name carries
+    # its History from the previous case (the progressive samples of one frame)
+    c3 = {}
+    hist = {}
+    for case in C3_CASES:
+        C = c3_camera(r, case, FRAME_W, FRAME_H)
+        H = r.trace(s7, C, FRAME_W, FRAME_H)
+        L, G = r.shade(s7, C, H, FRAME_W, FRAME_H)
+        F = r.filter(L, G, H, C, FRAME_W, FRAME_H)
+        seq = case["sequence"]
+        if seq not in hist:
+            hist[seq] = seeded_history(FRAME_W, FRAME_H, case["history_seed"])
+        Hist = hist[seq]
+        px = r.resolve(F, Hist, C, FRAME_W, FRAME_H)
+        name = case["name"]
+        c3[name] = {"C": [float(x) for x in C], "Hit": digest(H), "Linear": digest(L), "Guide": digest(G),
+                    "Filtered": digest(F), "History": digest(Hist), "Pixels": digest(px)}
+        for buf, arr in (("Hit", H), ("Linear", L), ("Filtered", F), ("History", Hist)):
+            crops["c3_%s_%s" % (name, buf)] = arr.reshape(FRAME_H, FRAME_W, 4)[CROP].copy()
+        crops["c3_%s_Pixels" % name] = px.reshape(FRAME_H, FRAME_W)[CROP].copy()
+    out["frames_c3"] = c3
 
     # single-tree frame (seed 1, camera 8 m from the trunk at eye height 1.72)
     s1 = r.generate(1)

@@ -157,6 +157,10 @@ class Oracle(_Common):
                                  [_f32p, _f32p, _u32p, _f32p, _i32p, _i, _i, _i, _f32p, _i])
         self._trace_list64 = _bind(L, "orc_trace_pixel_list_f64", _i,
                                    [_f64p, _f64p, _u32p, _f64p, _i32p, _i, _i, _i, _f64p, _i])
+        self._trace_off = _bind(L, "orc_trace_pixel_offsets_f32", _i,
+                                [_f32p, _f32p, _u32p, _f32p, _i32p, _f32p, _i, _i, _i, _f32p, _i, _i])
+        self._trace_off64 = _bind(L, "orc_trace_pixel_offsets_f64", _i,
+                                  [_f64p, _f64p, _u32p, _f64p, _i32p, _f32p, _i, _i, _i, _f64p, _i, _i])
         self._trace_tree64 = _bind(L, "orc_trace_tree_pixels_f64", _i, [_f64p, _f64p, _u32p, _f64p, _f64p, _i, _i, _i])
         self._shade64 = _bind(L, "orc_shade_pixels_f64", _i,
                               [_f64p, _f64p, _u32p, _f64p, _f64p, _f64p, _f64p, _i, _i, _i])
@@ -286,6 +290,42 @@ class Oracle(_Common):
             self._trace_list(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), xy, n, w, h, out, threads)
         return out
 
+    def trace_pixel_offsets(self, s, C, xy, dxy, w, h, threads=0, fp64=False, single=False):
+        """Hit records of primary rays through points offset by dxy (n x 2, in
+        pixels) from the listed pixels' sample positions; dxy = 0 gives the
+        pixels' own Hit records. single: the one-tree trace (traceTreePixels)."""
+        xy = np.ascontiguousarray(xy, np.int32).reshape(-1)
+        dxy = np.ascontiguousarray(dxy, np.float32).reshape(-1)
+        n = xy.size // 2
+        assert dxy.size == 2 * n
+        if fp64:
+            out = np.zeros(4 * n, np.float64)
+            self._trace_off64(s.P.astype(np.float64), s.Nodes.astype(np.float64), s.Order, C.astype(np.float64), xy,
+                              dxy, n, w, h, out, threads, int(single))
+        else:
+            out = np.zeros(4 * n, np.float32)
+            self._trace_off(s.P, s.Nodes, s.Order, np.ascontiguousarray(C, np.float32), xy, dxy, n, w, h, out,
+                            threads, int(single))
+        return out.reshape(n, 4)
+
+    def tie_answers(self, s, C, xy, w, h, eps, single=False):
+        """For each listed pixel, the set of (id, tree) answers that the FP32
+        and FP64 oracle give for the pixel's ray and for the 8 rays through
+        points eps pixels away from its sample position (axes and diagonals).
+        A ray is within eps of a tie when this set has more than one answer."""
+        xy = np.asarray(xy, np.int32).reshape(-1, 2)
+        offs = [(0.0, 0.0)] + [(eps * a, eps * b) for a in (-1, 0, 1) for b in (-1, 0, 1) if (a, b) != (0, 0)]
+        allxy = np.repeat(xy, len(offs), axis=0)
+        alld = np.tile(np.array(offs, np.float32), (len(xy), 1))
+        answers = [set() for _ in range(len(xy))]
+        for fp64 in (False, True):
+            hits = self.trace_pixel_offsets(s, C, allxy, alld, w, h, fp64=fp64, single=single)
+            hits = hits.reshape(len(xy), len(offs), 4)
+            for i in range(len(xy)):
+                for hrec in hits[i]:
+                    answers[i].add((int(hrec[1]), int(hrec[2])))
+        return answers
+
     def trace64(self, s, C, w, h, threads=0):
         Hit = np.zeros(w * h * 4, np.float64)
         self._trace64(s.P.astype(np.float64), s.Nodes.astype(np.float64), s.Order, C.astype(np.float64), Hit, w, h,
@@ -409,3 +449,85 @@ def digest(arr):
     import hashlib
 
     return hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest()
+
+
+# Frames of the shadowed, windy forest setting (seed 7, 96x64), written by the
+# reference harness into the golden fixtures and replayed by the oracle tests.
+# Cases sharing a "sequence" run in order and carry History from one to the
+# next: "prog" is one pixel's four progressive samples (C[9]=0..3, reset only
+# on the first), the form the 4-sample shadowed workload renders per frame.
+SUN_35_DEG = 0.610865
+C3_CASES = (
+    dict(name="prog0", sequence="prog", view=3, time=0.75, sample=0, reset=1, light=1, wind=0.6,
+         sun=SUN_35_DEG, debug=0, history_seed=11),
+    dict(name="prog1", sequence="prog", view=3, time=0.75, sample=1, reset=0, light=1, wind=0.6,
+         sun=SUN_35_DEG, debug=0, history_seed=11),
+    dict(name="prog2", sequence="prog", view=3, time=0.75, sample=2, reset=0, light=1, wind=0.6,
+         sun=SUN_35_DEG, debug=0, history_seed=11),
+    dict(name="prog3", sequence="prog", view=3, time=0.75, sample=3, reset=0, light=1, wind=0.6,
+         sun=SUN_35_DEG, debug=0, history_seed=11),
+    dict(name="hist2", sequence="hist2", view=1, time=1.3, sample=2, reset=0, light=1, wind=0.6,
+         sun=SUN_35_DEG, debug=0, history_seed=23),
+    dict(name="late37", sequence="late37", view=4, time=1.3, sample=37, reset=0, light=0, wind=0.6,
+         sun=SUN_35_DEG, debug=0, history_seed=5),
+    dict(name="calm1", sequence="calm1", view=1, time=0.0, sample=1, reset=0, light=1, wind=0.0,
+         sun=None, debug=0, history_seed=7),
+    dict(name="late200", sequence="late200", view=3, time=2.9, sample=200, reset=0, light=1, wind=0.6,
+         sun=SUN_35_DEG, debug=0, history_seed=3),
+    dict(name="debug1", sequence="debug1", view=3, time=0.75, sample=1, reset=0, light=0, wind=0.6,
+         sun=None, debug=1, history_seed=13),
+    dict(name="debug2", sequence="debug2", view=3, time=0.75, sample=1, reset=0, light=0, wind=0.6,
+         sun=None, debug=2, history_seed=13),
+    dict(name="debug3", sequence="debug3", view=3, time=0.75, sample=1, reset=0, light=1, wind=0.6,
+         sun=None, debug=3, history_seed=13),
+)
+
+
+def c3_camera(lib, case, w, h, seed=7):
+    """Camera state of one C3_CASES entry: initCamera, one updateCamera with the
+    preset view, then the wind time, sample, reset, sun and debug slots set
+    directly (Arbor.cu:181-182 slot meanings)."""
+    C = lib.init_camera(seed)
+    I = np.zeros(32, np.float32)
+    I[5] = case["view"]
+    lib.update_camera(C, I, 1.0 / 60.0, w, h)
+    C[8] = case["time"]
+    C[11] = case["wind"]
+    C[9] = case["sample"]
+    C[10] = case["reset"]
+    if case["sun"] is not None:
+        C[13] = case["sun"]
+    C[22] = case["light"]
+    C[15] = case["debug"]
+    return C
+
+
+def seeded_history(w, h, seed):
+    """A deterministic non-zero History buffer: rgb in [0, 2), w = a sample count."""
+    n = np.arange(w * h * 4, dtype=np.uint64)
+    v = ((n * np.uint64(2654435761) + np.uint64(seed) * np.uint64(40503)) % np.uint64(4093)).astype(np.float32)
+    out = v / np.float32(2048.0)
+    out[3::4] = np.float32(seed % 5)
+    return out
+
+
+def replay_c3(lib, scene, w, h):
+    """Runs every C3_CASES entry through trace, shade, filter and resolve of
+    `lib` (oracle or reference harness), carrying History along each sequence
+    as the fixture generator does. Returns per case C, Hit, Linear, Guide,
+    Filtered, the History before and after resolve, and Pixels."""
+    hist, out = {}, {}
+    for case in C3_CASES:
+        C = c3_camera(lib, case, w, h)
+        H = lib.trace(scene, C, w, h)
+        L, G = lib.shade(scene, C, H, w, h)
+        F = lib.filter(L, G, H, C, w, h)
+        seq = case["sequence"]
+        if seq not in hist:
+            hist[seq] = seeded_history(w, h, case["history_seed"])
+        Hist = hist[seq]
+        before = Hist.copy()
+        px = lib.resolve(F, Hist, C, w, h)
+        out[case["name"]] = dict(C=C, Hit=H, Linear=L, Guide=G, Filtered=F, before=before, History=Hist.copy(),
+                                 Pixels=px)
+    return out

@@ -211,6 +211,68 @@ def test_small_frames_bit_exact(oracle, scenes, golden, view):
     assert np.isfinite(L).all() and (L >= 0).all()  # native_harness.cpp:94
 
 
+@pytest.fixture(scope="module")
+def c3_frames(oracle, scenes, golden):
+    from oracle_lib import replay_c3
+
+    return replay_c3(oracle, scenes["7"], golden[0]["frames"]["width"], golden[0]["frames"]["height"])
+
+
+@pytest.mark.parametrize("name", ["prog0", "prog1", "prog2", "prog3", "hist2", "late37", "calm1", "late200",
+                                  "debug1", "debug2", "debug3"])
+def test_shadowed_windy_frames_bit_exact(c3_frames, golden, name):
+    """Breeze on (Arbor.cu:171-172), jittered samples and sun-disk samples for
+    C[9] > 0 (Arbor.cu:50-51,282), sun visibility at 35 degrees, the progressive
+    resolve mean over History (Arbor.cu:304) and the debug views: every buffer
+    bit-exact with the reference harness."""
+    g, crops = golden
+    want = g["frames_c3"][name]
+    got = c3_frames[name]
+    assert np.array_equal(got["C"], np.array(want["C"], np.float32))
+    for buf in ("Hit", "Linear", "Guide", "Filtered", "History", "Pixels"):
+        assert digest(got[buf]) == want[buf], buf
+    fr = g["frames"]
+    w, h = fr["width"], fr["height"]
+    rows, cols = slice(*fr["crop_rows"]), slice(*fr["crop_cols"])
+    for buf in ("Hit", "Linear", "Filtered", "History"):
+        assert np.array_equal(got[buf].reshape(h, w, 4)[rows, cols], crops["c3_%s_%s" % (name, buf)]), buf
+    assert np.array_equal(got["Pixels"].reshape(h, w)[rows, cols], crops["c3_%s_Pixels" % name])
+
+
+def test_shadowed_windy_cases_reach_their_branches(oracle, scenes, golden, c3_frames):
+    """The fixtures are not degenerate: each case changes what its branch computes."""
+    f = c3_frames
+    w, h = golden[0]["frames"]["width"], golden[0]["frames"]["height"]
+    calm = f["prog1"]["C"].copy()
+    calm[11] = 0.0
+    still = oracle.trace(scenes["7"], calm, w, h).reshape(-1, 4)
+    # the breeze branch decides hits: the same rays without wind hit elsewhere
+    assert (still[:, 1] != f["prog1"]["Hit"].reshape(-1, 4)[:, 1]).sum() > 20
+    hit = {k: v["Hit"].reshape(-1, 4) for k, v in f.items()}
+    # jitter moves the rays: the four progressive samples trace different t
+    for a, b in (("prog0", "prog1"), ("prog1", "prog2"), ("prog2", "prog3")):
+        assert (hit[a][:, 0] != hit[b][:, 0]).mean() > 0.3
+    # sun-disk samples and shadows: the same hits shade differently per sample
+    assert not np.array_equal(f["prog1"]["Linear"], f["prog2"]["Linear"])
+    # the progressive mean: History.w counts samples, rgb is the running mean
+    for k, n in (("prog0", 1), ("prog1", 2), ("prog2", 3), ("prog3", 4), ("hist2", 3), ("late200", 201)):
+        assert (f[k]["History"][3::4] == n).all(), k
+    h1 = f["hist2"]
+    cur = h1["Filtered"].reshape(-1, 4)[:, :3]
+    old = h1["before"].reshape(-1, 4)[:, :3]
+    mean = h1["History"].reshape(-1, 4)[:, :3]
+    # mix(old, cur, 1/3) lies between the two and is neither
+    lo, hi = np.minimum(old, cur), np.maximum(old, cur)
+    assert ((mean >= lo - 1e-6) & (mean <= hi + 1e-6)).all()
+    assert (np.abs(mean - cur) > 1e-4).mean() > 0.5 and (np.abs(mean - old) > 1e-4).mean() > 0.5
+    leaves = (hit["prog1"][:, 1] >= 5124) & (hit["prog1"][:, 1] < 5124 + 24192)
+    assert leaves.sum() > 100
+    # debug views replace the shading; the filter is skipped for them (C[15] >= 0.5)
+    for k in ("debug1", "debug2", "debug3"):
+        assert np.array_equal(f[k]["Filtered"].reshape(-1, 4)[:, :3], f[k]["Linear"].reshape(-1, 4)[:, :3]), k
+    assert not np.array_equal(f["debug1"]["Linear"], f["debug2"]["Linear"])
+
+
 def test_single_tree_frame(oracle, scenes, golden):
     g, crops = golden
     C = np.array(g["single_tree_64"]["C"], np.float32)
@@ -515,3 +577,34 @@ def test_fp64_run_agrees_with_fp32(oracle, scenes):
     rel = np.abs(h32[hit, 0] - h64[hit, 0]) / h64[hit, 0]
     assert np.median(rel) < 1e-6
     assert (rel <= 1e-5).mean() > 0.97
+
+
+def test_offset_rays_at_zero_are_the_pixel_rays(oracle, scenes):
+    """The tie definition's perturbed rays: at a zero offset they are the
+    pixel's own rays (forest and single tree, FP32 and FP64); at a small
+    offset only a few pixels change answer (they lie at decision boundaries)."""
+    s = scenes["7"]
+    w, h = 240, 136
+    C = oracle.init_camera(7)
+    I = np.zeros(32, np.float32)
+    I[5] = 3
+    oracle.update_camera(C, I, 1.0 / 60.0, w, h)
+    C[9] = 2.0
+    xy = np.array([[x, y] for y in range(0, h, 3) for x in range(0, w, 2)], np.int32)
+    zero = np.zeros(xy.shape, np.float32)
+    for fp64 in (False, True):
+        a = oracle.trace_pixels(s, C, xy, w, h, fp64=fp64).reshape(-1, 4)
+        b = oracle.trace_pixel_offsets(s, C, xy, zero, w, h, fp64=fp64)
+        assert np.array_equal(a, b)
+    t1 = oracle.trace_tree(scenes["1"], C, w, h).reshape(-1, 4)[xy[:, 1] * w + xy[:, 0]]
+    t2 = oracle.trace_pixel_offsets(scenes["1"], C, xy, zero, w, h, single=True)
+    assert np.array_equal(t1, t2)
+    # FP64 answers are stable under a 1e-4 pixel move (a tie lies within it
+    # for very few rays); the reference's FP32 arithmetic is not (its leaf
+    # Newton acceptance flips), which is why ties are judged on both runs
+    for fp64, most in ((True, 0.002), (False, 0.05)):
+        base = oracle.trace_pixels(s, C, xy, w, h, fp64=fp64).reshape(-1, 4)
+        moved = oracle.trace_pixel_offsets(s, C, xy, zero + np.float32(1e-4), w, h, fp64=fp64)
+        changed = (moved[:, 1] != base[:, 1]).mean()
+        print("fp64" if fp64 else "fp32", "answers changed by a 1e-4 pixel move: %.5f" % changed)
+        assert 0 < changed < most
