@@ -10,6 +10,7 @@
 
 #include "arbor_batch.h"
 #include "arbor_engine.h"
+#include "engine_passes.hpp"
 #include "launch.hpp"
 
 namespace {
@@ -46,7 +47,7 @@ struct arbor_engine {
   bool generated = false;
   hipEvent_t ev[6] = {};
   bool timing = false;
-  float stage_ms[6] = {0, 0, 0, 0, 0, 0};  // update, trace, shade, filter, resolve, whole frame
+  float stage_ms[6] = {0, 0, 0, 0, 0, 0};  // prologue, trace, shade, filter + resolve, 0, whole frame
   float input[ARBOR_I_FLOATS] = {};
   FILE* rec = nullptr;  // input recording (arbor_engine_record)
   bool accel = true;    // trace and sun visibility over the SAH hierarchy (arbor_engine_set_accel)
@@ -92,26 +93,55 @@ int recordFrame(arbor_engine* e, float dt, const float* device_input, int sample
   return ARBOR_OK;
 }
 
-// tracePixels for the engine: tile-scheduled, over the SAH hierarchy unless
-// arbor_engine_set_accel turned it off.
-int traceEngine(arbor_engine* e, int w, int h, hipStream_t stream) {
-  if (e->accel)
-    return arbor_tracePixelsAccel(e->b.P, e->b.Nodes, e->b.Order, e->b.Accel, e->b.C, e->b.Hit, w, h, e->b.Schedule,
-                                  stream);
-  return arbor_tracePixelsScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, w, h, e->b.Schedule, stream);
+namespace {
+
+// One sample of the engine's frame in five launches (the reference engine's
+// five dispatches, src/engine.js:48-50):
+//   1. the prologue: updateCamera when I is not null (delta seconds), the
+//      per-sample slots when sample >= 0 (light >= 0 also sets C[22]), and
+//      the tile orders of the trace and of the sun-visibility pass;
+//   2. tracePixels over the SAH hierarchy (or the heap) in the trace order;
+//   3. the sun-visibility pass in its order (it returns at once when C[22]
+//      is off);
+//   4. shadePixels from the visibility pass;
+//   5. filterLighting and resolvePixels as one kernel.
+// Every buffer gets the bits the separate entry points give. With `timed`,
+// HIP events on the engine's stream bracket the prologue, the trace, the
+// shading (3-4) and the filter + resolve.
+int frameLaunches(arbor_engine* e, const float* I, float delta, int sample, int light, bool timed) {
+  const int w = e->width, h = e->height;
+  const void* accel = e->accel ? e->b.Accel : nullptr;
+  int rc;
+  if (timed) (void)hipEventRecord(e->ev[0], e->stream);
+  if ((rc = arbor::framePrologue(e->b.C, I, delta, sample, light, w, h, e->b.Schedule, e->b.VisSchedule, e->stream)) !=
+      ARBOR_OK)
+    return rc;
+  if (timed) (void)hipEventRecord(e->ev[1], e->stream);
+  if ((rc = arbor::traceOrdered(e->b.P, e->b.Nodes, e->b.Order, accel, e->b.C, e->b.Hit, w, h, e->b.Schedule,
+                         e->stream)) != ARBOR_OK)
+    return rc;
+  if (timed) (void)hipEventRecord(e->ev[2], e->stream);
+  if ((rc = arbor::visibilityOrdered(e->b.P, e->b.Nodes, e->b.Order, accel, e->b.C, e->b.Hit, e->b.Vis, w, h,
+                              e->b.VisSchedule, e->stream)) != ARBOR_OK)
+    return rc;
+  if ((rc = arbor_shadePixelsVis(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, e->b.Linear,
+                                 e->b.Guide, w, h, e->stream)) != ARBOR_OK)
+    return rc;
+  if (timed) (void)hipEventRecord(e->ev[3], e->stream);
+  if ((rc = arbor::filterResolve(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w, h,
+                          e->stream)) != ARBOR_OK)
+    return rc;
+  if (timed) {
+    (void)hipEventRecord(e->ev[4], e->stream);
+    if (hipEventSynchronize(e->ev[4]) != hipSuccess) return ARBOR_ERR_DEVICE;
+    for (int s = 0; s < 4; s++) (void)hipEventElapsedTime(&e->stage_ms[s], e->ev[s], e->ev[s + 1]);
+    e->stage_ms[4] = 0.0f;
+    (void)hipEventElapsedTime(&e->stage_ms[5], e->ev[0], e->ev[4]);
+  }
+  return ARBOR_OK;
 }
 
-// shadePixels for the engine: the sun-visibility pass (tile-scheduled) and
-// shading from it; Linear and Guide equal arbor_shadePixels'.
-int shadeEngine(arbor_engine* e, int w, int h, hipStream_t stream) {
-  int rc = e->accel ? arbor_sunVisibilityAccel(e->b.P, e->b.Nodes, e->b.Order, e->b.Accel, e->b.C, e->b.Hit,
-                                               e->b.Vis, w, h, e->b.VisSchedule, stream)
-                    : arbor_sunVisibilityScheduled(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, w, h,
-                                                   e->b.VisSchedule, stream);
-  if (rc != ARBOR_OK) return rc;
-  return arbor_shadePixelsVis(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, e->b.Linear, e->b.Guide, w,
-                              h, stream);
-}
+}  // namespace
 
 extern "C" {
 
@@ -244,9 +274,8 @@ int arbor_engine_set_accel(arbor_engine* e, int on) {
   return ARBOR_OK;
 }
 
-// One frame: updateCamera -> tracePixels -> shadePixels -> filterLighting ->
-// resolvePixels, five dispatches (engine.js:45-53). With device_input NULL
-// the host input set by arbor_engine_set_input is uploaded first.
+// One frame: the five launches of frameLaunches. With device_input NULL the
+// host input set by arbor_engine_set_input is uploaded first.
 int arbor_engine_frame(arbor_engine* e, float dt, const float* device_input) {
   if (!e) return ARBOR_ERR_NULL;
   if (!e->generated || !e->b.Hit) return ARBOR_ERR_STATE;
@@ -259,28 +288,8 @@ int arbor_engine_frame(arbor_engine* e, float dt, const float* device_input) {
     I = e->b.I;
   }
   float delta = dt < 0.0f ? 0.0f : (dt > 0.1f ? 0.1f : dt);
-  const int w = e->width, h = e->height;
-  int rc;
-  if (e->timing) (void)hipEventRecord(e->ev[0], e->stream);
-  if ((rc = arbor_updateCamera(e->b.C, I, e->b.P, delta, w, h, e->stream)) != ARBOR_OK) return rc;
-  if (e->timing) (void)hipEventRecord(e->ev[1], e->stream);
-  if ((rc = traceEngine(e, w, h, e->stream)) != ARBOR_OK) return rc;
-  if (e->timing) (void)hipEventRecord(e->ev[2], e->stream);
-  if ((rc = shadeEngine(e, w, h, e->stream)) != ARBOR_OK)
-    return rc;
-  if (e->timing) (void)hipEventRecord(e->ev[3], e->stream);
-  if ((rc = arbor_filterLighting(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.C, w, h, e->stream)) !=
-      ARBOR_OK)
-    return rc;
-  if (e->timing) (void)hipEventRecord(e->ev[4], e->stream);
-  if ((rc = arbor_resolvePixels(e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w, h, e->stream)) != ARBOR_OK)
-    return rc;
-  if (e->timing) {
-    (void)hipEventRecord(e->ev[5], e->stream);
-    if (hipEventSynchronize(e->ev[5]) != hipSuccess) return ARBOR_ERR_DEVICE;
-    for (int s = 0; s < 5; s++) (void)hipEventElapsedTime(&e->stage_ms[s], e->ev[s], e->ev[s + 1]);
-    (void)hipEventElapsedTime(&e->stage_ms[5], e->ev[0], e->ev[5]);
-  }
+  int rc = frameLaunches(e, I, delta, -1, -1, e->timing);
+  if (rc != ARBOR_OK) return rc;
   e->frames++;
   return ARBOR_OK;
 }
@@ -396,18 +405,6 @@ int arbor_batch_walk_views(int seed, int world_seed, int n, float* cameras, int*
   }
   return ARBOR_OK;
 }
-namespace {
-// Per-sample camera slots of the harness render loop (native_harness.cpp:62).
-// light < 0 keeps the sun visibility flag.
-__global__ void k_setSample(float* C, int sample, int light) {
-  if (threadIdx.x == 0) {
-    C[9] = (float)sample;
-    C[10] = sample == 0 ? 1.0f : 0.0f;
-    if (light >= 0) C[22] = light ? 1.0f : 0.0f;
-  }
-}
-}  // namespace
-
 int arbor_engine_render(arbor_engine* e, int view, int samples, int light) {
   if (!e) return ARBOR_ERR_NULL;
   if (!e->generated || !e->b.Hit) return ARBOR_ERR_STATE;
@@ -417,17 +414,10 @@ int arbor_engine_render(arbor_engine* e, int view, int samples, int light) {
   if (hipMemcpyAsync(e->b.I, input, sizeof(input), hipMemcpyHostToDevice, e->stream) != hipSuccess)
     return ARBOR_ERR_DEVICE;
   if (hipStreamSynchronize(e->stream) != hipSuccess) return ARBOR_ERR_DEVICE;  // `input` is on this stack
-  const int w = e->width, h = e->height;
-  int rc = arbor_updateCamera(e->b.C, e->b.I, e->b.P, 1.0f / 60.0f, w, h, e->stream);
-  for (int i = 0; i < samples && rc == ARBOR_OK; i++) {
-    hipLaunchKernelGGL(k_setSample, dim3(1), dim3(64), 0, e->stream, e->b.C, i, light);
-    if ((rc = arbor::launch_status()) != ARBOR_OK) break;
-    rc = traceEngine(e, w, h, e->stream);
-    if (rc == ARBOR_OK)
-      rc = shadeEngine(e, w, h, e->stream);
-    if (rc == ARBOR_OK) rc = arbor_filterLighting(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.C, w, h, e->stream);
-    if (rc == ARBOR_OK) rc = arbor_resolvePixels(e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w, h, e->stream);
-  }
+  // per-sample camera slots of the harness render loop (native_harness.cpp:62)
+  int rc = ARBOR_OK;
+  for (int i = 0; i < samples && rc == ARBOR_OK; i++)
+    rc = frameLaunches(e, i == 0 ? e->b.I : nullptr, 1.0f / 60.0f, i, light ? 1 : 0, false);
   if (rc == ARBOR_OK) e->frames += samples;
   return rc;
 }
@@ -445,17 +435,8 @@ int arbor_engine_frame_samples(arbor_engine* e, float dt, const float* device_in
     I = e->b.I;
   }
   float delta = dt < 0.0f ? 0.0f : (dt > 0.1f ? 0.1f : dt);
-  const int w = e->width, h = e->height;
-  int rc = arbor_updateCamera(e->b.C, I, e->b.P, delta, w, h, e->stream);
-  for (int i = 0; i < samples && rc == ARBOR_OK; i++) {
-    hipLaunchKernelGGL(k_setSample, dim3(1), dim3(64), 0, e->stream, e->b.C, i, -1);
-    if ((rc = arbor::launch_status()) != ARBOR_OK) break;
-    rc = traceEngine(e, w, h, e->stream);
-    if (rc == ARBOR_OK)
-      rc = shadeEngine(e, w, h, e->stream);
-    if (rc == ARBOR_OK) rc = arbor_filterLighting(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.C, w, h, e->stream);
-    if (rc == ARBOR_OK) rc = arbor_resolvePixels(e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w, h, e->stream);
-  }
+  int rc = ARBOR_OK;
+  for (int i = 0; i < samples && rc == ARBOR_OK; i++) rc = frameLaunches(e, i == 0 ? I : nullptr, delta, i, -1, false);
   if (rc == ARBOR_OK) e->frames++;
   return rc;
 }

@@ -12,8 +12,14 @@
 #include <math.h>
 #include <string.h>
 
+#include <hip/hip_cooperative_groups.h>
+
+#include <atomic>
+
+#include "arbor_camera.hpp"
 #include "arbor_shade.hpp"
 #include "arbor_wavefront.hpp"
+#include "engine_passes.hpp"
 #include "launch.hpp"
 #include "tuning.hpp"
 
@@ -111,21 +117,24 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(kMi
 // through the previous camera; eye motion between frames is ignored); tiles
 // with no prediction (first frame, or directions outside the previous view)
 // go first. Tiles are bucketed by log2(cost) with two fraction bits and laid
-// out bucket by bucket, highest first, by three grid-wide passes: keys and a
-// bucket histogram, bucket starts, scatter. The last pass keeps the current
-// camera for the next frame and leaves the histogram zeroed.
+// out bucket by bucket, highest first: keys and a bucket histogram, then
+// (after one grid barrier) the bucket starts, which every block derives from
+// the histogram, and the scatter. A bucket's 64-bit accumulator holds its
+// tile count in the low word and the scatter cursor in the high word; calls
+// alternate between two accumulator sets, each call zeroing the set the next
+// call uses, so nothing waits for a reset. The call keeps its camera for the
+// next frame's reprojection.
 constexpr int kOrderBuckets = 128;
 constexpr int kOrderBlock = 256;
 struct TileSchedule {
   float prevC[64];
   uint32_t valid;
-  uint32_t pad[63];
-  uint32_t hist[kOrderBuckets];    // zero between calls
-  uint32_t starts[kOrderBuckets];  // next free slot of each bucket during the scatter
-  uint32_t pad2[128];
+  uint32_t parity;  // accumulator set of the next call
+  uint32_t pad[62];
+  unsigned long long acc[2][kOrderBuckets];  // low word: tiles in the bucket; high word: scatter cursor
   // followed by cost[tiles], order[tiles], key[tiles]
 };
-static_assert(sizeof(TileSchedule) == 2048, "schedule header");
+static_assert(sizeof(TileSchedule) == 2560, "schedule header");
 __device__ inline int tileKey(const View& cur, const View& prev, bool valid, const uint32_t* cost, int t, int tilesX,
                               int tilesY, int width, int height) {
   if (!valid) return kOrderBuckets - 1;
@@ -144,62 +153,122 @@ __device__ inline int tileKey(const View& cur, const View& prev, bool valid, con
   int sub = k >= 2 ? (int)((c >> (k - 2)) & 3u) : 0;
   return min(kOrderBuckets - 2, 4 * k + sub);
 }
-__global__ __launch_bounds__(kOrderBlock) void k_tileKeys(const float* C, TileSchedule* s, int tilesX, int tilesY,
-                                                          int width, int height) {
-  __shared__ uint32_t count[kOrderBuckets];
+// Ordering, before the grid barrier: every tile's bucket key and the bucket
+// histogram into accumulator set `par`; block 0 zeroes the other set.
+// `count` is a kOrderBuckets LDS array of the block; cam the current camera.
+__device__ void orderKeys(const float* cam, TileSchedule* s, int par, int tilesX, int tilesY, int width, int height,
+                          uint32_t* count) {
   const int tiles = tilesX * tilesY;
   const uint32_t* cost = reinterpret_cast<const uint32_t*>(s + 1);
-  uint32_t* key = const_cast<uint32_t*>(cost) + 2 * (size_t)tiles;
+  uint32_t* key = reinterpret_cast<uint32_t*>(s + 1) + 2 * (size_t)tiles;
   for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) count[b] = 0;
+  if (blockIdx.x == 0)
+    for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) s->acc[par ^ 1][b] = 0ull;
   __syncthreads();
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < tiles) {
-    const View cur = makeView(C), prev = makeView(s->prevC);
-    int k = tileKey(cur, prev, s->valid == 1u, cost, t, tilesX, tilesY, width, height);
+  const View cur = makeView(cam), prev = makeView(s->prevC);
+  const bool valid = s->valid == 1u;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += gridDim.x * blockDim.x) {
+    int k = tileKey(cur, prev, valid, cost, t, tilesX, tilesY, width, height);
     key[t] = (uint32_t)k;
     atomicAdd(&count[k], 1u);
   }
   __syncthreads();
   for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x)
-    if (count[b]) atomicAdd(&s->hist[b], count[b]);
-}
-__global__ __launch_bounds__(kOrderBuckets) void k_tileStarts(TileSchedule* s) {
-  __shared__ uint32_t h[kOrderBuckets];
-  h[threadIdx.x] = s->hist[threadIdx.x];
+    if (count[b]) atomicAdd(&s->acc[par][b], (unsigned long long)count[b]);
   __syncthreads();
-  if (threadIdx.x == 0) {  // highest bucket first
-    uint32_t at = 0;
-    for (int b = kOrderBuckets - 1; b >= 0; b--) {
-      uint32_t n = h[b];
-      h[b] = at;
-      at += n;
+}
+// Ordering, after the grid barrier: the bucket starts (highest bucket first)
+// from the histogram, then the tiles scattered bucket by bucket into the
+// order, a chunk of blockDim tiles at a time (the chunk loop's bound is
+// uniform per block). start, count and base are kOrderBuckets LDS arrays.
+__device__ void orderScatter(TileSchedule* s, int par, int tiles, uint32_t* start, uint32_t* count,
+                             uint32_t* base) {
+  const uint32_t* key = reinterpret_cast<const uint32_t*>(s + 1) + 2 * (size_t)tiles;
+  uint32_t* order = reinterpret_cast<uint32_t*>(s + 1) + tiles;
+  unsigned long long* acc = s->acc[par];
+  // start[b] = number of tiles in buckets above b: a suffix scan of the counts
+  for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) start[b] = (uint32_t)(acc[b] & 0xffffffffull);
+  __syncthreads();
+  for (int d = 1; d < kOrderBuckets; d <<= 1) {
+    uint32_t v = 0;
+    const bool mine = threadIdx.x < kOrderBuckets;
+    if (mine) v = start[threadIdx.x] + (threadIdx.x + d < kOrderBuckets ? start[threadIdx.x + d] : 0u);
+    __syncthreads();
+    if (mine) start[threadIdx.x] = v;
+    __syncthreads();
+  }
+  for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x)
+    start[b] -= (uint32_t)(acc[b] & 0xffffffffull);  // inclusive -> exclusive
+  __syncthreads();
+  for (int c0 = blockIdx.x * blockDim.x; c0 < tiles; c0 += gridDim.x * blockDim.x) {
+    for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) count[b] = 0;
+    __syncthreads();
+    const int t = c0 + threadIdx.x;
+    uint32_t k = 0, local = 0;
+    if (t < tiles) {
+      k = key[t];
+      local = atomicAdd(&count[k], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x)
+      base[b] = count[b] ? start[b] + (uint32_t)(atomicAdd(&acc[b], (unsigned long long)count[b] << 32) >> 32)
+                         : 0u;
+    __syncthreads();
+    const uint32_t at = base[k] + local;
+    if (t < tiles && at < (uint32_t)tiles) order[at] = (uint32_t)t;  // always true for consistent inputs
+    __syncthreads();
+  }
+}
+// Frame prologue: optionally updateCamera (I != nullptr) and the per-sample
+// camera slots (sample >= 0: C[9] = sample, C[10] = sample == 0, and C[22] =
+// light unless light < 0), then the tile order of one or two schedules (s1
+// may be null), in one cooperative launch with one grid barrier. Every block
+// computes the new camera itself in LDS from the old C (the same bits in
+// every block); block 0 stores it after the barrier, when no block reads the
+// old one any more, and keeps it in each schedule for the next reprojection.
+__global__ __launch_bounds__(kOrderBlock) void k_framePrologue(float* C, const float* I, float delta, int sample,
+                                                               int light, int width, int height, TileSchedule* s0,
+                                                               TileSchedule* s1) {
+  __shared__ float cam[64];
+  __shared__ uint32_t la[kOrderBuckets], lb[kOrderBuckets], lc[kOrderBuckets];
+  __shared__ int par[2];
+  const int tilesX = (width + 7) / 8, tilesY = (height + 7) / 8, tiles = tilesX * tilesY;
+  if (threadIdx.x < 64) cam[threadIdx.x] = C[threadIdx.x];
+  if (threadIdx.x == 0) {
+    par[0] = (int)(s0->parity & 1u);
+    par[1] = s1 ? (int)(s1->parity & 1u) : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (I) updateCameraState(cam, I, delta, width, height);
+    if (sample >= 0) {
+      cam[9] = (float)sample;
+      cam[10] = sample == 0 ? 1.0f : 0.0f;
+      if (light >= 0) cam[22] = light ? 1.0f : 0.0f;
     }
   }
   __syncthreads();
-  s->starts[threadIdx.x] = h[threadIdx.x];
-  s->hist[threadIdx.x] = 0u;
-}
-__global__ __launch_bounds__(kOrderBlock) void k_tileScatter(const float* C, TileSchedule* s, int tiles) {
-  __shared__ uint32_t count[kOrderBuckets], base[kOrderBuckets];
-  const uint32_t* key = reinterpret_cast<const uint32_t*>(s + 1) + 2 * (size_t)tiles;
-  uint32_t* order = reinterpret_cast<uint32_t*>(s + 1) + tiles;
-  for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) count[b] = 0;
-  __syncthreads();
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t k = 0, local = 0;
-  if (t < tiles) {
-    k = key[t];
-    local = atomicAdd(&count[k], 1u);
-  }
-  __syncthreads();
-  for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x)
-    base[b] = count[b] ? atomicAdd(&s->starts[b], count[b]) : 0u;
-  __syncthreads();
-  const uint32_t at = base[k] + local;
-  if (t < tiles && at < (uint32_t)tiles) order[at] = (uint32_t)t;  // always true unless an earlier pass failed
+  const int p0 = par[0], p1 = par[1];
+  orderKeys(cam, s0, p0, tilesX, tilesY, width, height, la);
+  if (s1) orderKeys(cam, s1, p1, tilesX, tilesY, width, height, la);
+  cooperative_groups::this_grid().sync();
+  orderScatter(s0, p0, tiles, la, lb, lc);
+  if (s1) orderScatter(s1, p1, tiles, la, lb, lc);
   if (blockIdx.x == 0) {
-    if (threadIdx.x < 64) s->prevC[threadIdx.x] = C[threadIdx.x];
-    if (threadIdx.x == 0) s->valid = 1u;
+    if (threadIdx.x < 64) {
+      const float c = cam[threadIdx.x];
+      if (I || sample >= 0) C[threadIdx.x] = c;
+      s0->prevC[threadIdx.x] = c;
+      if (s1) s1->prevC[threadIdx.x] = c;
+    }
+    if (threadIdx.x == 0) {
+      s0->valid = 1u;
+      s0->parity = (uint32_t)(p0 ^ 1);
+      if (s1) {
+        s1->valid = 1u;
+        s1->parity = (uint32_t)(p1 ^ 1);
+      }
+    }
   }
 }
 
@@ -292,6 +361,27 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTra
   }
 }
 
+AD float aces(float x) { return sat((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f)); }
+
+// One pixel of resolvePixels (Arbor.cu:304) given its Filtered colour.
+__device__ __forceinline__ void resolveOne(v3 current, float4* History, uint32_t* Pixels, const float* C, int x,
+                                           int y, int width, int height, int i) {
+  float count = C[10] > 0.5f ? 0.0f : C[9];
+  v3 mean = current;
+  if (!(count < 0.5f)) {
+    float4 h = History[i];
+    mean = mix(mk(h.x, h.y, h.z), current, 1.0f / (count + 1.0f));
+  }
+  History[i] = make_float4(mean.x, mean.y, mean.z, count + 1.0f);
+  v3 col = mean * C[20];
+  col = mk(aces(col.x), aces(col.y), aces(col.z));
+  const float g = 1.0f / 2.2f;
+  col = mk(powf(col.x, g), powf(col.y, g), powf(col.z, g));
+  float u = ((float)x / (float)width - 0.5f) * 2.0f, v = ((float)y / (float)height - 0.5f) * 2.0f;
+  col = col * (1.0f - 0.09f * (u * u + v * v));
+  Pixels[i] = packRGB(col);
+}
+
 // filterLighting (Arbor.cu:292-302): 5x5 cross-bilateral, same id/tree/depth.
 // One block filters a 32x8 tile. The tile plus a 2-pixel halo (36x12) is
 // staged in LDS as {t, id, tree, Linear/(Guide+0.02)}: the 25 taps read LDS
@@ -300,11 +390,16 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTra
 // The 6 distinct tap weights expf(-0.3*(j*j+k*k)) arrive precomputed.
 constexpr int kFilterW = 32, kFilterH = 8, kHalo = 2;
 constexpr int kTileW = kFilterW + 2 * kHalo, kTileH = kFilterH + 2 * kHalo;
+// With kResolve the same kernel then resolves the pixel from the colour it
+// just filtered (History and Pixels as resolvePixels writes them): the
+// engine's fused filter + resolve, with the same bits as the two launches.
 struct FilterWeights { float w[9]; };  // indexed by j*j+k*k (0,1,2,4,5,8 used)
+template <bool kResolve>
 __global__ __launch_bounds__(kFilterW * kFilterH) void k_filterLighting(const float4* Linear, const float4* Guide,
                                                                        const float4* Hit, float4* Filtered,
                                                                        const float* C, int width, int height,
-                                                                       FilterWeights fw) {
+                                                                       FilterWeights fw, float4* History,
+                                                                       uint32_t* Pixels) {
   __shared__ float st[kTileH][kTileW];
   __shared__ int sid[kTileH][kTileW], stree[kTileH][kTileW];
   __shared__ float sdr[kTileH][kTileW], sdg[kTileH][kTileW], sdb[kTileH][kTileW];
@@ -370,9 +465,8 @@ __global__ __launch_bounds__(kFilterW * kFilterH) void k_filterLighting(const fl
     if (weight > 0.0f) col = mix(col, (sum / weight) * base, 0.86f);
   }
   Filtered[b] = make_float4(col.x, col.y, col.z, 1.0f);
+  if (kResolve) resolveOne(col, History, Pixels, C, x, y, width, height, b);
 }
-
-AD float aces(float x) { return sat((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f)); }
 
 // resolvePixels (Arbor.cu:304).
 __global__ __launch_bounds__(256) void k_resolvePixels(const float4* Filtered, float4* History, uint32_t* Pixels,
@@ -381,26 +475,28 @@ __global__ __launch_bounds__(256) void k_resolvePixels(const float4* Filtered, f
   if (x >= width || y >= height) return;
   int i = y * width + x;
   float4 f = Filtered[i];
-  v3 current = mk(f.x, f.y, f.z);
-  float count = C[10] > 0.5f ? 0.0f : C[9];
-  v3 mean = current;
-  if (!(count < 0.5f)) {
-    float4 h = History[i];
-    mean = mix(mk(h.x, h.y, h.z), current, 1.0f / (count + 1.0f));
-  }
-  History[i] = make_float4(mean.x, mean.y, mean.z, count + 1.0f);
-  v3 col = mean * C[20];
-  col = mk(aces(col.x), aces(col.y), aces(col.z));
-  const float g = 1.0f / 2.2f;
-  col = mk(powf(col.x, g), powf(col.y, g), powf(col.z, g));
-  float u = ((float)x / (float)width - 0.5f) * 2.0f, v = ((float)y / (float)height - 0.5f) * 2.0f;
-  col = col * (1.0f - 0.09f * (u * u + v * v));
-  Pixels[i] = packRGB(col);
+  resolveOne(mk(f.x, f.y, f.z), History, Pixels, C, x, y, width, height, i);
 }
 
 }  // namespace arbor
 
 using namespace arbor;
+
+namespace arbor {
+// Compute units per device, read once per device and kept in atomics (any
+// host thread may launch).
+int deviceCUs() {
+  static std::atomic<int> cus[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  int n = cus[dev].load(std::memory_order_relaxed);
+  if (n <= 0) {
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev].store(n, std::memory_order_relaxed);
+  }
+  return n;
+}
+}  // namespace arbor
 
 namespace {
 // Grid for the tile kernels: one wave per 8x8 tile up to the cap (a tuning
@@ -496,18 +592,40 @@ int arbor_tracePixels(const float* P, const float* Nodes, const unsigned* Order,
 }  // extern "C"
 
 namespace {
-// The three ordering passes over a schedule; its cost and order arrays out.
+// Launches the frame prologue (k_framePrologue) cooperatively: every block
+// resident at once, so its grid barriers complete.
+int launchPrologue(float* C, const float* I, float delta, int sample, int light, int width, int height,
+                   void* s0, void* s1, hipStream_t stream) {
+  const int tiles = ((width + 7) / 8) * ((height + 7) / 8);
+  int blocks = (tiles + kOrderBlock - 1) / kOrderBlock;
+  const int cus = deviceCUs();
+  if (blocks > cus) blocks = cus;
+  TileSchedule* a = static_cast<TileSchedule*>(s0);
+  TileSchedule* b = static_cast<TileSchedule*>(s1);
+  void* args[] = {&C, &I, &delta, &sample, &light, &width, &height, &a, &b};
+  if (hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_framePrologue), dim3(blocks), dim3(kOrderBlock),
+                                 args, 0, stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return ARBOR_ERR_LAUNCH;
+  }
+  return launch_status();
+}
+// Cost and order arrays of a schedule for a width x height view.
+void scheduleArrays(void* schedule, int width, int height, const uint32_t** order, uint32_t** cost) {
+  const int tiles = ((width + 7) / 8) * ((height + 7) / 8);
+  *cost = reinterpret_cast<uint32_t*>(static_cast<TileSchedule*>(schedule) + 1);
+  *order = *cost + (size_t)tiles;
+}
+// The ordering of one schedule (one launch); its cost and order arrays out.
 int orderTiles(const float* C, void* schedule, int width, int height, hipStream_t stream, const uint32_t** order,
                uint32_t** cost) {
-  const int tx = (width + 7) / 8, ty = (height + 7) / 8;
-  TileSchedule* s = static_cast<TileSchedule*>(schedule);
-  const int tiles = tx * ty, blocks = (tiles + kOrderBlock - 1) / kOrderBlock;
-  hipLaunchKernelGGL(k_tileKeys, dim3(blocks), dim3(kOrderBlock), 0, stream, C, s, tx, ty, width, height);
-  hipLaunchKernelGGL(k_tileStarts, dim3(1), dim3(kOrderBuckets), 0, stream, s);
-  hipLaunchKernelGGL(k_tileScatter, dim3(blocks), dim3(kOrderBlock), 0, stream, C, s, tiles);
-  *cost = reinterpret_cast<uint32_t*>(s + 1);
-  *order = *cost + (size_t)tiles;
-  return launch_status();
+  scheduleArrays(schedule, width, height, order, cost);
+  return launchPrologue(const_cast<float*>(C), nullptr, 0.0f, -1, -1, width, height, schedule, nullptr, stream);
+}
+FilterWeights filterWeights() {
+  FilterWeights fw;
+  for (int i = 0; i < 9; i++) fw.w[i] = expf(-(float)i * 0.30f);  // host libm, as the reference harness
+  return fw;
 }
 }  // namespace
 
@@ -546,15 +664,8 @@ int arbor_traceTreePixels(const float* P, const float* Nodes, const unsigned* Or
 // per CU against 200 us with one block per 4 tiles (k_shadePixels<true>: 186
 // against 235 us with 2 per CU). Results do not depend on the grid.
 inline int shadeGrid(int tiles, int perCU) {
-  static int cus[64] = {};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-  if (cus[dev] == 0) {
-    int n = 0;
-    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-    cus[dev] = n;
-  }
-  const int g = tileGrid(tiles, kWavesPerBlock), cap = perCU * cus[dev];
+  const int cus = deviceCUs();
+  const int g = tileGrid(tiles, kWavesPerBlock), cap = perCU * cus;
   return g < cap ? g : cap;
 }
 
@@ -706,11 +817,9 @@ int arbor_filterLighting(const float* Linear, const float* Guide, const float* H
   ARBOR_REQUIRE(Linear && Guide && Hit && Filtered && C);
   if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
   dim3 block(kFilterW, kFilterH), grid((width + kFilterW - 1) / kFilterW, (height + kFilterH - 1) / kFilterH);
-  FilterWeights fw;
-  for (int i = 0; i < 9; i++) fw.w[i] = expf(-(float)i * 0.30f);  // host libm, as the reference harness
-  hipLaunchKernelGGL(k_filterLighting, grid, block, 0, stream, reinterpret_cast<const float4*>(Linear),
+  hipLaunchKernelGGL(k_filterLighting<false>, grid, block, 0, stream, reinterpret_cast<const float4*>(Linear),
                      reinterpret_cast<const float4*>(Guide), reinterpret_cast<const float4*>(Hit),
-                     reinterpret_cast<float4*>(Filtered), C, width, height, fw);
+                     reinterpret_cast<float4*>(Filtered), C, width, height, filterWeights(), nullptr, nullptr);
   return launch_status();
 }
 
@@ -879,3 +988,48 @@ int arbor_autotune_trace_accel(const float* P, const float* Nodes, const unsigne
 }
 
 }  // extern "C"
+
+namespace arbor {
+
+int framePrologue(float* C, const float* I, float delta, int sample, int light, int width, int height,
+                  void* traceSchedule, void* visSchedule, hipStream_t stream) {
+  if (!C || !traceSchedule) return ARBOR_ERR_NULL;
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  return launchPrologue(C, I, delta, sample, light, width, height, traceSchedule, visSchedule, stream);
+}
+
+int traceOrdered(const float* P, const float* Nodes, const unsigned* Order, const void* accel, const float* C,
+                 float* Hit, int width, int height, void* schedule, hipStream_t stream) {
+  if (!P || !Nodes || !Order || !C || !Hit || !schedule) return ARBOR_ERR_NULL;
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  const uint32_t* order = nullptr;
+  uint32_t* cost = nullptr;
+  scheduleArrays(schedule, width, height, &order, &cost);
+  if (accel) return launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, order, cost);
+  return launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream, order, cost);
+}
+
+int visibilityOrdered(const float* P, const float* Nodes, const unsigned* Order, const void* accel, const float* C,
+                      const float* Hit, float* Vis, int width, int height, void* schedule, hipStream_t stream) {
+  if (!P || !Nodes || !Order || !C || !Hit || !Vis || !schedule) return ARBOR_ERR_NULL;
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  const uint32_t* order = nullptr;
+  uint32_t* cost = nullptr;
+  scheduleArrays(schedule, width, height, &order, &cost);
+  if (accel) return launchVisibilityAccel(P, Nodes, Order, accel, C, Hit, Vis, width, height, order, cost, stream);
+  return launchVisibility(P, Nodes, Order, C, Hit, Vis, width, height, order, cost, stream);
+}
+
+int filterResolve(const float* Linear, const float* Guide, const float* Hit, float* Filtered, float* History,
+                  unsigned* Pixels, const float* C, int width, int height, hipStream_t stream) {
+  if (!Linear || !Guide || !Hit || !Filtered || !History || !Pixels || !C) return ARBOR_ERR_NULL;
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  dim3 block(kFilterW, kFilterH), grid((width + kFilterW - 1) / kFilterW, (height + kFilterH - 1) / kFilterH);
+  hipLaunchKernelGGL(k_filterLighting<true>, grid, block, 0, stream, reinterpret_cast<const float4*>(Linear),
+                     reinterpret_cast<const float4*>(Guide), reinterpret_cast<const float4*>(Hit),
+                     reinterpret_cast<float4*>(Filtered), C, width, height, filterWeights(),
+                     reinterpret_cast<float4*>(History), Pixels);
+  return launch_status();
+}
+
+}  // namespace arbor

@@ -238,16 +238,15 @@ def trace(scene, C, fb):
 
 class TileSchedule:
     """Device workspace of arbor_tracePixelsScheduled for one view size:
-    a 2048-byte header, then per 8x8 tile its recorded cost, its place in the
-    order and its bucket key (u32 each)."""
-
-    HEADER = 2048
+    a header, then per 8x8 tile its recorded cost, its place in the order and
+    its bucket key (u32 each). Zeroed at creation, as the library requires."""
 
     def __init__(self, width, height, device="cuda"):
         import torch
 
         n = lib().arbor_tile_schedule_bytes(width, height)
         check(n if n < 0 else 0, "arbor_tile_schedule_bytes")
+        self.HEADER = lib().arbor_tile_schedule_bytes(8, 8) - 12  # one tile: 12 bytes after the header
         self.buf = torch.zeros(n, dtype=torch.uint8, device=device)
         self.width, self.height = width, height
 

@@ -4,9 +4,10 @@ Workload (the full-forest configuration): seed 7, 18x18 = 324 instanced trees
 sharing one resident BVH, 1920x1080, 1 spp primary rays, shade, filterLighting
 and resolve, tree-to-tree sun visibility off, breeze off, along a seeded
 120-frame WASD walk (dt = 1/60 s). A step is one frame of the C++ engine:
-updateCamera -> tracePixels -> shadePixels -> filterLighting -> resolvePixels,
-inputs resident in HBM (the walk's 32-float input rows are uploaded before
-the timed region).
+updateCamera -> tracePixels -> shadePixels -> filterLighting -> resolvePixels
+in five kernel launches, inputs resident in HBM (the walk's 32-float input
+rows are uploaded before the timed region). The timed steps are walk frames
+0..steps-1; the whole walk's per-frame GPU time is reported beside them.
 
 Prints one JSON line. `value` is end-to-end primary Mrays/s (W*H per frame /
 frame time); the trace kernel's own rate, per-stage ms, its roofline fraction
@@ -41,6 +42,8 @@ FP32_PEAK_TFLOPS = 157.3
 # One non-packed FP32 VALU op per lane per clock: 256 CUs x 64 lanes x 2.4 GHz.
 FP32_SINGLE_ISSUE_TOPS = 39.32
 HBM_PEAK_GBS = 8000.0
+# Frames of the seeded walk (the full-forest workload's camera path).
+WALK_FRAMES = 120
 
 
 def parse():
@@ -343,16 +346,25 @@ def walk_inputs(px, seed, n):
     return np.stack(rows).astype(np.float32)
 
 
-def _cpu_frame(cpu, seed, width, height, threads):
-    """One frame (walk frame 0) of the workload on a CPU implementation:
-    (seconds for trace+shade+filter+resolve, seconds for the trace)."""
-    import numpy as np
-    import pyarbor
+# Walk frames the CPU baseline renders (the GPU's time on the same frames is
+# reported beside it): the first frame and the middle of the 120-frame walk.
+CPU_FRAMES = (0, 60)
 
-    scene = cpu.generate(seed)
+
+def _cpu_camera(cpu, pyarbor, seed, frame, width, height):
+    """Camera of walk frame `frame`: initCamera, then updateCamera over the
+    walk's inputs 0..frame (as the engine runs them)."""
     C = cpu.init_camera(seed)
-    I = walk_inputs(pyarbor, seed, 1)[0]
-    cpu.update_camera(C, I, 1.0 / 60.0, width, height)
+    for f, I in enumerate(walk_inputs(pyarbor, seed, frame + 1)):
+        cpu.update_camera(C, I, 1.0 / 60.0, width, height)
+    return C
+
+
+def _cpu_frame(cpu, scene, C, width, height, threads):
+    """One frame on a CPU implementation: (seconds for trace+shade+filter+
+    resolve, seconds for the trace)."""
+    import numpy as np
+
     t0 = time.perf_counter()
     H = cpu.trace(scene, C, width, height, threads)
     t1 = time.perf_counter()
@@ -364,27 +376,64 @@ def _cpu_frame(cpu, seed, width, height, threads):
     return t2 - t0, t1 - t0
 
 
+def _cpu_timed(cpu, seed, width, height, threads):
+    """Warm-up (walk frame 0 once), then the best of 3 runs of each frame in
+    CPU_FRAMES: per frame (frame s, trace s)."""
+    import pyarbor
+
+    scene = cpu.generate(seed)
+    cams = {f: _cpu_camera(cpu, pyarbor, seed, f, width, height) for f in CPU_FRAMES}
+    _cpu_frame(cpu, scene, cams[CPU_FRAMES[0]], width, height, threads)
+    out = {}
+    for f in CPU_FRAMES:
+        runs = [_cpu_frame(cpu, scene, cams[f], width, height, threads) for _ in range(3)]
+        out[f] = min(runs)
+    return out
+
+
+def _host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
 def cpu_baseline(seed, width, height, threads):
     """The reference CPU harness (compiled from its sources into
-    oracle/_ref/libarborref.so) on this host's cores, one frame (walk frame 0)
-    of the same workload: trace + shade + filter + resolve. The oracle's float
-    restatement (same outputs bit for bit) is timed beside it as "port"."""
+    oracle/_ref/libarborref.so) on this host's cores: walk frames 0 and 60 of
+    the same workload (trace + shade + filter + resolve), warm-up then best of
+    3 per frame. The oracle's float restatement (same outputs bit for bit) is
+    timed beside it as "port". Threads: the CPU share this process is given
+    (OMP_NUM_THREADS where set, else every core in its affinity mask)."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
     import oracle_lib
 
+    affinity = len(os.sched_getaffinity(0))
     if threads <= 0:
-        threads = min(16, len(os.sched_getaffinity(0)))
+        share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = min(affinity, share) if share > 0 else affinity
     rays = width * height
-    port_s, port_trace_s = _cpu_frame(oracle_lib.Oracle(), seed, width, height, threads)
+    port = _cpu_timed(oracle_lib.Oracle(), seed, width, height, threads)
     kind = "reference" if oracle_lib.ref_available() else "port"
-    if kind == "reference":
-        frame_s, trace_s = _cpu_frame(oracle_lib.RefHarness(), seed, width, height, threads)
-    else:
-        frame_s, trace_s = port_s, port_trace_s
+    ref = _cpu_timed(oracle_lib.RefHarness(), seed, width, height, threads) if kind == "reference" else port
+    frame_s = sum(v[0] for v in ref.values()) / len(ref)
+    trace_s = sum(v[1] for v in ref.values()) / len(ref)
+    port_s = sum(v[0] for v in port.values()) / len(port)
+    port_trace_s = sum(v[1] for v in port.values()) / len(port)
     return {"value": round(rays / frame_s / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": kind,
-            "sample": "walk frame 0 of the same workload (%dx%d, seed %d, shadows off), trace+shade+filter+resolve, "
-                      "OpenMP schedule(dynamic,2) rows" % (width, height, seed),
+            "sample": "walk frames %s of the same workload (%dx%d, seed %d, shadows off), trace+shade+filter+"
+                      "resolve, warm-up then best of 3 per frame, mean over the frames; OpenMP "
+                      "schedule(dynamic,2) rows" % (list(CPU_FRAMES), width, height, seed),
+            "nproc": os.cpu_count(), "affinity_cores": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": _host_cpu(),
             "ms_per_frame": round(frame_s * 1e3, 1), "trace_ms": round(trace_s * 1e3, 1),
+            "ms_by_frame": {str(f): round(v[0] * 1e3, 1) for f, v in ref.items()},
             "port": {"value": round(rays / port_s / 1e6, 4), "ms_per_frame": round(port_s * 1e3, 1),
                      "trace_ms": round(port_trace_s * 1e3, 1)}}
 
@@ -417,7 +466,7 @@ def main():
 
     W, H, K, Wm = args.width, args.height, args.steps, args.warmup
     eng = px.Engine(W, H, args.seed)
-    inputs = torch.from_numpy(walk_inputs(px, args.seed, max(K, Wm, 1))).cuda()
+    inputs = torch.from_numpy(walk_inputs(px, args.seed, max(K, Wm, WALK_FRAMES, 1))).cuda()
     dt = 1.0 / 60.0
 
     # warmup: the first Wm walk frames, then back to the initial camera
@@ -436,22 +485,34 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / K * 1e3
 
-    # per-stage kernel times: replay the same walk with HIP events around each
-    # stage on the engine's stream (synchronises per frame, outside the timed loop)
-    eng.reset_camera()
-    eng.set_timing(True)
-    stages = []
-    for f in range(K):
-        eng.frame(dt, inputs[f])
-        stages.append(eng.stage_ms())
-    eng.set_timing(False)
-    st = np.array(stages)
+    # per-stage kernel times: the whole 120-frame walk replayed with HIP
+    # events around each stage on the engine's stream (it synchronises per
+    # frame, so it runs outside the timed loop), over the SAH hierarchy (the
+    # engine's trace), then again over the heap BVH (arbor_tracePixels' walk)
+    walk = torch.from_numpy(walk_inputs(px, args.seed, WALK_FRAMES)).cuda()
+
+    def stage_pass():
+        eng.reset_camera()
+        eng.set_timing(True)
+        rows = []
+        for f in range(WALK_FRAMES):
+            eng.frame(dt, walk[f])
+            rows.append(eng.stage_ms())
+        eng.set_timing(False)
+        return np.array(rows)
+
+    st = stage_pass()
+    eng.set_accel(False)
+    st_heap = stage_pass()
+    eng.set_accel(True)
     mean = st.mean(axis=0)
     trace_ms = float(mean[1])
+    heap_trace_ms = float(st_heap[:, 1].mean())
 
-    roofline = {"bound": "mfma", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
+    roofline = {"bound": "fp32_valu", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
                 "kernel": "tracePixels (SAH walk, arbor_tracePixelsAccel)",
                 "compute": "FP32 vector ALU (ray traversal has no MFMA shape); peak is MI355X's FP32 157.3 TFLOP/s",
+                "duration_ms": round(trace_ms, 4), "duration_over": "mean of the %d-frame walk" % WALK_FRAMES,
                 "traffic": None}
     std_res = (W, H) == (1920, 1080) and args.seed == 7
     if std_res:
@@ -466,6 +527,7 @@ def main():
         roofline.update({"achieved": None, "frac": None})
 
     rays = W * H
+    walk_ms = float(mean[5])
     result = {
         "metric": "Mrays/s, ms/frame @1080p 324 trees",
         "value": round(rays * K / elapsed / 1e6, 3),
@@ -481,27 +543,36 @@ def main():
         "data": "synthetic (procedural forest from seed; seeded walk-path inputs)",
         "config": {"workload": "full forest plain method: 324 trees, 1920x1080, 1 spp primary+shade+filter+resolve, "
                                "shadows off, seeded 120-frame walk", "width": W, "height": H, "trees": 324,
-                   "seed": args.seed, "spp": 1, "frames": K},
+                   "seed": args.seed, "spp": 1, "frames": K, "timed_walk_frames": [0, K - 1]},
         "ms_per_frame": round(ms_per_step, 4),
+        "walk120": {"frames": WALK_FRAMES, "gpu_ms_per_frame": round(walk_ms, 4),
+                    "mrays_per_s": round(rays / (walk_ms * 1e-3) / 1e6, 2),
+                    "note": "HIP-event time of every frame of the walk (the timed steps cover frames 0..steps-1)"},
         "trace_mrays_per_s": round(rays / (trace_ms * 1e-3) / 1e6, 2),
-        "stage_ms": {"camera": round(float(mean[0]), 4), "trace": round(trace_ms, 4),
-                     "shade": round(float(mean[2]), 4), "filter": round(float(mean[3]), 4),
-                     "resolve": round(float(mean[4]), 4), "frame": round(float(mean[5]), 4)},
+        "heap_walk_trace": {"entry_point": "arbor_tracePixels' heap BVH walk, tile-scheduled",
+                            "ms": round(heap_trace_ms, 4),
+                            "mrays_per_s": round(rays / (heap_trace_ms * 1e-3) / 1e6, 2)},
+        "launches_per_frame": 5,
+        "stage_ms": {"prologue": round(float(mean[0]), 4), "trace": round(trace_ms, 4),
+                     "shade": round(float(mean[2]), 4), "filter_resolve": round(float(mean[3]), 4),
+                     "frame": round(walk_ms, 4)},
         "roofline": roofline,
     }
-    # HBM figures of the two bandwidth-bound kernels (64 and 52 algorithmic B/px)
-    result["hbm_kernels"] = {
-        "filterLighting": {"GB/s": round(64 * rays / (mean[3] * 1e-3) / 1e9, 1),
-                           "frac": round(64 * rays / (mean[3] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "resolvePixels": {"GB/s": round(52 * rays / (mean[4] * 1e-3) / 1e9, 1),
-                          "frac": round(52 * rays / (mean[4] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-    }
+    # HBM figure of the bandwidth-bound fused filter + resolve: 100 algorithmic
+    # B/px (Linear, Guide, Hit and History read; Filtered, History and Pixels written)
+    fr_ms = float(mean[3])
+    result["hbm_kernels"] = {"filterLighting+resolvePixels": {
+        "GB/s": round(100 * rays / (fr_ms * 1e-3) / 1e9, 1),
+        "frac": round(100 * rays / (fr_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
     eng.close()
     if not args.no_cpu:
         cb = cpu_baseline(args.seed, W, H, args.cpu_threads)
         result["cpu_baseline"] = cb
         result["gpu_vs_cpu"] = round(result["value"] / cb["value"], 1)
         result["gpu_vs_cpu_port"] = round(result["value"] / cb["port"]["value"], 1)
+        same = float(np.mean([st[f, 5] for f in CPU_FRAMES if f < WALK_FRAMES]))
+        result["gpu_vs_cpu_same_frames"] = {"frames": list(CPU_FRAMES), "gpu_ms_per_frame": round(same, 4),
+                                           "ratio": round(cb["ms_per_frame"] / same, 1)}
     print(json.dumps(result), flush=True)
 
 

@@ -36,11 +36,17 @@ int arbor_engine_set_timing(arbor_engine* e, int on);
  * the heap BVH (arbor_tracePixelsScheduled / arbor_sunVisibilityScheduled). */
 int arbor_engine_set_accel(arbor_engine* e, int on);
 /* engine.js:45-53: updateCamera, tracePixels, shadePixels, filterLighting,
- * resolvePixels. device_input, if not NULL, is a device array of 32 floats
- * used instead of the host input. */
+ * resolvePixels, issued as five kernel launches like the reference's five
+ * dispatches (engine.js:48-50, source.test.mjs:6): (1) updateCamera with the
+ * tile orders of the trace and of the sun-visibility pass (one cooperative
+ * launch), (2) tracePixels, (3) the sun-visibility pass, (4) shadePixels from
+ * it, (5) filterLighting and resolvePixels in one kernel. device_input, if
+ * not NULL, is a device array of 32 floats used instead of the host input
+ * (otherwise the host input is copied to the device first). */
 int arbor_engine_frame(arbor_engine* e, float dt, const float* device_input);
-/* Last timed frame: {camera, visibility, lighting, filter, resolve, frame} in
- * ms, measured with HIP events on the engine's stream. */
+/* Last timed frame: {prologue (camera and tile orders), trace, shading
+ * (visibility pass and shadePixels), filter + resolve, 0, whole frame} in ms,
+ * measured with HIP events on the engine's stream. */
 int arbor_engine_stage_ms(const arbor_engine* e, float* out6);
 int arbor_engine_sync(arbor_engine* e);
 /* Device pointer of a named buffer: B P Keys Order Nodes C I Hit Linear Guide
@@ -61,8 +67,9 @@ int arbor_walk_input(int seed, int frame, float* input32);
 int arbor_engine_render(arbor_engine* e, int view, int samples, int light);
 /* One frame of `samples` (1..512) jittered samples: updateCamera once, then per
  * sample C[9] = i, C[10] = (i == 0), tracePixels, shadePixels, filterLighting,
- * resolvePixels (History: the frame's sample mean). The multi-sample frame
- * of the 4-sample, sun-visibility configuration. */
+ * resolvePixels (History: the frame's sample mean), five launches per sample
+ * as in arbor_engine_frame. The multi-sample frame of the 4-sample,
+ * sun-visibility configuration. */
 int arbor_engine_frame_samples(arbor_engine* e, float dt, const float* device_input, int samples);
 /* Input recording (no counterpart in the reference, whose inputs come from
  * the browser each frame): with path non-NULL every later frame's dt, sample

@@ -172,10 +172,11 @@ def test_trace_accel_size_and_checks(L):
 
 
 def test_tile_schedule_size_and_checks(L):
-    # 2048-byte header (previous camera, valid flag, bucket histogram and starts)
+    # 2560-byte header (previous camera, valid flag, accumulator parity, two sets of
+    # 128 bucket accumulators: tile count and scatter cursor)
     # + cost, order and key per 8x8 tile
-    assert L.arbor_tile_schedule_bytes(1920, 1080) == 2048 + 12 * 240 * 135
-    assert L.arbor_tile_schedule_bytes(33, 9) == 2048 + 12 * 5 * 2
+    assert L.arbor_tile_schedule_bytes(1920, 1080) == 2560 + 12 * 240 * 135
+    assert L.arbor_tile_schedule_bytes(33, 9) == 2560 + 12 * 5 * 2
     assert L.arbor_tile_schedule_bytes(0, 10) == -2
     assert L.arbor_tracePixelsScheduled(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 64, 64, NULL, NULL) == -1
     assert L.arbor_tracePixelsScheduled(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 0, 64, DUMMY, NULL) == -2

@@ -122,3 +122,64 @@ def test_trace_autotune_keeps_results(px, scene, oracle):
     px.trace(scene, Cd, fb)
     assert np.array_equal(to_host(fb.Hit).view(np.uint32), before.view(np.uint32))
     px.tune_set("trace_variant", old)
+
+
+def test_batched_views_workload_full_size(px, scene, oracle):
+    """The batched-views workload at its size: 64 cameras at seeded positions
+    inside the 324-tree forest, resolutions drawn from {256x256, 640x360,
+    1280x720, 1920x1080, 2560x1440} (98,180,096 pixels), 1 spp, sun visibility
+    off, over the SAH hierarchy, rendered by two replays of one captured
+    hipGraph. Every view's Hit, Linear, Guide, Filtered, History and Pixels
+    equal its single-view render (arbor_tracePixelsAccel, shadePixels,
+    filterLighting, resolvePixels) bit for bit; 20,000 sampled pixels of a
+    1920x1080 view and of a 2560x1440 view meet the trace bar against the
+    FP32 and FP64 oracle."""
+    import torch
+
+    from gpu_util import check_hits
+
+    print("case: 64 batched views at native resolutions", flush=True)
+    cams, sizes = px.batch_walk_views(3, 7, 64)
+    total = sum(w * h for w, h in sizes)
+    assert total == 98_180_096 and len(set(sizes)) == 5, (total, set(sizes))
+    accel = px.TraceAccel(scene)
+    b = px.Batch(64, total)
+    b.set_views(cams, sizes)
+    b.set_accel(accel)
+    stream = torch.cuda.Stream()
+    b.capture(scene, stream)
+    for _ in range(2):
+        b.launch(stream)
+    stream.synchronize()
+    lib = px.lib()
+    fbs = {}
+    per = {"Hit": 16, "Linear": 16, "Guide": 16, "Filtered": 16, "History": 16, "Pixels": 4}
+    for i, (w, h) in enumerate(sizes):
+        fb = fbs.setdefault((w, h), px.FrameBuffers(w, h))
+        Cd = to_dev(cams[i])
+        fb.History.zero_()
+        px.trace_accel(scene, accel, Cd, fb)
+        px.shade(scene, Cd, fb)
+        px.filter_lighting(Cd, fb)
+        px.resolve(Cd, fb)
+        torch.cuda.synchronize()
+        for name, bpp in per.items():
+            ptr = lib.arbor_batch_buffer(b.h, i, name.encode())
+            got = px._wrap_device(ptr, w * h * bpp)
+            want = getattr(fb, name).view(torch.uint8).reshape(-1)
+            assert torch.equal(got, want), (i, (w, h), name)
+    print("64 views equal their single-view renders;", len(fbs), "sizes")
+    ref = oracle.generate(7)
+    for size in ((1920, 1080), (2560, 1440)):
+        i = next(k for k, s in enumerate(sizes) if s == size)
+        W, H = size
+        hit = b.download(i, "Hit").reshape(-1, 4)
+        rng = np.random.default_rng(4000 + W)
+        n = 20000
+        xy = np.stack([rng.integers(0, W, n), rng.integers(0, H, n)], axis=1).astype(np.int32)
+        gpu = hit[xy[:, 1] * W + xy[:, 0]].reshape(-1)
+        C = cams[i]
+        h32 = oracle.trace_pixels(ref, C, xy, W, H)
+        h64 = oracle.trace_pixels(ref, C, xy, W, H, fp64=True)
+        check_hits("batch view %d %dx%d (sampled)" % (i, W, H), gpu, h32, h64, (oracle, ref, C, W, H, xy))
+    b.close()

@@ -173,3 +173,53 @@ def test_training_reduces_loss_and_checkpoint_roundtrip(env, tmp_path):
     assert (o[:, 3] == 1).all()
     inf.close()
     dn.close()
+
+
+def test_checkpoint_on_1080p_tracer_frame(env):
+    """The learned-denoiser workload at its size: the committed checkpoint on a
+    1920x1080 frame the tracer renders (seed-7 forest, a held-out camera, sun
+    visibility on, one sample). Its output matches the fp32 PyTorch
+    restatement of the same graph (relative L2 <= 2e-2 on the runner's own
+    bf16 input), and after resolvePixels' display mapping it scores a higher
+    PSNR and SSIM than filterLighting against the 64-sample render."""
+    import os
+    import sys
+
+    torch, px = env
+    sys.path.insert(0, os.path.dirname(px.__file__))
+    import dn_train
+
+    print("case: denoiser checkpoint on a 1920x1080 tracer frame", flush=True)
+    W, Hh = 1920, 1080
+    scene = px.DeviceScene().generate(7)
+    cams, _ = px.batch_walk_views(11 + 777777, 7, 2)
+    cam = cams[0].copy()
+    cam[22] = 1.0
+    fb, normal, Cd = dn_train.render_inputs(px, torch, scene, cam, W, Hh)
+    target = dn_train.render_targets_batched(px, torch, scene, cam[None, :], W, Hh, 64)[0]
+    inf = px.Denoiser(1, 1088, 1920, train=False, seed=0)
+    inf.load(os.path.join(os.path.dirname(px.__file__), "weights", "denoiser_forest.bin"))
+    out = torch.zeros(W * Hh * 4, dtype=torch.float32, device="cuda")
+    inf.set_input(0, fb.Linear, fb.Guide, normal, fb.Hit, W, Hh)
+    inf.forward(1)
+    inf.output(0, out, W, Hh)
+    torch.cuda.synchronize()
+    x = torch.from_numpy(inf.tensor("input", 1088 * 1920 * 16, np.uint16).astype(np.int16).view(np.int16))
+    x = x.view(torch.bfloat16).float().reshape(1, 1088, 1920, 16).permute(0, 3, 1, 2)
+    y = torch.from_numpy(inf.tensor("output", 1088 * 1920 * 16, np.uint16).astype(np.int16).view(np.int16))
+    y = y.view(torch.bfloat16).float().reshape(1, 1088, 1920, 16).permute(0, 3, 1, 2)
+    Ws, Bs = _load(inf, torch)
+    with torch.no_grad():
+        ref = _ref_forward(torch, x, Ws, Bs)
+    rel = ((y[:, :3] - ref[:, :3]).norm() / ref[:, :3].norm()).item()
+    print("forward relative L2 error at 1088x1920: %.4g" % rel)
+    assert rel <= 2e-2
+    refimg = dn_train.display(px, torch, target, Cd, W, Hh)
+    scores = {}
+    for k, buf in (("1 spp", fb.Linear), ("filterLighting", fb.Filtered), ("learned", out)):
+        img = dn_train.display(px, torch, buf, Cd, W, Hh)
+        scores[k] = (dn_train.psnr(img, refimg), dn_train.ssim(img, refimg))
+    print("PSNR / SSIM against 64 samples:", {k: (round(a, 3), round(b, 4)) for k, (a, b) in scores.items()})
+    assert scores["learned"][0] > scores["filterLighting"][0] > scores["1 spp"][0]
+    assert scores["learned"][1] > scores["filterLighting"][1]
+    inf.close()

@@ -599,12 +599,12 @@ def test_offset_rays_at_zero_are_the_pixel_rays(oracle, scenes):
     t1 = oracle.trace_tree(scenes["1"], C, w, h).reshape(-1, 4)[xy[:, 1] * w + xy[:, 0]]
     t2 = oracle.trace_pixel_offsets(scenes["1"], C, xy, zero, w, h, single=True)
     assert np.array_equal(t1, t2)
-    # FP64 answers are stable under a 1e-4 pixel move (a tie lies within it
-    # for very few rays); the reference's FP32 arithmetic is not (its leaf
-    # Newton acceptance flips), which is why ties are judged on both runs
-    for fp64, most in ((True, 0.002), (False, 0.05)):
+    # FP64 answers are stable under a 1e-3 pixel move (a tie lies within it
+    # for few rays); the reference's FP32 arithmetic less so (its leaf Newton
+    # acceptance flips), which is why ties are judged on both runs
+    for fp64, most in ((True, 0.01), (False, 0.05)):
         base = oracle.trace_pixels(s, C, xy, w, h, fp64=fp64).reshape(-1, 4)
-        moved = oracle.trace_pixel_offsets(s, C, xy, zero + np.float32(1e-4), w, h, fp64=fp64)
+        moved = oracle.trace_pixel_offsets(s, C, xy, zero + np.float32(1e-3), w, h, fp64=fp64)
         changed = (moved[:, 1] != base[:, 1]).mean()
-        print("fp64" if fp64 else "fp32", "answers changed by a 1e-4 pixel move: %.5f" % changed)
+        print("fp64" if fp64 else "fp32", "answers changed by a 1e-3 pixel move: %.5f" % changed)
         assert 0 < changed < most
