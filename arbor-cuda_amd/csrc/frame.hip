@@ -15,6 +15,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <atomic>
+#include <mutex>
 
 #include "arbor_camera.hpp"
 #include "arbor_shade.hpp"
@@ -798,6 +799,51 @@ int arbor_sunVisibilityAccel(const float* P, const float* Nodes, const unsigned*
     if (rc != ARBOR_OK) return rc;
   }
   return launchVisibilityAccel(P, Nodes, Order, accel, C, Hit, Vis, width, height, order, cost, stream);
+}
+
+}  // extern "C"
+
+namespace {
+// Tile schedule of arbor_tracePixelsAccelNoAlloc: one per device, in a
+// file-scope device array (zero at module load, as a schedule must start),
+// for views of up to kNoAllocTiles 8x8 tiles. In host code `scratch` only
+// names the symbol; its device address comes from hipGetSymbolAddress, once
+// per device with that device current.
+constexpr int kNoAllocTiles = 512 * 512;  // 4096 x 4096 pixels
+constexpr size_t kNoAllocBytes = sizeof(TileSchedule) + 12 * (size_t)kNoAllocTiles;  // 3,148,288 bytes
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+void* scratchOnDevice() {
+  static std::mutex lock;
+  static void* ptr[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> g(lock);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    ptr[dev] = p;
+  }
+  return ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+long long arbor_no_alloc_scratch_bytes(void) { return (long long)kNoAllocBytes; }
+
+int arbor_tracePixelsAccelNoAlloc(const float* P, const float* Nodes, const unsigned* Order, const void* accel,
+                                  const float* C, float* Hit, int width, int height, hipStream_t stream) {
+  ARBOR_REQUIRE(P && Nodes && Order && accel && C && Hit);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  if ((long long)((width + 7) / 8) * ((height + 7) / 8) > kNoAllocTiles) return ARBOR_ERR_ARG;
+  void* schedule = scratchOnDevice();
+  if (!schedule) return ARBOR_ERR_DEVICE;
+  uint32_t* cost = nullptr;
+  const uint32_t* order = nullptr;
+  int rc = orderTiles(C, schedule, width, height, stream, &order, &cost);
+  if (rc != ARBOR_OK) return rc;
+  return launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, order, cost);
 }
 
 int arbor_shadePixelsAOV(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,

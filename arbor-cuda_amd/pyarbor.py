@@ -76,6 +76,8 @@ SIGNATURES = {
     "arbor_trace_accel_bytes": [],
     "arbor_buildTraceAccel": [_vp, _vp, _vp, _vp],
     "arbor_tracePixelsAccel": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
+    "arbor_no_alloc_scratch_bytes": [],
+    "arbor_tracePixelsAccelNoAlloc": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_sunVisibilityAccel": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "arbor_autotune_trace_accel": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, ct.c_char_p, _i],
     "arbor_shadePixelsVis": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -126,7 +128,7 @@ SIGNATURES = {
 _RESTYPES = {"arbor_version": ct.c_char_p, "arbor_engine_round_size": None, "arbor_engine_buffer": _vp,
              "arbor_engine_stream": _vp, "arbor_batch_buffer": _vp, "arbor_batch_pixels": ct.c_longlong,
              "arbor_tile_schedule_bytes": ct.c_longlong, "arbor_dn_wgrad_partial_floats": ct.c_longlong,
-             "arbor_trace_accel_bytes": ct.c_longlong,
+             "arbor_trace_accel_bytes": ct.c_longlong, "arbor_no_alloc_scratch_bytes": ct.c_longlong,
              "arbor_denoiser_parameters": ct.c_longlong, "arbor_denoiser_tensor": _vp}
 BATCH_TRACE_SHADE, BATCH_FILTER, BATCH_RESOLVE, BATCH_ALL = 1, 2, 4, 7
 

@@ -177,6 +177,18 @@ int arbor_tracePixelsAccel(const float* P, const float* Nodes, const unsigned* O
 int arbor_sunVisibilityAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,
                              const float* C, const float* Hit, float* Vis, int width, int height, void* schedule,
                              hipStream_t stream);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). arbor_tracePixelsAccel with a tile schedule, the schedule kept
+ * in the library's own device array (arbor_no_alloc_scratch_bytes() bytes per
+ * GPU, held until the process ends) instead of a caller workspace; it
+ * allocates nothing. Views of up to 512 x 512 tiles (4096 x 4096 pixels);
+ * larger views return ARBOR_ERR_ARG. Hit equals arbor_tracePixelsAccel's.
+ * Every call on one GPU shares its array (the previous call's tile times seed
+ * the order, whatever its view), so calls that overlap on one GPU race. */
+long long arbor_no_alloc_scratch_bytes(void);
+int arbor_tracePixelsAccelNoAlloc(const float* P, const float* Nodes, const unsigned* Order, const void* accel,
+                                  const float* C, float* Hit, int width, int height, hipStream_t stream);
 
 /* Single-specimen primary rays: traceTree in tree-local space with the ground
  * plane as initial best and no forest instancing (the single-tree workload of
