@@ -1,6 +1,7 @@
 // Batched multi-view rendering on gfx950: many camera views of uneven
-// resolution traced and shaded in ONE launch, then filtered and resolved per
-// view, the whole sequence captured once into a hipGraph and replayed.
+// resolution traced and shaded in ONE launch, then filtered and resolved in
+// one more over every view's pixels, the whole sequence captured once into a
+// hipGraph and replayed.
 //
 // Layout: a view table {width, height, tilesX, tileBase, pixelBase} and one
 // C[64] per view in device memory; Hit/Linear/Guide/Filtered/History are
@@ -20,6 +21,7 @@
 #include <vector>
 
 #include "arbor_batch.h"
+#include "arbor_filter.hpp"
 #include "arbor_shade.hpp"
 #include "launch.hpp"
 #include "tuning.hpp"
@@ -27,7 +29,8 @@
 namespace arbor {
 
 struct ViewDesc {
-  int width, height, tilesX, tileBase, pixelBase, pad0, pad1, pad2;
+  int width, height, tilesX, tileBase, pixelBase;
+  int filterBase, filterX, pad;  // first 32x8 filter block of the view, blocks per row
 };
 constexpr int kMaxViews = ARBOR_BATCH_MAX_VIEWS;
 
@@ -165,6 +168,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kTraceVis ?
   }
 }
 
+// filterLighting (and with kResolve resolvePixels) over every view's pixels
+// in one launch: one 32x8 block of one view per workgroup, found by a binary
+// search of the view table on the block index (uniform per block); each
+// view's pixels get the single-view kernels' arithmetic.
+__device__ __forceinline__ int findFilterView(const ViewDesc* views, int n, int block) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {  // last view with filterBase <= block
+    int mid = (lo + hi + 1) >> 1;
+    if (views[mid].filterBase <= block) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+template <bool kResolve>
+__global__ __launch_bounds__(kFilterW * kFilterH) void k_filterViews(const float4* Linear, const float4* Guide,
+                                                                    const float4* Hit, float4* Filtered,
+                                                                    const float* Cs, const ViewDesc* views,
+                                                                    int nviews, FilterWeights fw, float4* History,
+                                                                    uint32_t* Pixels) {
+  const int v = findFilterView(views, nviews, (int)blockIdx.x);
+  const ViewDesc d = views[v];
+  const int lb = (int)blockIdx.x - d.filterBase;
+  const size_t o = (size_t)d.pixelBase;
+  filterBlock<kResolve>(Linear + o, Guide + o, Hit + o, Filtered + o, Cs + 64 * v, d.width, d.height, fw,
+                        History + o, Pixels + o, lb % d.filterX, lb / d.filterX);
+}
+// resolvePixels alone over every view's pixels, in the same blocks.
+__global__ __launch_bounds__(kFilterW * kFilterH) void k_resolveViews(const float4* Filtered, float4* History,
+                                                                     uint32_t* Pixels, const float* Cs,
+                                                                     const ViewDesc* views, int nviews) {
+  const int v = findFilterView(views, nviews, (int)blockIdx.x);
+  const ViewDesc d = views[v];
+  const int lb = (int)blockIdx.x - d.filterBase;
+  const int x = (lb % d.filterX) * kFilterW + threadIdx.x, y = (lb / d.filterX) * kFilterH + threadIdx.y;
+  if (x >= d.width || y >= d.height) return;
+  const size_t o = (size_t)d.pixelBase;
+  const int i = y * d.width + x;
+  const float4 f = Filtered[o + i];
+  resolveOne(mk(f.x, f.y, f.z), History + o, Pixels + o, Cs + 64 * v, x, y, d.width, d.height, i);
+}
+
 typedef void (*BatchKernel)(const float*, const float4*, const uint32_t*, const float*, const ViewDesc*, int, int,
                             float4*, float4*, float4*, const float4*);
 struct BatchVariant {
@@ -209,6 +253,7 @@ struct arbor_batch {
   int nviews = 0;
   long long npixels = 0;
   int totalTiles = 0;
+  int filterBlocks = 0;  // 32x8 filter blocks over every view
   std::vector<ViewDesc> host;
   ViewDesc* dViews = nullptr;
   float* dC = nullptr;
@@ -263,22 +308,18 @@ int issue(arbor_batch* b, const float* P, const float* Nodes, const unsigned* Or
       if ((rc = launch_status()) != ARBOR_OK) return rc;
     }
   }
-  for (int i = 0; i < b->nviews; i++) {
-    const ViewDesc& d = b->host[i];
-    const float* C = b->dC + 64 * i;
-    size_t o = (size_t)d.pixelBase;
-    if (stages & ARBOR_BATCH_FILTER) {
-      int rc = arbor_filterLighting(reinterpret_cast<const float*>(b->Linear + o),
-                                    reinterpret_cast<const float*>(b->Guide + o),
-                                    reinterpret_cast<const float*>(b->Hit + o), reinterpret_cast<float*>(b->Filtered + o),
-                                    C, d.width, d.height, s);
-      if (rc != ARBOR_OK) return rc;
-    }
-    if (stages & ARBOR_BATCH_RESOLVE) {
-      int rc = arbor_resolvePixels(reinterpret_cast<const float*>(b->Filtered + o), reinterpret_cast<float*>(b->History + o),
-                                   b->Pixels + o, C, d.width, d.height, s);
-      if (rc != ARBOR_OK) return rc;
-    }
+  const bool filter = (stages & ARBOR_BATCH_FILTER) != 0, resolve = (stages & ARBOR_BATCH_RESOLVE) != 0;
+  if (filter) {
+    hipLaunchKernelGGL(resolve ? k_filterViews<true> : k_filterViews<false>, dim3(b->filterBlocks),
+                       dim3(kFilterW, kFilterH), 0, s, b->Linear, b->Guide, b->Hit, b->Filtered, b->dC, b->dViews,
+                       b->nviews, filterWeights(), b->History, b->Pixels);
+    int rc = launch_status();
+    if (rc != ARBOR_OK) return rc;
+  } else if (resolve) {
+    hipLaunchKernelGGL(k_resolveViews, dim3(b->filterBlocks), dim3(kFilterW, kFilterH), 0, s, b->Filtered,
+                       b->History, b->Pixels, b->dC, b->dViews, b->nviews);
+    int rc = launch_status();
+    if (rc != ARBOR_OK) return rc;
   }
   return ARBOR_OK;
 }
@@ -335,7 +376,7 @@ int arbor_batch_set_views(arbor_batch* b, int nviews, const float* cameras, cons
   if (nviews < 1 || nviews > b->maxViews) return ARBOR_ERR_ARG;
   std::vector<ViewDesc> h(nviews);
   long long pix = 0;
-  int tiles = 0;
+  int tiles = 0, fblocks = 0;
   for (int i = 0; i < nviews; i++) {
     int w = widths[i], hh = heights[i];
     if (!dims_ok(w, hh)) return ARBOR_ERR_ARG;
@@ -347,6 +388,9 @@ int arbor_batch_set_views(arbor_batch* b, int nviews, const float* cameras, cons
     d.tilesX = (w + 7) / 8;
     d.tileBase = tiles;
     d.pixelBase = (int)pix;
+    d.filterBase = fblocks;
+    d.filterX = (w + kFilterW - 1) / kFilterW;
+    fblocks += d.filterX * ((hh + kFilterH - 1) / kFilterH);
     tiles += d.tilesX * ((hh + 7) / 8);
     pix += (long long)w * hh;
     h[i] = d;
@@ -359,6 +403,7 @@ int arbor_batch_set_views(arbor_batch* b, int nviews, const float* cameras, cons
   b->nviews = nviews;
   b->npixels = pix;
   b->totalTiles = tiles;
+  b->filterBlocks = fblocks;
   dropGraph(b);  // the captured sequence depends on the layout
   return ARBOR_OK;
 }

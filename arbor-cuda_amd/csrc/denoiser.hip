@@ -527,6 +527,9 @@ int arbor_denoiser_save(const arbor_denoiser* d, const char* path) {
   return ok ? ARBOR_OK : ARBOR_ERR_DEVICE;
 }
 
+// The whole file is read and checked (magic, layer count, every layer's shape
+// and values, nothing after the last layer) before any device weight changes,
+// so a truncated or mismatched checkpoint leaves the network as it was.
 int arbor_denoiser_load(arbor_denoiser* d, const char* path) {
   ARBOR_REQUIRE(d && path);
   FILE* f = fopen(path, "rb");
@@ -535,17 +538,27 @@ int arbor_denoiser_load(arbor_denoiser* d, const char* path) {
   int32_t layers = 0;
   bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "ARBORDN1", 8) == 0 && fread(&layers, 4, 1, f) == 1 &&
             layers == (int32_t)d->conv.size();
+  std::vector<std::vector<float>> ws, bs;
   for (size_t li = 0; ok && li < d->conv.size(); li++) {
-    Conv& cv = d->conv[li];
+    const Conv& cv = d->conv[li];
     int32_t dims[2];
     ok = fread(dims, 4, 2, f) == 2 && dims[0] == cv.cin && dims[1] == cv.cout;
-    std::vector<float> w((size_t)cv.nw()), b((size_t)cv.cout);
-    ok = ok && fread(w.data(), 4, w.size(), f) == w.size() && fread(b.data(), 4, b.size(), f) == b.size();
-    ok = ok && hipMemcpy(cv.w, w.data(), w.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(cv.b, b.data(), b.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) break;
+    ws.emplace_back((size_t)cv.nw());
+    bs.emplace_back((size_t)cv.cout);
+    ok = fread(ws.back().data(), 4, ws.back().size(), f) == ws.back().size() &&
+         fread(bs.back().data(), 4, bs.back().size(), f) == bs.back().size();
   }
+  char extra;
+  ok = ok && fread(&extra, 1, 1, f) == 0;
   fclose(f);
   if (!ok) return ARBOR_ERR_ARG;
+  for (size_t li = 0; li < d->conv.size(); li++) {
+    Conv& cv = d->conv[li];
+    if (hipMemcpy(cv.w, ws[li].data(), ws[li].size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(cv.b, bs[li].data(), bs[li].size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return ARBOR_ERR_DEVICE;
+  }
   int rc = refreshBf16(d, nullptr);
   if (rc == ARBOR_OK && hipDeviceSynchronize() != hipSuccess) rc = ARBOR_ERR_DEVICE;
   return rc;

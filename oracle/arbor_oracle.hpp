@@ -663,11 +663,14 @@ template <class R> struct Arbor {
   // distance |q|; tests that end in such a discriminant are padded by
   // 1 mm + 4e-3*|q|. The leaf slab (boxSpan) and the wood SDF march (which
   // must reach within 2.5e-4 m of the surface) are well conditioned: 1 mm.
+  // Test infrastructure: scales both pads (1 = the GPU's; a test lowers it to
+  // show that the grazing-ray stress test catches a too-tight rejection).
+  static inline R cullPadScale = R(1);
   static bool sphereMiss(F3 ro, F3 rd, F3 centre, R radius, R padPerMetre) {
     F3 q = ro - centre;
     R b = dot(q, rd);
     F3 p = q - rd * b;
-    R r = radius + K(0.001) + padPerMetre * len(q);
+    R r = radius + (K(0.001) + padPerMetre * len(q)) * cullPadScale;
     return dot(p, p) > r * r || b > r;
   }
 

@@ -315,6 +315,97 @@ int orc_cull_mismatches(const float* P, const float* Nodes, const unsigned* Orde
   return bad;
 }
 
+// Grazing-ray stress of the bounding-sphere rejections in woodT / leafT:
+// raysPerObject rays per wood and leaf object, from a distance log-uniform in
+// [1, 500] m along a seeded direction, aimed either at a seeded point near
+// the object's surface (a capsule point of the wood segment; a point of the
+// leaf's Newton slab or petiole, swayed by the wind) jittered by up to
+// 1 mm + 2e-3 * distance, or at a point 0.9-1.1 times the padded radius of
+// one of its rejection spheres from the centre. Each ray's objectT, refined
+// (primary rays) and not (shadow rays), with the rejections off and on, at a
+// footprint of 8.33e-4 * distance (1080p pixels). out[0] = rays, out[1] =
+// rays with a hit without the rejections, out[2] = rays whose results differ
+// in any bit (0 expected), out[3] = the nearest distance (m) of a ray whose
+// hit a rejection removed (0 if none).
+void orc_set_cull_pad_scale(float s) { Arbor<float>::cullPadScale = s; }
+int orc_cull_stress(const float* P, int raysPerObject, int seed, float time, float wind, long long* out) {
+  using A = Arbor<float>;
+  const int objects = WOOD_COUNT + LEAF_COUNT;
+  const size_t n = (size_t)objects * raysPerObject;
+  struct Ray { V3<float> ro, rd; float fp, dist; int id; };
+  std::vector<Ray> rays(n);
+#pragma omp parallel for schedule(static)
+  for (int id = 0; id < objects; id++) {
+    const int b = id * PS;
+    for (int k = 0; k < raysPerObject; k++) {
+      auto r = [&](int j) { return A::rnd((id * raysPerObject + k) * 16 + j, 7001 + seed); };
+      const float dist = std::exp(std::log(500.0f) * r(0));
+      V3<float> target, centre;
+      float radius;
+      if (id < WOOD_COUNT) {
+        V3<float> a = A::ld3(P, b), e = A::ld3(P, b + 4), half = (e - a) * 0.5f;
+        centre = a + half;
+        radius = A::len(half) + std::fmax(P[b + 7], P[b + 11]);
+        const float s = r(1);
+        V3<float> axis = A::unit(e - a);
+        V3<float> side = A::unit(A::cross(axis, A::unit(A::c3(r(2) - 0.5f, r(3) - 0.5f, r(4) - 0.5f))));
+        target = a + (e - a) * s + side * (P[b + 7] + (P[b + 11] - P[b + 7]) * s);
+      } else {
+        V3<float> u = A::leafU(P, id, time, wind), nn = A::leafN(P, id, time, wind), v = A::ld3(P, b + 8);
+        const float stem = P[b + 16], ln = P[b + 7], w = P[b + 11];
+        if (r(1) < 0.25f) {  // petiole
+          centre = A::ld3(P, b) + u * (stem * 0.5f);
+          radius = stem * 0.5f + 0.0011f;
+          target = A::ld3(P, b) + u * (stem * r(2));
+        } else {  // Newton slab
+          const float hx = ln * 0.5f + 0.001f;
+          centre = A::ld3(P, b) + u * (stem + ln * 0.5f) + nn * 0.015f;
+          radius = std::sqrt(hx * hx + w * w + 0.05f * 0.05f);
+          target = A::ld3(P, b) + u * (stem + ln * r(2)) + v * (w * (2.0f * r(3) - 1.0f)) + nn * (0.1f * r(4) - 0.035f);
+        }
+      }
+      if (r(5) < 0.5f) {
+        const float jit = 0.001f + 0.002f * dist;
+        target = target + A::c3(r(6) - 0.5f, r(7) - 0.5f, r(8) - 0.5f) * (2.0f * jit);
+      } else {
+        const float padded = radius + 0.001f + 0.004f * dist;
+        target = centre + A::unit(A::c3(r(6) - 0.5f, r(7) - 0.5f, r(8) - 0.5f)) * (padded * (0.9f + 0.2f * r(9)));
+      }
+      Ray& ray = rays[(size_t)id * raysPerObject + k];
+      ray.rd = A::unit(A::c3(r(10) - 0.5f, r(11) - 0.5f, r(12) - 0.5f));
+      ray.ro = target - ray.rd * dist;
+      ray.fp = 8.33e-4f * dist;
+      ray.dist = dist;
+      ray.id = id;
+    }
+  }
+  std::vector<float> res[2] = {std::vector<float>(2 * n), std::vector<float>(2 * n)};
+  for (int on = 0; on < 2; on++) {
+    A::cullSpheres = on != 0;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long i = 0; i < (long long)n; i++) {
+      const Ray& ray = rays[i];
+      res[on][2 * i] = A::objectT(P, ray.id, ray.ro, ray.rd, ray.fp, time, wind, true);
+      res[on][2 * i + 1] = A::objectT(P, ray.id, ray.ro, ray.rd, ray.fp, time, wind, false);
+    }
+  }
+  A::cullSpheres = false;
+  long long hits = 0, bad = 0;
+  float worst = 0.0f;
+  for (size_t i = 0; i < n; i++) {
+    if (res[0][2 * i] < A::FAR() || res[0][2 * i + 1] < A::FAR()) hits++;
+    if (std::memcmp(&res[0][2 * i], &res[1][2 * i], 2 * sizeof(float)) != 0) {
+      bad++;
+      worst = worst == 0.0f ? rays[i].dist : std::fmin(worst, rays[i].dist);
+    }
+  }
+  out[0] = (long long)n;
+  out[1] = hits;
+  out[2] = bad;
+  out[3] = (long long)worst;
+  return 0;
+}
+
 // Reference walk vs lean walk over whole frames: returns the number of
 // pixels whose Hit differs in any bit (0 expected).
 // The SAH hierarchy of the GPU's accel walk over this heap BVH (the

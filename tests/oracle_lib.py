@@ -197,6 +197,20 @@ class Oracle(_Common):
         self._band_object_mismatch = _bind(L, "orc_band_skip_object_mismatches", _i, [_f32p, _i])
         self._wind_mismatch = _bind(L, "orc_wind_shortcut_mismatches", _i, [_f32p, _f32p, _u32p, _f32p, _i, _i])
         self._generate_cr = _bind(L, "orc_generate_cr", _i, [_i, _f32p, _f32p, _u32p, _u32p, _f32p])
+        self._cull_stress = _bind(L, "orc_cull_stress", _i,
+                                  [_f32p, _i, _i, _f, _f, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")])
+        self._cull_pad_scale = _bind(L, "orc_set_cull_pad_scale", None, [_f])
+
+    def cull_stress(self, P, rays_per_object, seed, time, wind):
+        """Grazing-ray stress of the rejection spheres (flops.cpp orc_cull_stress):
+        (rays, rays with a hit, rays whose results differ, nearest distance of a
+        removed hit)."""
+        out = np.zeros(4, np.int64)
+        assert self._cull_stress(P, rays_per_object, seed, time, wind, out) == 0
+        return tuple(int(v) for v in out)
+
+    def set_cull_pad_scale(self, scale):
+        self._cull_pad_scale(scale)
 
     def generate_cr(self, seed):
         """The build with correctly rounded sin/cos/pow instead of glibc's."""

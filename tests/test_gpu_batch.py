@@ -183,3 +183,31 @@ def test_batched_views_workload_full_size(px, scene, oracle):
         h64 = oracle.trace_pixels(ref, C, xy, W, H, fp64=True)
         check_hits("batch view %d %dx%d (sampled)" % (i, W, H), gpu, h32, h64, (oracle, ref, C, W, H, xy))
     b.close()
+
+
+def test_batch_rejects_mixed_worlds_and_launch_count(px, scene):
+    """All views of a batch see one forest: a view whose instancing seed C[31]
+    differs is rejected. A batched render (default split variant) is four
+    kernel launches whatever the number of views: trace, sun visibility,
+    shading, and filter + resolve over every view's pixels (profiler count)."""
+    import torch
+    from torch.profiler import ProfilerActivity, profile
+
+    print("case: batch world-seed check and launch count", flush=True)
+    cams, sizes = _views(px, 8)
+    b = px.Batch(8, sum(w * h for w, h in sizes))
+    bad = cams.copy()
+    bad[5, 31] = 8.0
+    with pytest.raises(px.ArborError):
+        b.set_views(bad, sizes)
+    b.set_views(cams, sizes)
+    b.render(scene)
+    torch.cuda.synchronize()
+    with profile(activities=[ProfilerActivity.CUDA]) as prof:
+        b.render(scene)
+        torch.cuda.synchronize()
+    names = [e.name for e in prof.events() if e.device_type == torch.autograd.DeviceType.CUDA
+             and "memcpy" not in e.name.lower() and "memset" not in e.name.lower()]
+    print(len(names), "kernels:", names)
+    assert px.tune_get("batch_variant") in (5, 6) and len(names) == 4
+    b.close()

@@ -223,3 +223,36 @@ def test_checkpoint_on_1080p_tracer_frame(env):
     assert scores["learned"][0] > scores["filterLighting"][0] > scores["1 spp"][0]
     assert scores["learned"][1] > scores["filterLighting"][1]
     inf.close()
+
+
+def test_bad_checkpoint_leaves_network_unchanged(env, tmp_path):
+    """A truncated checkpoint, or one with bytes after its last layer, is
+    rejected before any weight changes: the network's output stays bit for
+    bit what it was."""
+    torch, px = env
+    print("case: denoiser truncated / over-long checkpoint", flush=True)
+    dn = px.Denoiser(N, H, W, train=True, seed=7)
+    lin, gui, nor, hit, _ = _frames(torch, 3)
+    for i in range(N):
+        dn.set_input(i, lin[i], gui[i], nor[i], hit[i], w_img, h_img)
+    good = tmp_path / "good.bin"
+    dn.save(good)
+    dn.forward(N)
+    want = _output(dn, torch)
+    data = good.read_bytes()
+    other = px.Denoiser(N, H, W, train=True, seed=8)
+    other.save(tmp_path / "other.bin")
+    odata = (tmp_path / "other.bin").read_bytes()
+    for name, blob in (("trunc", odata[: len(odata) * 3 // 4]), ("long", odata + b"\0"), ("short", odata[:10])):
+        path = tmp_path / (name + ".bin")
+        path.write_bytes(blob)
+        with pytest.raises(px.ArborError):
+            dn.load(path)
+        dn.forward(N)
+        assert torch.equal(_output(dn, torch), want), name
+    assert len(data) == len(odata)
+    dn.load(tmp_path / "other.bin")
+    dn.forward(N)
+    assert not torch.equal(_output(dn, torch), want)
+    dn.close()
+    other.close()

@@ -389,8 +389,8 @@ def test_engine_render_mode_and_ppm(px, oracle, tmp_path):
         (d <= 1).mean(), (d <= 2).mean(), d.max(), int((d > 1).sum())))
     # four samples, each with its own near-tie hits and shadow rays: the
     # pixels off by more than one step are those a differing hit or
-    # visibility reaches through the filter
-    assert (d <= 1).mean() >= 0.997
+    # visibility reaches through the filter (measured 12 of 61,440)
+    assert (d <= 1).mean() >= 0.999
     path = tmp_path / "render.ppm"
     eng.save_ppm(path)
     data = path.read_bytes()

@@ -468,6 +468,27 @@ def test_sphere_culls_change_nothing(oracle, scenes, view, wind):
     assert oracle.cull_mismatches(scenes["7"], C, 200, 120) == 0
 
 
+@pytest.mark.parametrize("time,wind", [(0.0, 0.0), (1.3, 0.6)])
+def test_sphere_culls_exact_on_grazing_rays(oracle, scenes, time, wind):
+    """The rejection spheres under stress: 32 rays per wood and leaf object
+    (938,112 rays) from 1-500 m, aimed near the object's surface or at the
+    padded sphere's edge, each tested refined (primary) and not (shadow), with
+    the rejections off and on: every result bit-identical. With the pads cut
+    to a tenth (a planted mistake in the oracle) the same rays find culled
+    hits, so the shipped pads have a measured margin of 5-10x."""
+    P = scenes["7"].P
+    rays, hits, bad, _ = oracle.cull_stress(P, 32, 1, time, wind)
+    print("rays %d, with a hit %d, differing %d" % (rays, hits, bad))
+    assert rays == 32 * (5124 + 24192) and hits > 100000 and bad == 0
+    try:
+        oracle.set_cull_pad_scale(0.1)
+        _, _, bad_tight, nearest = oracle.cull_stress(P, 32, 1, time, wind)
+    finally:
+        oracle.set_cull_pad_scale(1.0)
+    print("pads x0.1: %d rays lose a hit (nearest at %d m)" % (bad_tight, nearest))
+    assert bad_tight > 20
+
+
 @pytest.mark.parametrize("frame,view", [(0, 0), (40, 0), (0, 2), (0, 4)])
 def test_early_exits_change_nothing(oracle, scenes, frame, view):
     """woodT/leafT stopping once their answer is known to be >= the caller's
