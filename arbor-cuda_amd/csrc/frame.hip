@@ -19,6 +19,7 @@
 
 #include "arbor_camera.hpp"
 #include "arbor_filter.hpp"
+#include "arbor_persist.hpp"
 #include "arbor_shade.hpp"
 #include "arbor_wavefront.hpp"
 #include "engine_passes.hpp"
@@ -131,8 +132,9 @@ constexpr int kOrderBlock = 256;
 struct TileSchedule {
   float prevC[64];
   uint32_t valid;
-  uint32_t parity;  // accumulator set of the next call
-  uint32_t pad[62];
+  uint32_t parity;      // accumulator set of the next call
+  uint32_t persist[2];  // tile counter and finished-wave count of the persistent trace (zero between launches)
+  uint32_t pad[60];
   unsigned long long acc[2][kOrderBuckets];  // low word: tiles in the bucket; high word: scatter cursor
   // followed by cost[tiles], order[tiles], key[tiles]
 };
@@ -414,10 +416,13 @@ inline int tileGrid(int tiles, int wavesPerBlock) {
 
 typedef void (*TraceKernel)(const float*, const float4*, const uint32_t*, const float*, float4*, int, int, int, int,
                             const uint32_t*, uint32_t*, const float4*);
+typedef void (*PersistKernel)(const float*, const float4*, const float*, float4*, int, int, int, int,
+                              const uint32_t*, uint32_t*, const float4*, uint32_t*);
 struct TraceVariant {
   TraceKernel forest, tree;
   int waves;
-  int ldsPerWave = 0;  // dynamic LDS bytes per wave (the SAH walk's stack entries)
+  int ldsPerWave = 0;              // dynamic LDS bytes per wave (the SAH walk's stack entries)
+  PersistKernel persist = nullptr;  // the persistent form (forest and tree unused)
 };
 // trace_variant index -> kernels. Identical results for every entry.
 const TraceVariant kTraceVariants[] = {
@@ -470,14 +475,59 @@ const TraceVariant kAccelTraceVariants[] = {
     {k_tracePixels<2, WalkSahL<8>, 4>, k_traceTreePixels<2, WalkSahL<8>>, 2, sahLdsPerWave(8)},
     {k_tracePixels<2, WalkSahG<8, 6>>, k_traceTreePixels<2, WalkSahG<8, 6>>, 2, sahLdsPerWave(8)},
     {k_tracePixels<2, WalkSahG<8, 5>, 4>, k_traceTreePixels<2, WalkSahG<8, 5>>, 2, sahLdsPerWave(8)},
+    {nullptr, nullptr, 2, sahLdsPerWave(8), k_tracePersistent<2, WalkSahG<8, 6>>},
+    {nullptr, nullptr, 1, sahLdsPerWave(8), k_tracePersistent<1, WalkSahG<8, 6>>},
+    {nullptr, nullptr, 2, sahLdsPerWave(8), k_tracePersistent<2, WalkSahG<8, 5>>},
 };
 constexpr int kAccelTraceVariantCount = sizeof(kAccelTraceVariants) / sizeof(kAccelTraceVariants[0]);
 
+// Counters of persistent launches without a tile schedule: one pair per
+// device in a device array (zero at module load; each launch leaves them at
+// zero), its device address from hipGetSymbolAddress once per device.
+static __device__ uint32_t persistCounters[2];
+uint32_t* unscheduledCounters() {
+  static std::mutex lock;
+  static void* ptr[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> g(lock);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(persistCounters)) != hipSuccess || !p) return nullptr;
+    ptr[dev] = p;
+  }
+  return static_cast<uint32_t*>(ptr[dev]);
+}
+// Blocks of a persistent launch: as many as the device holds at once.
+int residentBlocks(PersistKernel k, int waves, int ldsBytes) {
+  int perCU = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(k), waves * 64, ldsBytes) !=
+          hipSuccess ||
+      perCU < 1)
+    perCU = 1;
+  return perCU * deviceCUs();
+}
+
+void scheduleArrays(void* schedule, int width, int height, const uint32_t** order, uint32_t** cost);
+
+// The SAH-walk trace. schedule: an ordered tile schedule (its order and cost
+// arrays, and the persistent kernels' counters), or null.
 int launchTraceAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel, const float* C,
-                     float* Hit, int width, int height, hipStream_t stream, const uint32_t* tileOrder,
-                     uint32_t* tileCost) {
+                     float* Hit, int width, int height, hipStream_t stream, void* schedule) {
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  const uint32_t* tileOrder = nullptr;
+  uint32_t* tileCost = nullptr;
+  if (schedule) scheduleArrays(schedule, width, height, &tileOrder, &tileCost);
   const TraceVariant& v = kAccelTraceVariants[tuning().accel_variant];
+  if (v.persist) {
+    uint32_t* counters = schedule ? static_cast<TileSchedule*>(schedule)->persist : unscheduledCounters();
+    if (!counters) return ARBOR_ERR_DEVICE;
+    const int lds = v.ldsPerWave * v.waves;
+    hipLaunchKernelGGL(v.persist, dim3(residentBlocks(v.persist, v.waves, lds)), dim3(v.waves * 64), lds, stream, P,
+                       reinterpret_cast<const float4*>(Nodes), C, reinterpret_cast<float4*>(Hit), width, height, tx,
+                       tiles, tileOrder, tileCost, static_cast<const float4*>(accel), counters);
+    return launch_status();
+  }
   hipLaunchKernelGGL(v.forest, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), v.ldsPerWave * v.waves, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
                      tx, tiles, tileOrder, tileCost, static_cast<const float4*>(accel));
@@ -683,7 +733,7 @@ int arbor_tracePixelsAccel(const float* P, const float* Nodes, const unsigned* O
     int rc = orderTiles(C, schedule, width, height, stream, &order, &cost);
     if (rc != ARBOR_OK) return rc;
   }
-  return launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, order, cost);
+  return launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, schedule);
 }
 
 int arbor_sunVisibilityAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,
@@ -742,7 +792,7 @@ int arbor_tracePixelsAccelNoAlloc(const float* P, const float* Nodes, const unsi
   const uint32_t* order = nullptr;
   int rc = orderTiles(C, schedule, width, height, stream, &order, &cost);
   if (rc != ARBOR_OK) return rc;
-  return launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, order, cost);
+  return launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, schedule);
 }
 
 int arbor_shadePixelsAOV(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
@@ -908,10 +958,10 @@ int arbor_autotune_trace_accel(const float* P, const float* Nodes, const unsigne
   int best = 0, rc = ARBOR_OK;
   for (int v = 0; v < kAccelTraceVariantCount && rc == ARBOR_OK; v++) {
     t.accel_variant = v;
-    rc = launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, nullptr, nullptr);
+    rc = launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, nullptr);
     (void)hipEventRecord(e0, stream);
     for (int r = 0; r < reps && rc == ARBOR_OK; r++)
-      rc = launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, nullptr, nullptr);
+      rc = launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, nullptr);
     (void)hipEventRecord(e1, stream);
     if (hipEventSynchronize(e1) != hipSuccess) rc = ARBOR_ERR_DEVICE;
     (void)hipEventElapsedTime(&ms[v], e0, e1);
@@ -950,7 +1000,7 @@ int traceOrdered(const float* P, const float* Nodes, const unsigned* Order, cons
   const uint32_t* order = nullptr;
   uint32_t* cost = nullptr;
   scheduleArrays(schedule, width, height, &order, &cost);
-  if (accel) return launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, order, cost);
+  if (accel) return launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, schedule);
   return launchTrace(true, P, Nodes, Order, C, Hit, width, height, stream, order, cost);
 }
 

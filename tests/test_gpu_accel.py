@@ -17,6 +17,9 @@ sun-visibility passes over it.
 import numpy as np
 import pytest
 
+# accel_variant 0-9: tile-per-wave SAH kernels; 10-12: the persistent form
+ACCEL_VARIANTS = 13
+
 from accel_check import check_accel_structure
 from gpu_util import require_gpu, to_dev, to_host
 
@@ -126,7 +129,7 @@ def test_accel_launch_shapes_and_schedule_change_nothing(px, oracle, scene):
         px.trace_accel(dev, acc, C, ref_fb)
         want = to_host(ref_fb.Hit)
         sched = px.TileSchedule(W, H)
-        for v in range(10):
+        for v in range(ACCEL_VARIANTS):
             assert lib.arbor_tune_set(b"accel_variant", v) == 0
             for s in (None, sched, sched):  # unscheduled, first and second scheduled call
                 fb = px.FrameBuffers(W, H)
@@ -230,7 +233,7 @@ def test_accel_autotune(px, oracle, scene):
         px.trace_accel(dev, acc, C, ref)
         fb = px.FrameBuffers(W, H)
         r = px.autotune_trace_accel(dev, acc, C, fb, reps=1)
-        assert len(r["ms"]) == 10 and r["best"] == int(np.argmin(r["ms"]))
+        assert len(r["ms"]) == ACCEL_VARIANTS and r["best"] == int(np.argmin(r["ms"]))
         assert lib.arbor_tune_get(b"accel_variant") == r["best"]
         px.trace_accel(dev, acc, C, fb)
         assert np.array_equal(to_host(fb.Hit).view(np.uint32), to_host(ref.Hit).view(np.uint32))
