@@ -1,17 +1,20 @@
-// The learned denoiser's network: a 5-level 3x3-convolution encoder-decoder
-// (U-Net) with 32/48/64/96/128 channels, BF16 activations and weights, FP32
-// accumulation, master weights and Adam state, trained with an L1 loss on
-// log(1 + radiance), and used for inference on whole frames.
+// The learned denoiser's network: an L-level 3x3-convolution encoder-decoder
+// (U-Net, L = 1..5; by default 5 levels of 32/48/64/96/128 channels), BF16
+// activations and weights, FP32 accumulation, master weights and Adam state,
+// trained with an L1 loss on log(1 + radiance), and used for inference on
+// whole frames.
 //
 // Input per pixel (16 channels, 10 used): log(1 + 1-spp radiance) rgb,
 // albedo rgb (Guide), shading normal xyz, and depth log(1 + min(t, 1000))/7
 // (1 for sky). Output: log(1 + radiance) rgb.
 //
-// Graph (s = ReLU conv pairs kept as skips, p = 2x2 average pool, u = nearest
-// 2x upsample, c = channel concatenation [u, s]):
+// Graph of the 5-level net (s = ReLU conv pairs kept as skips, p = 2x2
+// average pool, u = nearest 2x upsample, c = channel concatenation [u, s]):
 //   in -> L0 -> L1 = s0 -> p -> L2 -> L3 = s1 -> p -> L4 -> L5 = s2 -> p ->
 //   L6 -> L7 = s3 -> p -> L8 -> L9 = e4 -> u,c(s3) -> L10 -> L11 -> u,c(s2) ->
 //   L12 -> L13 -> u,c(s1) -> L14 -> L15 -> u,c(s0) -> L16 -> L17 -> L18 (linear)
+// With L levels: encoder layers 0..2L-1 (two per level), decoder layers
+// 2L..4L-3 (two per level above the bottom), output layer 4L-2.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -29,8 +32,8 @@ namespace {
 
 using bf16 = __bf16;
 
-constexpr int kInC = 16, kOutC = 16, kLevels = 5;
-constexpr int kWidths[kLevels] = {32, 48, 64, 96, 128};
+constexpr int kInC = 16, kOutC = 16, kMaxLevels = 5;
+constexpr int kDefaultWidths[kMaxLevels] = {32, 48, 64, 96, 128};
 constexpr float kFar = 100000.0f;
 
 // ---- data kernels ----
@@ -154,9 +157,15 @@ struct arbor_denoiser {
   int maxBatch = 0, H = 0, W = 0;
   bool train = false;
   int step = 0;
+  int levels = kMaxLevels;
+  int widths[kMaxLevels] = {};
   std::vector<Conv> conv;
-  // activations: index by name
-  Act in, a[9], s[4], p[4], e4, u[4], c[4], dA[4], d[4], out;
+  // activations (level k): a = first conv of level k, s = skip (second conv
+  // of level k < L-1), e4 = bottleneck (second conv of level L-1), p = pooled
+  // skip, u / c = upsampled / concatenated decoder input, dA / d = the
+  // decoder's two convs
+  Act in, a[kMaxLevels], s[kMaxLevels - 1], p[kMaxLevels - 1], e4, u[kMaxLevels - 1], c[kMaxLevels - 1],
+      dA[kMaxLevels - 1], d[kMaxLevels - 1], out;
   uint16_t* gpool = nullptr;  // pool-gradient temporary (largest: level 0 x 32)
   float* target = nullptr;    // [maxBatch][H][W] float4
   float* partial = nullptr;
@@ -189,23 +198,27 @@ struct arbor_denoiser {
 
 namespace {
 
-// Layer table: (cin, cout, level, relu) in graph order L0..L18.
+// Layer table: (cin, cout, level, relu) in graph order: the encoder pairs,
+// the decoder pairs (deepest first), the linear output layer.
 void buildLayers(arbor_denoiser* d) {
-  const int* c = kWidths;
-  int spec[19][4] = {
-      {kInC, c[0], 0, 1}, {c[0], c[0], 0, 1}, {c[0], c[1], 1, 1}, {c[1], c[1], 1, 1}, {c[1], c[2], 2, 1},
-      {c[2], c[2], 2, 1}, {c[2], c[3], 3, 1}, {c[3], c[3], 3, 1}, {c[3], c[4], 4, 1}, {c[4], c[4], 4, 1},
-      {c[4] + c[3], c[3], 3, 1}, {c[3], c[3], 3, 1}, {c[3] + c[2], c[2], 2, 1}, {c[2], c[2], 2, 1},
-      {c[2] + c[1], c[1], 1, 1}, {c[1], c[1], 1, 1}, {c[1] + c[0], c[0], 0, 1}, {c[0], c[0], 0, 1},
-      {c[0], kOutC, 0, 0}};
-  for (auto& sp : spec) {
+  const int L = d->levels, *c = d->widths;
+  auto add = [&](int cin, int cout, int level, bool relu) {
     Conv cv;
-    cv.cin = sp[0];
-    cv.cout = sp[1];
-    cv.level = sp[2];
-    cv.relu = sp[3] != 0;
+    cv.cin = cin;
+    cv.cout = cout;
+    cv.level = level;
+    cv.relu = relu;
     d->conv.push_back(cv);
+  };
+  for (int k = 0; k < L; k++) {
+    add(k == 0 ? kInC : c[k - 1], c[k], k, true);
+    add(c[k], c[k], k, true);
   }
+  for (int k = L - 2; k >= 0; k--) {
+    add(c[k + 1] + c[k], c[k], k, true);
+    add(c[k], c[k], k, true);
+  }
+  add(c[0], kOutC, 0, false);
 }
 
 int initWeights(arbor_denoiser* d, int seed) {
@@ -249,37 +262,42 @@ int runConv(arbor_denoiser* d, int li, const Act& x, Act& y, int n, hipStream_t 
                           cv.cout, s);
 }
 
+// The decoder's input at level k (the coarser result it upsamples).
+const Act& decoderLow(const arbor_denoiser* d, int k) { return k + 1 < d->levels - 1 ? d->d[k + 1] : d->e4; }
+// The layer index of decoder level k's first convolution.
+int decoderLayer(const arbor_denoiser* d, int k) { return 2 * d->levels + 2 * (d->levels - 2 - k); }
+
 int forward(arbor_denoiser* d, int n, hipStream_t s) {
+  const int L = d->levels;
   // encoder
   const Act* x = &d->in;
-  for (int k = 0; k < kLevels; k++) {
+  for (int k = 0; k < L; k++) {
     Act& first = d->a[k];
-    Act& second = k < 4 ? d->s[k] : d->e4;
+    Act& second = k < L - 1 ? d->s[k] : d->e4;
     DN_TRY(runConv(d, 2 * k, *x, first, n, s));
     DN_TRY(runConv(d, 2 * k + 1, first, second, n, s));
-    if (k < 4) {
+    if (k < L - 1) {
       DN_TRY(arbor_dn_pool2(second.p, n, d->hl(k), d->wl(k), second.c, d->p[k].p, s));
       x = &d->p[k];
     }
   }
-  // decoder: level k = 3..0
-  const Act* low = &d->e4;
-  for (int k = 3; k >= 0; k--) {
-    const int j = 3 - k;  // decoder stage
+  // decoder: level k = L-2..0
+  for (int k = L - 2; k >= 0; k--) {
+    const Act& low = decoderLow(d, k);
+    const int li = decoderLayer(d, k);
     if (d->train) {  // the weight gradient needs the concatenated input in memory
-      DN_TRY(arbor_dn_up2(low->p, n, d->hl(k + 1), d->wl(k + 1), low->c, d->u[k].p, s));
+      DN_TRY(arbor_dn_up2(low.p, n, d->hl(k + 1), d->wl(k + 1), low.c, d->u[k].p, s));
       DN_TRY(arbor_dn_concat(d->u[k].p, d->u[k].c, d->s[k].p, d->s[k].c, (long long)n * d->hl(k) * d->wl(k),
                              d->c[k].p, s));
-      DN_TRY(runConv(d, 10 + 2 * j, d->c[k], d->dA[k], n, s));
+      DN_TRY(runConv(d, li, d->c[k], d->dA[k], n, s));
     } else {  // upsampling and concatenation inside the convolution's staging
-      const Conv& cv = d->conv[10 + 2 * j];
-      DN_TRY(arbor_dn_conv3x3_cat(low->p, low->c, d->s[k].p, d->s[k].c, n, d->hl(k), d->wl(k), cv.w16, cv.b, 1,
+      const Conv& cv = d->conv[li];
+      DN_TRY(arbor_dn_conv3x3_cat(low.p, low.c, d->s[k].p, d->s[k].c, n, d->hl(k), d->wl(k), cv.w16, cv.b, 1,
                                   d->dA[k].p, cv.cout, s));
     }
-    DN_TRY(runConv(d, 11 + 2 * j, d->dA[k], d->d[k], n, s));
-    low = &d->d[k];
+    DN_TRY(runConv(d, li + 1, d->dA[k], d->d[k], n, s));
   }
-  return runConv(d, 18, d->d[0], d->out, n, s);
+  return runConv(d, 4 * L - 2, L > 1 ? d->d[0] : d->e4, d->out, n, s);
 }
 
 // dX of conv li from dY (into gx; gx may be NULL for the input layer), and
@@ -294,20 +312,22 @@ int backConv(arbor_denoiser* d, int li, const Act& x, const Act& y, uint16_t* gx
 }
 
 int backward(arbor_denoiser* d, int n, hipStream_t s) {
+  const int L = d->levels;
   // output layer and decoder
-  DN_TRY(backConv(d, 18, d->d[0], d->out, d->d[0].g, n, s));
-  for (int k = 0; k <= 3; k++) {
-    const int j = 3 - k;
-    DN_TRY(backConv(d, 11 + 2 * j, d->dA[k], d->d[k], d->dA[k].g, n, s));
-    DN_TRY(backConv(d, 10 + 2 * j, d->c[k], d->dA[k], d->c[k].g, n, s));
+  Act& top = L > 1 ? d->d[0] : d->e4;
+  DN_TRY(backConv(d, 4 * L - 2, top, d->out, top.g, n, s));
+  for (int k = 0; k <= L - 2; k++) {
+    const int li = decoderLayer(d, k);
+    DN_TRY(backConv(d, li + 1, d->dA[k], d->d[k], d->dA[k].g, n, s));
+    DN_TRY(backConv(d, li, d->c[k], d->dA[k], d->c[k].g, n, s));
     const long long px = (long long)n * d->hl(k) * d->wl(k);
     DN_TRY(arbor_dn_split(d->c[k].g, d->u[k].c, d->s[k].c, px, d->u[k].g, d->s[k].g, s));
-    Act& low = k < 3 ? d->d[k + 1] : d->e4;
+    const Act& low = decoderLow(d, k);
     DN_TRY(arbor_dn_up2_bwd(d->u[k].g, n, d->hl(k + 1), d->wl(k + 1), low.c, low.g, s));
   }
   // bottleneck and encoder, deepest first
-  for (int k = 4; k >= 0; k--) {
-    Act& second = k < 4 ? d->s[k] : d->e4;
+  for (int k = L - 1; k >= 0; k--) {
+    Act& second = k < L - 1 ? d->s[k] : d->e4;
     const Act& input = k == 0 ? d->in : d->p[k - 1];
     DN_TRY(backConv(d, 2 * k + 1, d->a[k], second, d->a[k].g, n, s));
     DN_TRY(backConv(d, 2 * k, input, d->a[k], k == 0 ? nullptr : d->p[k - 1].g, n, s));
@@ -328,17 +348,24 @@ bool slotOk(const arbor_denoiser* d, int index, int w, int h) {
 
 extern "C" {
 
-int arbor_denoiser_create(arbor_denoiser** out, int max_batch, int height, int width, int train, int seed) {
-  if (!out) return ARBOR_ERR_NULL;
+int arbor_denoiser_create_net(arbor_denoiser** out, int max_batch, int height, int width, int train, int seed,
+                              int levels, const int* widths) {
+  if (!out || !widths) return ARBOR_ERR_NULL;
   *out = nullptr;
   if (max_batch < 1 || max_batch > 1024 || height < 16 || width < 16 || height % 16 || width % 16 ||
-      height > ARBOR_MAX_DIM || width > ARBOR_MAX_DIM)
+      height > ARBOR_MAX_DIM || width > ARBOR_MAX_DIM || levels < 1 || levels > kMaxLevels)
     return ARBOR_ERR_ARG;
+  for (int k = 0; k < levels; k++) {
+    const int below = k + 1 < levels ? widths[k + 1] : 0;  // the decoder concatenates the level below
+    if (widths[k] < 16 || widths[k] % 16 || widths[k] + below > ARBOR_DN_MAX_CHANNELS) return ARBOR_ERR_ARG;
+  }
   arbor_denoiser* d = new arbor_denoiser();
   d->maxBatch = max_batch;
   d->H = height;
   d->W = width;
   d->train = train != 0;
+  d->levels = levels;
+  for (int k = 0; k < levels; k++) d->widths[k] = widths[k];
   buildLayers(d);
   int rc = ARBOR_OK;
   for (Conv& cv : d->conv) {
@@ -355,10 +382,11 @@ int arbor_denoiser_create(arbor_denoiser** out, int max_batch, int height, int w
       rc |= d->alloc(&cv.vb, cv.cout * 4);
     }
   }
-  const int* c = kWidths;
+  const int* c = d->widths;
+  const int L = levels;
   rc |= d->allocAct(d->in, kInC, 0);
-  for (int k = 0; k < kLevels; k++) rc |= d->allocAct(d->a[k], c[k], k);
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < L; k++) rc |= d->allocAct(d->a[k], c[k], k);
+  for (int k = 0; k < L - 1; k++) {
     rc |= d->allocAct(d->s[k], c[k], k);
     rc |= d->allocAct(d->p[k], c[k], k + 1);
     if (d->train) {
@@ -373,10 +401,15 @@ int arbor_denoiser_create(arbor_denoiser** out, int max_batch, int height, int w
     rc |= d->allocAct(d->dA[k], c[k], k);
     rc |= d->allocAct(d->d[k], c[k], k);
   }
-  rc |= d->allocAct(d->e4, c[4], 4);
+  rc |= d->allocAct(d->e4, c[L - 1], L - 1);
   rc |= d->allocAct(d->out, kOutC, 0);
   if (d->train) {
-    rc |= d->alloc(&d->gpool, (size_t)max_batch * height * width * c[0] * 2);
+    size_t poolElems = 0;  // the largest skip a pooled gradient returns to
+    for (int k = 0; k < L - 1; k++) {
+      size_t e = (size_t)max_batch * d->hl(k) * d->wl(k) * c[k];
+      poolElems = e > poolElems ? e : poolElems;
+    }
+    if (poolElems) rc |= d->alloc(&d->gpool, poolElems * 2);
     rc |= d->alloc(&d->target, (size_t)max_batch * height * width * 16);
     long long most = 0;
     for (Conv& cv : d->conv) {
@@ -396,6 +429,42 @@ int arbor_denoiser_create(arbor_denoiser** out, int max_batch, int height, int w
     return ARBOR_ERR_DEVICE;
   }
   *out = d;
+  return ARBOR_OK;
+}
+
+int arbor_denoiser_create(arbor_denoiser** out, int max_batch, int height, int width, int train, int seed) {
+  return arbor_denoiser_create_net(out, max_batch, height, width, train, seed, kMaxLevels, kDefaultWidths);
+}
+
+int arbor_denoiser_shape(const arbor_denoiser* d, int* levels, int* widths) {
+  if (!d || !levels || !widths) return ARBOR_ERR_NULL;
+  *levels = d->levels;
+  for (int k = 0; k < kMaxLevels; k++) widths[k] = k < d->levels ? d->widths[k] : 0;
+  return ARBOR_OK;
+}
+
+// The architecture a checkpoint file was saved from: its layer count gives the
+// levels (4L - 1 layers), the encoder layers' output widths the channels.
+int arbor_denoiser_checkpoint_shape(const char* path, int* levels, int* widths) {
+  if (!path || !levels || !widths) return ARBOR_ERR_NULL;
+  FILE* f = fopen(path, "rb");
+  if (!f) return ARBOR_ERR_ARG;
+  char magic[8];
+  int32_t layers = 0;
+  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "ARBORDN1", 8) == 0 && fread(&layers, 4, 1, f) == 1 &&
+            layers >= 3 && (layers + 1) % 4 == 0 && (layers + 1) / 4 <= kMaxLevels;
+  const int L = ok ? (layers + 1) / 4 : 0;
+  int w[kMaxLevels] = {};
+  for (int li = 0; ok && li < 2 * L; li++) {
+    int32_t dims[2];
+    ok = fread(dims, 4, 2, f) == 2 && dims[0] > 0 && dims[1] > 0 &&
+         fseek(f, (long)(9LL * dims[0] * dims[1] + dims[1]) * 4, SEEK_CUR) == 0;
+    if (ok && li % 2 == 0) w[li / 2] = dims[1];
+  }
+  fclose(f);
+  if (!ok) return ARBOR_ERR_ARG;
+  *levels = L;
+  for (int k = 0; k < kMaxLevels; k++) widths[k] = w[k];
   return ARBOR_OK;
 }
 

@@ -20,6 +20,11 @@ by PSNR and SSIM:
 The times of filterLighting and of the network (features, forward, output) at
 1080p are measured with HIP events.
 
+Several networks can be trained on the same data and evaluated on the same
+held-out frames: --nets "5:32,48,64,96,128;2:16,32" (levels:widths, ';'
+between networks); the i-th network is saved as <out> with _i before the
+extension when there is more than one.
+
 Usage: python arbor-cuda_amd/dn_train.py --crops 4096 --size 512 --spp 64 --steps 6000
        --out denoiser.bin --report report.json
 (the committed checkpoint's configuration; about 12 minutes on one MI355X)
@@ -147,6 +152,8 @@ def main():
     ap.add_argument("--views-seed", type=int, default=11)
     ap.add_argument("--eval-views", type=int, default=2)
     ap.add_argument("--eval-only", default="", help="skip training: evaluate this checkpoint")
+    ap.add_argument("--nets", default="5:32,48,64,96,128",
+                    help="networks to train, levels:widths separated by ';' (default: the 5-level network)")
     ap.add_argument("--out", default="denoiser.bin")
     ap.add_argument("--report", default="denoiser_report.json")
     args = ap.parse_args()
@@ -195,37 +202,42 @@ def main():
         report["data"] = {"crops": n, "size": [w, h], "spp": args.spp, "input_s": round(t1 - t0, 2),
                           "target_s": round(t2 - t1, 2), "target_mrays_per_s": round(rays / (t2 - t1) / 1e6, 1)}
 
-        # ---- training ----
-        dn = px.Denoiser(args.batch, h, w, train=True, seed=args.seed)
-        losses = []
-        t3 = time.time()
-        for step in range(args.steps):
-            lr = args.lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi * step / args.steps)))
-            idx = rng.choice(n, args.batch, replace=False)
-            for j, i in enumerate(idx):
-                dn.set_input(j, L0[i], G0[i], N0[i], H0[i], w, h)
-                dn.set_target(j, T[i], w, h)
-            dn.train_step(args.batch, lr)
-            if step % 100 == 0 or step == args.steps - 1:
-                losses.append((step, dn.loss()))
-                _log("step %5d loss %.5f lr %.2e  %.1f s" % (step, losses[-1][1], lr, time.time() - t3))
-        torch.cuda.synchronize()
-        t4 = time.time()
-        dn.save(args.out)
-        report["train"] = {"steps": args.steps, "batch": args.batch, "seconds": round(t4 - t3, 1),
-                           "ms_per_step": round((t4 - t3) / args.steps * 1e3, 2), "loss": losses,
-                           "parameters": dn.parameters()}
-        dn.close()
+        # ---- training, one network after another on the same data ----
+        nets = [(int(a), [int(x) for x in b.split(",")]) for a, b in (n.split(":") for n in args.nets.split(";"))]
+        root, ext = os.path.splitext(args.out)
+        ckpts, report["train"] = [], []
+        for ni, (levels, widths) in enumerate(nets):
+            path = args.out if len(nets) == 1 else "%s_%d%s" % (root, ni, ext)
+            dn = px.Denoiser(args.batch, h, w, train=True, seed=args.seed, levels=levels, widths=widths)
+            losses = []
+            t3 = time.time()
+            for step in range(args.steps):
+                lr = args.lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi * step / args.steps)))
+                idx = rng.choice(n, args.batch, replace=False)
+                for j, i in enumerate(idx):
+                    dn.set_input(j, L0[i], G0[i], N0[i], H0[i], w, h)
+                    dn.set_target(j, T[i], w, h)
+                dn.train_step(args.batch, lr)
+                if step % 100 == 0 or step == args.steps - 1:
+                    losses.append((step, dn.loss()))
+                    _log("net %d step %5d loss %.5f lr %.2e  %.1f s" % (ni, step, losses[-1][1], lr, time.time() - t3))
+            torch.cuda.synchronize()
+            t4 = time.time()
+            dn.save(path)
+            report["train"].append({"levels": levels, "widths": widths, "checkpoint": os.path.basename(path),
+                                    "steps": args.steps, "batch": args.batch, "seconds": round(t4 - t3, 1),
+                                    "ms_per_step": round((t4 - t3) / args.steps * 1e3, 2), "loss": losses,
+                                    "parameters": dn.parameters()})
+            dn.close()
+            ckpts.append(path)
         del L0, G0, N0, H0, T
         torch.cuda.empty_cache()
-        ckpt = args.out
     else:
-        ckpt = args.eval_only
+        ckpts = args.eval_only.split(";")
 
     # ---- evaluation at 1920x1080 ----
     W, H = 1920, 1080
-    inf = px.Denoiser(1, 1088, 1920, train=False, seed=0)
-    inf.load(ckpt)
+    nets = [px.Denoiser.from_checkpoint(c, 1, 1088, 1920) for c in ckpts]
     eng = px.Engine(W, H, args.seed)
     I = px.walk_input(args.seed, 0)
     eng.set_input(I)
@@ -242,25 +254,31 @@ def main():
         cam[22] = 1.0
         fb, normal, Cd = render_inputs(px, torch, scene, cam, W, H)
         target = render_targets_batched(px, torch, scene, cam[None, :], W, H, args.spp)[0]
-        out = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
-
-        def run_net():
-            inf.set_input(0, fb.Linear, fb.Guide, normal, fb.Hit, W, H)
-            inf.forward(1)
-            inf.output(0, out, W, H)
-
-        run_net()
         ref = display(px, torch, target, Cd, W, H)
-        imgs = {"1 spp": display(px, torch, fb.Linear, Cd, W, H),
-                "filterLighting": display(px, torch, fb.Filtered, Cd, W, H),
-                "learned": display(px, torch, out, Cd, W, H)}
         r = {"camera": name}
-        for k, img in imgs.items():
+        for k, buf in (("1 spp", fb.Linear), ("filterLighting", fb.Filtered)):
+            img = display(px, torch, buf, Cd, W, H)
             r[k] = {"psnr": round(psnr(img, ref), 3), "ssim": round(ssim(img, ref), 4)}
         r["filter_ms"] = round(event_ms(torch, lambda: px.filter_lighting(Cd, fb)), 4)
-        r["denoiser_ms"] = round(event_ms(torch, run_net, reps=10), 3)
+        r["learned"] = []
+        for ni, inf in enumerate(nets):
+            out = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+
+            def run_net():
+                inf.set_input(0, fb.Linear, fb.Guide, normal, fb.Hit, W, H)
+                inf.forward(1)
+                inf.output(0, out, W, H)
+
+            run_net()
+            img = display(px, torch, out, Cd, W, H)
+            levels, widths = inf.shape()
+            r["learned"].append({"checkpoint": os.path.basename(ckpts[ni]), "levels": levels, "widths": widths,
+                                 "psnr": round(psnr(img, ref), 3), "ssim": round(ssim(img, ref), 4),
+                                 "denoiser_ms": round(event_ms(torch, run_net, reps=10), 4)})
         _log(json.dumps(r))
         results.append(r)
+    for inf in nets:
+        inf.close()
     report["eval"] = results
     with open(args.report, "w") as f:
         json.dump(report, f, indent=1)

@@ -52,6 +52,9 @@ SIGNATURES = {
     "arbor_dn_add": [_vp, _vp, ct.c_longlong, _vp],
     "arbor_shadePixelsAOV": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_denoiser_create": [ct.POINTER(_vp), _i, _i, _i, _i, _i],
+    "arbor_denoiser_create_net": [ct.POINTER(_vp), _i, _i, _i, _i, _i, _i, ct.POINTER(_i)],
+    "arbor_denoiser_shape": [_vp, ct.POINTER(_i), ct.POINTER(_i)],
+    "arbor_denoiser_checkpoint_shape": [ct.c_char_p, ct.POINTER(_i), ct.POINTER(_i)],
     "arbor_denoiser_destroy": [_vp],
     "arbor_denoiser_parameters": [_vp],
     "arbor_denoiser_set_input": [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -456,14 +459,41 @@ def shade_aov(scene, C, fb, normal):
          fb.width, fb.height)
 
 
-class Denoiser:
-    """The learned denoiser's network runner (include/arbor_denoise.h)."""
+def checkpoint_shape(path):
+    """(levels, widths) of the network a denoiser checkpoint was saved from."""
+    lv = ct.c_int()
+    w = (ct.c_int * 5)()
+    check(lib().arbor_denoiser_checkpoint_shape(str(path).encode(), ct.byref(lv), w), "arbor_denoiser_checkpoint_shape")
+    return lv.value, list(w)[:lv.value]
 
-    def __init__(self, max_batch, height, width, train=True, seed=1):
+
+class Denoiser:
+    """The learned denoiser's network runner (include/arbor_denoise.h). levels
+    and widths choose the U-Net (default: 5 levels of 32/48/64/96/128)."""
+
+    def __init__(self, max_batch, height, width, train=True, seed=1, levels=None, widths=None):
         h = ct.c_void_p()
-        check(lib().arbor_denoiser_create(ct.byref(h), max_batch, height, width, int(train), seed),
-              "arbor_denoiser_create")
+        if levels is None:
+            check(lib().arbor_denoiser_create(ct.byref(h), max_batch, height, width, int(train), seed),
+                  "arbor_denoiser_create")
+        else:
+            w = (ct.c_int * 5)(*(list(widths) + [0] * (5 - len(widths))))
+            check(lib().arbor_denoiser_create_net(ct.byref(h), max_batch, height, width, int(train), seed, levels, w),
+                  "arbor_denoiser_create_net")
         self.h, self.max_batch, self.height, self.width, self.train = h, max_batch, height, width, train
+
+    @classmethod
+    def from_checkpoint(cls, path, max_batch, height, width, train=False, seed=0):
+        levels, widths = checkpoint_shape(path)
+        d = cls(max_batch, height, width, train=train, seed=seed, levels=levels, widths=widths)
+        d.load(path)
+        return d
+
+    def shape(self):
+        lv = ct.c_int()
+        w = (ct.c_int * 5)()
+        check(lib().arbor_denoiser_shape(self.h, ct.byref(lv), w), "arbor_denoiser_shape")
+        return lv.value, list(w)[:lv.value]
 
     def close(self):
         if self.h:

@@ -60,10 +60,11 @@ int arbor_dn_relu_bwd(uint16_t* G, const uint16_t* Yact, long long count, hipStr
 int arbor_dn_add(uint16_t* A, const uint16_t* B, long long count, hipStream_t stream);
 
 /* ---- the network ----
- * 5-level U-Net, 3x3 convolutions with 32/48/64/96/128 channels (ReLU), 2x2
- * average pooling, nearest upsampling and skip concatenation, a linear 3x3
- * output layer: 1,021,904 parameters stored (1,016,419 excluding the zero
- * padding of the input and output channels). Input per pixel: log(1 + radiance) rgb
+ * An L-level U-Net (L = 1..5), 3x3 convolutions (ReLU), 2x2 average pooling,
+ * nearest upsampling and skip concatenation, a linear 3x3 output layer. The
+ * default is 5 levels of 32/48/64/96/128 channels: 1,021,904 parameters stored
+ * (1,016,419 excluding the zero padding of the input and output channels).
+ * Input per pixel: log(1 + radiance) rgb
  * of one sample, albedo rgb (Guide), shading normal (arbor_shadePixelsAOV)
  * and depth log(1 + min(t, 1000))/7 (1 for sky); output log(1 + radiance).
  * The runner holds a batch of max_batch slots of height x width (multiples
@@ -72,6 +73,14 @@ typedef struct arbor_denoiser arbor_denoiser;
 /* train != 0 also allocates gradients, Adam state and targets. Weights are
  * He-initialised from seed. */
 int arbor_denoiser_create(arbor_denoiser** out, int max_batch, int height, int width, int train, int seed);
+/* The same with `levels` levels of widths[0..levels-1] channels (multiples of
+ * 16; a width plus the next one at most ARBOR_DN_MAX_CHANNELS). */
+int arbor_denoiser_create_net(arbor_denoiser** out, int max_batch, int height, int width, int train, int seed,
+                              int levels, const int* widths);
+/* The network's levels and widths (widths: 5 ints, unused levels 0). */
+int arbor_denoiser_shape(const arbor_denoiser* d, int* levels, int* widths);
+/* The levels and widths a checkpoint file was saved from (widths: 5 ints). */
+int arbor_denoiser_checkpoint_shape(const char* path, int* levels, int* widths);
 int arbor_denoiser_destroy(arbor_denoiser* d);
 long long arbor_denoiser_parameters(const arbor_denoiser* d);
 /* Input features of slot `index` from one frame's buffers (float4 per pixel,

@@ -17,11 +17,21 @@ from gpu_util import require_gpu
 
 pytestmark = pytest.mark.gpu
 
-# (cin, cout, level, relu) of L0..L18, as the runner builds them
-LAYERS = [(16, 32, 0, 1), (32, 32, 0, 1), (32, 48, 1, 1), (48, 48, 1, 1), (48, 64, 2, 1), (64, 64, 2, 1),
-          (64, 96, 3, 1), (96, 96, 3, 1), (96, 128, 4, 1), (128, 128, 4, 1), (224, 96, 3, 1), (96, 96, 3, 1),
-          (160, 64, 2, 1), (64, 64, 2, 1), (112, 48, 1, 1), (48, 48, 1, 1), (80, 32, 0, 1), (32, 32, 0, 1),
-          (32, 16, 0, 0)]
+def net_layers(widths):
+    """(cin, cout, level, relu) of every layer, as the runner builds an
+    L-level network: encoder pairs, decoder pairs (deepest first), output."""
+    L = len(widths)
+    out = []
+    for k in range(L):
+        out += [(16 if k == 0 else widths[k - 1], widths[k], k, 1), (widths[k], widths[k], k, 1)]
+    for k in range(L - 2, -1, -1):
+        out += [(widths[k + 1] + widths[k], widths[k], k, 1), (widths[k], widths[k], k, 1)]
+    return out + [(widths[0], 16, 0, 0)]
+
+
+# (cin, cout, level, relu) of L0..L18 of the default network
+LAYERS = net_layers([32, 48, 64, 96, 128])
+assert LAYERS[10] == (224, 96, 3, 1) and LAYERS[16] == (80, 32, 0, 1) and len(LAYERS) == 19
 N, H, W, w_img, h_img = 2, 64, 96, 90, 61
 
 
@@ -48,9 +58,9 @@ def _frames(torch, seed):
     return [t.contiguous().cuda() for t in (lin, gui, nor, hit, tgt)]
 
 
-def _load(dn, torch):
+def _load(dn, torch, layers=LAYERS):
     Ws, Bs = [], []
-    for i, (ci, co, _, _) in enumerate(LAYERS):
+    for i, (ci, co, _, _) in enumerate(layers):
         w = torch.from_numpy(dn.tensor("weight:%d" % i, 9 * co * ci, np.float32))
         b = torch.from_numpy(dn.tensor("bias:%d" % i, co, np.float32))
         Ws.append(w.reshape(3, 3, co, ci).permute(2, 3, 0, 1).contiguous())
@@ -58,32 +68,33 @@ def _load(dn, torch):
     return Ws, Bs
 
 
-def _ref_forward(torch, x, Ws, Bs):
+def _ref_forward(torch, x, Ws, Bs, layers=LAYERS):
     F = torch.nn.functional
+    L = (len(layers) + 1) // 4
 
     def bfr(t):
         return t.to(torch.bfloat16).float()
 
     def conv(i, t):
         y = F.conv2d(t, bfr(Ws[i]), Bs[i], padding=1)
-        return bfr(F.relu(y) if LAYERS[i][3] else y)
+        return bfr(F.relu(y) if layers[i][3] else y)
 
     def up(t):
         return t.repeat_interleave(2, 2).repeat_interleave(2, 3)
 
     skips, t = [], x
-    for k in range(5):
+    for k in range(L):
         a = conv(2 * k, t)
         s = conv(2 * k + 1, a)
-        if k < 4:
+        if k < L - 1:
             skips.append(s)
             t = bfr(F.avg_pool2d(s, 2))
         else:
             t = s
-    for j, k in enumerate(range(3, -1, -1)):
+    for j, k in enumerate(range(L - 2, -1, -1)):
         c = torch.cat([up(t), skips[k]], 1)
-        t = conv(11 + 2 * j, conv(10 + 2 * j, c))
-    return conv(18, t)
+        t = conv(2 * L + 1 + 2 * j, conv(2 * L + 2 * j, c))
+    return conv(4 * L - 2, t)
 
 
 def _input(dn, torch):
@@ -256,3 +267,62 @@ def test_bad_checkpoint_leaves_network_unchanged(env, tmp_path):
     assert not torch.equal(_output(dn, torch), want)
     dn.close()
     other.close()
+
+
+@pytest.mark.parametrize("widths", [[16, 32], [16]])
+def test_small_network_forward_and_gradients(env, widths):
+    """A network of other levels and widths (arbor_denoiser_create_net): the
+    same forward and gradient bars against the PyTorch restatement of its
+    graph, and its shape read back from the runner and from a checkpoint."""
+    torch, px = env
+    print("case: denoiser %d levels %s, batch 2, 64x96 slots" % (len(widths), widths), flush=True)
+    layers = net_layers(widths)
+    dn = px.Denoiser(N, H, W, train=True, seed=4, levels=len(widths), widths=widths)
+    assert dn.shape() == (len(widths), widths)
+    assert dn.parameters() == sum(9 * ci * co + co for ci, co, _, _ in layers)
+    lin, gui, nor, hit, tgt = _frames(torch, 5)
+    for i in range(N):
+        dn.set_input(i, lin[i], gui[i], nor[i], hit[i], w_img, h_img)
+        dn.set_target(i, tgt[i], w_img, h_img)
+    dn.forward(N)
+    x = _input(dn, torch)
+    Ws, Bs = _load(dn, torch, layers)
+    ref = _ref_forward(torch, x, Ws, Bs, layers)
+    got = _output(dn, torch)
+    rel = ((got[:, :3] - ref[:, :3]).norm() / ref[:, :3].norm()).item()
+    print("forward relative L2 error %.3g" % rel)
+    assert rel <= 2e-2
+    Ws_g = [w.clone().requires_grad_(True) for w in Ws]
+    Bs_g = [b.clone().requires_grad_(True) for b in Bs]
+    out = _ref_forward(torch, x, Ws_g, Bs_g, layers)
+    t = torch.from_numpy(dn.tensor("target", N * H * W * 4, np.float32)).reshape(N, H, W, 4).permute(0, 3, 1, 2)
+    loss = ((out[:, :3] - t[:, :3]).abs() * t[:, 3:4]).sum() / (N * H * W * 3)
+    loss.backward()
+    dn.train_step(N, 1e-3)
+    assert abs(dn.loss() - loss.item()) <= 0.02 * loss.item()
+    for i, (ci, co, _, _) in enumerate(layers):
+        gw = torch.from_numpy(dn.tensor("wgrad:%d" % i, 9 * co * ci, np.float32)).reshape(3, 3, co, ci).permute(
+            2, 3, 0, 1)
+        gb = torch.from_numpy(dn.tensor("bgrad:%d" % i, co, np.float32))
+        for name, a, b in (("w", gw, Ws_g[i].grad), ("b", gb, Bs_g[i].grad)):
+            nb = b.norm().item()
+            if nb == 0:
+                assert a.norm().item() == 0
+                continue
+            cos = (a * b).sum().item() / (a.norm().item() * nb)
+            print("L%d %s grad cos %.5f" % (i, name, cos))
+            assert cos >= 0.99, (i, name)
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as tmp:
+        path = tmp + "/small.bin"
+        dn.save(path)
+        assert px.checkpoint_shape(path) == (len(widths), widths)
+        inf = px.Denoiser.from_checkpoint(path, N, H, W)
+        for i in range(N):
+            inf.set_input(i, lin[i], gui[i], nor[i], hit[i], w_img, h_img)
+        inf.forward(N)
+        dn.forward(N)
+        assert torch.equal(_output(inf, torch), _output(dn, torch))
+        inf.close()
+    dn.close()
