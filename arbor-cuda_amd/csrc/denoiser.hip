@@ -495,6 +495,14 @@ int arbor_denoiser_destroy(arbor_denoiser* d) {
   return ARBOR_OK;
 }
 
+int arbor_denoiser_slot(const arbor_denoiser* d, int* max_batch, int* height, int* width) {
+  if (!d || !max_batch || !height || !width) return ARBOR_ERR_NULL;
+  *max_batch = d->maxBatch;
+  *height = d->H;
+  *width = d->W;
+  return ARBOR_OK;
+}
+
 long long arbor_denoiser_parameters(const arbor_denoiser* d) {
   if (!d) return ARBOR_ERR_NULL;
   long long n = 0;

@@ -9,6 +9,7 @@
 #include <math.h>
 
 #include "arbor_batch.h"
+#include "arbor_denoise.h"
 #include "arbor_engine.h"
 #include "engine_passes.hpp"
 #include "launch.hpp"
@@ -24,6 +25,7 @@ struct Buffers {
   void* VisSchedule = nullptr;  // tile schedule of the sun-visibility pass
   float* Vis = nullptr;         // sun visibility per pixel
   void* Accel = nullptr;        // SAH trace hierarchy (arbor_buildTraceAccel)
+  float* Normal = nullptr;      // shading normal, the denoiser's input (only with a denoiser attached)
 };
 
 template <class T>
@@ -51,6 +53,8 @@ struct arbor_engine {
   float input[ARBOR_I_FLOATS] = {};
   FILE* rec = nullptr;  // input recording (arbor_engine_record)
   bool accel = true;    // trace and sun visibility over the SAH hierarchy (arbor_engine_set_accel)
+  arbor_denoiser* dn = nullptr;  // learned denoiser in place of filterLighting (arbor_engine_set_denoiser)
+  bool denoised = false;         // the last frame's denoiser: the network (true) or filterLighting
 };
 
 namespace {
@@ -60,6 +64,16 @@ size_t pixel_bytes(int w, int h) { return (size_t)w * h * 16; }
 void free_pixels(arbor_engine* e) {
   release(e->b.Hit); release(e->b.Linear); release(e->b.Guide); release(e->b.Filtered);
   release(e->b.History); release(e->b.Pixels); release(e->b.Schedule); release(e->b.VisSchedule); release(e->b.Vis);
+  release(e->b.Normal);
+}
+
+// Whether this frame denoises with the attached network: only when the
+// frame fits one of its slots; otherwise filterLighting.
+bool useNetwork(const arbor_engine* e) {
+  if (!e->dn || !e->b.Normal) return false;
+  int n = 0, h = 0, w = 0;
+  if (arbor_denoiser_slot(e->dn, &n, &h, &w) != ARBOR_OK) return false;
+  return n >= 1 && e->width <= w && e->height <= h;
 }
 
 }  // namespace
@@ -108,6 +122,9 @@ namespace {
 // Every buffer gets the bits the separate entry points give. With `timed`,
 // HIP events on the engine's stream bracket the prologue, the trace, the
 // shading (3-4) and the filter + resolve.
+// With a denoiser attached and a frame that fits its slot, shading also
+// writes the normal and step 5 becomes the network's input features, its
+// forward pass and output into Filtered (slot 0), then resolvePixels.
 int frameLaunches(arbor_engine* e, const float* I, float delta, int sample, int light, bool timed) {
   const int w = e->width, h = e->height;
   const void* accel = e->accel ? e->b.Accel : nullptr;
@@ -124,13 +141,27 @@ int frameLaunches(arbor_engine* e, const float* I, float delta, int sample, int 
   if ((rc = arbor::visibilityOrdered(e->b.P, e->b.Nodes, e->b.Order, accel, e->b.C, e->b.Hit, e->b.Vis, w, h,
                               e->b.VisSchedule, e->stream)) != ARBOR_OK)
     return rc;
-  if ((rc = arbor_shadePixelsVis(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, e->b.Linear,
-                                 e->b.Guide, w, h, e->stream)) != ARBOR_OK)
-    return rc;
+  const bool net = useNetwork(e);
+  if (net)
+    rc = arbor::shadeVisNormal(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, e->b.Linear, e->b.Guide,
+                               e->b.Normal, w, h, e->stream);
+  else
+    rc = arbor_shadePixelsVis(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, e->b.Linear, e->b.Guide, w,
+                              h, e->stream);
+  if (rc != ARBOR_OK) return rc;
   if (timed) (void)hipEventRecord(e->ev[3], e->stream);
-  if ((rc = arbor::filterResolve(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w, h,
-                          e->stream)) != ARBOR_OK)
+  if (net) {
+    if ((rc = arbor_denoiser_set_input(e->dn, 0, e->b.Linear, e->b.Guide, e->b.Normal, e->b.Hit, w, h, e->stream)) !=
+            ARBOR_OK ||
+        (rc = arbor_denoiser_forward(e->dn, 1, e->stream)) != ARBOR_OK ||
+        (rc = arbor_denoiser_get_output(e->dn, 0, e->b.Filtered, w, h, e->stream)) != ARBOR_OK ||
+        (rc = arbor_resolvePixels(e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w, h, e->stream)) != ARBOR_OK)
+      return rc;
+  } else if ((rc = arbor::filterResolve(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.History, e->b.Pixels,
+                                        e->b.C, w, h, e->stream)) != ARBOR_OK) {
     return rc;
+  }
+  e->denoised = net;
   if (timed) {
     (void)hipEventRecord(e->ev[4], e->stream);
     if (hipEventSynchronize(e->ev[4]) != hipSuccess) return ARBOR_ERR_DEVICE;
@@ -236,6 +267,7 @@ int arbor_engine_resize(arbor_engine* e, int width, int height, int round_size) 
   rc |= alloc(&e->b.Schedule, (size_t)sched);
   rc |= alloc(&e->b.VisSchedule, (size_t)sched);
   rc |= alloc(&e->b.Vis, n);
+  if (e->dn) rc |= alloc(&e->b.Normal, n);
   if (rc != ARBOR_OK) {
     free_pixels(e);
     e->width = e->height = 0;
@@ -271,6 +303,19 @@ int arbor_engine_set_timing(arbor_engine* e, int on) {
 int arbor_engine_set_accel(arbor_engine* e, int on) {
   if (!e) return ARBOR_ERR_NULL;
   e->accel = on != 0;
+  return ARBOR_OK;
+}
+
+int arbor_engine_set_denoiser(arbor_engine* e, arbor_denoiser* d) {
+  if (!e) return ARBOR_ERR_NULL;
+  if (d && !e->b.Normal && e->b.Hit) {
+    if (alloc(&e->b.Normal, pixel_bytes(e->width, e->height)) != ARBOR_OK) return ARBOR_ERR_DEVICE;
+  }
+  if (!d && e->b.Normal) {
+    (void)hipStreamSynchronize(e->stream);
+    release(e->b.Normal);
+  }
+  e->dn = d;
   return ARBOR_OK;
 }
 
@@ -312,7 +357,7 @@ void* arbor_engine_buffer(arbor_engine* e, const char* name) {
       {"B", b.B}, {"P", b.P}, {"Keys", b.Keys}, {"Order", b.Order}, {"Nodes", b.Nodes}, {"C", b.C}, {"I", b.I},
       {"Hit", b.Hit}, {"Linear", b.Linear}, {"Guide", b.Guide}, {"Filtered", b.Filtered}, {"History", b.History},
       {"Pixels", b.Pixels}, {"Schedule", b.Schedule}, {"VisSchedule", b.VisSchedule}, {"Vis", b.Vis},
-      {"Accel", b.Accel}};
+      {"Accel", b.Accel}, {"Normal", b.Normal}};
   for (auto& t : table)
     if (strcmp(t.n, name) == 0) return t.p;
   return nullptr;
@@ -332,18 +377,22 @@ int arbor_engine_inspect(const arbor_engine* e, char* out, int cap) {
   if (!e || !out || cap <= 0) return ARBOR_ERR_NULL;
   size_t scene = (size_t)ARBOR_CURVES * ARBOR_CS * 4 + (size_t)ARBOR_CAPACITY * ARBOR_PS * 4 +
                  (size_t)ARBOR_CAPACITY * 8 + (size_t)ARBOR_NODE_COUNT * 32;
-  // Hit, Linear, Guide, Filtered, History and Vis at 16 B a pixel, Pixels at
-  // 4 B, the two tile schedules, and the SAH hierarchy beside the scene.
+  // Hit, Linear, Guide, Filtered, History, Vis (and Normal with a denoiser
+  // attached) at 16 B a pixel, Pixels at 4 B, the two tile schedules, and
+  // the SAH hierarchy beside the scene.
   const size_t sched = e->width > 0 ? (size_t)arbor_tile_schedule_bytes(e->width, e->height) : 0;
   size_t allocated = scene + ARBOR_C_FLOATS * 4 + ARBOR_I_FLOATS * 4 + (size_t)arbor_trace_accel_bytes() +
-                     6 * pixel_bytes(e->width, e->height) + (size_t)e->width * e->height * 4 + 2 * sched;
+                     (e->b.Normal ? 7 : 6) * pixel_bytes(e->width, e->height) + (size_t)e->width * e->height * 4 +
+                     2 * sched;
   int n = snprintf(out, (size_t)cap,
-                   "{\"width\":%d,\"height\":%d,\"seed\":%d,\"frames\":%ld,\"accel\":%s,\"sceneBytes\":%zu,"
+                   "{\"width\":%d,\"height\":%d,\"seed\":%d,\"frames\":%ld,\"accel\":%s,\"denoiser\":\"%s\","
+                   "\"sceneBytes\":%zu,"
                    "\"allocatedBytes\":%zu,\"timings_ms\":{\"camera\":%.4f,\"visibility\":%.4f,\"lighting\":%.4f,"
                    "\"filter\":%.4f,\"resolve\":%.4f,\"frame\":%.4f},\"abi\":{\"CAPACITY\":%d,\"NODE_COUNT\":%d,"
                    "\"PS\":%d,\"CS\":%d,\"CURVES\":%d,\"WOOD_COUNT\":%d,\"LEAF_COUNT\":%d,\"ROCK_COUNT\":%d,"
                    "\"OBJECT_COUNT\":%d}}",
-                   e->width, e->height, e->seed, e->frames, e->accel ? "true" : "false", scene, allocated, e->stage_ms[0], e->stage_ms[1],
+                   e->width, e->height, e->seed, e->frames, e->accel ? "true" : "false",
+                   e->denoised ? "network" : "filterLighting", scene, allocated, e->stage_ms[0], e->stage_ms[1],
                    e->stage_ms[2], e->stage_ms[3], e->stage_ms[4], e->stage_ms[5], ARBOR_CAPACITY, ARBOR_NODE_COUNT, ARBOR_PS,
                    ARBOR_CS, ARBOR_CURVES, ARBOR_WOOD_COUNT, ARBOR_LEAF_COUNT, ARBOR_ROCK_COUNT, ARBOR_OBJECT_COUNT);
   return (n >= 0 && n < cap) ? ARBOR_OK : ARBOR_ERR_ARG;

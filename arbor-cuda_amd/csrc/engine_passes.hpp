@@ -24,6 +24,11 @@ int traceOrdered(const float* P, const float* Nodes, const unsigned* Order, cons
                  float* Hit, int width, int height, void* schedule, hipStream_t stream);
 int visibilityOrdered(const float* P, const float* Nodes, const unsigned* Order, const void* accel, const float* C,
                       const float* Hit, float* Vis, int width, int height, void* schedule, hipStream_t stream);
+// arbor_shadePixelsVis that also writes the shading normal (Normal as
+// arbor_shadePixelsAOV writes it); Linear and Guide are the same bits.
+int shadeVisNormal(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                   const float* Vis, float* Linear, float* Guide, float* Normal, int width, int height,
+                   hipStream_t stream);
 int filterResolve(const float* Linear, const float* Guide, const float* Hit, float* Filtered, float* History,
                   unsigned* Pixels, const float* C, int width, int height, hipStream_t stream);
 // Compute units of the current device (cached per device).

@@ -1015,6 +1015,19 @@ int visibilityOrdered(const float* P, const float* Nodes, const unsigned* Order,
   return launchVisibility(P, Nodes, Order, C, Hit, Vis, width, height, order, cost, stream);
 }
 
+int shadeVisNormal(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                   const float* Vis, float* Linear, float* Guide, float* Normal, int width, int height,
+                   hipStream_t stream) {
+  if (!P || !Nodes || !Order || !C || !Hit || !Vis || !Linear || !Guide || !Normal) return ARBOR_ERR_NULL;
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  hipLaunchKernelGGL(k_shadePixels<false>, dim3(shadeGrid(tiles, 3)), dim3(kTileBlock), 0, stream, P,
+                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
+                     reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
+                     reinterpret_cast<float4*>(Normal), reinterpret_cast<const float4*>(Vis));
+  return launch_status();
+}
+
 int filterResolve(const float* Linear, const float* Guide, const float* Hit, float* Filtered, float* History,
                   unsigned* Pixels, const float* C, int width, int height, hipStream_t stream) {
   if (!Linear || !Guide || !Hit || !Filtered || !History || !Pixels || !C) return ARBOR_ERR_NULL;

@@ -55,6 +55,7 @@ SIGNATURES = {
     "arbor_denoiser_create_net": [ct.POINTER(_vp), _i, _i, _i, _i, _i, _i, ct.POINTER(_i)],
     "arbor_denoiser_shape": [_vp, ct.POINTER(_i), ct.POINTER(_i)],
     "arbor_denoiser_checkpoint_shape": [ct.c_char_p, ct.POINTER(_i), ct.POINTER(_i)],
+    "arbor_denoiser_slot": [_vp, ct.POINTER(_i), ct.POINTER(_i), ct.POINTER(_i)],
     "arbor_denoiser_destroy": [_vp],
     "arbor_denoiser_parameters": [_vp],
     "arbor_denoiser_set_input": [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -119,6 +120,7 @@ SIGNATURES = {
     "arbor_engine_set_input": [_vp, ct.POINTER(_f)],
     "arbor_engine_set_timing": [_vp, _i],
     "arbor_engine_set_accel": [_vp, _i],
+    "arbor_engine_set_denoiser": [_vp, _vp],
     "arbor_engine_frame": [_vp, _f, _vp],
     "arbor_engine_stage_ms": [_vp, ct.POINTER(_f)],
     "arbor_engine_sync": [_vp],
@@ -495,6 +497,12 @@ class Denoiser:
         check(lib().arbor_denoiser_shape(self.h, ct.byref(lv), w), "arbor_denoiser_shape")
         return lv.value, list(w)[:lv.value]
 
+    def slot(self):
+        """(max_batch, height, width) of the runner's slots."""
+        n, h, w = ct.c_int(), ct.c_int(), ct.c_int()
+        check(lib().arbor_denoiser_slot(self.h, ct.byref(n), ct.byref(h), ct.byref(w)), "arbor_denoiser_slot")
+        return n.value, h.value, w.value
+
     def close(self):
         if self.h:
             check(lib().arbor_denoiser_destroy(self.h), "arbor_denoiser_destroy")
@@ -565,6 +573,7 @@ class Engine:
         check(self._lib.arbor_engine_create(ct.byref(h), width, height, seed, int(round_size), s),
               "arbor_engine_create")
         self.h = h
+        self._dn = None
 
     def close(self):
         if self.h:
@@ -601,6 +610,12 @@ class Engine:
     def set_accel(self, on):
         """Trace over the SAH hierarchy (default) or over the heap BVH."""
         check(self._lib.arbor_engine_set_accel(self.h, int(on)), "arbor_engine_set_accel")
+
+    def set_denoiser(self, dn):
+        """Denoise frames that fit dn's slot with the network (a Denoiser, or
+        None for filterLighting); the engine keeps a reference while attached."""
+        check(self._lib.arbor_engine_set_denoiser(self.h, None if dn is None else dn.h), "arbor_engine_set_denoiser")
+        self._dn = dn
 
     def frame(self, dt=1.0 / 60.0, device_input=None):
         dev = None if device_input is None else ct.c_void_p(device_input.data_ptr())

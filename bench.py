@@ -72,26 +72,65 @@ def parse():
                          "sun visibility on at 35 deg elevation, breeze on, static initial camera (use --steps 10)")
     ap.add_argument("--profile-denoiser", type=int, default=0, metavar="N",
                     help="profiling driver: N learned-denoiser passes (features, forward, output) at 1920x1080 "
-                         "on fixed random inputs with the committed checkpoint; prints no JSON")
-    ap.add_argument("--denoiser", nargs="?", const=DENOISER_WEIGHTS, default="", metavar="CHECKPOINT",
+                         "on fixed random inputs with the network of --denoiser (default: the 5-level "
+                         "checkpoint); prints no JSON")
+    ap.add_argument("--denoiser", nargs="?", const=",".join(DENOISER_NETS), default="", metavar="CHECKPOINTS",
                     help="learned-denoiser workload instead: held-out 1920x1080 frames (sun visibility on), "
-                         "1-sample inputs against 64-sample targets; filterLighting vs the network")
+                         "1-sample inputs against 64-sample targets; filterLighting vs each network of the "
+                         "comma-separated checkpoints (default: the 1-level equal-PSNR network, then the "
+                         "5-level one); the first one's ms is the value")
     return ap.parse_args()
 
 
-DENOISER_WEIGHTS = os.path.join(HERE, "arbor-cuda_amd", "weights", "denoiser_forest.bin")
+WEIGHTS = os.path.join(HERE, "arbor-cuda_amd", "weights")
+DENOISER_WEIGHTS = os.path.join(WEIGHTS, "denoiser_forest.bin")
+# The smallest network at or above filterLighting's PSNR on the held-out frames.
+DENOISER_SMALL = os.path.join(WEIGHTS, "denoiser_forest_small.bin")
+DENOISER_NETS = (DENOISER_SMALL, DENOISER_WEIGHTS)
 # BF16 dense MFMA peak of MI355X (TFLOP/s).
 BF16_PEAK_TFLOPS = 2500.0
 
 
-def denoiser_flops(height, width):
-    """Algorithmic FLOPs of one forward pass at height x width (multiples of 16):
-    2 * 9 * cin * cout per output pixel of every convolution, counting the 10
-    used input and 3 used output channels."""
-    layers = [(10, 32, 0), (32, 32, 0), (32, 48, 1), (48, 48, 1), (48, 64, 2), (64, 64, 2), (64, 96, 3), (96, 96, 3),
-              (96, 128, 4), (128, 128, 4), (224, 96, 3), (96, 96, 3), (160, 64, 2), (64, 64, 2), (112, 48, 1),
-              (48, 48, 1), (80, 32, 0), (32, 32, 0), (32, 3, 0)]
-    return sum(2.0 * 9 * ci * co * (height >> lv) * (width >> lv) for ci, co, lv in layers)
+def denoiser_layers(widths):
+    """(cin, cout, level, cat) of every convolution of the U-Net of `widths`
+    channels per level, in the runner's order (encoder pairs, decoder pairs
+    deepest first, output layer); cat = channels that come upsampled from the
+    level below (0 for a plain convolution). cin counts the 10 used input
+    features, cout of the output layer its 3 used channels."""
+    L = len(widths)
+    out = []
+    for k in range(L):
+        out += [(10 if k == 0 else widths[k - 1], widths[k], k, 0), (widths[k], widths[k], k, 0)]
+    for k in range(L - 2, -1, -1):
+        out += [(widths[k + 1] + widths[k], widths[k], k, widths[k + 1]), (widths[k], widths[k], k, 0)]
+    return out + [(widths[0], 3, 0, 0)]
+
+
+def denoiser_flops(height, width, widths=(32, 48, 64, 96, 128)):
+    """Algorithmic FLOPs of one forward pass at height x width (multiples of
+    16): 2 * 9 * cin * cout per output pixel of every convolution."""
+    return sum(2.0 * 9 * ci * co * (height >> lv) * (width >> lv) for ci, co, lv, _ in denoiser_layers(widths))
+
+
+def denoiser_bytes(height, width, h, w, widths):
+    """Algorithmic HBM bytes of one denoiser pass (input features, forward,
+    output) for an h x w image in a height x width slot, tensors as the runner
+    stores them (bf16, channels padded to 16 at input and output): the
+    feature kernel reads Linear, Guide, Normal and Hit (64 B a pixel) and
+    writes the 16-channel input; each convolution reads its input once (the
+    concatenating ones the coarse tensor at its own size) and writes its
+    output; pooling reads a skip and writes a quarter; the output kernel reads
+    3 channels and writes float4 radiance."""
+    px = lambda lv: (height >> lv) * (width >> lv)
+    total = 64 * h * w + 32 * px(0) + 6 * h * w + 16 * h * w
+    layers = denoiser_layers(widths)
+    for i, (ci, co, lv, cat) in enumerate(layers):
+        cin = 16 if i == 0 else ci
+        cout = 16 if i == len(layers) - 1 else co
+        total += 2 * (cin - cat) * px(lv) + 2 * cat * px(lv + 1) + 2 * cout * px(lv)
+    for k in range(len(widths) - 1):  # 2x2 pooling of every skip
+        total += 2 * widths[k] * px(k) + 2 * widths[k] * px(k + 1)
+    return float(total)
 
 
 def bench_shadows4k(args):
@@ -153,8 +192,8 @@ def profile_denoiser(args):
     import pyarbor as px
 
     W, H = 1920, 1080
-    inf = px.Denoiser(1, 1088, 1920, train=False, seed=0)
-    inf.load(DENOISER_WEIGHTS)
+    path = args.denoiser.split(",")[0] if args.denoiser else DENOISER_WEIGHTS
+    inf = px.Denoiser.from_checkpoint(path, 1, 1088, 1920)
     g = torch.Generator(device="cuda").manual_seed(0)
     bufs = [torch.rand(W * H * 4, device="cuda", generator=g) for _ in range(4)]
     out = torch.zeros(W * H * 4, device="cuda")
@@ -163,15 +202,17 @@ def profile_denoiser(args):
         inf.forward(1)
         inf.output(0, out, W, H)
     torch.cuda.synchronize()
+    levels, widths = inf.shape()
     inf.close()
-    print("profile driver: %d denoiser passes at %dx%d" % (args.profile_denoiser, W, H))
+    print("profile driver: %d passes of the %d-level %s denoiser (%s) at %dx%d"
+          % (args.profile_denoiser, levels, widths, os.path.basename(path), W, H))
 
 
 def bench_denoiser(args):
-    """Learned denoiser vs filterLighting on held-out frames: quality against
+    """Learned denoisers vs filterLighting on held-out frames: quality against
     the 64-sample image (PSNR, SSIM after resolvePixels' display mapping) and
-    time per 1920x1080 frame (HIP events); the network's FLOP rate against the
-    BF16 MFMA peak."""
+    time per 1920x1080 frame (HIP events) of each network; each network's
+    FLOP rate against the BF16 MFMA peak and its byte rate against HBM."""
     import numpy as np
     import torch
 
@@ -179,9 +220,9 @@ def bench_denoiser(args):
     import pyarbor as px
 
     W, H, spp = 1920, 1080, 64
+    paths = [p for p in args.denoiser.split(",") if p]
     scene = px.DeviceScene().generate(args.seed)
-    inf = px.Denoiser(1, 1088, 1920, train=False, seed=0)
-    inf.load(args.denoiser)
+    nets = [px.Denoiser.from_checkpoint(p, 1, 1088, 1920) for p in paths]
     eng = px.Engine(W, H, args.seed)
     eng.set_input(px.walk_input(args.seed, 0))
     eng.frame(1.0 / 60.0)
@@ -195,44 +236,78 @@ def bench_denoiser(args):
         cam[22] = 1.0
         fb, normal, Cd = dn_train.render_inputs(px, torch, scene, cam, W, H)
         target = dn_train.render_targets_batched(px, torch, scene, cam[None, :], W, H, spp)[0]
-        out = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
-
-        def net():
-            inf.set_input(0, fb.Linear, fb.Guide, normal, fb.Hit, W, H)
-            inf.forward(1)
-            inf.output(0, out, W, H)
-
-        net()
         ref = dn_train.display(px, torch, target, Cd, W, H)
+        out = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
         row = {"camera": name}
-        for k, buf in (("1 spp", fb.Linear), ("filterLighting", fb.Filtered), ("learned", out)):
+        for k, buf in (("1 spp", fb.Linear), ("filterLighting", fb.Filtered)):
             img = dn_train.display(px, torch, buf, Cd, W, H)
             row[k] = {"psnr": round(dn_train.psnr(img, ref), 3), "ssim": round(dn_train.ssim(img, ref), 4)}
         row["filter_ms"] = round(dn_train.event_ms(torch, lambda: px.filter_lighting(Cd, fb)), 4)
-        row["denoiser_ms"] = round(dn_train.event_ms(torch, net, reps=10), 4)
-        row["forward_ms"] = round(dn_train.event_ms(torch, lambda: inf.forward(1), reps=10), 4)
+        row["networks"] = []
+        for path, inf in zip(paths, nets):
+            def net():
+                inf.set_input(0, fb.Linear, fb.Guide, normal, fb.Hit, W, H)
+                inf.forward(1)
+                inf.output(0, out, W, H)
+
+            net()
+            img = dn_train.display(px, torch, out, Cd, W, H)
+            row["networks"].append({
+                "checkpoint": os.path.basename(path),
+                "psnr": round(dn_train.psnr(img, ref), 3), "ssim": round(dn_train.ssim(img, ref), 4),
+                "denoiser_ms": round(dn_train.event_ms(torch, net, reps=10), 4),
+                "forward_ms": round(dn_train.event_ms(torch, lambda: inf.forward(1), reps=10), 4)})
         rows.append(row)
-    fwd = float(np.mean([r["forward_ms"] for r in rows]))
-    den = float(np.mean([r["denoiser_ms"] for r in rows]))
     flt = float(np.mean([r["filter_ms"] for r in rows]))
-    flops = denoiser_flops(1088, 1920)
-    achieved = flops / (fwd * 1e-3) / 1e12
+    summary = []
+    for i, (path, inf) in enumerate(zip(paths, nets)):
+        levels, widths = inf.shape()
+        den = float(np.mean([r["networks"][i]["denoiser_ms"] for r in rows]))
+        fwd = float(np.mean([r["networks"][i]["forward_ms"] for r in rows]))
+        flops = denoiser_flops(1088, 1920, widths)
+        nbytes = denoiser_bytes(1088, 1920, H, W, widths)
+        tflops = flops / (fwd * 1e-3) / 1e12
+        gbs = nbytes / (den * 1e-3) / 1e9
+        beats = all(r["networks"][i]["psnr"] >= r["filterLighting"]["psnr"] for r in rows)
+        summary.append({
+            "checkpoint": os.path.basename(path), "levels": levels, "widths": widths,
+            "parameters": int(inf.parameters()), "denoiser_ms": round(den, 4), "forward_ms": round(fwd, 4),
+            "vs_filter_time": round(den / flt, 2),
+            "mean_psnr": round(float(np.mean([r["networks"][i]["psnr"] for r in rows])), 3),
+            "psnr_at_or_above_filter_on_every_frame": beats,
+            "flop_per_frame": flops, "forward_tflops": round(tflops, 2),
+            "mfma_frac": round(tflops / BF16_PEAK_TFLOPS, 4),
+            "algorithmic_bytes_per_frame": nbytes, "pass_gbs": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            # operational intensity against the ridge point decides the bound
+            "bound": "mfma" if flops / nbytes > BF16_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9) else "hbm"})
+        inf.close()
+    first = summary[0]
+    if first["bound"] == "hbm":
+        roof = {"bound": "hbm", "kernel": "denoiser pass (features, %d convolutions, output), %d-level %s"
+                % (len(denoiser_layers(first["widths"])), first["levels"], first["widths"]),
+                "achieved": first["pass_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": first["hbm_frac"],
+                "algorithmic_bytes": first["algorithmic_bytes_per_frame"], "traffic": None}
+    else:
+        roof = {"bound": "mfma", "kernel": "denoiser forward (%d convolutions + resampling), %d-level %s"
+                % (len(denoiser_layers(first["widths"])), first["levels"], first["widths"]),
+                "achieved": first["forward_tflops"], "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": first["mfma_frac"], "flop_per_frame": first["flop_per_frame"], "traffic": None}
     result = {
         "metric": "learned vs filter denoise ms at equal PSNR",
-        "value": round(den, 4),
+        "value": first["denoiser_ms"],
         "unit": "ms/frame (learned denoiser, 1920x1080)",
-        "n_gpus": 1, "steps": len(rows), "warmup": 1, "ms_per_step": round(den, 4),
+        "n_gpus": 1, "steps": len(rows), "warmup": 1, "ms_per_step": first["denoiser_ms"],
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
         "data": "synthetic (tracer-rendered held-out frames of the seed-7 forest, sun visibility on)",
-        "config": {"workload": "learned denoiser (5-level U-Net, 32/48/64/96/128 channels, BF16 MFMA) vs "
-                               "filterLighting on 1-sample 1920x1080 frames, against 64-sample targets",
-                   "checkpoint": os.path.basename(args.denoiser)},
+        "config": {"workload": "learned denoisers (BF16 MFMA U-Nets) vs filterLighting on 1-sample 1920x1080 "
+                               "frames, against 64-sample targets",
+                   "checkpoints": [os.path.basename(p) for p in paths]},
         "filter_ms": round(flt, 4),
-        "learned_vs_filter_time": round(den / flt, 2),
+        "learned_vs_filter_time": first["vs_filter_time"],
+        "networks": summary,
         "quality": rows,
-        "roofline": {"bound": "mfma", "kernel": "denoiser forward (19 convolutions + resampling)",
-                     "flop_per_frame": flops, "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": None},
+        "roofline": roof,
     }
     print(json.dumps(result), flush=True)
 

@@ -81,6 +81,8 @@ int arbor_denoiser_create_net(arbor_denoiser** out, int max_batch, int height, i
 int arbor_denoiser_shape(const arbor_denoiser* d, int* levels, int* widths);
 /* The levels and widths a checkpoint file was saved from (widths: 5 ints). */
 int arbor_denoiser_checkpoint_shape(const char* path, int* levels, int* widths);
+/* The runner's slot count and slot size (height x width). */
+int arbor_denoiser_slot(const arbor_denoiser* d, int* max_batch, int* height, int* width);
 int arbor_denoiser_destroy(arbor_denoiser* d);
 long long arbor_denoiser_parameters(const arbor_denoiser* d);
 /* Input features of slot `index` from one frame's buffers (float4 per pixel,

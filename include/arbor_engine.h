@@ -6,6 +6,7 @@
 #define ARBOR_ENGINE_H
 
 #include "arbor.h"
+#include "arbor_denoise.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -35,6 +36,17 @@ int arbor_engine_set_timing(arbor_engine* e, int on);
  * (arbor_tracePixelsAccel / arbor_sunVisibilityAccel; on by default) or over
  * the heap BVH (arbor_tracePixelsScheduled / arbor_sunVisibilityScheduled). */
 int arbor_engine_set_accel(arbor_engine* e, int on);
+/* Denoise with a learned network in place of filterLighting (no counterpart
+ * in the reference, which filters only, Arbor.cu:292-302). With d not NULL,
+ * every later frame whose size fits d's slot (arbor_denoiser_slot) shades
+ * with the normal as well (a Normal buffer, 16 B a pixel, is allocated),
+ * runs the network on slot 0 (input features, forward, output into Filtered)
+ * and resolves; a frame larger than the slot falls back to filterLighting,
+ * as does every frame with d NULL (the default). inspect() names the last
+ * frame's denoiser. The engine does not own d: the caller loads its weights,
+ * keeps it alive while attached and does not use slot 0 of it meanwhile
+ * from another stream. */
+int arbor_engine_set_denoiser(arbor_engine* e, arbor_denoiser* d);
 /* engine.js:45-53: updateCamera, tracePixels, shadePixels, filterLighting,
  * resolvePixels, issued as five kernel launches like the reference's five
  * dispatches (engine.js:48-50, source.test.mjs:6): (1) updateCamera with the
@@ -45,13 +57,14 @@ int arbor_engine_set_accel(arbor_engine* e, int on);
  * (otherwise the host input is copied to the device first). */
 int arbor_engine_frame(arbor_engine* e, float dt, const float* device_input);
 /* Last timed frame: {prologue (camera and tile orders), trace, shading
- * (visibility pass and shadePixels), filter + resolve, 0, whole frame} in ms,
+ * (visibility pass and shadePixels), filter + resolve (or the network and
+ * resolve), 0, whole frame} in ms,
  * measured with HIP events on the engine's stream. */
 int arbor_engine_stage_ms(const arbor_engine* e, float* out6);
 int arbor_engine_sync(arbor_engine* e);
 /* Device pointer of a named buffer: B P Keys Order Nodes C I Hit Linear Guide
- * Filtered History Pixels Schedule VisSchedule Vis Accel; NULL for an unknown
- * name. */
+ * Filtered History Pixels Schedule VisSchedule Vis Accel Normal; NULL for an
+ * unknown name (and for Normal without a denoiser attached). */
 void* arbor_engine_buffer(arbor_engine* e, const char* name);
 hipStream_t arbor_engine_stream(arbor_engine* e);
 int arbor_engine_size(const arbor_engine* e, int* width, int* height);

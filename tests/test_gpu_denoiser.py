@@ -326,3 +326,77 @@ def test_small_network_forward_and_gradients(env, widths):
         assert torch.equal(_output(inf, torch), _output(dn, torch))
         inf.close()
     dn.close()
+
+
+def test_engine_denoises_with_network_and_falls_back(env):
+    """arbor_engine_set_denoiser: a frame that fits the network's slot equals
+    the entry points chained one by one on the same camera (trace, sun
+    visibility, shadePixelsVis, the normal of shadePixelsAOV, the network's
+    input / forward / output into Filtered, resolvePixels), bit for bit; a
+    frame larger than the slot, and every frame after detaching, equals the
+    filterLighting engine's bit for bit."""
+    torch, px = env
+    from gpu_util import to_dev, to_host
+
+    print("case: engine with a 2-level network attached, 328x184 in a 336x192 slot", flush=True)
+    w, h = 328, 184
+    dn = px.Denoiser(1, 192, 336, train=False, seed=5, levels=2, widths=[16, 32])
+    assert dn.slot() == (1, 192, 336)
+    eng, plain = px.Engine(w, h, 7), px.Engine(w, h, 7)
+    assert not eng.buffer_ptr("Normal")
+    eng.set_denoiser(dn)
+    assert eng.buffer_ptr("Normal")
+    for e in (eng, plain):
+        e.set_input(px.walk_input(7, 3))
+        e.frame(1.0 / 60.0)
+        e.set_input(np.zeros(32, np.float32))
+    before = eng.download("History", w * h * 4).copy()
+    eng.frame(1.0 / 60.0)
+    plain.frame(1.0 / 60.0)
+    assert eng.inspect()["denoiser"] == "network" and plain.inspect()["denoiser"] == "filterLighting"
+    C = eng.download("C", 64).copy()
+    assert C[22] == 1.0 and C[9] == 1.0
+    dev = px.DeviceScene().generate(7)
+    acc = px.TraceAccel(dev)
+    fb, aov = px.FrameBuffers(w, h), px.FrameBuffers(w, h)
+    fb.History.copy_(to_dev(before))
+    Cd = to_dev(C)
+    px.trace_accel(dev, acc, Cd, fb)
+    vis = torch.zeros(w * h * 4, dtype=torch.float32, device="cuda")
+    normal = torch.zeros(w * h * 4, dtype=torch.float32, device="cuda")
+    px.sun_visibility_accel(dev, acc, Cd, fb, vis)
+    px.shade_vis(dev, Cd, fb, vis)
+    aov.Hit.copy_(fb.Hit)
+    px.shade_aov(dev, Cd, aov, normal)
+    dn.set_input(0, fb.Linear, fb.Guide, normal, fb.Hit, w, h)
+    dn.forward(1)
+    dn.output(0, fb.Filtered, w, h)
+    px.resolve(Cd, fb)
+    for name in ("Hit", "Linear", "Guide", "Filtered", "History"):
+        want = to_host(getattr(fb, name)).view(np.uint32)
+        got = eng.download(name, w * h * 4).view(np.uint32)
+        assert np.array_equal(got, want), (name, int((got != want).sum()))
+    assert np.array_equal(eng.download("Normal", w * h * 4).view(np.uint32), to_host(normal).view(np.uint32))
+    assert np.array_equal(eng.download("Pixels", w * h, np.uint32), to_host(fb.Pixels, np.uint32))
+    # the network's image is not the filter's
+    assert not np.array_equal(eng.download("Filtered", w * h * 4), plain.download("Filtered", w * h * 4))
+
+    print("case: engine frame larger than the slot, then detached", flush=True)
+    W2, H2 = 384, 200
+    for e in (eng, plain):
+        e.resize(W2, H2)
+        e.set_input(px.walk_input(7, 4))
+    for step in range(2):
+        if step == 1:
+            eng.set_denoiser(None)
+            assert not eng.buffer_ptr("Normal")
+        eng.frame(1.0 / 60.0)
+        plain.frame(1.0 / 60.0)
+        assert eng.inspect()["denoiser"] == "filterLighting"
+        for name in ("Filtered", "History"):
+            assert np.array_equal(eng.download(name, W2 * H2 * 4).view(np.uint32),
+                                  plain.download(name, W2 * H2 * 4).view(np.uint32)), (step, name)
+        assert np.array_equal(eng.download("Pixels", W2 * H2, np.uint32), plain.download("Pixels", W2 * H2, np.uint32))
+    eng.close()
+    plain.close()
+    dn.close()
