@@ -215,6 +215,42 @@ AD TreeInst makeTree(int tree, int ws) {
   t.phase = (float)(t.seed & 8191) * 0.0017f;
   return t;
 }
+// The same records in device memory for the kernels that keep LDS for their
+// traversal stacks: three float4 per tree, {px, pz, sc, cl} {sl, cw, sw,
+// phase} {seed (int bits), 0, 0, 0}, written by the frame prologue.
+constexpr int kTreeRecF4 = 3;
+struct TreeRecs {
+  const float4* rec;
+};
+// What a DDA cell's broad test reads ({px, pz, sc} and cl).
+struct TreeHead {
+  float px, pz, sc, cl;
+};
+AD TreeHead treeHead(const TreeInst* t, int i) { return {t[i].px, t[i].pz, t[i].sc, t[i].cl}; }
+AD TreeInst treeRecord(const TreeInst* t, int i, const TreeHead&) { return t[i]; }
+AD TreeHead treeHead(TreeRecs t, int i) {
+  const float4 a = t.rec[kTreeRecF4 * i];
+  return {a.x, a.y, a.z, a.w};
+}
+AD TreeInst treeRecord(TreeRecs t, int i, const TreeHead& h) {
+  const float4 b = t.rec[kTreeRecF4 * i + 1];
+  TreeInst r;
+  r.px = h.px;
+  r.pz = h.pz;
+  r.sc = h.sc;
+  r.cl = h.cl;
+  r.sl = b.x;
+  r.cw = b.y;
+  r.sw = b.z;
+  r.phase = b.w;
+  r.seed = __float_as_int(t.rec[kTreeRecF4 * i + 2].x);
+  return r;
+}
+AD void storeTreeRec(float4* rec, int i, const TreeInst& t) {
+  rec[kTreeRecF4 * i] = make_float4(t.px, t.pz, t.sc, t.cl);
+  rec[kTreeRecF4 * i + 1] = make_float4(t.sl, t.cw, t.sw, t.phase);
+  rec[kTreeRecF4 * i + 2] = make_float4(__int_as_float(t.seed), 0.0f, 0.0f, 0.0f);
+}
 AD v3 toLocalPoint(v3 p, const TreeInst& t) {
   v3 q = p - mk(t.px, 0.0f, t.pz);
   return mk(q.x * t.cl - q.z * t.sl, q.y, q.x * t.sl + q.z * t.cl) / t.sc;

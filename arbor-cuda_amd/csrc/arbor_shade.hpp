@@ -93,9 +93,9 @@ AD float specGGX(v3 n, v3 v, v3 l, float rough) {
 // sun-visibility pass inlines this (its kernel then needs 121 VGPRs, 4 waves
 // per SIMD); the fused shading paths call the out-of-line lightVisibility
 // below, so their own register budget is not added to the traversal's.
-template <class W>
+template <class W, class Trees>
 __device__ __forceinline__ v3 lightVisibilityInline(const float* P, const float4* Nodes4, const float4* top, const uint32_t* Order,
-                                     const TreeInst* trees, v3 p, v3 direction, float time, float wind) {
+                                     Trees trees, v3 p, v3 direction, float time, float wind) {
   v3 transmission = mk(1.0f, 1.0f, 1.0f);
   v3 ro = p;
   for (int j = 0; j < 4; j++) {
@@ -112,7 +112,7 @@ __device__ __forceinline__ v3 lightVisibilityInline(const float* P, const float4
 template <class W>
 __device__ __noinline__ v3 lightVisibility(const float* P, const float4* Nodes4, const float4* top, const uint32_t* Order,
                                            const TreeInst* trees, v3 p, v3 direction, float time, float wind) {
-  return lightVisibilityInline<W>(P, Nodes4, top, Order, trees, p, direction, time, wind);
+  return lightVisibilityInline<W, const TreeInst*>(P, Nodes4, top, Order, trees, p, direction, time, wind);
 }
 
 
@@ -157,9 +157,9 @@ AD SunRay sunRayOf(const ShadeCtx& sc, int pixel, v3 p, int id) {
 
 // Sun visibility of one pixel from its Hit record: lightVisibility along its
 // sun ray, or 1 for sky or with tree-to-tree visibility off.
-template <class W>
+template <class W, class Trees>
 __device__ inline v3 sunVisibilityOne(const float* P, const float4* Nodes4, const float4* top, const uint32_t* Order,
-                                      const TreeInst* trees, const ShadeCtx& sc, int x, int y, float4 hit) {
+                                      Trees trees, const ShadeCtx& sc, int x, int y, float4 hit) {
   if (!sc.shadows || !(hit.x < kFar)) return mk(1.0f, 1.0f, 1.0f);
   v3 rd = viewRay(sc.view, x, y, sc.width, sc.height);
   v3 p = sc.view.eye + rd * hit.x;

@@ -664,11 +664,12 @@ __device__ __forceinline__ v2 traceTreeSah(const float* P, const float4* Nodes4,
 }
 
 // ---- forest DDA (Arbor.cu:235-251) ----
-// `trees` holds the 324 instancing records (LDS); every value is what the
-// reference recomputes inline per cell.
-template <class W>
+// `trees` holds the 324 instancing records (a TreeInst array in LDS, or
+// TreeRecs in device memory); every value is what the reference recomputes
+// inline per cell.
+template <class W, class Trees>
 __device__ inline v2 traceForest(const float* P, const float4* Nodes4, const float4* top, const uint32_t* Order,
-                                 const TreeInst* trees, v3 ro, v3 rd, float cone, float time, float wind, float best,
+                                 Trees trees, v3 ro, v3 rd, float cone, float time, float wind, float best,
                                  int* treeOut, bool shadows) {
   int foundTree = -1;
   float found = -1.0f;
@@ -688,11 +689,12 @@ __device__ inline v2 traceForest(const float* P, const float4* Nodes4, const flo
   for (int step = 0; step < 48; step++) {
     if (ix < 0 || iz < 0 || ix >= kSide || iz >= kSide) break;
     int tree = iz * kSide + ix;
-    const TreeInst& ti = trees[tree];
-    float sc = ti.sc;
-    v3 centre = mk(ti.px, 0.0f, ti.pz) + mk(0.0f, 4.4f * sc, 0.0f);
+    const TreeHead hd = treeHead(trees, tree);
+    float sc = hd.sc;
+    v3 centre = mk(hd.px, 0.0f, hd.pz) + mk(0.0f, 4.4f * sc, 0.0f);
     float broad = sphereT(ro, rd, centre, 6.6f * sc);
     if (broad < best) {
+      const TreeInst ti = treeRecord(trees, tree, hd);
       v3 lr = toLocalPoint(ro, ti), ld = toLocalDir(rd, ti);
       v2 h = traceTree<W>(P, Nodes4, top, Order, lr, ld, cone / sc, time + ti.phase, wind, best / sc, -1, shadows);
       if (h.y >= 0.0f && h.x * sc < best) { best = h.x * sc; found = h.y; foundTree = tree; }

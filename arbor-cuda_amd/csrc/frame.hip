@@ -34,6 +34,12 @@ constexpr int kWavesPerBlock = kTileBlock / 64;
 __device__ __forceinline__ void fillTrees(TreeInst* trees, int ws) {
   for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, ws);
 }
+// The tree records a kernel reads: its LDS copy, or the device-memory ones.
+template <bool kRecTrees>
+__device__ __forceinline__ auto treeSource(const TreeInst* lds, const float4* recs) {
+  if constexpr (kRecTrees) return TreeRecs{recs};
+  else return lds;
+}
 template <class W>
 __device__ __forceinline__ void fillTop(float4* top, const float4* Nodes4) {
   if (W::ldsTop)
@@ -45,17 +51,21 @@ __device__ __forceinline__ void fillTop(float4* top, const float4* Nodes4) {
 // tile took (wall clock ticks, >= 1) in tileCost; the order changes only which
 // wave traces which tile, never a pixel's result.
 // With a SAH walk policy, accel is the hierarchy of arbor_buildTraceAccel.
-// kMinWaves > 1 caps the registers at that many waves per SIMD.
-template <int kWaves, class W, int kMinWaves = 1>
+// kMinWaves > 1 caps the registers at that many waves per SIMD. With
+// kRecTrees the tree records come from treeRecs (device memory, the tile
+// schedule's copy) instead of LDS, which then holds only the walk's stack.
+template <int kWaves, class W, int kMinWaves = 1, bool kRecTrees = false>
 __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(kMinWaves))) void k_tracePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
                                                              const float* C, float4* Hit, int width, int height,
                                                              int tilesX, int tiles, const uint32_t* tileOrder,
-                                                             uint32_t* tileCost, const float4* accel) {
-  __shared__ TreeInst trees[kTrees];
+                                                             uint32_t* tileCost, const float4* accel,
+                                                             const float4* treeRecs) {
+  __shared__ TreeInst ldsTrees[kRecTrees ? 1 : kTrees];
   __shared__ float4 top[W::ldsTop ? 2 * kTopNodes : 1];
-  fillTrees(trees, (int)C[31]);
+  if constexpr (!kRecTrees) fillTrees(ldsTrees, (int)C[31]);
   fillTop<W>(top, Nodes4);
   __syncthreads();
+  const auto trees = treeSource<kRecTrees>(ldsTrees, treeRecs);
   const View view = makeView(C);
   const float cone = C[14] / (float)height, time = C[8], wind = C[11];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -83,18 +93,20 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(kMi
 // runs the trace kernel's launch shape (one wave64 per 8x8 tile, the walk
 // inlined, optional tile schedule), so shading's register budget no longer
 // limits how many shadow traversals are in flight.
-template <int kWaves, class W, int kMinWaves = 1>
+template <int kWaves, class W, int kMinWaves = 1, bool kRecTrees = false>
 __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(kMinWaves))) void k_sunVisibility(const float* P, const float4* Nodes4,
                                                                const uint32_t* Order, const float* C, const float4* Hit,
                                                                float4* Vis, int width, int height, int tilesX,
                                                                int tiles, const uint32_t* tileOrder,
-                                                               uint32_t* tileCost, const float4* accel) {
-  __shared__ TreeInst trees[kTrees];
+                                                               uint32_t* tileCost, const float4* accel,
+                                                               const float4* treeRecs) {
+  __shared__ TreeInst ldsTrees[kRecTrees ? 1 : kTrees];
   __shared__ float4 top[W::ldsTop ? 2 * kTopNodes : 1];
   if (!(C[22] > 0.5f)) return;  // visibility off: shading never reads Vis
-  fillTrees(trees, (int)C[31]);
+  if constexpr (!kRecTrees) fillTrees(ldsTrees, (int)C[31]);
   fillTop<W>(top, Nodes4);
   __syncthreads();
+  const auto trees = treeSource<kRecTrees>(ldsTrees, treeRecs);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = blockIdx.x * kWaves + wave; i < tiles; i += gridDim.x * kWaves) {
     const int t = tileOrder ? (int)tileOrder[i] : i;
@@ -136,9 +148,10 @@ struct TileSchedule {
   uint32_t persist[2];  // tile counter and finished-wave count of the persistent trace (zero between launches)
   uint32_t pad[60];
   unsigned long long acc[2][kOrderBuckets];  // low word: tiles in the bucket; high word: scatter cursor
+  float4 trees[kTreeRecF4 * kTrees];         // tree records of the world seed C[31] (storeTreeRec)
   // followed by cost[tiles], order[tiles], key[tiles]
 };
-static_assert(sizeof(TileSchedule) == 2560, "schedule header");
+static_assert(sizeof(TileSchedule) == 2560 + 16 * kTreeRecF4 * kTrees, "schedule header");
 __device__ inline int tileKey(const View& cur, const View& prev, bool valid, const uint32_t* cost, int t, int tilesX,
                               int tilesY, int width, int height) {
   if (!valid) return kOrderBuckets - 1;
@@ -230,6 +243,8 @@ __device__ void orderScatter(TileSchedule* s, int par, int tiles, uint32_t* star
 // computes the new camera itself in LDS from the old C (the same bits in
 // every block); block 0 stores it after the barrier, when no block reads the
 // old one any more, and keeps it in each schedule for the next reprojection.
+// Each schedule also gets the tree records of the world seed C[31], which
+// the kRecTrees trace and visibility kernels read.
 __global__ __launch_bounds__(kOrderBlock) void k_framePrologue(float* C, const float* I, float delta, int sample,
                                                                int light, int width, int height, TileSchedule* s0,
                                                                TileSchedule* s1) {
@@ -252,6 +267,12 @@ __global__ __launch_bounds__(kOrderBlock) void k_framePrologue(float* C, const f
     }
   }
   __syncthreads();
+  // the tree records of the world seed, a few trees per block
+  for (int i = threadIdx.x * gridDim.x + blockIdx.x; i < kTrees; i += blockDim.x * gridDim.x) {
+    const TreeInst t = makeTree(i, (int)cam[31]);
+    storeTreeRec(s0->trees, i, t);
+    if (s1) storeTreeRec(s1->trees, i, t);
+  }
   const int p0 = par[0], p1 = par[1];
   orderKeys(cam, s0, p0, tilesX, tilesY, width, height, la);
   if (s1) orderKeys(cam, s1, p1, tilesX, tilesY, width, height, la);
@@ -281,7 +302,8 @@ template <int kWaves, class W>
 __global__ __launch_bounds__(kWaves * 64) void k_traceTreePixels(const float* P, const float4* Nodes4,
                                                                  const uint32_t* Order, const float* C, float4* Hit,
                                                                  int width, int height, int tilesX, int tiles,
-                                                                 const uint32_t*, uint32_t*, const float4* accel) {
+                                                                 const uint32_t*, uint32_t*, const float4* accel,
+                                                                 const float4*) {
   __shared__ float4 top[W::ldsTop ? 2 * kTopNodes : 1];
   fillTop<W>(top, Nodes4);
   __syncthreads();
@@ -415,7 +437,7 @@ inline int tileGrid(int tiles, int wavesPerBlock) {
 }
 
 typedef void (*TraceKernel)(const float*, const float4*, const uint32_t*, const float*, float4*, int, int, int, int,
-                            const uint32_t*, uint32_t*, const float4*);
+                            const uint32_t*, uint32_t*, const float4*, const float4*);
 typedef void (*PersistKernel)(const float*, const float4*, const float*, float4*, int, int, int, int,
                               const uint32_t*, uint32_t*, const float4*, uint32_t*);
 struct TraceVariant {
@@ -423,6 +445,7 @@ struct TraceVariant {
   int waves;
   int ldsPerWave = 0;              // dynamic LDS bytes per wave (the SAH walk's stack entries)
   PersistKernel persist = nullptr;  // the persistent form (forest and tree unused)
+  bool recTrees = false;            // reads the schedule's tree records (scheduled calls only)
 };
 // trace_variant index -> kernels. Identical results for every entry.
 const TraceVariant kTraceVariants[] = {
@@ -456,14 +479,22 @@ int launchTrace(bool forest, const float* P, const float* Nodes, const unsigned*
   const TraceVariant& v = kTraceVariants[tu.trace_variant < kTraceVariantCount ? tu.trace_variant : 5];
   hipLaunchKernelGGL(forest ? v.forest : v.tree, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
-                     tx, tiles, tileOrder, tileCost, nullptr);
+                     tx, tiles, tileOrder, tileCost, nullptr, nullptr);
   return launch_status();
 }
 
-// accel_variant index -> primary-ray kernel over the SAH hierarchy. 8 (the
-// default) runs a wave's object tests once 6/8 of its live lanes wait at a
-// leaf: 3.26 ms for a scheduled 1080p walk frame against 3.37 ms with the
-// if-if order of 3; a threshold of 2/8 took 3.80 ms, 8/8 3.46 ms.
+// accel_variant index -> primary-ray kernel over the SAH hierarchy. 8 runs a
+// wave's object tests once 6/8 of its live lanes wait at a leaf: 3.26 ms for
+// a scheduled 1080p walk frame against 3.37 ms with the if-if order of 3; a
+// threshold of 2/8 took 3.80 ms, 8/8 3.46 ms. The record-tree shapes (13,
+// 14, 16-21) read the 324 tree records from the tile schedule instead of
+// building a copy in LDS per block, so LDS holds the stack. 16 (the
+// default: one-wave blocks, 20 LDS entries, 3 waves per SIMD) traces a
+// 1080p walk frame in 3.22 ms against 3.31 for 8 and, for a hierarchy of
+// depth <= 21, keeps every stack entry out of scratch (PMC WRITE_SIZE about
+// Hit's bytes). 19 and 20 (4 waves per SIMD, registers spilled to scratch)
+// are as fast at 1080p and 12% faster at 4K (7.5-7.7 ms against 8.8), at
+// four times the write traffic.
 const TraceVariant kAccelTraceVariants[] = {
     {k_tracePixels<1, WalkSah>, k_traceTreePixels<1, WalkSah>, 1},
     {k_tracePixels<2, WalkSah>, k_traceTreePixels<2, WalkSah>, 2},
@@ -478,7 +509,30 @@ const TraceVariant kAccelTraceVariants[] = {
     {nullptr, nullptr, 2, sahLdsPerWave(8), k_tracePersistent<2, WalkSahG<8, 6>>},
     {nullptr, nullptr, 1, sahLdsPerWave(8), k_tracePersistent<1, WalkSahG<8, 6>>},
     {nullptr, nullptr, 2, sahLdsPerWave(8), k_tracePersistent<2, WalkSahG<8, 5>>},
+    // 13-15: the tree records in device memory (the schedule's copy), LDS
+    // free for 20 or 24 stack entries per lane; 15 keeps them in LDS with 14
+    {k_tracePixels<2, WalkSahG<20, 6>, 1, true>, k_traceTreePixels<2, WalkSahG<20, 6>>, 2, sahLdsPerWave(20),
+     nullptr, true},
+    {k_tracePixels<2, WalkSahG<24, 6>, 1, true>, k_traceTreePixels<2, WalkSahG<24, 6>>, 2, sahLdsPerWave(24),
+     nullptr, true},
+    {k_tracePixels<2, WalkSahG<14, 6>>, k_traceTreePixels<2, WalkSahG<14, 6>>, 2, sahLdsPerWave(14)},
+    // 16-18: more record-tree shapes (1-wave blocks; 16 entries; 4 waves per SIMD)
+    {k_tracePixels<1, WalkSahG<20, 6>, 1, true>, k_traceTreePixels<1, WalkSahG<20, 6>>, 1, sahLdsPerWave(20),
+     nullptr, true},
+    {k_tracePixels<2, WalkSahG<16, 6>, 1, true>, k_traceTreePixels<2, WalkSahG<16, 6>>, 2, sahLdsPerWave(16),
+     nullptr, true},
+    {k_tracePixels<2, WalkSahG<16, 5>, 4, true>, k_traceTreePixels<2, WalkSahG<16, 5>>, 2, sahLdsPerWave(16),
+     nullptr, true},
+    // 19-21: record-tree shapes with 1-wave blocks
+    {k_tracePixels<1, WalkSahG<16, 5>, 4, true>, k_traceTreePixels<1, WalkSahG<16, 5>>, 1, sahLdsPerWave(16),
+     nullptr, true},
+    {k_tracePixels<1, WalkSahG<20, 5>, 4, true>, k_traceTreePixels<1, WalkSahG<20, 5>>, 1, sahLdsPerWave(20),
+     nullptr, true},
+    {k_tracePixels<1, WalkSahG<24, 6>, 1, true>, k_traceTreePixels<1, WalkSahG<24, 6>>, 1, sahLdsPerWave(24),
+     nullptr, true},
 };
+// the shape an unscheduled call runs in place of a record-tree one
+constexpr int kAccelUnscheduled = 8;
 constexpr int kAccelTraceVariantCount = sizeof(kAccelTraceVariants) / sizeof(kAccelTraceVariants[0]);
 
 // Counters of persistent launches without a tile schedule: one pair per
@@ -518,7 +572,9 @@ int launchTraceAccel(const float* P, const float* Nodes, const unsigned* Order, 
   const uint32_t* tileOrder = nullptr;
   uint32_t* tileCost = nullptr;
   if (schedule) scheduleArrays(schedule, width, height, &tileOrder, &tileCost);
-  const TraceVariant& v = kAccelTraceVariants[tuning().accel_variant];
+  const int tuned = tuning().accel_variant;
+  // an unscheduled call has no tree records: it runs the default shape instead
+  const TraceVariant& v = kAccelTraceVariants[kAccelTraceVariants[tuned].recTrees && !schedule ? kAccelUnscheduled : tuned];
   if (v.persist) {
     uint32_t* counters = schedule ? static_cast<TileSchedule*>(schedule)->persist : unscheduledCounters();
     if (!counters) return ARBOR_ERR_DEVICE;
@@ -530,7 +586,8 @@ int launchTraceAccel(const float* P, const float* Nodes, const unsigned* Order, 
   }
   hipLaunchKernelGGL(v.forest, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), v.ldsPerWave * v.waves, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<float4*>(Hit), width, height,
-                     tx, tiles, tileOrder, tileCost, static_cast<const float4*>(accel));
+                     tx, tiles, tileOrder, tileCost, static_cast<const float4*>(accel),
+                     schedule ? static_cast<const TileSchedule*>(schedule)->trees : nullptr);
   return launch_status();
 }
 }  // namespace
@@ -645,11 +702,12 @@ int arbor_shadePixelsVis(const float* P, const float* Nodes, const unsigned* Ord
 
 namespace {
 typedef void (*VisKernel)(const float*, const float4*, const uint32_t*, const float*, const float4*, float4*, int, int,
-                          int, int, const uint32_t*, uint32_t*, const float4*);
+                          int, int, const uint32_t*, uint32_t*, const float4*, const float4*);
 struct VisVariant {
   VisKernel kernel;
   int waves;
-  int ldsPerWave = 0;  // dynamic LDS bytes per wave (the SAH walk's stack entries)
+  int ldsPerWave = 0;     // dynamic LDS bytes per wave (the SAH walk's stack entries)
+  bool recTrees = false;  // reads the schedule's tree records (scheduled calls only)
 };
 // vis_variant index -> kernel. Identical results for every entry. The
 // kernel keeps the 324-entry tree table (11.7 KB) in LDS per block, so
@@ -670,13 +728,16 @@ int launchVisibility(const float* P, const float* Nodes, const unsigned* Order, 
   const VisVariant& v = kVisVariants[tuning().vis_variant];
   hipLaunchKernelGGL(v.kernel, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
-                     reinterpret_cast<float4*>(Vis), width, height, tx, tiles, tileOrder, tileCost, nullptr);
+                     reinterpret_cast<float4*>(Vis), width, height, tx, tiles, tileOrder, tileCost, nullptr, nullptr);
   return launch_status();
 }
 
 // accel_vis_variant index -> sun-visibility kernel over the SAH hierarchy. 8
-// (the default) gathers object tests at 5/8 of the live lanes: 7.8 ms for
-// 1080p with shadows against 8.8 ms for 0, the same kernel without it.
+// gathers object tests at 5/8 of the live lanes: 7.8 ms for 1080p with
+// shadows against 8.8 ms for 0, the same kernel without it. 15 (the
+// default) reads the tree records from the schedule and keeps 14 stack
+// entries per lane in LDS, one-wave blocks at 5 waves per SIMD: 118-120 ms
+// for a 4-sample 4K frame with sun visibility against 131 ms with 8.
 const VisVariant kAccelVisVariants[] = {
     {k_sunVisibility<2, WalkSah, 5>, 2},
     {k_sunVisibility<1, WalkSah>, 1},
@@ -688,18 +749,45 @@ const VisVariant kAccelVisVariants[] = {
     {k_sunVisibility<4, WalkSahL<4>, 5>, 4, sahLdsPerWave(4)},
     {k_sunVisibility<2, WalkSahG<0, 5>, 5>, 2},
     {k_sunVisibility<2, WalkSahG<8, 5>, 4>, 2, sahLdsPerWave(8)},
+    // 10-12: the tree records in device memory (the schedule's copy), LDS
+    // free for the stack entries
+    {k_sunVisibility<2, WalkSahG<14, 5>, 5, true>, 2, sahLdsPerWave(14), true},
+    {k_sunVisibility<2, WalkSahG<18, 5>, 4, true>, 2, sahLdsPerWave(18), true},
+    {k_sunVisibility<2, WalkSahG<20, 5>, 1, true>, 2, sahLdsPerWave(20), true},
+    // 13-17: more record-tree shapes (16 entries; 6 waves per SIMD; 1-wave
+    // blocks; gathering at 4/8 and 6/8)
+    {k_sunVisibility<2, WalkSahG<16, 5>, 5, true>, 2, sahLdsPerWave(16), true},
+    {k_sunVisibility<2, WalkSahG<12, 5>, 6, true>, 2, sahLdsPerWave(12), true},
+    {k_sunVisibility<1, WalkSahG<14, 5>, 5, true>, 1, sahLdsPerWave(14), true},
+    {k_sunVisibility<2, WalkSahG<14, 4>, 5, true>, 2, sahLdsPerWave(14), true},
+    {k_sunVisibility<2, WalkSahG<14, 6>, 5, true>, 2, sahLdsPerWave(14), true},
+    // 18-21: record-tree shapes with 1-wave blocks
+    {k_sunVisibility<1, WalkSahG<16, 5>, 5, true>, 1, sahLdsPerWave(16), true},
+    {k_sunVisibility<1, WalkSahG<12, 5>, 6, true>, 1, sahLdsPerWave(12), true},
+    {k_sunVisibility<1, WalkSahG<14, 6>, 5, true>, 1, sahLdsPerWave(14), true},
+    {k_sunVisibility<1, WalkSahG<20, 5>, 4, true>, 1, sahLdsPerWave(20), true},
 };
+// the shape an unscheduled call runs in place of a record-tree one
+constexpr int kAccelVisUnscheduled = 8;
 constexpr int kAccelVisVariantCount = sizeof(kAccelVisVariants) / sizeof(kAccelVisVariants[0]);
 
+// schedule: the ordered tile schedule whose order, cost and tree records the
+// pass uses, or null.
 int launchVisibilityAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,
-                          const float* C, const float* Hit, float* Vis, int width, int height,
-                          const uint32_t* tileOrder, uint32_t* tileCost, hipStream_t stream) {
+                          const float* C, const float* Hit, float* Vis, int width, int height, void* schedule,
+                          hipStream_t stream) {
   int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
-  const VisVariant& v = kAccelVisVariants[tuning().accel_vis_variant];
+  const uint32_t* tileOrder = nullptr;
+  uint32_t* tileCost = nullptr;
+  if (schedule) scheduleArrays(schedule, width, height, &tileOrder, &tileCost);
+  const int tuned = tuning().accel_vis_variant;
+  // an unscheduled call has no tree records: it runs the default shape instead
+  const VisVariant& v = kAccelVisVariants[kAccelVisVariants[tuned].recTrees && !schedule ? kAccelVisUnscheduled : tuned];
   hipLaunchKernelGGL(v.kernel, dim3(tileGrid(tiles, v.waves)), dim3(v.waves * 64), v.ldsPerWave * v.waves, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Vis), width, height, tx, tiles, tileOrder, tileCost,
-                     static_cast<const float4*>(accel));
+                     static_cast<const float4*>(accel),
+                     schedule ? static_cast<const TileSchedule*>(schedule)->trees : nullptr);
   return launch_status();
 }
 }  // namespace
@@ -747,7 +835,7 @@ int arbor_sunVisibilityAccel(const float* P, const float* Nodes, const unsigned*
     int rc = orderTiles(C, schedule, width, height, stream, &order, &cost);
     if (rc != ARBOR_OK) return rc;
   }
-  return launchVisibilityAccel(P, Nodes, Order, accel, C, Hit, Vis, width, height, order, cost, stream);
+  return launchVisibilityAccel(P, Nodes, Order, accel, C, Hit, Vis, width, height, schedule, stream);
 }
 
 }  // extern "C"
@@ -759,7 +847,7 @@ namespace {
 // names the symbol; its device address comes from hipGetSymbolAddress, once
 // per device with that device current.
 constexpr int kNoAllocTiles = 512 * 512;  // 4096 x 4096 pixels
-constexpr size_t kNoAllocBytes = sizeof(TileSchedule) + 12 * (size_t)kNoAllocTiles;  // 3,148,288 bytes
+constexpr size_t kNoAllocBytes = sizeof(TileSchedule) + 12 * (size_t)kNoAllocTiles;  // 3,163,840 bytes
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 void* scratchOnDevice() {
@@ -956,18 +1044,35 @@ int arbor_autotune_trace_accel(const float* P, const float* Nodes, const unsigne
   const int saved = t.accel_variant;
   float ms[kAccelTraceVariantCount] = {};
   int best = 0, rc = ARBOR_OK;
+  // Every shape traces in the same tile order, the one the engine would use
+  // for this view: a schedule of its own, ordered twice around one trace of
+  // shape kAccelUnscheduled (the second ordering from that trace's costs).
+  const long long schedBytes = arbor_tile_schedule_bytes(width, height);
+  void* sched = nullptr;
+  if (hipMalloc(&sched, (size_t)schedBytes) != hipSuccess) rc = ARBOR_ERR_DEVICE;
+  if (rc == ARBOR_OK && hipMemsetAsync(sched, 0, (size_t)schedBytes, stream) != hipSuccess) rc = ARBOR_ERR_DEVICE;
+  const uint32_t* order = nullptr;
+  uint32_t* cost = nullptr;
+  if (rc == ARBOR_OK) rc = orderTiles(C, sched, width, height, stream, &order, &cost);
+  if (rc == ARBOR_OK) {
+    t.accel_variant = kAccelUnscheduled;
+    rc = launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, sched);
+  }
+  if (rc == ARBOR_OK) rc = orderTiles(C, sched, width, height, stream, &order, &cost);
   for (int v = 0; v < kAccelTraceVariantCount && rc == ARBOR_OK; v++) {
     t.accel_variant = v;
-    rc = launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, nullptr);
+    rc = launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, sched);
     (void)hipEventRecord(e0, stream);
     for (int r = 0; r < reps && rc == ARBOR_OK; r++)
-      rc = launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, nullptr);
+      rc = launchTraceAccel(P, Nodes, Order, accel, C, Hit, width, height, stream, sched);
     (void)hipEventRecord(e1, stream);
     if (hipEventSynchronize(e1) != hipSuccess) rc = ARBOR_ERR_DEVICE;
     (void)hipEventElapsedTime(&ms[v], e0, e1);
     ms[v] /= (float)reps;
     if (ms[v] < ms[best]) best = v;
   }
+  (void)hipStreamSynchronize(stream);
+  if (sched) (void)hipFree(sched);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   t.accel_variant = rc == ARBOR_OK ? best : saved;
@@ -1011,7 +1116,7 @@ int visibilityOrdered(const float* P, const float* Nodes, const unsigned* Order,
   const uint32_t* order = nullptr;
   uint32_t* cost = nullptr;
   scheduleArrays(schedule, width, height, &order, &cost);
-  if (accel) return launchVisibilityAccel(P, Nodes, Order, accel, C, Hit, Vis, width, height, order, cost, stream);
+  if (accel) return launchVisibilityAccel(P, Nodes, Order, accel, C, Hit, Vis, width, height, schedule, stream);
   return launchVisibility(P, Nodes, Order, C, Hit, Vis, width, height, order, cost, stream);
 }
 

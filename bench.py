@@ -34,10 +34,11 @@ sys.path.insert(0, os.path.join(HERE, "arbor-cuda_amd"))
 TRACE_FLOP_PER_RAY = 5030.3
 TRACE_FLOP_PER_RAY_HEAP = 8785.9
 # HBM bytes per scheduled SAH-walk trace launch on this workload (walk frame
-# 0): rocprofv3 --pmc FETCH_SIZE 40,390 KiB x 2.0 + WRITE_SIZE 53,180 KiB x
-# 1.0, the factors being what the same counters read for a 256 MiB device copy
-# (131,097 KiB and 262,144 KiB); profiles/tracePixels_forest1080p_pmc.md.
-TRACE_HBM_BYTES = (40390.0 * 2.0 + 53180.0 * 1.0) * 1024
+# 0, the default launch shape): rocprofv3 --pmc FETCH_SIZE 33,595 KiB x 2.0 +
+# WRITE_SIZE 33,370 KiB x 1.0, the factors being what the same counters read
+# for a 256 MiB device copy (131,090 KiB and 262,144 KiB);
+# profiles/tracePixels_forest1080p_pmc.md.
+TRACE_HBM_BYTES = (33595.0 * 2.0 + 33370.0 * 1.0) * 1024
 FP32_PEAK_TFLOPS = 157.3
 # One non-packed FP32 VALU op per lane per clock: 256 CUs x 64 lanes x 2.4 GHz.
 FP32_SINGLE_ISSUE_TOPS = 39.32
@@ -63,6 +64,9 @@ def parse():
     ap.add_argument("--accel-variant", type=int, default=-1, metavar="V",
                     help="SAH-walk trace launch shape (the tuning key accel_variant) for this run; -1 keeps the "
                          "library default")
+    ap.add_argument("--accel-vis-variant", type=int, default=-1, metavar="V",
+                    help="SAH-walk sun-visibility launch shape (the tuning key accel_vis_variant) for this run; -1 "
+                         "keeps the library default")
     ap.add_argument("--views", type=int, default=0, metavar="N",
                     help="batched-views workload instead: N seeded walk views of uneven resolution rendered by one "
                          "hipGraph-captured batch per step (about 0.3 s per step at N=64; use --steps 10)")
@@ -526,6 +530,10 @@ def main():
         import pyarbor
 
         pyarbor.tune_set("accel_variant", args.accel_variant)
+    if args.accel_vis_variant >= 0:
+        import pyarbor
+
+        pyarbor.tune_set("accel_vis_variant", args.accel_vis_variant)
     if args.profile_trace:
         profile_trace(args)
         return

@@ -135,7 +135,10 @@ int arbor_shadePixelsAOV(const float* P, const float* Nodes, const unsigned* Ord
  * longest-predicted-first (the previous call's per-tile trace times,
  * reprojected through the camera change), traces them in that order and
  * records this call's times. Hit is identical to arbor_tracePixels'. One
- * schedule per view size; concurrent calls must not share it. */
+ * schedule per view size; concurrent calls must not share it. Each ordering
+ * also writes the 324 tree records of the world seed C[31] into it, which
+ * the SAH walk's record-tree launch shapes read (an unscheduled call of
+ * such a shape runs the default shape instead). */
 long long arbor_tile_schedule_bytes(int width, int height);
 int arbor_tracePixelsScheduled(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
                                int width, int height, void* schedule, hipStream_t stream);
@@ -225,7 +228,9 @@ int arbor_tune_load(const char* path);
 int arbor_autotune_trace(const float* P, const float* Nodes, const unsigned* Order, const float* C, float* Hit,
                          int width, int height, int reps, hipStream_t stream, char* json, int cap);
 /* The same over the SAH hierarchy: times every "accel_variant" and keeps the
- * fastest (no counterpart in the reference). */
+ * fastest (no counterpart in the reference). The shapes trace scheduled, in
+ * the tile order the engine would use for the view (a tile schedule the call
+ * allocates and frees). */
 int arbor_autotune_trace_accel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,
                                const float* C, float* Hit, int width, int height, int reps, hipStream_t stream,
                                char* json, int cap);

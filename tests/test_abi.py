@@ -164,7 +164,7 @@ def test_trace_accel_size_and_checks(L):
     assert L.arbor_engine_set_accel(NULL, 1) == E
     assert L.arbor_autotune_trace_accel(DUMMY, DUMMY, DUMMY, NULL, DUMMY, DUMMY, 64, 64, 1, NULL, NULL, 0) == E
     assert L.arbor_autotune_trace_accel(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 64, 64, 0, NULL, NULL, 0) == -2
-    for key, n in ((b"accel_variant", 13), (b"accel_vis_variant", 10)):
+    for key, n in ((b"accel_variant", 22), (b"accel_vis_variant", 22)):
         old = L.arbor_tune_get(key)
         assert L.arbor_tune_set(key, n) == -2 and L.arbor_tune_set(key, -1) == -2
         assert L.arbor_tune_set(key, n - 1) == 0 and L.arbor_tune_get(key) == n - 1
@@ -172,11 +172,12 @@ def test_trace_accel_size_and_checks(L):
 
 
 def test_tile_schedule_size_and_checks(L):
-    # 2560-byte header (previous camera, valid flag, accumulator parity, two sets of
-    # 128 bucket accumulators: tile count and scatter cursor)
+    # 18,112-byte header (previous camera, valid flag, accumulator parity, two sets of
+    # 128 bucket accumulators: tile count and scatter cursor; 324 tree records of 48 B)
     # + cost, order and key per 8x8 tile
-    assert L.arbor_tile_schedule_bytes(1920, 1080) == 2560 + 12 * 240 * 135
-    assert L.arbor_tile_schedule_bytes(33, 9) == 2560 + 12 * 5 * 2
+    header = 2560 + 324 * 48
+    assert L.arbor_tile_schedule_bytes(1920, 1080) == header + 12 * 240 * 135
+    assert L.arbor_tile_schedule_bytes(33, 9) == header + 12 * 5 * 2
     assert L.arbor_tile_schedule_bytes(0, 10) == -2
     assert L.arbor_tracePixelsScheduled(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 64, 64, NULL, NULL) == -1
     assert L.arbor_tracePixelsScheduled(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 0, 64, DUMMY, NULL) == -2

@@ -17,8 +17,11 @@ sun-visibility passes over it.
 import numpy as np
 import pytest
 
-# accel_variant 0-9: tile-per-wave SAH kernels; 10-12: the persistent form
-ACCEL_VARIANTS = 13
+# accel_variant 0-9 and 13-21: tile-per-wave SAH kernels (13, 14 and 16-21 read
+# the tree records from the schedule); 10-12: the persistent form
+ACCEL_VARIANTS = 22
+# accel_vis_variant 0-21 (10-21 read the tree records from the schedule)
+ACCEL_VIS_VARIANTS = 22
 
 from accel_check import check_accel_structure
 from gpu_util import require_gpu, to_dev, to_host
@@ -137,12 +140,17 @@ def test_accel_launch_shapes_and_schedule_change_nothing(px, oracle, scene):
                 assert np.array_equal(to_host(fb.Hit).view(np.uint32), want.view(np.uint32)), (v, s is None)
         import torch
 
-        vis = [torch.zeros(W * H * 4, dtype=torch.float32, device="cuda") for _ in range(10)]
-        for v in range(10):
+        vis = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+        assert lib.arbor_tune_set(b"accel_vis_variant", 0) == 0
+        px.sun_visibility_accel(dev, acc, C, ref_fb, vis)
+        want_vis = to_host(vis).view(np.uint32).copy()
+        vsched = px.TileSchedule(W, H)
+        for v in range(ACCEL_VIS_VARIANTS):
             assert lib.arbor_tune_set(b"accel_vis_variant", v) == 0
-            px.sun_visibility_accel(dev, acc, C, ref_fb, vis[v], sched if v % 2 else None)
-        for v in range(1, 10):
-            assert np.array_equal(to_host(vis[v]).view(np.uint32), to_host(vis[0]).view(np.uint32)), v
+            for s in (None, vsched):
+                vis.zero_()
+                px.sun_visibility_accel(dev, acc, C, ref_fb, vis, s)
+                assert np.array_equal(to_host(vis).view(np.uint32), want_vis), (v, s is None)
     finally:
         lib.arbor_tune_set(b"accel_variant", old_t)
         lib.arbor_tune_set(b"accel_vis_variant", old_v)

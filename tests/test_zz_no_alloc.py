@@ -21,7 +21,7 @@ def test_no_alloc_trace_equals_scheduled_trace(oracle):
     scene = px.DeviceScene().generate(7)
     acc = px.TraceAccel(scene)
     lib = px.lib()
-    assert lib.arbor_no_alloc_scratch_bytes() == 2560 + 12 * 512 * 512
+    assert lib.arbor_no_alloc_scratch_bytes() == 2560 + 324 * 48 + 12 * 512 * 512
     C = oracle.init_camera(7)
     for (w, h), frames in (((640, 360), 4), ((328, 184), 3)):
         print("case: no-alloc trace %dx%d after %d walk frames" % (w, h, frames), flush=True)
