@@ -26,17 +26,63 @@
 #include <vector>
 
 #include "arbor_denoise.h"
+#include "arbor_filter.hpp"
+#include "engine_passes.hpp"
 #include "launch.hpp"
 
 namespace {
 
 using bf16 = __bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kInC = 16, kOutC = 16, kMaxLevels = 5;
 constexpr int kDefaultWidths[kMaxLevels] = {32, 48, 64, 96, 128};
 constexpr float kFar = 100000.0f;
 
 // ---- data kernels ----
+// log(1 + x) for x >= 0 within about 1e-6 relative: below 1/16 its Taylor
+// series to x^6 (truncation x^6 / 7 relative, under 2e-8), above it the
+// hardware logarithm of 1 + x (the rounding of 1 + x costs at most 2^-24 /
+// log(1 + 1/16), 1e-6 relative). The inputs are rounded to bf16 (2^-8).
+// Both forms are evaluated and one selected (no branch: neighbouring pixels
+// take either side).
+__device__ __forceinline__ float log1pFast(float x) {
+  const float series = x * (1.0f - x * (0.5f - x * (1.0f / 3.0f - x * (0.25f - x * (0.2f - x * (1.0f / 6.0f))))));
+  const float hw = __logf(1.0f + x);
+  return x < 0.0625f ? series : hw;
+}
+// exp(y) - 1 within about 2e-6 relative: below |y| = 1/16 its Taylor series
+// to y^6 (truncation under 1e-10 relative), above it the hardware
+// exponential (1 ulp) minus 1, which magnifies its error at most
+// e^(1/16) / (e^(1/16) - 1) = 16.5 times.
+__device__ __forceinline__ float expm1Fast(float y) {
+  const float series =
+      y * (1.0f + y * (0.5f + y * (1.0f / 6.0f + y * (1.0f / 24.0f + y * (1.0f / 120.0f + y * (1.0f / 720.0f))))));
+  const float hw = __expf(y) - 1.0f;
+  return fabsf(y) < 0.0625f ? series : hw;
+}
+// The ten used input features of one pixel from its Linear, Guide, Normal
+// and Hit values.
+__device__ __forceinline__ void featuresOf(const float4 l, const float4 g, const float4 n, const float4 t, float* f) {
+  f[0] = log1pFast(fmaxf(l.x, 0.0f));
+  f[1] = log1pFast(fmaxf(l.y, 0.0f));
+  f[2] = log1pFast(fmaxf(l.z, 0.0f));
+  f[3] = g.x;
+  f[4] = g.y;
+  f[5] = g.z;
+  f[6] = n.x;
+  f[7] = n.y;
+  f[8] = n.z;
+  f[9] = t.x < kFar ? log1pFast(fmaxf(fminf(t.x, 1000.0f), 0.0f)) * (1.0f / 7.0f) : 1.0f;
+}
+// The same of pixel i of the frame buffers.
+__device__ __forceinline__ void pixelFeatures(const float4* __restrict__ Linear, const float4* __restrict__ Guide,
+                                              const float4* __restrict__ Normal, const float4* __restrict__ Hit,
+                                              int i, float* f) {
+  featuresOf(Linear[i], Guide[i], Normal[i], Hit[i], f);
+}
 __global__ void k_features(const float4* __restrict__ Linear, const float4* __restrict__ Guide,
                            const float4* __restrict__ Normal, const float4* __restrict__ Hit, int w, int h,
                            bf16* __restrict__ X, int H, int W) {
@@ -44,20 +90,7 @@ __global__ void k_features(const float4* __restrict__ Linear, const float4* __re
   if (x >= W) return;
   float f[kInC];
   for (int c = 0; c < kInC; c++) f[c] = 0.0f;
-  if (x < w && y < h) {
-    const int i = y * w + x;
-    const float4 l = Linear[i], g = Guide[i], n = Normal[i], t = Hit[i];
-    f[0] = log1pf(fmaxf(l.x, 0.0f));
-    f[1] = log1pf(fmaxf(l.y, 0.0f));
-    f[2] = log1pf(fmaxf(l.z, 0.0f));
-    f[3] = g.x;
-    f[4] = g.y;
-    f[5] = g.z;
-    f[6] = n.x;
-    f[7] = n.y;
-    f[8] = n.z;
-    f[9] = t.x < kFar ? log1pf(fminf(t.x, 1000.0f)) * (1.0f / 7.0f) : 1.0f;
-  }
+  if (x < w && y < h) pixelFeatures(Linear, Guide, Normal, Hit, y * w + x, f);
   bf16* o = X + ((size_t)y * W + x) * kInC;
   for (int c = 0; c < kInC; c++) o[c] = (bf16)f[c];
 }
@@ -106,13 +139,190 @@ __global__ void k_sumTo(const float* __restrict__ part, int n, float scale, floa
     *out = s * scale;
   }
 }
+// ---- a 1-level network of 16 channels in one pass ----
+// Input features, the three convolutions and the output of one frame for a
+// 32x16-pixel tile per block, the intermediate tensors in LDS: each layer is
+// computed on the tile plus the halo the later layers read (38x22 features,
+// then 36x20 and 34x18 activations, 32 B per pixel). Every convolution issues
+// the v_mfma_f32_16x16x32_bf16 sequence of arbor_dn_conv3x3 for 16 input
+// channels (per tap, in tap order: 16 output channels x 16 pixels, K = the 16
+// channels and 16 zero slots whose weights repeat channels 8-15), adds the
+// bias, applies ReLU and rounds to bf16 as it does, and positions outside
+// the slot are zero as its padding is: the output has the layered network's
+// bits. With kResolve each pixel is also resolved (resolvePixels).
+namespace fused {
+constexpr int kTW = 32, kTH = 16, kC = 16;
+constexpr int kFW = kTW + 6, kFH = kTH + 6;  // features
+constexpr int kAW = kTW + 4, kAH = kTH + 4;  // first convolution
+constexpr int kBW = kTW + 2, kBH = kTH + 2;  // second convolution
+constexpr int kFPix = kFW * kFH, kAPix = kAW * kAH, kBPix = kBW * kBH;
+constexpr int kThreads = 256;
+constexpr int kXcds = 8;  // accelerator complex dies, each with its own L2
+static_assert(kBPix <= kFPix, "the second convolution's region reuses the features' LDS");
+// Zero pixels read by the lanes whose K slots hold no channel: every tap's
+// offset from the region's first pixel stays inside.
+constexpr int kZeroPix = 2 * kFW + 3;
+
+__device__ __forceinline__ bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int j = 0; j < 8; j++) z[j] = (bf16)0.0f;
+  return z;
+}
+// The nine taps' weight fragments of lane (row = output channel, g) for a
+// 16 -> 16 convolution [9][16][16]: channels 8g..8g+7, and 8-15 for g >= 2
+// (those K slots multiply zero pixels).
+__device__ __forceinline__ void loadTaps(const bf16* __restrict__ Wt, int row, int g, bf16x8* w) {
+  const int c = 8 * (g < 2 ? g : 1);
+#pragma unroll
+  for (int tap = 0; tap < 9; tap++) w[tap] = *reinterpret_cast<const bf16x8*>(Wt + (tap * kC + row) * kC + c);
+}
+// The 3x3 convolutions of target pixels p0 and p1 (region wd wide) from the
+// source region (wd + 2 wide) at src: lane (row, g) gets channels 4g..4g+3
+// of the pixel with index p0 (p1) of its 16-pixel group (row = its place in
+// the group). Lanes g >= 2 read their zero operand from the zero pixels, so
+// every lane issues the same LDS reads (no branch) before the products; the
+// two groups' products alternate, so neither waits on its own previous one.
+__device__ __forceinline__ void convPair(const bf16* src, const bf16* zeros, int wd, int p0, int p1, int g,
+                                         const bf16x8* w, f32x4& acc0, f32x4& acc1) {
+  const int ws = wd + 2;
+  const int ty0 = p0 / wd, tx0 = p0 - ty0 * wd, ty1 = p1 / wd, tx1 = p1 - ty1 * wd;
+  const bf16* b0 = g < 2 ? src + (ty0 * ws + tx0) * kC + 8 * g : zeros;
+  const bf16* b1 = g < 2 ? src + (ty1 * ws + tx1) * kC + 8 * g : zeros;
+  bf16x8 a0[9], a1[9];
+#pragma unroll
+  for (int tap = 0; tap < 9; tap++) {
+    const int off = ((tap / 3) * ws + tap - 3 * (tap / 3)) * kC;
+    a0[tap] = *reinterpret_cast<const bf16x8*>(b0 + off);
+    a1[tap] = *reinterpret_cast<const bf16x8*>(b1 + off);
+  }
+  acc0 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  acc1 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int tap = 0; tap < 9; tap++) {
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tap], a0[tap], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tap], a1[tap], acc1, 0, 0, 0);
+  }
+}
+// One ReLU convolution of a whole region: dst (wd x hd, origin (yo, xo) in
+// slot coordinates) from src; two 16-pixel groups per wave at a time.
+__device__ __forceinline__ void convRegion(const bf16* src, const bf16* zeros, bf16* dst, int wd, int hd, int yo,
+                                           int xo, const bf16* __restrict__ Wt, const float* __restrict__ bias,
+                                           int H, int W) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, row = lane & 15, g = lane >> 4;
+  bf16x8 w[9];
+  loadTaps(Wt, row, g, w);
+  float bv[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) bv[i] = bias[4 * g + i];
+  const int n = wd * hd;
+  for (int base = 32 * wave; base < n; base += 32 * (kThreads / 64)) {
+    const int p0 = min(base + row, n - 1), p1 = min(base + 16 + row, n - 1);
+    f32x4 a0, a1;
+    convPair(src, zeros, wd, p0, p1, g, w, a0, a1);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int q = base + 16 * k + row;
+      if (q >= n) continue;
+      const f32x4 acc = k ? a1 : a0;
+      const int ty = q / wd, tx = q - ty * wd, y = yo + ty, x = xo + tx;
+      const bool inSlot = y >= 0 && y < H && x >= 0 && x < W;
+      bf16x4 o;
+#pragma unroll
+      for (int i = 0; i < 4; i++) o[i] = inSlot ? (bf16)fmaxf(acc[i] + bv[i], 0.0f) : (bf16)0.0f;
+      *reinterpret_cast<bf16x4*>(dst + q * kC + 4 * g) = o;
+    }
+  }
+}
+}  // namespace fused
+
+template <bool kResolve>
+__global__ __launch_bounds__(fused::kThreads) void k_denoise1(
+    const float4* __restrict__ Linear, const float4* __restrict__ Guide, const float4* __restrict__ Normal,
+    const float4* __restrict__ Hit, int w, int h, int H, int W, const bf16* __restrict__ W0,
+    const float* __restrict__ B0, const bf16* __restrict__ W1, const float* __restrict__ B1,
+    const bf16* __restrict__ W2, const float* __restrict__ B2, float4* __restrict__ Out, const float* C,
+    float4* History, uint32_t* Pixels) {
+  using namespace fused;
+  __shared__ __attribute__((aligned(16))) bf16 featB[kFPix * kC];  // features, then the second convolution
+  __shared__ __attribute__((aligned(16))) bf16 actA[kAPix * kC];   // the first convolution
+  __shared__ __attribute__((aligned(16))) bf16 zeros[kZeroPix * kC];
+  // Blocks are dealt to the 8 XCDs in turn: block b takes tile t so that
+  // each XCD walks a contiguous run of tiles (row-major), and the halo a
+  // tile shares with its neighbours is read through that XCD's L2.
+  const int tilesX = (w + kTW - 1) / kTW, tiles = tilesX * ((h + kTH - 1) / kTH);
+  const int per = (tiles + kXcds - 1) / kXcds, t = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  if (t >= tiles) return;
+  const int x0 = (t % tilesX) * kTW, y0 = (t / tilesX) * kTH;
+  for (int i = threadIdx.x; i < kZeroPix * 2; i += kThreads) *reinterpret_cast<bf16x8*>(zeros + 8 * i) = zero8();
+  // Features: every load of the thread's pixels is issued before the first
+  // is used (pixel 0 stands in outside the image).
+  constexpr int kIter = (kFPix + kThreads - 1) / kThreads;
+  float4 lq[kIter], gq[kIter], nq[kIter], tq[kIter];
+  bool in[kIter];
+#pragma unroll
+  for (int s = 0; s < kIter; s++) {
+    const int p = threadIdx.x + s * kThreads, fy = p / kFW, y = y0 - 3 + fy, x = x0 - 3 + (p - fy * kFW);
+    in[s] = p < kFPix && y >= 0 && y < h && x >= 0 && x < w;
+    const int i = in[s] ? y * w + x : 0;
+    lq[s] = Linear[i];
+    gq[s] = Guide[i];
+    nq[s] = Normal[i];
+    tq[s] = Hit[i];
+  }
+#pragma unroll
+  for (int s = 0; s < kIter; s++) {
+    const int p = threadIdx.x + s * kThreads;
+    if (p >= kFPix) continue;
+    float f[10];
+#pragma unroll
+    for (int c = 0; c < 10; c++) f[c] = 0.0f;
+    if (in[s]) featuresOf(lq[s], gq[s], nq[s], tq[s], f);
+    bf16x8 lo, hi = zero8();
+#pragma unroll
+    for (int c = 0; c < 8; c++) lo[c] = (bf16)f[c];
+    hi[0] = (bf16)f[8];
+    hi[1] = (bf16)f[9];
+    *reinterpret_cast<bf16x8*>(featB + p * kC) = lo;
+    *reinterpret_cast<bf16x8*>(featB + p * kC + 8) = hi;
+  }
+  __syncthreads();
+  convRegion(featB, zeros, actA, kAW, kAH, y0 - 2, x0 - 2, W0, B0, H, W);
+  __syncthreads();
+  convRegion(actA, zeros, featB, kBW, kBH, y0 - 1, x0 - 1, W1, B1, H, W);
+  __syncthreads();
+  // the linear output convolution; lanes of group 0 hold channels 0-3
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, row = lane & 15, g = lane >> 4;
+  bf16x8 wt[9];
+  loadTaps(W2, row, g, wt);
+  float bv[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) bv[i] = B2[4 * g + i];
+  for (int base = 32 * wave; base < kTW * kTH; base += 32 * (kThreads / 64)) {
+    f32x4 a0, a1;
+    convPair(featB, zeros, kTW, base + row, base + 16 + row, g, wt, a0, a1);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int q = base + 16 * k + row, ty = q / kTW, y = y0 + ty, x = x0 + (q - ty * kTW);
+      if (g != 0 || y >= h || x >= w) continue;
+      const f32x4 acc = k ? a1 : a0;
+      const float r = fmaxf(expm1Fast((float)(bf16)(acc[0] + bv[0])), 0.0f);
+      const float gg = fmaxf(expm1Fast((float)(bf16)(acc[1] + bv[1])), 0.0f);
+      const float b = fmaxf(expm1Fast((float)(bf16)(acc[2] + bv[2])), 0.0f);
+      const int i = y * w + x;
+      Out[i] = make_float4(r, gg, b, 1.0f);
+      if constexpr (kResolve) arbor::resolveOne(arbor::mk(r, gg, b), History, Pixels, C, x, y, w, h, i);
+    }
+  }
+}
+
 // Output slot -> radiance (expm1 of the prediction, clamped at 0), alpha 1.
 __global__ void k_output(const bf16* __restrict__ Y, int H, int W, float4* __restrict__ Out, int w, int h) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   if (x >= w || y >= h) return;
   const bf16* o = Y + ((size_t)y * W + x) * kOutC;
-  Out[y * w + x] = make_float4(fmaxf(expm1f((float)o[0]), 0.0f), fmaxf(expm1f((float)o[1]), 0.0f),
-                               fmaxf(expm1f((float)o[2]), 0.0f), 1.0f);
+  Out[y * w + x] = make_float4(fmaxf(expm1Fast((float)o[0]), 0.0f), fmaxf(expm1Fast((float)o[1]), 0.0f),
+                               fmaxf(expm1Fast((float)o[2]), 0.0f), 1.0f);
 }
 // Adam on fp32 master values; also writes the bf16 copy when w16 != NULL.
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
@@ -536,6 +746,57 @@ int arbor_denoiser_forward(arbor_denoiser* d, int n, hipStream_t stream) {
   ARBOR_REQUIRE(d);
   if (n < 1 || n > d->maxBatch) return ARBOR_ERR_ARG;
   return forward(d, n, stream);
+}
+
+}  // extern "C"
+
+namespace {
+// Whether the network runs as the one-pass kernel (1 level of 16 channels).
+bool fusedShape(const arbor_denoiser* d) { return d->levels == 1 && d->widths[0] == fused::kC; }
+
+// One frame through the network into Out (and, with History and Pixels, the
+// resolve): the one-pass kernel for the 1-level 16-channel shape, else the
+// input features, forward and output on slot 0 (then resolvePixels).
+int denoiseFrame(arbor_denoiser* d, const float* Linear, const float* Guide, const float* Normal, const float* Hit,
+                 int w, int h, float* Out, float* History, unsigned* Pixels, const float* C, hipStream_t stream) {
+  if (!slotOk(d, 0, w, h)) return ARBOR_ERR_ARG;
+  const bool resolve = History != nullptr;
+  if (fusedShape(d)) {
+    const Conv &c0 = d->conv[0], &c1 = d->conv[1], &c2 = d->conv[2];
+    const int tiles = ((w + fused::kTW - 1) / fused::kTW) * ((h + fused::kTH - 1) / fused::kTH);
+    const dim3 grid(fused::kXcds * ((tiles + fused::kXcds - 1) / fused::kXcds));
+    auto kernel = resolve ? k_denoise1<true> : k_denoise1<false>;
+    hipLaunchKernelGGL(kernel, grid, dim3(fused::kThreads), 0, stream, reinterpret_cast<const float4*>(Linear),
+                       reinterpret_cast<const float4*>(Guide), reinterpret_cast<const float4*>(Normal),
+                       reinterpret_cast<const float4*>(Hit), w, h, d->H, d->W,
+                       reinterpret_cast<const bf16*>(c0.w16), c0.b, reinterpret_cast<const bf16*>(c1.w16), c1.b,
+                       reinterpret_cast<const bf16*>(c2.w16), c2.b, reinterpret_cast<float4*>(Out), C,
+                       reinterpret_cast<float4*>(History), reinterpret_cast<uint32_t*>(Pixels));
+    return arbor::launch_status();
+  }
+  DN_TRY(arbor_denoiser_set_input(d, 0, Linear, Guide, Normal, Hit, w, h, stream));
+  DN_TRY(arbor_denoiser_forward(d, 1, stream));
+  DN_TRY(arbor_denoiser_get_output(d, 0, Out, w, h, stream));
+  if (resolve) return arbor_resolvePixels(Out, History, Pixels, C, w, h, stream);
+  return ARBOR_OK;
+}
+}  // namespace
+
+namespace arbor {
+int denoiseResolve(arbor_denoiser* d, const float* Linear, const float* Guide, const float* Normal, const float* Hit,
+                   float* Filtered, float* History, unsigned* Pixels, const float* C, int w, int h,
+                   hipStream_t stream) {
+  if (!d || !Linear || !Guide || !Normal || !Hit || !Filtered || !History || !Pixels || !C) return ARBOR_ERR_NULL;
+  return denoiseFrame(d, Linear, Guide, Normal, Hit, w, h, Filtered, History, Pixels, C, stream);
+}
+}  // namespace arbor
+
+extern "C" {
+
+int arbor_denoiser_denoise(arbor_denoiser* d, const float* Linear, const float* Guide, const float* Normal,
+                           const float* Hit, int w, int h, float* Out, hipStream_t stream) {
+  ARBOR_REQUIRE(d && Linear && Guide && Normal && Hit && Out);
+  return denoiseFrame(d, Linear, Guide, Normal, Hit, w, h, Out, nullptr, nullptr, nullptr, stream);
 }
 
 int arbor_denoiser_get_output(arbor_denoiser* d, int index, float* Out, int w, int h, hipStream_t stream) {

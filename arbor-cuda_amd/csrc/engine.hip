@@ -124,7 +124,8 @@ namespace {
 // shading (3-4) and the filter + resolve.
 // With a denoiser attached and a frame that fits its slot, shading also
 // writes the normal and step 5 becomes the network's input features, its
-// forward pass and output into Filtered (slot 0), then resolvePixels.
+// forward pass and output into Filtered (slot 0), then resolvePixels: one
+// launch for a 1-level 16-channel network, so the frame stays five.
 int frameLaunches(arbor_engine* e, const float* I, float delta, int sample, int light, bool timed) {
   const int w = e->width, h = e->height;
   const void* accel = e->accel ? e->b.Accel : nullptr;
@@ -151,11 +152,8 @@ int frameLaunches(arbor_engine* e, const float* I, float delta, int sample, int 
   if (rc != ARBOR_OK) return rc;
   if (timed) (void)hipEventRecord(e->ev[3], e->stream);
   if (net) {
-    if ((rc = arbor_denoiser_set_input(e->dn, 0, e->b.Linear, e->b.Guide, e->b.Normal, e->b.Hit, w, h, e->stream)) !=
-            ARBOR_OK ||
-        (rc = arbor_denoiser_forward(e->dn, 1, e->stream)) != ARBOR_OK ||
-        (rc = arbor_denoiser_get_output(e->dn, 0, e->b.Filtered, w, h, e->stream)) != ARBOR_OK ||
-        (rc = arbor_resolvePixels(e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w, h, e->stream)) != ARBOR_OK)
+    if ((rc = arbor::denoiseResolve(e->dn, e->b.Linear, e->b.Guide, e->b.Normal, e->b.Hit, e->b.Filtered,
+                                    e->b.History, e->b.Pixels, e->b.C, w, h, e->stream)) != ARBOR_OK)
       return rc;
   } else if ((rc = arbor::filterResolve(e->b.Linear, e->b.Guide, e->b.Hit, e->b.Filtered, e->b.History, e->b.Pixels,
                                         e->b.C, w, h, e->stream)) != ARBOR_OK) {

@@ -13,6 +13,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "arbor_denoise.h"
+
 namespace arbor {
 
 // I may be null (no camera update); sample < 0 leaves C[9], C[10] and C[22];
@@ -31,6 +33,11 @@ int shadeVisNormal(const float* P, const float* Nodes, const unsigned* Order, co
                    hipStream_t stream);
 int filterResolve(const float* Linear, const float* Guide, const float* Hit, float* Filtered, float* History,
                   unsigned* Pixels, const float* C, int width, int height, hipStream_t stream);
+// The learned denoiser in place of filterResolve: the network's output into
+// Filtered, then resolvePixels (one launch for the 1-level 16-channel shape).
+int denoiseResolve(arbor_denoiser* d, const float* Linear, const float* Guide, const float* Normal, const float* Hit,
+                   float* Filtered, float* History, unsigned* Pixels, const float* C, int w, int h,
+                   hipStream_t stream);
 // Compute units of the current device (cached per device).
 int deviceCUs();
 

@@ -56,6 +56,7 @@ SIGNATURES = {
     "arbor_denoiser_shape": [_vp, ct.POINTER(_i), ct.POINTER(_i)],
     "arbor_denoiser_checkpoint_shape": [ct.c_char_p, ct.POINTER(_i), ct.POINTER(_i)],
     "arbor_denoiser_slot": [_vp, ct.POINTER(_i), ct.POINTER(_i), ct.POINTER(_i)],
+    "arbor_denoiser_denoise": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "arbor_denoiser_destroy": [_vp],
     "arbor_denoiser_parameters": [_vp],
     "arbor_denoiser_set_input": [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -529,6 +530,10 @@ class Denoiser:
     def output(self, index, out, w, h):
         call("arbor_denoiser_get_output", self.h, index, out, w, h)
 
+    def denoise(self, linear, guide, normal, hit, w, h, out):
+        """One frame: features, forward and output (slot 0) into out."""
+        call("arbor_denoiser_denoise", self.h, linear, guide, normal, hit, w, h, out)
+
     def train_step(self, n, lr):
         call("arbor_denoiser_train_step", self.h, n, ct.c_float(lr))
 
@@ -542,6 +547,13 @@ class Denoiser:
 
     def load(self, path):
         check(lib().arbor_denoiser_load(self.h, str(path).encode()), "arbor_denoiser_load")
+
+    def tensor_ptr(self, name):
+        """Device address of a named tensor (arbor_denoiser_tensor)."""
+        ptr = lib().arbor_denoiser_tensor(self.h, name.encode())
+        if not ptr:
+            raise ArborError("no denoiser tensor " + name)
+        return ptr
 
     def tensor(self, name, count, dtype):
         """Host copy of `count` elements of a named device tensor."""

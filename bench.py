@@ -202,9 +202,7 @@ def profile_denoiser(args):
     bufs = [torch.rand(W * H * 4, device="cuda", generator=g) for _ in range(4)]
     out = torch.zeros(W * H * 4, device="cuda")
     for _ in range(args.profile_denoiser):
-        inf.set_input(0, bufs[0], bufs[1], bufs[2], bufs[3], W, H)
-        inf.forward(1)
-        inf.output(0, out, W, H)
+        inf.denoise(bufs[0], bufs[1], bufs[2], bufs[3], W, H, out)
     torch.cuda.synchronize()
     levels, widths = inf.shape()
     inf.close()
@@ -250,6 +248,9 @@ def bench_denoiser(args):
         row["networks"] = []
         for path, inf in zip(paths, nets):
             def net():
+                inf.denoise(fb.Linear, fb.Guide, normal, fb.Hit, W, H, out)
+
+            def layered():
                 inf.set_input(0, fb.Linear, fb.Guide, normal, fb.Hit, W, H)
                 inf.forward(1)
                 inf.output(0, out, W, H)
@@ -260,6 +261,7 @@ def bench_denoiser(args):
                 "checkpoint": os.path.basename(path),
                 "psnr": round(dn_train.psnr(img, ref), 3), "ssim": round(dn_train.ssim(img, ref), 4),
                 "denoiser_ms": round(dn_train.event_ms(torch, net, reps=10), 4),
+                "layered_ms": round(dn_train.event_ms(torch, layered, reps=10), 4),
                 "forward_ms": round(dn_train.event_ms(torch, lambda: inf.forward(1), reps=10), 4)})
         rows.append(row)
     flt = float(np.mean([r["filter_ms"] for r in rows]))
@@ -267,15 +269,19 @@ def bench_denoiser(args):
     for i, (path, inf) in enumerate(zip(paths, nets)):
         levels, widths = inf.shape()
         den = float(np.mean([r["networks"][i]["denoiser_ms"] for r in rows]))
+        lay = float(np.mean([r["networks"][i]["layered_ms"] for r in rows]))
         fwd = float(np.mean([r["networks"][i]["forward_ms"] for r in rows]))
+        one_pass = levels == 1 and widths == [16]
         flops = denoiser_flops(1088, 1920, widths)
-        nbytes = denoiser_bytes(1088, 1920, H, W, widths)
-        tflops = flops / (fwd * 1e-3) / 1e12
+        # the one-pass kernel reads the four frame buffers and writes Filtered
+        nbytes = 80.0 * W * H if one_pass else denoiser_bytes(1088, 1920, H, W, widths)
+        tflops = flops / ((den if one_pass else fwd) * 1e-3) / 1e12
         gbs = nbytes / (den * 1e-3) / 1e9
         beats = all(r["networks"][i]["psnr"] >= r["filterLighting"]["psnr"] for r in rows)
         summary.append({
             "checkpoint": os.path.basename(path), "levels": levels, "widths": widths,
-            "parameters": int(inf.parameters()), "denoiser_ms": round(den, 4), "forward_ms": round(fwd, 4),
+            "parameters": int(inf.parameters()), "one_pass": one_pass, "denoiser_ms": round(den, 4),
+            "layered_ms": round(lay, 4), "forward_ms": round(fwd, 4),
             "vs_filter_time": round(den / flt, 2),
             "mean_psnr": round(float(np.mean([r["networks"][i]["psnr"] for r in rows])), 3),
             "psnr_at_or_above_filter_on_every_frame": beats,
@@ -288,8 +294,9 @@ def bench_denoiser(args):
         inf.close()
     first = summary[0]
     if first["bound"] == "hbm":
-        roof = {"bound": "hbm", "kernel": "denoiser pass (features, %d convolutions, output), %d-level %s"
-                % (len(denoiser_layers(first["widths"])), first["levels"], first["widths"]),
+        roof = {"bound": "hbm", "kernel": "denoiser pass (features, %d convolutions, output%s), %d-level %s"
+                % (len(denoiser_layers(first["widths"])), ": one kernel" if first["one_pass"] else "",
+                   first["levels"], first["widths"]),
                 "achieved": first["pass_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": first["hbm_frac"],
                 "algorithmic_bytes": first["algorithmic_bytes_per_frame"], "traffic": None}
     else:

@@ -96,6 +96,13 @@ int arbor_denoiser_forward(arbor_denoiser* d, int n, hipStream_t stream);
 /* Radiance of slot `index` (float4, alpha 1, w x h): a drop-in for Filtered
  * ahead of resolvePixels. */
 int arbor_denoiser_get_output(arbor_denoiser* d, int index, float* Out, int w, int h, hipStream_t stream);
+/* One frame through the network: input features, forward and output on
+ * slot 0 into Out (float4, alpha 1, w x h; a drop-in for Filtered). A
+ * 1-level network of 16 channels runs as one kernel with its intermediate
+ * tensors on chip (the same bits as the three calls); other shapes run the
+ * three calls. */
+int arbor_denoiser_denoise(arbor_denoiser* d, const float* Linear, const float* Guide, const float* Normal,
+                           const float* Hit, int w, int h, float* Out, hipStream_t stream);
 /* Forward, L1 loss on log(1 + radiance) against the targets, backward, one
  * Adam step (beta 0.9 / 0.999) over slots 0..n-1. */
 int arbor_denoiser_train_step(arbor_denoiser* d, int n, float lr, hipStream_t stream);

@@ -41,7 +41,9 @@ int arbor_engine_set_accel(arbor_engine* e, int on);
  * every later frame whose size fits d's slot (arbor_denoiser_slot) shades
  * with the normal as well (a Normal buffer, 16 B a pixel, is allocated),
  * runs the network on slot 0 (input features, forward, output into Filtered)
- * and resolves; a frame larger than the slot falls back to filterLighting,
+ * and resolves (arbor_denoiser_denoise, then resolvePixels; for a 1-level
+ * 16-channel network one kernel does both, and the frame stays five
+ * launches); a frame larger than the slot falls back to filterLighting,
  * as does every frame with d NULL (the default). inspect() names the last
  * frame's denoiser. The engine does not own d: the caller loads its weights,
  * keeps it alive while attached and does not use slot 0 of it meanwhile

@@ -328,19 +328,21 @@ def test_small_network_forward_and_gradients(env, widths):
     dn.close()
 
 
-def test_engine_denoises_with_network_and_falls_back(env):
+@pytest.mark.parametrize("levels,widths", [(2, [16, 32]), (1, [16])])
+def test_engine_denoises_with_network_and_falls_back(env, levels, widths):
     """arbor_engine_set_denoiser: a frame that fits the network's slot equals
     the entry points chained one by one on the same camera (trace, sun
     visibility, shadePixelsVis, the normal of shadePixelsAOV, the network's
     input / forward / output into Filtered, resolvePixels), bit for bit; a
     frame larger than the slot, and every frame after detaching, equals the
-    filterLighting engine's bit for bit."""
+    filterLighting engine's bit for bit. The 1-level 16-channel network runs
+    as the one-pass kernel with the resolve fused."""
     torch, px = env
     from gpu_util import to_dev, to_host
 
-    print("case: engine with a 2-level network attached, 328x184 in a 336x192 slot", flush=True)
+    print("case: engine with a %d-level network attached, 328x184 in a 336x192 slot" % levels, flush=True)
     w, h = 328, 184
-    dn = px.Denoiser(1, 192, 336, train=False, seed=5, levels=2, widths=[16, 32])
+    dn = px.Denoiser(1, 192, 336, train=False, seed=5, levels=levels, widths=widths)
     assert dn.slot() == (1, 192, 336)
     eng, plain = px.Engine(w, h, 7), px.Engine(w, h, 7)
     assert not eng.buffer_ptr("Normal")
@@ -399,4 +401,89 @@ def test_engine_denoises_with_network_and_falls_back(env):
         assert np.array_equal(eng.download("Pixels", W2 * H2, np.uint32), plain.download("Pixels", W2 * H2, np.uint32))
     eng.close()
     plain.close()
+    dn.close()
+
+
+def test_one_level_network_in_one_pass_equals_layered(env):
+    """arbor_denoiser_denoise runs a 1-level 16-channel network as one fused
+    kernel (features, three convolutions and output, intermediates in LDS).
+    Its output equals the layered runner's (set_input, forward, get_output)
+    bit for bit: the committed small checkpoint and a seeded He-initialised
+    network with random biases, on random frames in a zero-padded slot (90x61
+    in 96x64: image and slot borders inside the tiles) and on a 1920x1080
+    tracer frame in a 1088x1920 slot. A 2-level network takes the layered
+    path through the same call."""
+    import os
+    import sys
+
+    torch, px = env
+    sys.path.insert(0, os.path.dirname(px.__file__))
+    import dn_train
+
+    small = os.path.join(os.path.dirname(px.__file__), "weights", "denoiser_forest_small.bin")
+    lin, gui, nor, hit, _ = _frames(torch, 11)
+    nets = [("checkpoint", px.Denoiser.from_checkpoint(small, 1, H, W)),
+            ("seeded", px.Denoiser(1, H, W, train=False, seed=9, levels=1, widths=[16])),
+            ("2-level", px.Denoiser(1, H, W, train=False, seed=9, levels=2, widths=[16, 32]))]
+    g = torch.Generator(device="cuda").manual_seed(5)
+    for li in range(3):  # random biases: a bias mix-up shows
+        b = px._wrap_device(nets[1][1].tensor_ptr("bias:%d" % li), 16 * 4).view(torch.float32)
+        b.copy_(torch.randn(16, device="cuda", generator=g) * 0.3)
+    for name, dn in nets:
+        for k in range(N):
+            print("case: one-pass denoise, %s network, random frame %d, %dx%d in %dx%d" % (name, k, w_img, h_img, W, H),
+                  flush=True)
+            a = torch.zeros(w_img * h_img * 4, dtype=torch.float32, device="cuda")
+            b = torch.full_like(a, -1.0)
+            dn.set_input(0, lin[k], gui[k], nor[k], hit[k], w_img, h_img)
+            dn.forward(1)
+            dn.output(0, a, w_img, h_img)
+            dn.denoise(lin[k], gui[k], nor[k], hit[k], w_img, h_img, b)
+            torch.cuda.synchronize()
+            assert torch.equal(a.view(torch.int32), b.view(torch.int32)), (name, k, int((a != b).sum()))
+        dn.close()
+
+    print("case: one-pass denoise, checkpoint, 1920x1080 tracer frame", flush=True)
+    Wf, Hf = 1920, 1080
+    scene = px.DeviceScene().generate(7)
+    cams, _ = px.batch_walk_views(11 + 777777, 7, 2)
+    cam = cams[1].copy()
+    cam[22] = 1.0
+    fb, normal, _ = dn_train.render_inputs(px, torch, scene, cam, Wf, Hf)
+    dn = px.Denoiser.from_checkpoint(small, 1, 1088, 1920)
+    a = torch.zeros(Wf * Hf * 4, dtype=torch.float32, device="cuda")
+    b = torch.full_like(a, -1.0)
+    dn.set_input(0, fb.Linear, fb.Guide, normal, fb.Hit, Wf, Hf)
+    dn.forward(1)
+    dn.output(0, a, Wf, Hf)
+    dn.denoise(fb.Linear, fb.Guide, normal, fb.Hit, Wf, Hf, b)
+    torch.cuda.synchronize()
+    assert torch.equal(a.view(torch.int32), b.view(torch.int32)), int((a != b).sum())
+    dn.close()
+
+
+def test_engine_frame_with_one_pass_network_is_five_launches(env):
+    """With a 1-level 16-channel network attached the engine's frame stays
+    five kernel launches: the network and the resolve are one kernel."""
+    torch, px = env
+    from torch.profiler import ProfilerActivity, profile
+
+    print("case: launches per engine frame with the one-pass network, 320x192", flush=True)
+    eng = px.Engine(320, 192, 7)
+    dn = px.Denoiser(1, 192, 320, train=False, seed=5, levels=1, widths=[16])
+    eng.set_denoiser(dn)
+    dev_in = torch.from_numpy(px.walk_input(7, 0)).cuda()
+    eng.frame(1.0 / 60.0, dev_in)
+    eng.sync()
+    torch.cuda.synchronize()
+    with profile(activities=[ProfilerActivity.CUDA]) as prof:
+        eng.frame(1.0 / 60.0, dev_in)
+        eng.sync()
+        torch.cuda.synchronize()
+    names = [e.name for e in prof.events() if e.device_type == torch.autograd.DeviceType.CUDA
+             and "memcpy" not in e.name.lower() and "memset" not in e.name.lower()]
+    print(len(names), "kernels:", names)
+    assert len(names) == 5 and any("denoise1" in n for n in names)
+    assert eng.inspect()["denoiser"] == "network"
+    eng.close()
     dn.close()
