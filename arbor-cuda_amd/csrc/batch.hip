@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_traceShadeViews(const float* P,
                                                                  const uint32_t* Order, const float* Cs,
                                                                  const ViewDesc* views, int nviews, int totalTiles,
                                                                  float4* Hit, float4* Linear, float4* Guide,
-                                                                 const float4* accel) {
+                                                                 const float4* accel, const float4*) {
   __shared__ TreeInst trees[kTrees];
   __shared__ ViewDesc sv[kMaxViews];
   for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, (int)Cs[31]);
@@ -88,7 +88,7 @@ template <int kWaves, class W>
 __global__ __launch_bounds__(kWaves * 64) void k_traceViews(const float* P, const float4* Nodes4, const uint32_t* Order,
                                                             const float* Cs, const ViewDesc* views, int nviews,
                                                             int totalTiles, float4* Hit, float4*, float4*,
-                                                            const float4* accel) {
+                                                            const float4* accel, const float4*) {
   __shared__ TreeInst trees[kTrees];
   __shared__ ViewDesc sv[kMaxViews];
   for (int i = threadIdx.x; i < kTrees; i += blockDim.x) trees[i] = makeTree(i, (int)Cs[31]);
@@ -108,6 +108,38 @@ __global__ __launch_bounds__(kWaves * 64) void k_traceViews(const float* P, cons
     int tree = -1;
     v2 h = traceForest<W>(P, Nodes4, W::sah ? accel : nullptr, Order, trees, ro, rd, C[14] / (float)d.height, C[8],
                           C[11], ground > 0.0f ? ground : kFar, &tree, false);
+    Hit[d.pixelBase + y * d.width + x] = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)d.height);
+  }
+}
+// The split trace over the SAH hierarchy with the tree records in device
+// memory (the batch's copy of the world seed's records): the block's dynamic
+// LDS holds kWaves x W::ldsStack stack entries per lane, then the view
+// table. Hit equals k_traceViews'.
+template <int kWaves, class W>
+__global__ __launch_bounds__(kWaves * 64) void k_traceViewsRec(const float* P, const float4* Nodes4,
+                                                               const uint32_t* Order, const float* Cs,
+                                                               const ViewDesc* views, int nviews, int totalTiles,
+                                                               float4* Hit, float4*, float4*, const float4* accel,
+                                                               const float4* treeRecs) {
+  static_assert(W::sah && W::ldsStack > 0, "a record-tree batch trace walks the SAH hierarchy with an LDS stack");
+  ViewDesc* sv = reinterpret_cast<ViewDesc*>(sahLdsStack + (size_t)kWaves * W::ldsStack * 64);
+  for (int i = threadIdx.x; i < nviews; i += blockDim.x) sv[i] = views[i];
+  __syncthreads();
+  const TreeRecs trees{treeRecs};
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x * kWaves + wave; t < totalTiles; t += gridDim.x * kWaves) {
+    const int v = findView(sv, nviews, t);
+    const ViewDesc d = sv[v];
+    const float* C = Cs + 64 * v;
+    const int lt = t - d.tileBase;
+    const int x = (lt % d.tilesX) * 8 + (lane & 7), y = (lt / d.tilesX) * 8 + (lane >> 3);
+    if (x >= d.width || y >= d.height) continue;
+    const View view = makeView(C);
+    v3 ro = view.eye, rd = viewRay(view, x, y, d.width, d.height);
+    float ground = rd.y < -0.00001f ? -ro.y / rd.y : kFar;
+    int tree = -1;
+    v2 h = traceForest<W>(P, Nodes4, accel, Order, trees, ro, rd, C[14] / (float)d.height, C[8], C[11],
+                          ground > 0.0f ? ground : kFar, &tree, false);
     Hit[d.pixelBase + y * d.width + x] = make_float4(h.x, h.y, (float)tree, C[14] * h.x / (float)d.height);
   }
 }
@@ -137,6 +169,39 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5))) void k
     v3 vis = sunVisibilityOne<W>(P, Nodes4, W::sah ? accel : nullptr, Order, trees, sc, x, y, Hit[pixel]);
     Vis[pixel] = make_float4(vis.x, vis.y, vis.z, 1.0f);
   }
+}
+
+// The same over the SAH hierarchy with the batch's tree records, one wave
+// per block; dynamic LDS: W::ldsStack stack entries per lane, then the view
+// table.
+template <class W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_sunVisibilityViewsRec(
+    const float* P, const float4* Nodes4, const uint32_t* Order, const float* Cs, const ViewDesc* views, int nviews,
+    int totalTiles, const float4* Hit, float4* Vis, const float4* accel, const float4* treeRecs) {
+  static_assert(W::sah && W::ldsStack > 0, "a record-tree batch pass walks the SAH hierarchy with an LDS stack");
+  ViewDesc* sv = reinterpret_cast<ViewDesc*>(sahLdsStack + (size_t)W::ldsStack * 64);
+  for (int i = threadIdx.x; i < nviews; i += blockDim.x) sv[i] = views[i];
+  __syncthreads();
+  const TreeRecs trees{treeRecs};
+  const int lane = threadIdx.x;
+  for (int t = blockIdx.x; t < totalTiles; t += gridDim.x) {
+    const int v = findView(sv, nviews, t);
+    const ViewDesc d = sv[v];
+    const float* C = Cs + 64 * v;
+    if (!(C[22] > 0.5f)) continue;
+    const int lt = t - d.tileBase;
+    const int x = (lt % d.tilesX) * 8 + (lane & 7), y = (lt / d.tilesX) * 8 + (lane >> 3);
+    if (x >= d.width || y >= d.height) continue;
+    const ShadeCtx sc = makeShadeCtx(C, d.width, d.height);
+    const int pixel = d.pixelBase + y * d.width + x;
+    v3 vis = sunVisibilityOne<W>(P, Nodes4, accel, Order, trees, sc, x, y, Hit[pixel]);
+    Vis[pixel] = make_float4(vis.x, vis.y, vis.z, 1.0f);
+  }
+}
+// The world seed's tree records for the record-tree kernels.
+__global__ void k_batchTreeRecs(const float* Cs, float4* recs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kTrees) storeTreeRec(recs, i, makeTree(i, (int)Cs[31]));
 }
 
 // Shading of every view's pixels from Hit; with Vis, the sun visibility comes
@@ -210,12 +275,14 @@ __global__ __launch_bounds__(kFilterW * kFilterH) void k_resolveViews(const floa
 }
 
 typedef void (*BatchKernel)(const float*, const float4*, const uint32_t*, const float*, const ViewDesc*, int, int,
-                            float4*, float4*, float4*, const float4*);
+                            float4*, float4*, float4*, const float4*, const float4*);
 struct BatchVariant {
   BatchKernel kernel;  // fused trace + shade, or (split) the trace followed by k_shadeViews
   bool split;
   int waves;
   const char* name;
+  int ldsPerWave = 0;     // dynamic LDS bytes per wave (stack entries) of a record-tree trace
+  bool recTrees = false;  // tree records from the batch's copy; stack and view table in dynamic LDS
 };
 const BatchVariant kBatchVariants[] = {
     {k_traceShadeViews<1, Walk<true, false, false>>, false, 1, "1 wave/block, culled walk"},
@@ -225,6 +292,12 @@ const BatchVariant kBatchVariants[] = {
     {k_traceShadeViews<4, Walk<false, false, false>>, false, 4, "4 waves/block, plain walk"},
     {k_traceViews<1, Walk<true, false, false>>, true, 1, "split: trace 1 wave/block culled, visibility, shade"},
     {k_traceViews<2, Walk<true, false, false>>, true, 2, "split: trace 2 waves/block culled, visibility, shade"},
+    // over the heap these are 6 and 5; over the SAH hierarchy their traces
+    // read the tree records from device memory and keep the stack in LDS
+    {k_traceViews<2, Walk<true, false, false>>, true, 2,
+     "split as 6 (SAH walk: 1 wave/block, record trees, 20-entry LDS stack)"},
+    {k_traceViews<1, Walk<true, false, false>>, true, 1,
+     "split as 5 (SAH walk: 2 waves/block, record trees, 16-entry LDS stack)"},
 };
 constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 int batchVariantCount() { return kBatchVariantCount; }
@@ -235,11 +308,20 @@ int batchVariantCount() { return kBatchVariantCount; }
 // kernels.
 using WalkSahB = Walk<true, false, false, true, 0, 6>;
 using WalkSahBVis = Walk<true, false, false, true, 0, 5>;
+// The record-tree split passes: the trace with 20 (1-wave blocks) or 16
+// (2-wave blocks) LDS stack entries per lane, the visibility pass with 14
+// (1-wave blocks, 5 waves per SIMD).
+using WalkSahBRec1 = Walk<true, false, false, true, 20, 6>;
+using WalkSahBRec2 = Walk<true, false, false, true, 16, 6>;
+using WalkSahBVisRec = Walk<true, false, false, true, 14, 5>;
+constexpr int kBatchLds(int k) { return k * 64 * 8; }
 const BatchVariant kBatchVariantsSah[] = {
     {k_traceShadeViews<1, WalkSahB>, false, 1, ""}, {k_traceShadeViews<2, WalkSahB>, false, 2, ""},
     {k_traceShadeViews<4, WalkSahB>, false, 4, ""}, {k_traceShadeViews<1, WalkSahB>, false, 1, ""},
     {k_traceShadeViews<4, WalkSahB>, false, 4, ""}, {k_traceViews<1, WalkSahB>, true, 1, ""},
     {k_traceViews<2, WalkSahB>, true, 2, ""},
+    {k_traceViewsRec<1, WalkSahBRec1>, true, 1, "", kBatchLds(20), true},
+    {k_traceViewsRec<2, WalkSahBRec2>, true, 2, "", kBatchLds(16), true},
 };
 static_assert(sizeof(kBatchVariantsSah) == sizeof(kBatchVariants), "one SAH entry per batch variant");
 
@@ -263,6 +345,7 @@ struct arbor_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const float4* accel = nullptr;  // SAH hierarchy (arbor_batch_set_accel), or NULL for the heap walk
+  float4* trees = nullptr;        // the world seed's tree records (set_views), for the record-tree kernels
 };
 
 namespace {
@@ -286,21 +369,31 @@ void dropGraph(arbor_batch* b) {
 int issue(arbor_batch* b, const float* P, const float* Nodes, const unsigned* Order, int stages, hipStream_t s) {
   if (stages & ARBOR_BATCH_TRACE_SHADE) {
     const BatchVariant& v = (b->accel ? kBatchVariantsSah : kBatchVariants)[tuning().batch_variant];
+    const size_t table = sizeof(ViewDesc) * (size_t)b->nviews;
     int blocks = (b->totalTiles + v.waves - 1) / v.waves;
     if (blocks > tuning().grid_cap) blocks = tuning().grid_cap;
-    hipLaunchKernelGGL(v.kernel, dim3(blocks), dim3(v.waves * 64), 0, s, P, reinterpret_cast<const float4*>(Nodes),
-                       Order, b->dC, b->dViews, b->nviews, b->totalTiles, b->Hit, b->Linear, b->Guide, b->accel);
+    hipLaunchKernelGGL(v.kernel, dim3(blocks), dim3(v.waves * 64), v.recTrees ? v.waves * v.ldsPerWave + table : 0, s,
+                       P, reinterpret_cast<const float4*>(Nodes), Order, b->dC, b->dViews, b->nviews, b->totalTiles,
+                       b->Hit, b->Linear, b->Guide, b->accel, b->trees);
     int rc = launch_status();
     if (rc != ARBOR_OK) return rc;
     if (v.split) {
       int vblocks = (b->totalTiles + 1) / 2, sblocks = (b->totalTiles + 3) / 4;
       if (vblocks > tuning().grid_cap) vblocks = tuning().grid_cap;
       if (sblocks > tuning().grid_cap) sblocks = tuning().grid_cap;
-      typedef void (*VisViewsKernel)(const float*, const float4*, const uint32_t*, const float*, const ViewDesc*, int,
-                                     int, const float4*, float4*, const float4*);
-      const VisViewsKernel vk = b->accel ? k_sunVisibilityViews<WalkSahBVis> : k_sunVisibilityViews<Walk<true, false, false>>;
-      hipLaunchKernelGGL(vk, dim3(vblocks), dim3(128), 0, s, P, reinterpret_cast<const float4*>(Nodes), Order, b->dC,
-                         b->dViews, b->nviews, b->totalTiles, b->Hit, b->Vis, b->accel);
+      if (b->accel && v.recTrees) {
+        int rblocks = b->totalTiles < tuning().grid_cap ? b->totalTiles : tuning().grid_cap;
+        hipLaunchKernelGGL(k_sunVisibilityViewsRec<WalkSahBVisRec>, dim3(rblocks), dim3(64),
+                           kBatchLds(WalkSahBVisRec::ldsStack) + table, s, P, reinterpret_cast<const float4*>(Nodes),
+                           Order, b->dC, b->dViews, b->nviews, b->totalTiles, b->Hit, b->Vis, b->accel, b->trees);
+      } else {
+        typedef void (*VisViewsKernel)(const float*, const float4*, const uint32_t*, const float*, const ViewDesc*,
+                                       int, int, const float4*, float4*, const float4*);
+        const VisViewsKernel vk =
+            b->accel ? k_sunVisibilityViews<WalkSahBVis> : k_sunVisibilityViews<Walk<true, false, false>>;
+        hipLaunchKernelGGL(vk, dim3(vblocks), dim3(128), 0, s, P, reinterpret_cast<const float4*>(Nodes), Order,
+                           b->dC, b->dViews, b->nviews, b->totalTiles, b->Hit, b->Vis, b->accel);
+      }
       if ((rc = launch_status()) != ARBOR_OK) return rc;
       hipLaunchKernelGGL(k_shadeViews<false>, dim3(sblocks), dim3(256), 0, s, P,
                          reinterpret_cast<const float4*>(Nodes), Order, b->dC, b->dViews, b->nviews, b->totalTiles,
@@ -346,6 +439,7 @@ int arbor_batch_create(arbor_batch** out, int max_views, long long max_pixels) {
   rc |= dalloc(&b->History, 16 * n);
   rc |= dalloc(&b->Vis, 16 * n);
   rc |= dalloc(&b->Pixels, 4 * n);
+  rc |= dalloc(&b->trees, sizeof(float4) * kTreeRecF4 * kTrees);
   if (rc != ARBOR_OK || hipMemset(b->History, 0, 16 * n) != hipSuccess) {
     arbor_batch_destroy(b);
     return ARBOR_ERR_DEVICE;
@@ -366,7 +460,7 @@ int arbor_batch_destroy(arbor_batch* b) {
   (void)hipDeviceSynchronize();
   dropGraph(b);
   dfree(b->dViews); dfree(b->dC); dfree(b->Hit); dfree(b->Linear); dfree(b->Guide); dfree(b->Filtered);
-  dfree(b->History); dfree(b->Pixels); dfree(b->Vis);
+  dfree(b->History); dfree(b->Pixels); dfree(b->Vis); dfree(b->trees);
   delete b;
   return ARBOR_OK;
 }
@@ -399,6 +493,9 @@ int arbor_batch_set_views(arbor_batch* b, int nviews, const float* cameras, cons
   if (hipMemcpy(b->dViews, h.data(), sizeof(ViewDesc) * nviews, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->dC, cameras, sizeof(float) * 64 * nviews, hipMemcpyHostToDevice) != hipSuccess)
     return ARBOR_ERR_DEVICE;
+  // the tree records of the views' world seed (synchronously, as the copies)
+  hipLaunchKernelGGL(k_batchTreeRecs, dim3((kTrees + 255) / 256), dim3(256), 0, nullptr, b->dC, b->trees);
+  if (launch_status() != ARBOR_OK || hipStreamSynchronize(nullptr) != hipSuccess) return ARBOR_ERR_DEVICE;
   b->host = h;
   b->nviews = nviews;
   b->npixels = pix;

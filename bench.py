@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--accel-vis-variant", type=int, default=-1, metavar="V",
                     help="SAH-walk sun-visibility launch shape (the tuning key accel_vis_variant) for this run; -1 "
                          "keeps the library default")
+    ap.add_argument("--batch-variant", type=int, default=-1, metavar="V",
+                    help="batched-views launch shape (the tuning key batch_variant) for this run; -1 keeps the "
+                         "library default")
     ap.add_argument("--views", type=int, default=0, metavar="N",
                     help="batched-views workload instead: N seeded walk views of uneven resolution rendered by one "
                          "hipGraph-captured batch per step (about 0.3 s per step at N=64; use --steps 10)")
@@ -360,16 +363,18 @@ def bench_views(args):
     graph_ms = ev0.elapsed_time(ev1) / K
 
     # the same views, one at a time through the single-view entry points
-    fbs, cds = {}, []
+    # (each view with a tile schedule of its own, as the engine traces)
+    fbs, cds, scheds = {}, [], []
     for i, (w, h) in enumerate(sizes):
         fbs.setdefault((w, h), px.FrameBuffers(w, h))
         cds.append(torch.from_numpy(cams[i].copy()).cuda())
-    for rep in range(2):  # the first pass warms the single-view kernels
+        scheds.append(px.TileSchedule(w, h))
+    for rep in range(2):  # the first pass warms the single-view kernels and seeds the schedules
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         for i, (w, h) in enumerate(sizes):
             fb = fbs[(w, h)]
-            px.trace_accel(scene, accel, cds[i], fb)
+            px.trace_accel(scene, accel, cds[i], fb, scheds[i])
             px.shade(scene, cds[i], fb)
             px.filter_lighting(cds[i], fb)
             px.resolve(cds[i], fb)
@@ -541,6 +546,10 @@ def main():
         import pyarbor
 
         pyarbor.tune_set("accel_vis_variant", args.accel_vis_variant)
+    if args.batch_variant >= 0:
+        import pyarbor
+
+        pyarbor.tune_set("batch_variant", args.batch_variant)
     if args.profile_trace:
         profile_trace(args)
         return

@@ -43,7 +43,9 @@ int arbor_batch_launch(arbor_batch* b, hipStream_t stream);
 /* Trace and sun visibility over a SAH hierarchy of arbor_buildTraceAccel
  * (arbor.h) for the scene passed to render/capture, or over the heap BVH
  * with accel NULL (the default). batch_variant keeps selecting the launch
- * shape. Invalidates a captured graph. No counterpart in the reference. */
+ * shape; over the hierarchy the default (7) and 8 read the views' tree
+ * records from a copy set_views writes and keep the walk's stack in LDS.
+ * Invalidates a captured graph. No counterpart in the reference. */
 int arbor_batch_set_accel(arbor_batch* b, const void* accel);
 /* Device pointer of view `view`'s slice of a buffer: Hit Linear Guide
  * Filtered History Pixels C; NULL for an unknown name or view. */

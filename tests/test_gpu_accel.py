@@ -198,7 +198,7 @@ def test_batch_accel_equals_batch_heap(px, scene):
     """The batched views over the SAH hierarchy (every batch_variant, shadows
     on) give the heap batch's Hit, Linear and Guide, up to the
     walk-order-dependent pixels."""
-    print("case: batch of 6 walk views, accel vs heap, batch_variant 0..6, shadows on", flush=True)
+    print("case: batch of 6 walk views, accel vs heap, batch_variant 0..8, shadows on", flush=True)
     dev, acc = scene
     cams, sizes = px.batch_walk_views(3, 7, 6)
     sizes = [(min(w, 640), min(h, 360)) for w, h in sizes]
@@ -207,7 +207,7 @@ def test_batch_accel_equals_batch_heap(px, scene):
     lib = px.lib()
     old = lib.arbor_tune_get(b"batch_variant")
     try:
-        for v in range(7):
+        for v in range(9):
             assert lib.arbor_tune_set(b"batch_variant", v) == 0
             out = []
             for use in (None, acc):

@@ -10,6 +10,10 @@ from gpu_util import require_gpu, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
+# batch_variant 0-4: fused trace + shade; 5-8: split (7 and 8 read the tree
+# records from device memory over the SAH hierarchy)
+BATCH_VARIANTS = 9
+
 SIZES = [(256, 256), (320, 180), (200, 120), (64, 64), (333, 97), (128, 72), (96, 200), (40, 24)]
 
 
@@ -63,7 +67,7 @@ def test_batch_equals_single_views(px, scene, shadows):
     assert b.pixels() == sum(w * h for w, h in sizes)
     old = px.tune_get("batch_variant")
     try:
-        for v in range(7):
+        for v in range(BATCH_VARIANTS):
             px.tune_set("batch_variant", v)
             b.set_views(cams, sizes)
             b.render(scene)
@@ -209,5 +213,22 @@ def test_batch_rejects_mixed_worlds_and_launch_count(px, scene):
     names = [e.name for e in prof.events() if e.device_type == torch.autograd.DeviceType.CUDA
              and "memcpy" not in e.name.lower() and "memset" not in e.name.lower()]
     print(len(names), "kernels:", names)
-    assert px.tune_get("batch_variant") in (5, 6) and len(names) == 4
+    assert px.tune_get("batch_variant") in (5, 6, 7, 8) and len(names) == 4
+    # over the SAH hierarchy with the default variant: the record-tree trace
+    old = px.tune_get("batch_variant")
+    try:
+        px.tune_set("batch_variant", 7)
+        acc = px.TraceAccel(scene)  # held until the batch is closed
+        b.set_accel(acc)
+        b.render(scene)
+        torch.cuda.synchronize()
+        with profile(activities=[ProfilerActivity.CUDA]) as prof:
+            b.render(scene)
+            torch.cuda.synchronize()
+        names = [e.name for e in prof.events() if e.device_type == torch.autograd.DeviceType.CUDA
+                 and "memcpy" not in e.name.lower() and "memset" not in e.name.lower()]
+        print(len(names), "kernels over the hierarchy:", names)
+        assert len(names) == 4 and "k_traceViewsRec" in names[0] and "k_sunVisibilityViewsRec" in names[1]
+    finally:
+        px.tune_set("batch_variant", old)
     b.close()
