@@ -65,7 +65,11 @@ __device__ __forceinline__ int ldsSlot(int pix, int grp) { return pix * kChunk +
 // for 128; 1 leaves the allocation unforced). Over the network's layer shapes
 // at 1088x1920 these targets give the lowest total time.
 constexpr int convWaves(int nsub) { return nsub == 2 ? 5 : (nsub == 3 ? 4 : (nsub == 8 ? 3 : 1)); }
-template <int NSUB, int MSUB>
+// kPair (16 -> 16, no concatenation): two taps share a matrix instruction,
+// K slots 0-15 holding tap 2k's 16 channels and 16-31 tap 2k+1's; the fifth
+// instruction holds tap 8 and zero weights. Five instructions per output
+// subtile instead of nine half-empty ones, and no LDS read of zeros.
+template <int NSUB, int MSUB, bool kPair = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(convWaves(NSUB)))) void k_conv3x3(
     const __bf16* __restrict__ X, const __bf16* __restrict__ Xc, int Cc, int N, int H, int W, int Cin,
     const __bf16* __restrict__ Wt, const float* __restrict__ bias, int relu, __bf16* __restrict__ Y, int CoutTotal,
@@ -119,6 +123,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(convWaves(N
       if (i < kGroups) *reinterpret_cast<bf16x8*>(&tile[ldsSlot(i >> 2, i & 3)]) = ok[s] ? sv[s] : zero8();
     }
     __syncthreads();
+    if constexpr (kPair) {
+      // lane (row, g): tap 2k + (g >> 1) of pair k, channels 8 (g & 1) ..;
+      // the next pair's weights are loaded while the current one computes
+      const int grp = g & 1;
+      auto loadPair = [&](int k, bf16x8* w8) {
+        const int tap = 2 * k + (g >> 1);
+#pragma unroll
+        for (int j = 0; j < NSUB; j++)
+          w8[j] = tap < 9 ? *reinterpret_cast<const bf16x8*>(Wt + ((size_t)tap * CoutTotal + co0 + j * 16 + row) * Cin +
+                                                             8 * grp)
+                          : zero8();
+      };
+      auto pairTaps = [&](int k, const bf16x8* w8) {
+        const int tl = min(2 * k + (g >> 1), 8), dy = tl / 3, dx = tl - 3 * (tl / 3);
+#pragma unroll
+        for (int m = 0; m < MSUB; m++) {
+          const bf16x8 a =
+              *reinterpret_cast<const bf16x8*>(&tile[ldsSlot((MSUB * wave + m + dy) * kHaloW + row + dx, grp)]);
+#pragma unroll
+          for (int j = 0; j < NSUB; j++)
+            acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w8[j], a, acc[m][j], 0, 0, 0);
+        }
+      };
+      bf16x8 w0[NSUB], w1[NSUB];
+      loadPair(0, w0);
+      loadPair(1, w1);
+      asm volatile("" ::: "memory");
+      pairTaps(0, w0);
+      loadPair(2, w0);
+      asm volatile("" ::: "memory");
+      pairTaps(1, w1);
+      loadPair(3, w1);
+      asm volatile("" ::: "memory");
+      pairTaps(2, w0);
+      loadPair(4, w0);
+      asm volatile("" ::: "memory");
+      pairTaps(3, w1);
+      pairTaps(4, w0);
+      __syncthreads();
+      continue;
+    }
     // Weight fragments come from global memory (L2); the next tap's are
     // loaded while the current one computes. A lane whose channel group lies
     // past Cin reads group 0's weights instead: its staged pixels are zero,
@@ -389,8 +434,14 @@ void launchConv(const __bf16* X, const __bf16* Xc, int Cc, int N, int H, int W, 
   // narrow outputs: 16-row tiles (each weight fragment feeds 4 MFMAs); wide: 8-row tiles
   constexpr int MSUB = NSUB <= 4 ? 4 : 2;
   const long long blocks = (long long)N * ((H + 4 * MSUB - 1) / (4 * MSUB)) * ((W + kTileW - 1) / kTileW);
-  hipLaunchKernelGGL((k_conv3x3<NSUB, MSUB>), dim3((unsigned)blocks), dim3(256), 0, s, X, Xc, Cc, N, H, W, Cin, Wt,
-                     bias, relu, Y, coutTotal, co0);
+  // paired taps for 16 -> 16 only: with wider outputs the mixed-tap LDS reads
+  // conflict (16 -> 32 took 117 against 66-82 us at 1088x1920)
+  if (NSUB == 1 && Cin == 16 && !Xc)
+    hipLaunchKernelGGL((k_conv3x3<NSUB, MSUB, true>), dim3((unsigned)blocks), dim3(256), 0, s, X, Xc, Cc, N, H, W,
+                       Cin, Wt, bias, relu, Y, coutTotal, co0);
+  else
+    hipLaunchKernelGGL((k_conv3x3<NSUB, MSUB>), dim3((unsigned)blocks), dim3(256), 0, s, X, Xc, Cc, N, H, W, Cin,
+                       Wt, bias, relu, Y, coutTotal, co0);
 }
 int conv(const __bf16* x, const __bf16* xc, int cc, int n, int h, int w, int cin, const __bf16* wt, const float* bias,
          int relu, __bf16* y, int cout, hipStream_t stream) {

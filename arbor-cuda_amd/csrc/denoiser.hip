@@ -145,8 +145,8 @@ __global__ void k_sumTo(const float* __restrict__ part, int n, float scale, floa
 // computed on the tile plus the halo the later layers read (38x22 features,
 // then 36x20 and 34x18 activations, 32 B per pixel). Every convolution issues
 // the v_mfma_f32_16x16x32_bf16 sequence of arbor_dn_conv3x3 for 16 input
-// channels (per tap, in tap order: 16 output channels x 16 pixels, K = the 16
-// channels and 16 zero slots whose weights repeat channels 8-15), adds the
+// channels (five instructions per 16 output channels x 16 pixels, taps paired
+// in K: taps 0-1, 2-3, 4-5, 6-7, then tap 8 with zero weights), adds the
 // bias, applies ReLU and rounds to bf16 as it does, and positions outside
 // the slot are zero as its padding is: the output has the layered network's
 // bits. With kResolve each pixel is also resolved (resolvePixels).
@@ -159,9 +159,6 @@ constexpr int kFPix = kFW * kFH, kAPix = kAW * kAH, kBPix = kBW * kBH;
 constexpr int kThreads = 256;
 constexpr int kXcds = 8;  // accelerator complex dies, each with its own L2
 static_assert(kBPix <= kFPix, "the second convolution's region reuses the features' LDS");
-// Zero pixels read by the lanes whose K slots hold no channel: every tap's
-// offset from the region's first pixel stays inside.
-constexpr int kZeroPix = 2 * kFW + 3;
 
 __device__ __forceinline__ bf16x8 zero8() {
   bf16x8 z;
@@ -169,67 +166,76 @@ __device__ __forceinline__ bf16x8 zero8() {
   for (int j = 0; j < 8; j++) z[j] = (bf16)0.0f;
   return z;
 }
-// The nine taps' weight fragments of lane (row = output channel, g) for a
-// 16 -> 16 convolution [9][16][16]: channels 8g..8g+7, and 8-15 for g >= 2
-// (those K slots multiply zero pixels).
+// The tap of K-slot group g in paired instruction k (9: none).
+__device__ __forceinline__ int pairTap(int k, int g) { return 2 * k + (g >> 1); }
+// The five paired instructions' weight fragments of lane (row = output
+// channel, g) for a 16 -> 16 convolution [9][16][16]: tap pairTap(k, g),
+// channels 8 (g & 1) .. +7; zero for the slot without a tap.
 __device__ __forceinline__ void loadTaps(const bf16* __restrict__ Wt, int row, int g, bf16x8* w) {
-  const int c = 8 * (g < 2 ? g : 1);
 #pragma unroll
-  for (int tap = 0; tap < 9; tap++) w[tap] = *reinterpret_cast<const bf16x8*>(Wt + (tap * kC + row) * kC + c);
+  for (int k = 0; k < 5; k++) {
+    const int tap = pairTap(k, g);
+    w[k] = tap < 9 ? *reinterpret_cast<const bf16x8*>(Wt + (tap * kC + row) * kC + 8 * (g & 1)) : zero8();
+  }
 }
 // The 3x3 convolutions of target pixels p0 and p1 (region wd wide) from the
 // source region (wd + 2 wide) at src: lane (row, g) gets channels 4g..4g+3
 // of the pixel with index p0 (p1) of its 16-pixel group (row = its place in
-// the group). Lanes g >= 2 read their zero operand from the zero pixels, so
-// every lane issues the same LDS reads (no branch) before the products; the
-// two groups' products alternate, so neither waits on its own previous one.
-__device__ __forceinline__ void convPair(const bf16* src, const bf16* zeros, int wd, int p0, int p1, int g,
-                                         const bf16x8* w, f32x4& acc0, f32x4& acc1) {
+// the group). Every lane reads its paired tap's 8 channels (tap 8 again for
+// the slot without a tap, whose weights are zero); the two groups' products
+// alternate, so neither waits on its own previous one.
+__device__ __forceinline__ void convPair(const bf16* src, int wd, int p0, int p1, int g, const bf16x8* w,
+                                         f32x4& acc0, f32x4& acc1) {
   const int ws = wd + 2;
   const int ty0 = p0 / wd, tx0 = p0 - ty0 * wd, ty1 = p1 / wd, tx1 = p1 - ty1 * wd;
-  const bf16* b0 = g < 2 ? src + (ty0 * ws + tx0) * kC + 8 * g : zeros;
-  const bf16* b1 = g < 2 ? src + (ty1 * ws + tx1) * kC + 8 * g : zeros;
-  bf16x8 a0[9], a1[9];
+  const bf16* b0 = src + (ty0 * ws + tx0) * kC + 8 * (g & 1);
+  const bf16* b1 = src + (ty1 * ws + tx1) * kC + 8 * (g & 1);
+  bf16x8 a0[5], a1[5];
 #pragma unroll
-  for (int tap = 0; tap < 9; tap++) {
-    const int off = ((tap / 3) * ws + tap - 3 * (tap / 3)) * kC;
-    a0[tap] = *reinterpret_cast<const bf16x8*>(b0 + off);
-    a1[tap] = *reinterpret_cast<const bf16x8*>(b1 + off);
+  for (int k = 0; k < 5; k++) {
+    const int tap = min(pairTap(k, g), 8), off = ((tap / 3) * ws + tap - 3 * (tap / 3)) * kC;
+    a0[k] = *reinterpret_cast<const bf16x8*>(b0 + off);
+    a1[k] = *reinterpret_cast<const bf16x8*>(b1 + off);
   }
   acc0 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   acc1 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-  for (int tap = 0; tap < 9; tap++) {
-    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tap], a0[tap], acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tap], a1[tap], acc1, 0, 0, 0);
+  for (int k = 0; k < 5; k++) {
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[k], a0[k], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[k], a1[k], acc1, 0, 0, 0);
   }
 }
 // One ReLU convolution of a whole region: dst (wd x hd, origin (yo, xo) in
 // slot coordinates) from src; two 16-pixel groups per wave at a time.
-__device__ __forceinline__ void convRegion(const bf16* src, const bf16* zeros, bf16* dst, int wd, int hd, int yo,
-                                           int xo, const bf16* __restrict__ Wt, const float* __restrict__ bias,
-                                           int H, int W) {
+__device__ __forceinline__ void convRegion(const bf16* src, bf16* dst, int wd, int hd, int yo, int xo,
+                                           const bf16* __restrict__ Wt, const float* __restrict__ bias, int H,
+                                           int W) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, row = lane & 15, g = lane >> 4;
-  bf16x8 w[9];
+  bf16x8 w[5];
   loadTaps(Wt, row, g, w);
   float bv[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) bv[i] = bias[4 * g + i];
   const int n = wd * hd;
+  // a region wholly inside the slot needs no per-pixel padding test
+  const bool interior = yo >= 0 && xo >= 0 && yo + hd <= H && xo + wd <= W;
   for (int base = 32 * wave; base < n; base += 32 * (kThreads / 64)) {
     const int p0 = min(base + row, n - 1), p1 = min(base + 16 + row, n - 1);
     f32x4 a0, a1;
-    convPair(src, zeros, wd, p0, p1, g, w, a0, a1);
+    convPair(src, wd, p0, p1, g, w, a0, a1);
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       const int q = base + 16 * k + row;
       if (q >= n) continue;
       const f32x4 acc = k ? a1 : a0;
-      const int ty = q / wd, tx = q - ty * wd, y = yo + ty, x = xo + tx;
-      const bool inSlot = y >= 0 && y < H && x >= 0 && x < W;
-      bf16x4 o;
+      f32x4 v;
 #pragma unroll
-      for (int i = 0; i < 4; i++) o[i] = inSlot ? (bf16)fmaxf(acc[i] + bv[i], 0.0f) : (bf16)0.0f;
+      for (int i = 0; i < 4; i++) v[i] = fmaxf(acc[i] + bv[i], 0.0f);
+      bf16x4 o = __builtin_convertvector(v, bf16x4);
+      if (!interior) {
+        const int ty = q / wd, tx = q - ty * wd, y = yo + ty, x = xo + tx;
+        if (!(y >= 0 && y < H && x >= 0 && x < W)) o = __builtin_convertvector(f32x4{0.0f, 0.0f, 0.0f, 0.0f}, bf16x4);
+      }
       *reinterpret_cast<bf16x4*>(dst + q * kC + 4 * g) = o;
     }
   }
@@ -246,7 +252,6 @@ __global__ __launch_bounds__(fused::kThreads) void k_denoise1(
   using namespace fused;
   __shared__ __attribute__((aligned(16))) bf16 featB[kFPix * kC];  // features, then the second convolution
   __shared__ __attribute__((aligned(16))) bf16 actA[kAPix * kC];   // the first convolution
-  __shared__ __attribute__((aligned(16))) bf16 zeros[kZeroPix * kC];
   // Blocks are dealt to the 8 XCDs in turn: block b takes tile t so that
   // each XCD walks a contiguous run of tiles (row-major), and the halo a
   // tile shares with its neighbours is read through that XCD's L2.
@@ -254,7 +259,6 @@ __global__ __launch_bounds__(fused::kThreads) void k_denoise1(
   const int per = (tiles + kXcds - 1) / kXcds, t = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
   if (t >= tiles) return;
   const int x0 = (t % tilesX) * kTW, y0 = (t / tilesX) * kTH;
-  for (int i = threadIdx.x; i < kZeroPix * 2; i += kThreads) *reinterpret_cast<bf16x8*>(zeros + 8 * i) = zero8();
   // Features: every load of the thread's pixels is issued before the first
   // is used (pixel 0 stands in outside the image).
   constexpr int kIter = (kFPix + kThreads - 1) / kThreads;
@@ -287,32 +291,46 @@ __global__ __launch_bounds__(fused::kThreads) void k_denoise1(
     *reinterpret_cast<bf16x8*>(featB + p * kC + 8) = hi;
   }
   __syncthreads();
-  convRegion(featB, zeros, actA, kAW, kAH, y0 - 2, x0 - 2, W0, B0, H, W);
+  convRegion(featB, actA, kAW, kAH, y0 - 2, x0 - 2, W0, B0, H, W);
   __syncthreads();
-  convRegion(actA, zeros, featB, kBW, kBH, y0 - 1, x0 - 1, W1, B1, H, W);
+  convRegion(actA, featB, kBW, kBH, y0 - 1, x0 - 1, W1, B1, H, W);
   __syncthreads();
-  // the linear output convolution; lanes of group 0 hold channels 0-3
+  // The linear output convolution. Lanes of group 0 hold channels 0-3 of
+  // their pixel; they leave the three used ones, rounded to bf16 as the
+  // layered output is, in LDS (the first convolution's, no longer read), and
+  // every thread then finishes two pixels: expm1, Filtered, the resolve.
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, row = lane & 15, g = lane >> 4;
-  bf16x8 wt[9];
+  bf16x8 wt[5];
   loadTaps(W2, row, g, wt);
   float bv[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) bv[i] = B2[4 * g + i];
+  bf16* rgb = actA;  // [kTW * kTH][4]
   for (int base = 32 * wave; base < kTW * kTH; base += 32 * (kThreads / 64)) {
     f32x4 a0, a1;
-    convPair(featB, zeros, kTW, base + row, base + 16 + row, g, wt, a0, a1);
+    convPair(featB, kTW, base + row, base + 16 + row, g, wt, a0, a1);
+    if (g == 0) {
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
-      const int q = base + 16 * k + row, ty = q / kTW, y = y0 + ty, x = x0 + (q - ty * kTW);
-      if (g != 0 || y >= h || x >= w) continue;
-      const f32x4 acc = k ? a1 : a0;
-      const float r = fmaxf(expm1Fast((float)(bf16)(acc[0] + bv[0])), 0.0f);
-      const float gg = fmaxf(expm1Fast((float)(bf16)(acc[1] + bv[1])), 0.0f);
-      const float b = fmaxf(expm1Fast((float)(bf16)(acc[2] + bv[2])), 0.0f);
-      const int i = y * w + x;
-      Out[i] = make_float4(r, gg, b, 1.0f);
-      if constexpr (kResolve) arbor::resolveOne(arbor::mk(r, gg, b), History, Pixels, C, x, y, w, h, i);
+      for (int k = 0; k < 2; k++) {
+        const f32x4 acc = k ? a1 : a0;
+        f32x4 v;
+#pragma unroll
+        for (int i = 0; i < 4; i++) v[i] = acc[i] + bv[i];
+        *reinterpret_cast<bf16x4*>(rgb + (base + 16 * k + row) * 4) = __builtin_convertvector(v, bf16x4);
+      }
     }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < kTW * kTH; q += kThreads) {
+    const int ty = q / kTW, y = y0 + ty, x = x0 + (q - ty * kTW);
+    if (y >= h || x >= w) continue;
+    const bf16x4 o = *reinterpret_cast<const bf16x4*>(rgb + q * 4);
+    const float r = fmaxf(expm1Fast((float)o[0]), 0.0f);
+    const float gg = fmaxf(expm1Fast((float)o[1]), 0.0f);
+    const float b = fmaxf(expm1Fast((float)o[2]), 0.0f);
+    const int i = y * w + x;
+    Out[i] = make_float4(r, gg, b, 1.0f);
+    if constexpr (kResolve) arbor::resolveOne(arbor::mk(r, gg, b), History, Pixels, C, x, y, w, h, i);
   }
 }
 
