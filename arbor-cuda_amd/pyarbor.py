@@ -80,6 +80,7 @@ SIGNATURES = {
     "arbor_sunVisibilityScheduled": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "arbor_trace_accel_bytes": [],
     "arbor_buildTraceAccel": [_vp, _vp, _vp, _vp],
+    "arbor_buildTraceAccelLimit": [_vp, _vp, _vp, ct.c_int, _vp],
     "arbor_tracePixelsAccel": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "arbor_no_alloc_scratch_bytes": [],
     "arbor_tracePixelsAccelNoAlloc": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -276,14 +277,28 @@ def trace_scheduled(scene, C, fb, schedule):
 
 
 class TraceAccel:
-    """Device SAH hierarchy of one scene (arbor_buildTraceAccel)."""
+    """Device SAH hierarchy of one scene (arbor_buildTraceAccel; with
+    depth_limit, arbor_buildTraceAccelLimit)."""
 
-    def __init__(self, scene, device="cuda"):
+    # u32 words of the hierarchy at the start of the buffer (the build's
+    # workspace follows them)
+    WORDS = 16 + 16 * (CAPACITY - 1) + CAPACITY
+
+    def __init__(self, scene, device="cuda", depth_limit=None):
         import torch
 
         n = lib().arbor_trace_accel_bytes()
         self.buf = torch.zeros(n, dtype=torch.uint8, device=device)
-        call("arbor_buildTraceAccel", scene.Nodes, scene.Order, self.buf)
+        if depth_limit is None:
+            call("arbor_buildTraceAccel", scene.Nodes, scene.Order, self.buf)
+        else:
+            call("arbor_buildTraceAccelLimit", scene.Nodes, scene.Order, self.buf, int(depth_limit))
+
+    def words(self):
+        """The hierarchy's u32 words, on the host."""
+        import numpy as np
+
+        return self.buf[: 4 * self.WORDS].cpu().numpy().view(np.uint32).copy()
 
 
 def trace_accel(scene, accel, C, fb, schedule=None):

@@ -159,11 +159,13 @@ int arbor_shadePixelsVis(const float* P, const float* Nodes, const unsigned* Ord
                          const float* Vis, float* Linear, float* Guide, int width, int height, hipStream_t stream);
 
 /* SAH trace hierarchy (no counterpart in the reference; for callers that trace
- * one scene many times). arbor_buildTraceAccel builds, on the host, a
- * binned surface-area-heuristic hierarchy over the heap BVH's leaf boxes (the
- * Nodes and Order of buildBVH) into `accel`, arbor_trace_accel_bytes() bytes
- * of device memory; it synchronises the stream, and the same Nodes/Order give
- * the same bytes. The Accel trace and visibility entry points walk it in place
+ * one scene many times). arbor_buildTraceAccel builds, on the device and
+ * asynchronously on `stream`, a binned surface-area-heuristic hierarchy over
+ * the heap BVH's leaf boxes (the Nodes and Order of buildBVH) into `accel`,
+ * arbor_trace_accel_bytes() bytes of device memory: the hierarchy's words
+ * (16 + 16 (ARBOR_CAPACITY - 1) + ARBOR_CAPACITY u32) first, then the build's
+ * workspace. The same Nodes/Order give the same words, and every leaf lies
+ * within the walk's 32-entry stack. The Accel trace and visibility entry points walk it in place
  * of the heap (about a third of the box tests and 30% fewer object tests).
  * They take the heap too: objects hit at exactly the same t resolve in the
  * heap walk's order, decided at their common heap ancestor. Results equal
@@ -175,6 +177,12 @@ int arbor_shadePixelsVis(const float* P, const float* Nodes, const unsigned* Ord
  * and Order passed with it (it is not checked). */
 long long arbor_trace_accel_bytes(void);
 int arbor_buildTraceAccel(const float* Nodes, const unsigned* Order, void* accel, hipStream_t stream);
+/* The same build with every leaf within depth_limit (15..32) of the root:
+ * nodes whose primitives would not fit below the limit by a surface-area
+ * split are split at the median instead. depth_limit 32 is
+ * arbor_buildTraceAccel. */
+int arbor_buildTraceAccelLimit(const float* Nodes, const unsigned* Order, void* accel, int depth_limit,
+                               hipStream_t stream);
 int arbor_tracePixelsAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,
                            const float* C, float* Hit, int width, int height, void* schedule, hipStream_t stream);
 int arbor_sunVisibilityAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel,

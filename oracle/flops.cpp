@@ -11,7 +11,7 @@
 #include <cstdint>
 
 #include "arbor_oracle.hpp"
-#include "../arbor-cuda_amd/csrc/sah_build.hpp"
+#include "sah_build.hpp"
 
 namespace orc {
 
@@ -409,7 +409,7 @@ int orc_cull_stress(const float* P, int raysPerObject, int seed, float time, flo
 // Reference walk vs lean walk over whole frames: returns the number of
 // pixels whose Hit differs in any bit (0 expected).
 // The SAH hierarchy of the GPU's accel walk over this heap BVH (the
-// library's own builder, sah_build.hpp) into out[words]: 16 + 16 * (CAPACITY -
+// host restatement of the device builder, sah_build.hpp) into out[words]: 16 + 16 * (CAPACITY -
 // 1) + CAPACITY words. Returns 0, or -1 if out is too small or the depth
 // limit (32 as the walk's stack; lower in the limit form, to test the
 // builder's bound) failed.

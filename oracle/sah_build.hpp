@@ -1,5 +1,7 @@
-// Host-only builder of the SAH trace hierarchy (plain C++, no HIP types): a
-// binned surface-area-heuristic binary tree over the heap BVH's leaf boxes,
+// Oracle (test infrastructure): the host restatement of the device SAH
+// builder (arbor-cuda_amd/csrc/accel.hip), recursive and serial, which the
+// GPU tests compare the device build against word for word. A binned
+// surface-area-heuristic binary tree over the heap BVH's leaf boxes,
 // flattened into the device layout the SAH walk reads (arbor_trace.hpp):
 //   float4 [0] {root lo, root ref}, [1] {root hi, depth}, [2] {nodes, prims,
 //   0, 0}, [3] 0; node j at float4 4 + 4j: {left lo, left ref} {left hi, 0}

@@ -1,5 +1,5 @@
-"""Structure of the SAH trace hierarchy (device layout of
-arbor-cuda_amd/csrc/sah_build.hpp), checked against the heap BVH it was built
+"""Structure of the SAH trace hierarchy (the device layout that
+arbor-cuda_amd/csrc/accel.hip writes), checked against the heap BVH it was built
 from: every primitive of the heap appears once, with its heap leaf box; every
 child box of an interior node is the exact union of that child's own
 children's boxes; the depth in the header is the deepest leaf's and within
@@ -55,3 +55,46 @@ def check_accel_structure(words, nodes, order):
     for s in np.nonzero(used)[0]:
         assert slots[order[s]] == s
     return depth
+
+
+class LeafSet:
+    """Nodes (heap layout, leaf records from CAPACITY on) and Order of a
+    synthetic leaf set, on the host."""
+
+
+def skewed_leaf_set(shape):
+    """Leaf sets built to defeat the surface-area heuristic: every box the
+    same ("identical": no split separates them), centres spaced
+    exponentially along one axis ("exponential": each SAH split would peel
+    off one leaf), nested boxes growing geometrically ("nested"), and all
+    CAPACITY slots used ("full", random boxes). Ids are a seeded permutation
+    of the used slots."""
+    cap = CAPACITY
+    rng = np.random.default_rng(5)
+    n = {"identical": 20000, "exponential": 30000, "nested": 4000, "full": cap}[shape]
+    nodes = np.zeros((2 * cap, 8), np.float32)
+    leaves = nodes[cap:]
+    if shape == "identical":
+        c = np.zeros((n, 3), np.float32)
+    elif shape == "exponential":
+        c = np.zeros((n, 3), np.float32)
+        c[:, 0] = np.float32(1.0009) ** np.arange(n, dtype=np.float32)
+    else:
+        c = rng.uniform(-10, 10, (n, 3)).astype(np.float32)
+    half = np.float32(0.01)
+    if shape == "nested":  # each box holds every smaller one: the heuristic would peel one per level
+        g = np.float32(1.004) ** np.arange(n, dtype=np.float32)
+        c = np.zeros((n, 3), np.float32)
+        c[:, 0] = g
+        half = (g * np.float32(2.0))[:, None]
+    leaves[:n, 0:3] = c - half
+    leaves[:n, 4:7] = c + half
+    leaves[:n, 3] = 1.0
+    order = np.arange(cap, dtype=np.uint32)
+    perm = rng.permutation(n).astype(np.uint32)
+    order[:n] = perm
+    leaves[:n, 7] = perm.astype(np.float32)
+    s = LeafSet()
+    s.Nodes = np.ascontiguousarray(nodes.reshape(-1))
+    s.Order = order
+    return s

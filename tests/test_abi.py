@@ -152,11 +152,17 @@ def test_tune_cache_roundtrip(L, tmp_path):
 
 def test_trace_accel_size_and_checks(L):
     # header + 4 float4 per interior node of a binary tree over CAPACITY
-    # leaves, then a u32 heap slot per primitive id
-    assert L.arbor_trace_accel_bytes() == (4 + 4 * (32768 - 1)) * 16 + 4 * 32768
+    # leaves, then a u32 heap slot per primitive id; the build's workspace
+    # (two arrays of CAPACITY primitives of 40 bytes, the node queues)
+    # follows those words
+    words = (4 + 4 * (32768 - 1)) * 16 + 4 * 32768
+    assert words + 2 * 32768 * 40 < L.arbor_trace_accel_bytes() < words + 4 * 1024 * 1024
     E = -1
     assert L.arbor_buildTraceAccel(NULL, DUMMY, DUMMY, NULL) == E
     assert L.arbor_buildTraceAccel(DUMMY, DUMMY, NULL, NULL) == E
+    assert L.arbor_buildTraceAccelLimit(NULL, DUMMY, DUMMY, 32, NULL) == E
+    for bad in (14, 33):
+        assert L.arbor_buildTraceAccelLimit(DUMMY, DUMMY, DUMMY, bad, NULL) == -2
     assert L.arbor_tracePixelsAccel(DUMMY, DUMMY, DUMMY, NULL, DUMMY, DUMMY, 64, 64, NULL, NULL) == E
     assert L.arbor_tracePixelsAccel(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 0, 64, NULL, NULL) == -2
     assert L.arbor_sunVisibilityAccel(DUMMY, DUMMY, DUMMY, NULL, DUMMY, DUMMY, DUMMY, 64, 64, NULL, NULL) == E

@@ -1,6 +1,10 @@
-"""The SAH trace hierarchy (arbor_buildTraceAccel) and the trace and
-sun-visibility passes over it.
+"""The SAH trace hierarchy (arbor_buildTraceAccel, built on the device) and
+the trace and sun-visibility passes over it.
 
+* Build: the device builder's words equal the oracle's host builder's (the
+  same binned SAH, costs, partition and median fallback) bit for bit on four
+  scenes and on leaf sets built to defeat the heuristic, at the walk's depth
+  bound 32 and at lower bounds (where the median splits take over).
 * Structure: deterministic bytes; every primitive of the heap BVH appears once
   with its heap leaf box; every interior child box is the exact union of its
   children's; depth within the walk's 32-entry stack; the slot table maps
@@ -23,7 +27,7 @@ ACCEL_VARIANTS = 22
 # accel_vis_variant 0-21 (10-21 read the tree records from the schedule)
 ACCEL_VIS_VARIANTS = 22
 
-from accel_check import check_accel_structure
+from accel_check import LeafSet, check_accel_structure, skewed_leaf_set
 from gpu_util import require_gpu, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
@@ -58,10 +62,55 @@ def _walk_camera(px, oracle, frame, w, h, shadows):
 def test_accel_structure(px, scene):
     print("case: accel structure, seed 7", flush=True)
     dev, acc = scene
-    buf = acc.buf.cpu().numpy()
-    again = px.TraceAccel(dev).buf.cpu().numpy()
-    assert np.array_equal(buf, again), "the build is not deterministic"
-    check_accel_structure(buf.view(np.uint32), to_host(dev.Nodes), to_host(dev.Order))
+    words = acc.words()
+    again = px.TraceAccel(dev).words()
+    assert np.array_equal(words, again), "the build is not deterministic"
+    check_accel_structure(words, to_host(dev.Nodes), to_host(dev.Order))
+
+
+def _host_leaf_set(dev):
+    s = LeafSet()
+    s.Nodes = to_host(dev.Nodes)
+    s.Order = to_host(dev.Order).view(np.uint32)
+    return s
+
+
+@pytest.mark.parametrize("seed", [7, 1, 42, 1788])
+def test_device_build_equals_oracle_builder(px, oracle, seed):
+    """The device build of a scene's hierarchy equals the oracle's host
+    build word for word, at depth bounds 32 (arbor_buildTraceAccel) and 15
+    (arbor_buildTraceAccelLimit: median splits over the upper levels)."""
+    dev = px.DeviceScene().generate(seed)
+    host = _host_leaf_set(dev)
+    for limit in (None, 15):
+        print("case: device SAH build, seed", seed, "depth limit", limit or 32, flush=True)
+        got = px.TraceAccel(dev, depth_limit=limit).words()
+        want = oracle.build_sah(host, depth_limit=limit)
+        bad = np.nonzero(got != want)[0]
+        assert bad.size == 0, (limit, bad[:8], got[bad[:8]], want[bad[:8]])
+        depth = check_accel_structure(got, host.Nodes, host.Order)
+        assert depth <= (limit or 32)
+
+
+@pytest.mark.parametrize("shape", ["identical", "exponential", "nested", "full"])
+def test_device_build_on_skewed_leaf_sets(px, oracle, shape):
+    """Leaf sets built to defeat the heuristic (identical boxes, centres
+    spaced exponentially, nested boxes, every slot used): the device build
+    equals the oracle's at depth bounds 32 and 16 and keeps every leaf
+    within the bound."""
+    s = skewed_leaf_set(shape)
+
+    class Dev:
+        Nodes = to_dev(s.Nodes)
+        Order = to_dev(s.Order)
+
+    for limit in (None, 16):
+        print("case: device SAH build,", shape, "leaf set, depth limit", limit or 32, flush=True)
+        got = px.TraceAccel(Dev, depth_limit=limit).words()
+        want = oracle.build_sah(s, depth_limit=limit)
+        bad = np.nonzero(got != want)[0]
+        assert bad.size == 0, (limit, bad[:8], got[bad[:8]], want[bad[:8]])
+        assert check_accel_structure(got, s.Nodes, s.Order) <= (limit or 32)
 
 
 @pytest.mark.parametrize("seed", [1, 42, 1788])
@@ -74,7 +123,7 @@ def test_accel_other_scenes(px, oracle, seed):
 
     dev = px.DeviceScene().generate(seed)
     acc = px.TraceAccel(dev)
-    check_accel_structure(acc.buf.cpu().numpy().view(np.uint32), to_host(dev.Nodes), to_host(dev.Order))
+    check_accel_structure(acc.words(), to_host(dev.Nodes), to_host(dev.Order))
     W, H = 640, 360
     C = oracle.init_camera(seed)
     I = px.walk_input(seed, 0)

@@ -386,40 +386,9 @@ def test_sah_depth_bound_on_skewed_leaf_sets(oracle, shape):
     (each SAH split would peel off one leaf), nested boxes growing
     geometrically, and all 32,768 slots used. The
     hierarchy stays a valid one of exactly those leaf boxes."""
-    from accel_check import check_accel_structure
+    from accel_check import check_accel_structure, skewed_leaf_set
 
-    cap = 32768
-    rng = np.random.default_rng(5)
-    n = {"identical": 20000, "exponential": 30000, "nested": 4000, "full": cap}[shape]
-    nodes = np.zeros((2 * cap, 8), np.float32)
-    leaves = nodes[cap:]
-    if shape == "identical":
-        c = np.zeros((n, 3), np.float32)
-    elif shape == "exponential":
-        c = np.zeros((n, 3), np.float32)
-        c[:, 0] = np.float32(1.0009) ** np.arange(n, dtype=np.float32)
-    else:
-        c = rng.uniform(-10, 10, (n, 3)).astype(np.float32)
-    half = np.float32(0.01)
-    if shape == "nested":  # each box holds every smaller one: the heuristic would peel one per level
-        g = np.float32(1.004) ** np.arange(n, dtype=np.float32)
-        c = np.zeros((n, 3), np.float32)
-        c[:, 0] = g
-        half = (g * np.float32(2.0))[:, None]
-    leaves[:n, 0:3] = c - half
-    leaves[:n, 4:7] = c + half
-    leaves[:n, 3] = 1.0
-    order = np.arange(cap, dtype=np.uint32)
-    perm = rng.permutation(n).astype(np.uint32)
-    order[:n] = perm
-    leaves[:n, 7] = perm.astype(np.float32)
-
-    class S:
-        pass
-
-    s = S()
-    s.Nodes = np.ascontiguousarray(nodes.reshape(-1))
-    s.Order = order
+    s = skewed_leaf_set(shape)
     a = oracle.build_sah(s)
     depth = check_accel_structure(a, s.Nodes, s.Order)
     # the same leaves with the bound at 16: the median splits near the limit
