@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "arbor_conv1x1.hpp"
 #include "arbor_denoise.h"
 #include "launch.hpp"
 
@@ -65,11 +66,14 @@ __device__ __forceinline__ int ldsSlot(int pix, int grp) { return pix * kChunk +
 // for 128; 1 leaves the allocation unforced). Over the network's layer shapes
 // at 1088x1920 these targets give the lowest total time.
 constexpr int convWaves(int nsub) { return nsub == 2 ? 5 : (nsub == 3 ? 4 : (nsub == 8 ? 3 : 1)); }
-// kPair (16 -> 16, no concatenation): two taps share a matrix instruction,
-// K slots 0-15 holding tap 2k's 16 channels and 16-31 tap 2k+1's; the fifth
-// instruction holds tap 8 and zero weights. Five instructions per output
-// subtile instead of nine half-empty ones, and no LDS read of zeros.
-template <int NSUB, int MSUB, bool kPair = false>
+// kPack taps share a matrix instruction (no concatenation):
+//  * 2 (16 -> 16): K slots 0-15 hold tap 2k's 16 channels and 16-31 tap
+//    2k+1's; the fifth instruction holds tap 8 and zero weights. Five
+//    instructions per output subtile instead of nine half-empty ones.
+//  * 4 (8 input channels): K slots 8g..8g+7 hold tap 4k+g's 8 channels; the
+//    third instruction holds tap 8 and zero weights. Three instructions
+//    instead of nine quarter-full ones.
+template <int NSUB, int MSUB, int kPack = 1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(convWaves(NSUB)))) void k_conv3x3(
     const __bf16* __restrict__ X, const __bf16* __restrict__ Xc, int Cc, int N, int H, int W, int Cin,
     const __bf16* __restrict__ Wt, const float* __restrict__ bias, int relu, __bf16* __restrict__ Y, int CoutTotal,
@@ -123,7 +127,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(convWaves(N
       if (i < kGroups) *reinterpret_cast<bf16x8*>(&tile[ldsSlot(i >> 2, i & 3)]) = ok[s] ? sv[s] : zero8();
     }
     __syncthreads();
-    if constexpr (kPair) {
+    if constexpr (kPack == 4) {
+      // lane (row, g): tap 4k + g of instruction k, channels 0-7 (staged
+      // group 0); every lane reads a staged pixel (tap 8 for the slots
+      // without a tap, whose weights are zero)
+      auto loadQuad = [&](int k, bf16x8* w8) {
+        const int tap = 4 * k + g;
+#pragma unroll
+        for (int j = 0; j < NSUB; j++)
+          w8[j] = tap < 9 ? *reinterpret_cast<const bf16x8*>(Wt + ((size_t)tap * CoutTotal + co0 + j * 16 + row) * Cin)
+                          : zero8();
+      };
+      auto quadTaps = [&](int k, const bf16x8* w8) {
+        const int tl = min(4 * k + g, 8), dy = tl / 3, dx = tl - 3 * (tl / 3);
+#pragma unroll
+        for (int m = 0; m < MSUB; m++) {
+          const bf16x8 a =
+              *reinterpret_cast<const bf16x8*>(&tile[ldsSlot((MSUB * wave + m + dy) * kHaloW + row + dx, 0)]);
+#pragma unroll
+          for (int j = 0; j < NSUB; j++)
+            acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w8[j], a, acc[m][j], 0, 0, 0);
+        }
+      };
+      bf16x8 w0[NSUB], w1[NSUB], w2[NSUB];
+      loadQuad(0, w0);
+      loadQuad(1, w1);
+      loadQuad(2, w2);
+      asm volatile("" ::: "memory");
+      quadTaps(0, w0);
+      quadTaps(1, w1);
+      quadTaps(2, w2);
+      __syncthreads();
+      continue;
+    }
+    if constexpr (kPack == 2) {
       // lane (row, g): tap 2k + (g >> 1) of pair k, channels 8 (g & 1) ..;
       // the next pair's weights are loaded while the current one computes
       const int grp = g & 1;
@@ -225,6 +262,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(convWaves(N
         o[i] = f2bf(v);
       }
       *reinterpret_cast<bf16x4*>(Y + p * CoutTotal + co0 + 16 * j + 4 * g) = o;
+    }
+  }
+}
+
+// ---- 1x1 convolution (the lean network's output layer) ----
+// Y[p, co] = act(sum_ci X[p, ci] * W1[co, ci] + bias[co]) over P pixels of
+// 16*J channels, W1 [Cout][16*J]: each wave takes 16-pixel groups, one
+// matrix instruction per 16 output channels (operands as arbor_conv1x1.hpp
+// lays them out).
+template <int J>
+__global__ __launch_bounds__(256) void k_conv1x1(const __bf16* __restrict__ X, long long P,
+                                                 const __bf16* __restrict__ W1, const float* __restrict__ bias,
+                                                 int relu, __bf16* __restrict__ Y, int Cout) {
+  using Op = dn1::Op1x1<J>;
+  constexpr int Cin = 16 * J;
+  const int lane = threadIdx.x & 63, row = lane & 15, g = lane >> 4;
+  const long long groups = (P + 15) / 16;
+  for (long long grp = (blockIdx.x * 256LL + threadIdx.x) >> 6; grp < groups; grp += gridDim.x * 4LL) {
+    const long long p = grp * 16 + row;
+    const bool ok = p < P;
+    const typename Op::T b = Op::load(X + (ok ? p : P - 1) * Cin, g);
+    for (int co0 = 0; co0 < Cout; co0 += 16) {
+      const f32x4 acc = Op::mma(Op::load(W1 + (size_t)(co0 + row) * Cin, g), b, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+      bf16x4 o;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        float v = acc[i] + (bias ? bias[co0 + 4 * g + i] : 0.0f);
+        if (relu) v = fmaxf(v, 0.0f);
+        o[i] = f2bf(v);
+      }
+      if (ok) *reinterpret_cast<bf16x4*>(Y + p * Cout + co0 + 4 * g) = o;
     }
   }
 }
@@ -437,7 +505,10 @@ void launchConv(const __bf16* X, const __bf16* Xc, int Cc, int N, int H, int W, 
   // paired taps for 16 -> 16 only: with wider outputs the mixed-tap LDS reads
   // conflict (16 -> 32 took 117 against 66-82 us at 1088x1920)
   if (NSUB == 1 && Cin == 16 && !Xc)
-    hipLaunchKernelGGL((k_conv3x3<NSUB, MSUB, true>), dim3((unsigned)blocks), dim3(256), 0, s, X, Xc, Cc, N, H, W,
+    hipLaunchKernelGGL((k_conv3x3<NSUB, MSUB, 2>), dim3((unsigned)blocks), dim3(256), 0, s, X, Xc, Cc, N, H, W,
+                       Cin, Wt, bias, relu, Y, coutTotal, co0);
+  else if (Cin == 8 && !Xc)  // four taps per instruction
+    hipLaunchKernelGGL((k_conv3x3<NSUB, MSUB, 4>), dim3((unsigned)blocks), dim3(256), 0, s, X, Xc, Cc, N, H, W,
                        Cin, Wt, bias, relu, Y, coutTotal, co0);
   else
     hipLaunchKernelGGL((k_conv3x3<NSUB, MSUB>), dim3((unsigned)blocks), dim3(256), 0, s, X, Xc, Cc, N, H, W, Cin,
@@ -482,6 +553,24 @@ int arbor_dn_conv3x3(const uint16_t* X, int n, int h, int w, int cin, const uint
   if (cin > ARBOR_DN_MAX_CHANNELS || (long long)n * h * w > (1LL << 31) / 256) return ARBOR_ERR_ARG;
   return conv(reinterpret_cast<const __bf16*>(X), nullptr, 0, n, h, w, cin, reinterpret_cast<const __bf16*>(Wt), bias,
               relu, reinterpret_cast<__bf16*>(Y), cout, stream);
+}
+
+int arbor_dn_conv1x1(const uint16_t* X, long long pixels, int cin, const uint16_t* W1, const float* bias, int relu,
+                     uint16_t* Y, int cout, hipStream_t stream) {
+  ARBOR_REQUIRE(X && W1 && Y);
+  if (pixels < 1 || !fits32(pixels * (cin > cout ? cin : cout)) || (cin != 16 && cin != 32) || cout < 16 ||
+      cout % 16 != 0 || cout > ARBOR_DN_MAX_CHANNELS)
+    return ARBOR_ERR_ARG;
+  const long long groups = (pixels + 15) / 16;
+  const int blocks = (int)((groups + 3) / 4 < 65535 ? (groups + 3) / 4 : 65535);
+  auto x = reinterpret_cast<const __bf16*>(X);
+  auto w = reinterpret_cast<const __bf16*>(W1);
+  auto y = reinterpret_cast<__bf16*>(Y);
+  if (cin == 16)
+    hipLaunchKernelGGL(k_conv1x1<1>, dim3(blocks), dim3(256), 0, stream, x, pixels, w, bias, relu, y, cout);
+  else
+    hipLaunchKernelGGL(k_conv1x1<2>, dim3(blocks), dim3(256), 0, stream, x, pixels, w, bias, relu, y, cout);
+  return launch_status();
 }
 
 int arbor_dn_conv3x3_cat(const uint16_t* Xc, int cc, const uint16_t* X, int cs, int n, int h, int w, const uint16_t* Wt,

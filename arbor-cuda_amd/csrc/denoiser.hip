@@ -15,6 +15,12 @@
 //   L12 -> L13 -> u,c(s1) -> L14 -> L15 -> u,c(s0) -> L16 -> L17 -> L18 (linear)
 // With L levels: encoder layers 0..2L-1 (two per level), decoder layers
 // 2L..4L-3 (two per level above the bottom), output layer 4L-2.
+//
+// The lean network (levels = 0): the compact input features (8 channels:
+// log(1 + radiance) rgb, albedo rgb, depth, zero), one 3x3 convolution of 16
+// or 32 channels with ReLU and a linear 1x1 output layer; layer 0 and
+// layer 1. Its one-pass kernel reads the 16-byte compact features the
+// shading pass can write directly (arbor_shadePixelsFeatures).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -25,7 +31,9 @@
 #include <string>
 #include <vector>
 
+#include "arbor_conv1x1.hpp"
 #include "arbor_denoise.h"
+#include "arbor_features.hpp"
 #include "arbor_filter.hpp"
 #include "engine_passes.hpp"
 #include "launch.hpp"
@@ -38,31 +46,13 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kInC = 16, kOutC = 16, kMaxLevels = 5;
+constexpr int kLeanInC = 8;  // the compact features (arbor::compactFeatures)
 constexpr int kDefaultWidths[kMaxLevels] = {32, 48, 64, 96, 128};
-constexpr float kFar = 100000.0f;
 
 // ---- data kernels ----
-// log(1 + x) for x >= 0 within about 1e-6 relative: below 1/16 its Taylor
-// series to x^6 (truncation x^6 / 7 relative, under 2e-8), above it the
-// hardware logarithm of 1 + x (the rounding of 1 + x costs at most 2^-24 /
-// log(1 + 1/16), 1e-6 relative). The inputs are rounded to bf16 (2^-8).
-// Both forms are evaluated and one selected (no branch: neighbouring pixels
-// take either side).
-__device__ __forceinline__ float log1pFast(float x) {
-  const float series = x * (1.0f - x * (0.5f - x * (1.0f / 3.0f - x * (0.25f - x * (0.2f - x * (1.0f / 6.0f))))));
-  const float hw = __logf(1.0f + x);
-  return x < 0.0625f ? series : hw;
-}
-// exp(y) - 1 within about 2e-6 relative: below |y| = 1/16 its Taylor series
-// to y^6 (truncation under 1e-10 relative), above it the hardware
-// exponential (1 ulp) minus 1, which magnifies its error at most
-// e^(1/16) / (e^(1/16) - 1) = 16.5 times.
-__device__ __forceinline__ float expm1Fast(float y) {
-  const float series =
-      y * (1.0f + y * (0.5f + y * (1.0f / 6.0f + y * (1.0f / 24.0f + y * (1.0f / 120.0f + y * (1.0f / 720.0f))))));
-  const float hw = __expf(y) - 1.0f;
-  return fabsf(y) < 0.0625f ? series : hw;
-}
+using arbor::depthFeature;
+using arbor::expm1Fast;
+using arbor::log1pFast;
 // The ten used input features of one pixel from its Linear, Guide, Normal
 // and Hit values.
 __device__ __forceinline__ void featuresOf(const float4 l, const float4 g, const float4 n, const float4 t, float* f) {
@@ -75,7 +65,7 @@ __device__ __forceinline__ void featuresOf(const float4 l, const float4 g, const
   f[6] = n.x;
   f[7] = n.y;
   f[8] = n.z;
-  f[9] = t.x < kFar ? log1pFast(fmaxf(fminf(t.x, 1000.0f), 0.0f)) * (1.0f / 7.0f) : 1.0f;
+  f[9] = depthFeature(t.x);
 }
 // The same of pixel i of the frame buffers.
 __device__ __forceinline__ void pixelFeatures(const float4* __restrict__ Linear, const float4* __restrict__ Guide,
@@ -83,11 +73,25 @@ __device__ __forceinline__ void pixelFeatures(const float4* __restrict__ Linear,
                                               int i, float* f) {
   featuresOf(Linear[i], Guide[i], Normal[i], Hit[i], f);
 }
+// Input slot (H x W) from a frame (w x h; zero outside): the ten features in
+// 16 channels, or with kCompact the compact set in 8 (Normal not read).
+template <bool kCompact>
 __global__ void k_features(const float4* __restrict__ Linear, const float4* __restrict__ Guide,
                            const float4* __restrict__ Normal, const float4* __restrict__ Hit, int w, int h,
                            bf16* __restrict__ X, int H, int W) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   if (x >= W) return;
+  if constexpr (kCompact) {
+    arbor::featx8 f;
+#pragma unroll
+    for (int c = 0; c < 8; c++) f[c] = (bf16)0.0f;
+    if (x < w && y < h) {
+      const int i = y * w + x;
+      f = arbor::compactFeatures(Linear[i], Guide[i], Hit[i].x);
+    }
+    *reinterpret_cast<arbor::featx8*>(X + ((size_t)y * W + x) * kLeanInC) = f;
+    return;
+  }
   float f[kInC];
   for (int c = 0; c < kInC; c++) f[c] = 0.0f;
   if (x < w && y < h) pixelFeatures(Linear, Guide, Normal, Hit, y * w + x, f);
@@ -334,6 +338,125 @@ __global__ __launch_bounds__(fused::kThreads) void k_denoise1(
   }
 }
 
+// ---- the lean network in one pass ----
+// A block computes a 32x16-pixel tile: it stages the compact features of the
+// tile plus a 1-pixel halo (34x18 pixels, 16 bytes each) in LDS, and each
+// wave takes eight 16-pixel groups. Per group, the 3x3 convolution is three
+// v_mfma_f32_16x16x32_bf16 per 16 output channels (four taps per
+// instruction: K slots 8g..8g+7 hold tap 4k+g's 8 channels; the third holds
+// tap 8 and zero weights), as arbor_dn_conv3x3 issues them for 8 input
+// channels; bias, ReLU and the rounding to bf16 leave the activations where
+// the 1x1 output layer takes its B operand (arbor_conv1x1.hpp), so that
+// layer is one more instruction on registers. The output, rounded to bf16
+// as the layered network's is, goes through LDS so that every thread then
+// finishes two pixels (expm1, Out, and with kResolve resolvePixels). Feat
+// holds the features of a w x h frame with row stride `stride` (zero outside
+// the frame, as the layered network's padding): the output has the layered
+// runner's bits.
+namespace lean {
+constexpr int kTW = 32, kTH = 16, kRW = kTW + 2, kRH = kTH + 2, kRPix = kRW * kRH;
+constexpr int kThreads = 256, kGroupsPerWave = kTW * kTH / 16 / (kThreads / 64);
+}  // namespace lean
+
+template <int J, bool kResolve>
+__global__ __launch_bounds__(lean::kThreads) void k_denoiseLean(
+    const arbor::featx8* __restrict__ Feat, int stride, int w, int h, const bf16* __restrict__ W0,
+    const float* __restrict__ B0, const bf16* __restrict__ W1, const float* __restrict__ B1, float4* __restrict__ Out,
+    const float* C, float4* History, uint32_t* Pixels) {
+  using namespace lean;
+  using Op = arbor::dn1::Op1x1<J>;
+  __shared__ arbor::featx8 feat[kRPix];
+  __shared__ bf16x4 rgb[kTW * kTH];
+  // XCD-contiguous tiles, as k_denoise1 takes them
+  const int tilesX = (w + kTW - 1) / kTW, tiles = tilesX * ((h + kTH - 1) / kTH);
+  const int per = (tiles + fused::kXcds - 1) / fused::kXcds;
+  const int t = (int)(blockIdx.x % fused::kXcds) * per + (int)(blockIdx.x / fused::kXcds);
+  if (t >= tiles) return;
+  const int x0 = (t % tilesX) * kTW, y0 = (t / tilesX) * kTH;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, row = lane & 15, g = lane >> 4;
+  // stage: every load issued before the first store (pixel 0 outside the frame)
+  constexpr int kIter = (kRPix + kThreads - 1) / kThreads;
+  arbor::featx8 v[kIter];
+  bool in[kIter];
+#pragma unroll
+  for (int s = 0; s < kIter; s++) {
+    const int p = threadIdx.x + s * kThreads, fy = p / kRW, y = y0 - 1 + fy, x = x0 - 1 + (p - fy * kRW);
+    in[s] = p < kRPix && y >= 0 && y < h && x >= 0 && x < w;
+    v[s] = Feat[in[s] ? (size_t)y * stride + x : 0];
+  }
+  // weight fragments: instruction k, output group j (lane row = channel
+  // 16j + row, K slots 8g.. = tap 4k+g), the 1x1 layer's, and the biases
+  bf16x8 w0[3][J];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+      const int tap = 4 * k + g;
+      w0[k][j] = tap < 9 ? *reinterpret_cast<const bf16x8*>(W0 + ((size_t)tap * 16 * J + 16 * j + row) * kLeanInC)
+                         : fused::zero8();
+    }
+  const typename Op::T w1 = Op::load(W1 + (size_t)row * 16 * J, g);
+  float b0[J][4], b1[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+#pragma unroll
+    for (int j = 0; j < J; j++) b0[j][i] = B0[16 * j + 4 * g + i];
+    b1[i] = B1[4 * g + i];
+  }
+#pragma unroll
+  for (int s = 0; s < kIter; s++) {
+    const int p = threadIdx.x + s * kThreads;
+    if (p < kRPix) {
+      if (!in[s]) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) v[s][c] = (bf16)0.0f;
+      }
+      feat[p] = v[s];
+    }
+  }
+  __syncthreads();
+  for (int s = 0; s < kGroupsPerWave; s++) {
+    const int grp = wave * kGroupsPerWave + s, ty = grp >> 1, tx = (grp & 1) * 16 + row;
+    f32x4 acc[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int tl = min(4 * k + g, 8), dy = tl / 3, dx = tl - 3 * (tl / 3);
+      const bf16x8 a = feat[(ty + dy) * kRW + tx + dx];
+#pragma unroll
+      for (int j = 0; j < J; j++) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0[k][j], a, acc[j], 0, 0, 0);
+    }
+    bf16x4 hid[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+      f32x4 r;
+#pragma unroll
+      for (int i = 0; i < 4; i++) r[i] = fmaxf(acc[j][i] + b0[j][i], 0.0f);
+      hid[j] = __builtin_convertvector(r, bf16x4);
+    }
+    const f32x4 o = Op::mma(w1, Op::fromGroups(hid), f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+    if (g == 0) {
+      f32x4 r;
+#pragma unroll
+      for (int i = 0; i < 4; i++) r[i] = o[i] + b1[i];
+      rgb[ty * kTW + tx] = __builtin_convertvector(r, bf16x4);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < kTW * kTH; q += kThreads) {
+    const int ty = q / kTW, y = y0 + ty, x = x0 + (q - ty * kTW);
+    if (y >= h || x >= w) continue;
+    const bf16x4 o = rgb[q];
+    const float r = fmaxf(expm1Fast((float)o[0]), 0.0f);
+    const float gg = fmaxf(expm1Fast((float)o[1]), 0.0f);
+    const float b = fmaxf(expm1Fast((float)o[2]), 0.0f);
+    const int i = y * w + x;
+    Out[i] = make_float4(r, gg, b, 1.0f);
+    if constexpr (kResolve) arbor::resolveOne(arbor::mk(r, gg, b), History, Pixels, C, x, y, w, h, i);
+  }
+}
+
 // Output slot -> radiance (expm1 of the prediction, clamped at 0), alpha 1.
 __global__ void k_output(const bf16* __restrict__ Y, int H, int W, float4* __restrict__ Out, int w, int h) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
@@ -360,6 +483,13 @@ __global__ void k_toBf16(const float* __restrict__ a, long long n, bf16* __restr
     b[i] = (bf16)a[i];
 }
 
+// Zeroes every tap but the centre one (4) of a [9][cout][cin] gradient
+// (per = cin * cout elements per tap).
+__global__ void k_keepCentre(float* __restrict__ g, long long per) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < 9 * per; i += (long long)gridDim.x * blockDim.x)
+    if (i / per != 4) g[i] = 0.0f;
+}
+
 int grid1(long long n) {
   long long b = (n + 255) / 256;
   return (int)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
@@ -368,6 +498,7 @@ int grid1(long long n) {
 struct Conv {
   int cin, cout, level;
   bool relu;
+  bool k1 = false;  // 1x1: only the centre tap (4) of the stored [9][cout][cin] weights is used, the rest stay 0
   float *w = nullptr, *b = nullptr, *gw = nullptr, *gb = nullptr, *mw = nullptr, *vw = nullptr, *mb = nullptr,
         *vb = nullptr;
   uint16_t *w16 = nullptr, *wflip = nullptr;
@@ -402,6 +533,9 @@ struct arbor_denoiser {
   float* loss = nullptr;
   std::vector<void*> owned;
 
+  // levels == 0: the lean network (widths[0] channels)
+  bool lean() const { return levels == 0; }
+  int inC() const { return lean() ? kLeanInC : kInC; }
   int hl(int level) const { return H >> level; }
   int wl(int level) const { return W >> level; }
   long long count(const Act& t, int n) const { return (long long)n * hl(t.level) * wl(t.level) * t.c; }
@@ -438,6 +572,12 @@ void buildLayers(arbor_denoiser* d) {
     cv.relu = relu;
     d->conv.push_back(cv);
   };
+  if (d->lean()) {
+    add(kLeanInC, c[0], 0, true);
+    add(c[0], kOutC, 0, false);
+    d->conv.back().k1 = true;
+    return;
+  }
   for (int k = 0; k < L; k++) {
     add(k == 0 ? kInC : c[k - 1], c[k], k, true);
     add(c[k], c[k], k, true);
@@ -454,14 +594,16 @@ int initWeights(arbor_denoiser* d, int seed) {
   for (size_t li = 0; li < d->conv.size(); li++) {
     Conv& cv = d->conv[li];
     const bool last = li + 1 == d->conv.size();
-    const int fanIn = 9 * (li == 0 ? 10 : cv.cin);
+    const int used = d->lean() ? 7 : 10;  // input channels in use
+    const int fanIn = (cv.k1 ? 1 : 9) * (li == 0 ? used : cv.cin);
     std::normal_distribution<float> nd(0.0f, last ? sqrtf(1.0f / fanIn) : sqrtf(2.0f / fanIn));
     std::vector<float> w((size_t)cv.nw());
     for (int tap = 0; tap < 9; tap++)
       for (int co = 0; co < cv.cout; co++)
         for (int ci = 0; ci < cv.cin; ci++) {
           float v = nd(rng);
-          if ((li == 0 && ci >= 10) || (last && co >= 3)) v = 0.0f;  // unused input / output channels
+          // unused input / output channels, and the taps a 1x1 layer has not
+          if ((li == 0 && ci >= used) || (last && co >= 3) || (cv.k1 && tap != 4)) v = 0.0f;
           w[((size_t)tap * cv.cout + co) * cv.cin + ci] = v;
         }
     if (hipMemcpy(cv.w, w.data(), w.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return ARBOR_ERR_DEVICE;
@@ -486,6 +628,9 @@ int refreshBf16(arbor_denoiser* d, hipStream_t s) {
 
 int runConv(arbor_denoiser* d, int li, const Act& x, Act& y, int n, hipStream_t s) {
   const Conv& cv = d->conv[li];
+  if (cv.k1)
+    return arbor_dn_conv1x1(x.p, (long long)n * d->hl(cv.level) * d->wl(cv.level), cv.cin,
+                            cv.w16 + 4LL * cv.cout * cv.cin, cv.b, cv.relu ? 1 : 0, y.p, cv.cout, s);
   return arbor_dn_conv3x3(x.p, n, d->hl(cv.level), d->wl(cv.level), cv.cin, cv.w16, cv.b, cv.relu ? 1 : 0, y.p,
                           cv.cout, s);
 }
@@ -496,6 +641,10 @@ const Act& decoderLow(const arbor_denoiser* d, int k) { return k + 1 < d->levels
 int decoderLayer(const arbor_denoiser* d, int k) { return 2 * d->levels + 2 * (d->levels - 2 - k); }
 
 int forward(arbor_denoiser* d, int n, hipStream_t s) {
+  if (d->lean()) {
+    DN_TRY(runConv(d, 0, d->in, d->a[0], n, s));
+    return runConv(d, 1, d->a[0], d->out, n, s);
+  }
   const int L = d->levels;
   // encoder
   const Act* x = &d->in;
@@ -535,11 +684,21 @@ int backConv(arbor_denoiser* d, int li, const Act& x, const Act& y, uint16_t* gx
   const int h = d->hl(cv.level), w = d->wl(cv.level);
   if (cv.relu) DN_TRY(arbor_dn_relu_bwd(y.g, y.p, (long long)n * h * w * cv.cout, s));
   DN_TRY(arbor_dn_conv3x3_wgrad(x.p, y.g, n, h, w, cv.cin, cv.cout, d->partial, d->partialFloats, cv.gw, cv.gb, s));
+  if (cv.k1) {  // a 1x1 layer's gradient is the centre tap's; its other taps stay 0
+    hipLaunchKernelGGL(k_keepCentre, dim3(grid1(cv.nw())), dim3(256), 0, s, cv.gw, (long long)cv.cin * cv.cout);
+    DN_TRY(arbor::launch_status());
+  }
+  // (for a 1x1 layer the flipped weights are zero outside the centre tap, so
+  // this 3x3 convolution is the 1x1 layer's transpose)
   if (gx) DN_TRY(arbor_dn_conv3x3(y.g, n, h, w, cv.cout, cv.wflip, nullptr, 0, gx, cv.cin, s));
   return ARBOR_OK;
 }
 
 int backward(arbor_denoiser* d, int n, hipStream_t s) {
+  if (d->lean()) {
+    DN_TRY(backConv(d, 1, d->a[0], d->out, d->a[0].g, n, s));
+    return backConv(d, 0, d->in, d->a[0], nullptr, n, s);
+  }
   const int L = d->levels;
   // output layer and decoder
   Act& top = L > 1 ? d->d[0] : d->e4;
@@ -568,6 +727,20 @@ int backward(arbor_denoiser* d, int n, hipStream_t s) {
   return ARBOR_OK;
 }
 
+// Checkpoint kinds: "ARBORDN1" files hold a U-Net; "ARBORDN2" files name
+// their kind after the magic.
+constexpr int32_t kKindUNet = 0, kKindLean = 1;
+// Reads the magic, the kind and the layer count.
+bool readHeader(FILE* f, int32_t* kind, int32_t* layers) {
+  char magic[8];
+  if (fread(magic, 1, 8, f) != 8) return false;
+  if (memcmp(magic, "ARBORDN1", 8) == 0)
+    *kind = kKindUNet;
+  else if (memcmp(magic, "ARBORDN2", 8) != 0 || fread(kind, 4, 1, f) != 1 || *kind != kKindLean)
+    return false;
+  return fread(layers, 4, 1, f) == 1;
+}
+
 bool slotOk(const arbor_denoiser* d, int index, int w, int h) {
   return index >= 0 && index < d->maxBatch && w >= 1 && h >= 1 && w <= d->W && h <= d->H;
 }
@@ -581,8 +754,9 @@ int arbor_denoiser_create_net(arbor_denoiser** out, int max_batch, int height, i
   if (!out || !widths) return ARBOR_ERR_NULL;
   *out = nullptr;
   if (max_batch < 1 || max_batch > 1024 || height < 16 || width < 16 || height % 16 || width % 16 ||
-      height > ARBOR_MAX_DIM || width > ARBOR_MAX_DIM || levels < 1 || levels > kMaxLevels)
+      height > ARBOR_MAX_DIM || width > ARBOR_MAX_DIM || levels < 0 || levels > kMaxLevels)
     return ARBOR_ERR_ARG;
+  if (levels == 0 && widths[0] != 16 && widths[0] != 32) return ARBOR_ERR_ARG;
   for (int k = 0; k < levels; k++) {
     const int below = k + 1 < levels ? widths[k + 1] : 0;  // the decoder concatenates the level below
     if (widths[k] < 16 || widths[k] % 16 || widths[k] + below > ARBOR_DN_MAX_CHANNELS) return ARBOR_ERR_ARG;
@@ -593,7 +767,7 @@ int arbor_denoiser_create_net(arbor_denoiser** out, int max_batch, int height, i
   d->W = width;
   d->train = train != 0;
   d->levels = levels;
-  for (int k = 0; k < levels; k++) d->widths[k] = widths[k];
+  for (int k = 0; k < (levels ? levels : 1); k++) d->widths[k] = widths[k];
   buildLayers(d);
   int rc = ARBOR_OK;
   for (Conv& cv : d->conv) {
@@ -612,7 +786,8 @@ int arbor_denoiser_create_net(arbor_denoiser** out, int max_batch, int height, i
   }
   const int* c = d->widths;
   const int L = levels;
-  rc |= d->allocAct(d->in, kInC, 0);
+  rc |= d->allocAct(d->in, d->inC(), 0);
+  if (d->lean()) rc |= d->allocAct(d->a[0], c[0], 0);
   for (int k = 0; k < L; k++) rc |= d->allocAct(d->a[k], c[k], k);
   for (int k = 0; k < L - 1; k++) {
     rc |= d->allocAct(d->s[k], c[k], k);
@@ -629,7 +804,7 @@ int arbor_denoiser_create_net(arbor_denoiser** out, int max_batch, int height, i
     rc |= d->allocAct(d->dA[k], c[k], k);
     rc |= d->allocAct(d->d[k], c[k], k);
   }
-  rc |= d->allocAct(d->e4, c[L - 1], L - 1);
+  if (!d->lean()) rc |= d->allocAct(d->e4, c[L - 1], L - 1);
   rc |= d->allocAct(d->out, kOutC, 0);
   if (d->train) {
     size_t poolElems = 0;  // the largest skip a pooled gradient returns to
@@ -667,7 +842,7 @@ int arbor_denoiser_create(arbor_denoiser** out, int max_batch, int height, int w
 int arbor_denoiser_shape(const arbor_denoiser* d, int* levels, int* widths) {
   if (!d || !levels || !widths) return ARBOR_ERR_NULL;
   *levels = d->levels;
-  for (int k = 0; k < kMaxLevels; k++) widths[k] = k < d->levels ? d->widths[k] : 0;
+  for (int k = 0; k < kMaxLevels; k++) widths[k] = k < (d->levels ? d->levels : 1) ? d->widths[k] : 0;
   return ARBOR_OK;
 }
 
@@ -677,12 +852,20 @@ int arbor_denoiser_checkpoint_shape(const char* path, int* levels, int* widths) 
   if (!path || !levels || !widths) return ARBOR_ERR_NULL;
   FILE* f = fopen(path, "rb");
   if (!f) return ARBOR_ERR_ARG;
-  char magic[8];
-  int32_t layers = 0;
-  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "ARBORDN1", 8) == 0 && fread(&layers, 4, 1, f) == 1 &&
-            layers >= 3 && (layers + 1) % 4 == 0 && (layers + 1) / 4 <= kMaxLevels;
-  const int L = ok ? (layers + 1) / 4 : 0;
+  int32_t kind = 0, layers = 0;
+  bool ok = readHeader(f, &kind, &layers);
   int w[kMaxLevels] = {};
+  if (ok && kind == kKindLean) {  // levels 0: the lean network, width = layer 0's output channels
+    int32_t dims[2];
+    ok = layers == 2 && fread(dims, 4, 2, f) == 2 && dims[0] == kLeanInC && (dims[1] == 16 || dims[1] == 32);
+    fclose(f);
+    if (!ok) return ARBOR_ERR_ARG;
+    *levels = 0;
+    for (int k = 0; k < kMaxLevels; k++) widths[k] = k == 0 ? dims[1] : 0;
+    return ARBOR_OK;
+  }
+  ok = ok && kind == kKindUNet && layers >= 3 && (layers + 1) % 4 == 0 && (layers + 1) / 4 <= kMaxLevels;
+  const int L = ok ? (layers + 1) / 4 : 0;
   for (int li = 0; ok && li < 2 * L; li++) {
     int32_t dims[2];
     ok = fread(dims, 4, 2, f) == 2 && dims[0] > 0 && dims[1] > 0 &&
@@ -740,11 +923,11 @@ long long arbor_denoiser_parameters(const arbor_denoiser* d) {
 
 int arbor_denoiser_set_input(arbor_denoiser* d, int index, const float* Linear, const float* Guide,
                              const float* Normal, const float* Hit, int w, int h, hipStream_t stream) {
-  ARBOR_REQUIRE(d && Linear && Guide && Normal && Hit);
+  ARBOR_REQUIRE(d && Linear && Guide && (Normal || d->lean()) && Hit);
   if (!slotOk(d, index, w, h)) return ARBOR_ERR_ARG;
-  auto X = reinterpret_cast<bf16*>(d->in.p) + (size_t)index * d->H * d->W * kInC;
-  hipLaunchKernelGGL(k_features, dim3((d->W + 63) / 64, d->H), dim3(64), 0, stream,
-                     reinterpret_cast<const float4*>(Linear), reinterpret_cast<const float4*>(Guide),
+  auto X = reinterpret_cast<bf16*>(d->in.p) + (size_t)index * d->H * d->W * d->inC();
+  hipLaunchKernelGGL(d->lean() ? k_features<true> : k_features<false>, dim3((d->W + 63) / 64, d->H), dim3(64), 0,
+                     stream, reinterpret_cast<const float4*>(Linear), reinterpret_cast<const float4*>(Guide),
                      reinterpret_cast<const float4*>(Normal), reinterpret_cast<const float4*>(Hit), w, h, X, d->H,
                      d->W);
   return arbor::launch_status();
@@ -772,9 +955,29 @@ namespace {
 // Whether the network runs as the one-pass kernel (1 level of 16 channels).
 bool fusedShape(const arbor_denoiser* d) { return d->levels == 1 && d->widths[0] == fused::kC; }
 
+// The lean network's one-pass kernel over compact features (w x h, row
+// stride `stride`) into Out (and, with History and Pixels, the resolve).
+int leanPass(arbor_denoiser* d, const void* Feat, int stride, int w, int h, float* Out, float* History,
+             unsigned* Pixels, const float* C, hipStream_t stream) {
+  const Conv &c0 = d->conv[0], &c1 = d->conv[1];
+  const int tiles = ((w + lean::kTW - 1) / lean::kTW) * ((h + lean::kTH - 1) / lean::kTH);
+  const dim3 grid(fused::kXcds * ((tiles + fused::kXcds - 1) / fused::kXcds));
+  const bool resolve = History != nullptr;
+  auto kernel = d->widths[0] == 32 ? (resolve ? k_denoiseLean<2, true> : k_denoiseLean<2, false>)
+                                   : (resolve ? k_denoiseLean<1, true> : k_denoiseLean<1, false>);
+  hipLaunchKernelGGL(kernel, grid, dim3(lean::kThreads), 0, stream, reinterpret_cast<const arbor::featx8*>(Feat),
+                     stride, w, h, reinterpret_cast<const bf16*>(c0.w16), c0.b,
+                     reinterpret_cast<const bf16*>(c1.w16) + 4LL * c1.cout * c1.cin, c1.b,
+                     reinterpret_cast<float4*>(Out), C, reinterpret_cast<float4*>(History),
+                     reinterpret_cast<uint32_t*>(Pixels));
+  return arbor::launch_status();
+}
+
 // One frame through the network into Out (and, with History and Pixels, the
-// resolve): the one-pass kernel for the 1-level 16-channel shape, else the
-// input features, forward and output on slot 0 (then resolvePixels).
+// resolve): the one-pass kernel for the 1-level 16-channel shape; for the
+// lean network its compact features into slot 0, then its one-pass kernel;
+// else the input features, forward and output on slot 0 (then
+// resolvePixels).
 int denoiseFrame(arbor_denoiser* d, const float* Linear, const float* Guide, const float* Normal, const float* Hit,
                  int w, int h, float* Out, float* History, unsigned* Pixels, const float* C, hipStream_t stream) {
   if (!slotOk(d, 0, w, h)) return ARBOR_ERR_ARG;
@@ -793,6 +996,7 @@ int denoiseFrame(arbor_denoiser* d, const float* Linear, const float* Guide, con
     return arbor::launch_status();
   }
   DN_TRY(arbor_denoiser_set_input(d, 0, Linear, Guide, Normal, Hit, w, h, stream));
+  if (d->lean()) return leanPass(d, d->in.p, d->W, w, h, Out, History, Pixels, C, stream);
   DN_TRY(arbor_denoiser_forward(d, 1, stream));
   DN_TRY(arbor_denoiser_get_output(d, 0, Out, w, h, stream));
   if (resolve) return arbor_resolvePixels(Out, History, Pixels, C, w, h, stream);
@@ -807,14 +1011,30 @@ int denoiseResolve(arbor_denoiser* d, const float* Linear, const float* Guide, c
   if (!d || !Linear || !Guide || !Normal || !Hit || !Filtered || !History || !Pixels || !C) return ARBOR_ERR_NULL;
   return denoiseFrame(d, Linear, Guide, Normal, Hit, w, h, Filtered, History, Pixels, C, stream);
 }
+bool denoiserLean(const arbor_denoiser* d) { return d && d->lean(); }
+int denoiseFeaturesResolve(arbor_denoiser* d, const void* Feat, float* Filtered, float* History, unsigned* Pixels,
+                           const float* C, int w, int h, hipStream_t stream) {
+  if (!d || !Feat || !Filtered || !History || !Pixels || !C) return ARBOR_ERR_NULL;
+  if (!d->lean()) return ARBOR_ERR_STATE;
+  if (!slotOk(d, 0, w, h)) return ARBOR_ERR_ARG;
+  return leanPass(d, Feat, w, w, h, Filtered, History, Pixels, C, stream);
+}
 }  // namespace arbor
 
 extern "C" {
 
 int arbor_denoiser_denoise(arbor_denoiser* d, const float* Linear, const float* Guide, const float* Normal,
                            const float* Hit, int w, int h, float* Out, hipStream_t stream) {
-  ARBOR_REQUIRE(d && Linear && Guide && Normal && Hit && Out);
+  ARBOR_REQUIRE(d && Linear && Guide && (Normal || d->lean()) && Hit && Out);
   return denoiseFrame(d, Linear, Guide, Normal, Hit, w, h, Out, nullptr, nullptr, nullptr, stream);
+}
+
+int arbor_denoiser_denoise_features(arbor_denoiser* d, const void* Feat, int w, int h, float* Out,
+                                    hipStream_t stream) {
+  ARBOR_REQUIRE(d && Feat && Out);
+  if (!d->lean()) return ARBOR_ERR_STATE;
+  if (!slotOk(d, 0, w, h)) return ARBOR_ERR_ARG;
+  return leanPass(d, Feat, w, w, h, Out, nullptr, nullptr, nullptr, stream);
 }
 
 int arbor_denoiser_get_output(arbor_denoiser* d, int index, float* Out, int w, int h, hipStream_t stream) {
@@ -861,14 +1081,16 @@ int arbor_denoiser_loss(arbor_denoiser* d, float* host_loss) {
   return ARBOR_OK;
 }
 
-// File: "ARBORDN1", int32 layer count, per layer int32 cin, cout, then fp32
-// weights [9][cout][cin] and biases [cout].
+// File: "ARBORDN1" (a U-Net) or "ARBORDN2" and an int32 network kind (1:
+// the lean network), then the int32 layer count, per layer int32 cin, cout,
+// then fp32 weights [9][cout][cin] and biases [cout].
 int arbor_denoiser_save(const arbor_denoiser* d, const char* path) {
   ARBOR_REQUIRE(d && path);
   if (hipDeviceSynchronize() != hipSuccess) return ARBOR_ERR_DEVICE;
   FILE* f = fopen(path, "wb");
   if (!f) return ARBOR_ERR_ARG;
-  bool ok = fwrite("ARBORDN1", 1, 8, f) == 8;
+  bool ok = fwrite(d->lean() ? "ARBORDN2" : "ARBORDN1", 1, 8, f) == 8;
+  if (d->lean()) ok = ok && fwrite(&kKindLean, 4, 1, f) == 1;
   int32_t layers = (int32_t)d->conv.size();
   ok = ok && fwrite(&layers, 4, 1, f) == 1;
   for (const Conv& cv : d->conv) {
@@ -890,9 +1112,8 @@ int arbor_denoiser_load(arbor_denoiser* d, const char* path) {
   ARBOR_REQUIRE(d && path);
   FILE* f = fopen(path, "rb");
   if (!f) return ARBOR_ERR_ARG;
-  char magic[8];
-  int32_t layers = 0;
-  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "ARBORDN1", 8) == 0 && fread(&layers, 4, 1, f) == 1 &&
+  int32_t kind = 0, layers = 0;
+  bool ok = readHeader(f, &kind, &layers) && kind == (d->lean() ? kKindLean : kKindUNet) &&
             layers == (int32_t)d->conv.size();
   std::vector<std::vector<float>> ws, bs;
   for (size_t li = 0; ok && li < d->conv.size(); li++) {

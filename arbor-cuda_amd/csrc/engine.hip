@@ -25,7 +25,8 @@ struct Buffers {
   void* VisSchedule = nullptr;  // tile schedule of the sun-visibility pass
   float* Vis = nullptr;         // sun visibility per pixel
   void* Accel = nullptr;        // SAH trace hierarchy (arbor_buildTraceAccel)
-  float* Normal = nullptr;      // shading normal, the denoiser's input (only with a denoiser attached)
+  float* Normal = nullptr;      // shading normal, the denoiser's input, or the lean network's compact features
+                                // (only with a denoiser attached)
 };
 
 template <class T>
@@ -125,7 +126,10 @@ namespace {
 // With a denoiser attached and a frame that fits its slot, shading also
 // writes the normal and step 5 becomes the network's input features, its
 // forward pass and output into Filtered (slot 0), then resolvePixels: one
-// launch for a 1-level 16-channel network, so the frame stays five.
+// launch for a 1-level 16-channel network, so the frame stays five. With the
+// lean network, shading writes the compact features (in the Normal buffer)
+// in place of Linear and Guide, which that frame leaves as they were, and
+// step 5 is the network's one-pass kernel from them with the resolve.
 int frameLaunches(arbor_engine* e, const float* I, float delta, int sample, int light, bool timed) {
   const int w = e->width, h = e->height;
   const void* accel = e->accel ? e->b.Accel : nullptr;
@@ -142,8 +146,11 @@ int frameLaunches(arbor_engine* e, const float* I, float delta, int sample, int 
   if ((rc = arbor::visibilityOrdered(e->b.P, e->b.Nodes, e->b.Order, accel, e->b.C, e->b.Hit, e->b.Vis, w, h,
                               e->b.VisSchedule, e->stream)) != ARBOR_OK)
     return rc;
-  const bool net = useNetwork(e);
-  if (net)
+  const bool net = useNetwork(e), lean = net && arbor::denoiserLean(e->dn);
+  if (lean)  // the compact features, in the normal's buffer (both 16 B per pixel)
+    rc = arbor::shadeVisFeatures(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, e->b.Normal, w, h,
+                                 e->stream);
+  else if (net)
     rc = arbor::shadeVisNormal(e->b.P, e->b.Nodes, e->b.Order, e->b.C, e->b.Hit, e->b.Vis, e->b.Linear, e->b.Guide,
                                e->b.Normal, w, h, e->stream);
   else
@@ -151,7 +158,11 @@ int frameLaunches(arbor_engine* e, const float* I, float delta, int sample, int 
                               h, e->stream);
   if (rc != ARBOR_OK) return rc;
   if (timed) (void)hipEventRecord(e->ev[3], e->stream);
-  if (net) {
+  if (lean) {
+    if ((rc = arbor::denoiseFeaturesResolve(e->dn, e->b.Normal, e->b.Filtered, e->b.History, e->b.Pixels, e->b.C, w,
+                                            h, e->stream)) != ARBOR_OK)
+      return rc;
+  } else if (net) {
     if ((rc = arbor::denoiseResolve(e->dn, e->b.Linear, e->b.Guide, e->b.Normal, e->b.Hit, e->b.Filtered,
                                     e->b.History, e->b.Pixels, e->b.C, w, h, e->stream)) != ARBOR_OK)
       return rc;

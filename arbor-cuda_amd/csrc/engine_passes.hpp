@@ -38,6 +38,16 @@ int filterResolve(const float* Linear, const float* Guide, const float* Hit, flo
 int denoiseResolve(arbor_denoiser* d, const float* Linear, const float* Guide, const float* Normal, const float* Hit,
                    float* Filtered, float* History, unsigned* Pixels, const float* C, int w, int h,
                    hipStream_t stream);
+// Whether d is the lean network (compact features in, one pass).
+bool denoiserLean(const arbor_denoiser* d);
+// The lean network's one-pass kernel over compact features (w x h, as
+// arbor_shadePixelsFeatures writes them) into Filtered, with resolvePixels.
+int denoiseFeaturesResolve(arbor_denoiser* d, const void* Feat, float* Filtered, float* History, unsigned* Pixels,
+                           const float* C, int w, int h, hipStream_t stream);
+// arbor_shadePixelsVis writing the compact features (as
+// arbor_shadePixelsFeatures does) in place of Linear and Guide.
+int shadeVisFeatures(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                     const float* Vis, void* Feat, int width, int height, hipStream_t stream);
 // Compute units of the current device (cached per device).
 int deviceCUs();
 

@@ -18,6 +18,7 @@
 #include <mutex>
 
 #include "arbor_camera.hpp"
+#include "arbor_features.hpp"
 #include "arbor_filter.hpp"
 #include "arbor_persist.hpp"
 #include "arbor_shade.hpp"
@@ -364,11 +365,14 @@ constexpr int sahLdsPerWave(int k) { return k * 64 * 8; }
 // shadePixels (Arbor.cu:277-289).
 // kTraceVis = false: Vis is given whenever sun visibility is on (the shadow
 // traversals are compiled out, and with them the call's register cost).
+// With Feat, each pixel's compact denoiser features (compactFeatures of its
+// Linear, Guide and Hit distance) are written in place of Linear and Guide.
 template <bool kTraceVis>
 __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTraceVis ? 1 : 3))) void k_shadePixels(const float* P, const float4* Nodes4, const uint32_t* Order,
                                                             const float* C, const float4* Hit, float4* Linear,
                                                             float4* Guide, int width, int height, int tilesX,
-                                                            int tiles, float4* Normal, const float4* Vis) {
+                                                            int tiles, float4* Normal, const float4* Vis,
+                                                            featx8* Feat) {
   __shared__ TreeInst trees[kTrees];
   fillTrees(trees, (int)C[31]);
   __syncthreads();
@@ -380,7 +384,12 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(kTra
     int pixel = y * width + x;
     float4 l, g;
     v3 n;
-    shadeOne<WalkShade, kTraceVis>(P, Nodes4, Order, trees, sc, x, y, Hit[pixel], l, g, Normal ? &n : nullptr, Vis);
+    const float4 hit = Hit[pixel];
+    shadeOne<WalkShade, kTraceVis>(P, Nodes4, Order, trees, sc, x, y, hit, l, g, Normal ? &n : nullptr, Vis);
+    if (Feat) {
+      Feat[pixel] = compactFeatures(l, g, hit.x);
+      continue;
+    }
     Linear[pixel] = l;
     Guide[pixel] = g;
     if (Normal) Normal[pixel] = make_float4(n.x, n.y, n.z, 0.0f);
@@ -684,7 +693,7 @@ int arbor_shadePixels(const float* P, const float* Nodes, const unsigned* Order,
   hipLaunchKernelGGL(k_shadePixels<true>, dim3(shadeGrid(tiles, 2)), dim3(kTileBlock), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
-                     nullptr, nullptr);
+                     nullptr, nullptr, nullptr);
   return launch_status();
 }
 
@@ -696,7 +705,7 @@ int arbor_shadePixelsVis(const float* P, const float* Nodes, const unsigned* Ord
   hipLaunchKernelGGL(k_shadePixels<false>, dim3(shadeGrid(tiles, 3)), dim3(kTileBlock), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
-                     nullptr, reinterpret_cast<const float4*>(Vis));
+                     nullptr, reinterpret_cast<const float4*>(Vis), nullptr);
   return launch_status();
 }
 
@@ -891,7 +900,18 @@ int arbor_shadePixelsAOV(const float* P, const float* Nodes, const unsigned* Ord
   hipLaunchKernelGGL(k_shadePixels<true>, dim3(shadeGrid(tiles, 2)), dim3(kTileBlock), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
-                     reinterpret_cast<float4*>(Normal), nullptr);
+                     reinterpret_cast<float4*>(Normal), nullptr, nullptr);
+  return launch_status();
+}
+
+int arbor_shadePixelsFeatures(const float* P, const float* Nodes, const unsigned* Order, const float* C,
+                              const float* Hit, void* Feat, int width, int height, hipStream_t stream) {
+  ARBOR_REQUIRE(P && Nodes && Order && C && Hit && Feat);
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  hipLaunchKernelGGL(k_shadePixels<true>, dim3(shadeGrid(tiles, 2)), dim3(kTileBlock), 0, stream, P,
+                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
+                     nullptr, nullptr, width, height, tx, tiles, nullptr, nullptr, reinterpret_cast<featx8*>(Feat));
   return launch_status();
 }
 
@@ -1129,7 +1149,19 @@ int shadeVisNormal(const float* P, const float* Nodes, const unsigned* Order, co
   hipLaunchKernelGGL(k_shadePixels<false>, dim3(shadeGrid(tiles, 3)), dim3(kTileBlock), 0, stream, P,
                      reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit),
                      reinterpret_cast<float4*>(Linear), reinterpret_cast<float4*>(Guide), width, height, tx, tiles,
-                     reinterpret_cast<float4*>(Normal), reinterpret_cast<const float4*>(Vis));
+                     reinterpret_cast<float4*>(Normal), reinterpret_cast<const float4*>(Vis), nullptr);
+  return launch_status();
+}
+
+int shadeVisFeatures(const float* P, const float* Nodes, const unsigned* Order, const float* C, const float* Hit,
+                     const float* Vis, void* Feat, int width, int height, hipStream_t stream) {
+  if (!P || !Nodes || !Order || !C || !Hit || !Vis || !Feat) return ARBOR_ERR_NULL;
+  if (!dims_ok(width, height)) return ARBOR_ERR_ARG;
+  int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  hipLaunchKernelGGL(k_shadePixels<false>, dim3(shadeGrid(tiles, 3)), dim3(kTileBlock), 0, stream, P,
+                     reinterpret_cast<const float4*>(Nodes), Order, C, reinterpret_cast<const float4*>(Hit), nullptr,
+                     nullptr, width, height, tx, tiles, nullptr, reinterpret_cast<const float4*>(Vis),
+                     reinterpret_cast<featx8*>(Feat));
   return launch_status();
 }
 

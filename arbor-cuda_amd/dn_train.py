@@ -21,9 +21,11 @@ The times of filterLighting and of the network (features, forward, output) at
 1080p are measured with HIP events.
 
 Several networks can be trained on the same data and evaluated on the same
-held-out frames: --nets "5:32,48,64,96,128;2:16,32" (levels:widths, ';'
-between networks); the i-th network is saved as <out> with _i before the
-extension when there is more than one.
+held-out frames: --nets "5:32,48,64,96,128;2:16,32;0:16" (levels:widths, ';'
+between networks; levels 0 is the lean network, whose time is that of its
+one-pass kernel from the compact features the shading pass writes); the i-th
+network is saved as <out> with _i before the extension when there is more
+than one.
 
 Usage: python arbor-cuda_amd/dn_train.py --crops 4096 --size 512 --spp 64 --steps 6000
        --out denoiser.bin --report report.json
@@ -261,10 +263,15 @@ def main():
             r[k] = {"psnr": round(psnr(img, ref), 3), "ssim": round(ssim(img, ref), 4)}
         r["filter_ms"] = round(event_ms(torch, lambda: px.filter_lighting(Cd, fb)), 4)
         r["learned"] = []
+        feat = torch.zeros(W * H * 8, dtype=torch.bfloat16, device="cuda")
+        px.shade_features(scene, Cd, fb, feat)  # the lean network's input, as the shading pass writes it
         for ni, inf in enumerate(nets):
             out = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
 
             def run_net():
+                if inf.shape()[0] == 0:
+                    inf.denoise_features(feat, W, H, out)
+                    return
                 inf.set_input(0, fb.Linear, fb.Guide, normal, fb.Hit, W, H)
                 inf.forward(1)
                 inf.output(0, out, W, H)

@@ -39,6 +39,7 @@ SIGNATURES = {
     "arbor_tracePixels": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_dn_conv3x3": [_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp],
     "arbor_dn_wgrad_partial_floats": [_i, _i, _i, _i, _i],
+    "arbor_dn_conv1x1": [_vp, ct.c_longlong, _i, _vp, _vp, _i, _vp, _i, _vp],
     "arbor_dn_conv3x3_cat": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp],
     "arbor_dn_conv3x3_wgrad": [_vp, _vp, _i, _i, _i, _i, _i, _vp, ct.c_longlong, _vp, _vp, _vp],
     "arbor_dn_flip_weights": [_vp, _i, _i, _vp, _vp],
@@ -57,6 +58,8 @@ SIGNATURES = {
     "arbor_denoiser_checkpoint_shape": [ct.c_char_p, ct.POINTER(_i), ct.POINTER(_i)],
     "arbor_denoiser_slot": [_vp, ct.POINTER(_i), ct.POINTER(_i), ct.POINTER(_i)],
     "arbor_denoiser_denoise": [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
+    "arbor_denoiser_denoise_features": [_vp, _vp, _i, _i, _vp, _vp],
+    "arbor_shadePixelsFeatures": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "arbor_denoiser_destroy": [_vp],
     "arbor_denoiser_parameters": [_vp],
     "arbor_denoiser_set_input": [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -477,17 +480,24 @@ def shade_aov(scene, C, fb, normal):
          fb.width, fb.height)
 
 
+def shade_features(scene, C, fb, feat):
+    """shadePixels writing the lean denoiser's compact features (8 bf16 per
+    pixel) into feat (a device tensor of at least 16 bytes per pixel)."""
+    call("arbor_shadePixelsFeatures", scene.P, scene.Nodes, scene.Order, C, fb.Hit, feat, fb.width, fb.height)
+
+
 def checkpoint_shape(path):
     """(levels, widths) of the network a denoiser checkpoint was saved from."""
     lv = ct.c_int()
     w = (ct.c_int * 5)()
     check(lib().arbor_denoiser_checkpoint_shape(str(path).encode(), ct.byref(lv), w), "arbor_denoiser_checkpoint_shape")
-    return lv.value, list(w)[:lv.value]
+    return lv.value, list(w)[:max(lv.value, 1)]
 
 
 class Denoiser:
     """The learned denoiser's network runner (include/arbor_denoise.h). levels
-    and widths choose the U-Net (default: 5 levels of 32/48/64/96/128)."""
+    and widths choose the U-Net (default: 5 levels of 32/48/64/96/128), or
+    with levels 0 the lean network of widths[0] (16 or 32) channels."""
 
     def __init__(self, max_batch, height, width, train=True, seed=1, levels=None, widths=None):
         h = ct.c_void_p()
@@ -511,7 +521,7 @@ class Denoiser:
         lv = ct.c_int()
         w = (ct.c_int * 5)()
         check(lib().arbor_denoiser_shape(self.h, ct.byref(lv), w), "arbor_denoiser_shape")
-        return lv.value, list(w)[:lv.value]
+        return lv.value, list(w)[:max(lv.value, 1)]
 
     def slot(self):
         """(max_batch, height, width) of the runner's slots."""
@@ -548,6 +558,10 @@ class Denoiser:
     def denoise(self, linear, guide, normal, hit, w, h, out):
         """One frame: features, forward and output (slot 0) into out."""
         call("arbor_denoiser_denoise", self.h, linear, guide, normal, hit, w, h, out)
+
+    def denoise_features(self, feat, w, h, out):
+        """The lean network's one pass from compact features into out."""
+        call("arbor_denoiser_denoise_features", self.h, feat, w, h, out)
 
     def train_step(self, n, lr):
         call("arbor_denoiser_train_step", self.h, n, ct.c_float(lr))

@@ -78,22 +78,25 @@ def parse():
                     help="sun-visibility workload instead: 3840x2160, 4 jittered samples per frame, tree-to-tree "
                          "sun visibility on at 35 deg elevation, breeze on, static initial camera (use --steps 10)")
     ap.add_argument("--profile-denoiser", type=int, default=0, metavar="N",
-                    help="profiling driver: N learned-denoiser passes (features, forward, output) at 1920x1080 "
-                         "on fixed random inputs with the network of --denoiser (default: the 5-level "
-                         "checkpoint); prints no JSON")
+                    help="profiling driver: N learned-denoiser passes (features, forward, output; for the lean "
+                         "network its one pass from compact features) at 1920x1080 on fixed random inputs with "
+                         "the network of --denoiser (default: the 5-level checkpoint); prints no JSON")
     ap.add_argument("--denoiser", nargs="?", const=",".join(DENOISER_NETS), default="", metavar="CHECKPOINTS",
                     help="learned-denoiser workload instead: held-out 1920x1080 frames (sun visibility on), "
                          "1-sample inputs against 64-sample targets; filterLighting vs each network of the "
-                         "comma-separated checkpoints (default: the 1-level equal-PSNR network, then the "
+                         "comma-separated checkpoints (default: the lean network, the 1-level U-Net, then the "
                          "5-level one); the first one's ms is the value")
     return ap.parse_args()
 
 
 WEIGHTS = os.path.join(HERE, "arbor-cuda_amd", "weights")
 DENOISER_WEIGHTS = os.path.join(WEIGHTS, "denoiser_forest.bin")
-# The smallest network at or above filterLighting's PSNR on the held-out frames.
+# The smallest U-Net at or above filterLighting's PSNR on the held-out frames.
 DENOISER_SMALL = os.path.join(WEIGHTS, "denoiser_forest_small.bin")
-DENOISER_NETS = (DENOISER_SMALL, DENOISER_WEIGHTS)
+# The lean network (one 3x3 convolution and a 1x1 output, compact features
+# written by the shading pass): at or above filterLighting's PSNR, one pass.
+DENOISER_LEAN = os.path.join(WEIGHTS, "denoiser_forest_lean.bin")
+DENOISER_NETS = (DENOISER_LEAN, DENOISER_SMALL, DENOISER_WEIGHTS)
 # BF16 dense MFMA peak of MI355X (TFLOP/s).
 BF16_PEAK_TFLOPS = 2500.0
 
@@ -117,6 +120,13 @@ def denoiser_flops(height, width, widths=(32, 48, 64, 96, 128)):
     """Algorithmic FLOPs of one forward pass at height x width (multiples of
     16): 2 * 9 * cin * cout per output pixel of every convolution."""
     return sum(2.0 * 9 * ci * co * (height >> lv) * (width >> lv) for ci, co, lv, _ in denoiser_layers(widths))
+
+
+def lean_flops(h, w, width):
+    """Algorithmic FLOPs of the lean network on an h x w image: the 3x3
+    convolution of the 7 used features into `width` channels and the 1x1
+    layer into the 3 used outputs."""
+    return 2.0 * (9 * 7 * width + width * 3) * h * w
 
 
 def denoiser_bytes(height, width, h, w, widths):
@@ -204,10 +214,14 @@ def profile_denoiser(args):
     g = torch.Generator(device="cuda").manual_seed(0)
     bufs = [torch.rand(W * H * 4, device="cuda", generator=g) for _ in range(4)]
     out = torch.zeros(W * H * 4, device="cuda")
-    for _ in range(args.profile_denoiser):
-        inf.denoise(bufs[0], bufs[1], bufs[2], bufs[3], W, H, out)
-    torch.cuda.synchronize()
     levels, widths = inf.shape()
+    feat = torch.rand(W * H * 8, device="cuda", generator=g).to(torch.bfloat16)  # the lean network's input
+    for _ in range(args.profile_denoiser):
+        if levels == 0:
+            inf.denoise_features(feat, W, H, out)
+        else:
+            inf.denoise(bufs[0], bufs[1], bufs[2], bufs[3], W, H, out)
+    torch.cuda.synchronize()
     inf.close()
     print("profile driver: %d passes of the %d-level %s denoiser (%s) at %dx%d"
           % (args.profile_denoiser, levels, widths, os.path.basename(path), W, H))
@@ -248,10 +262,23 @@ def bench_denoiser(args):
             img = dn_train.display(px, torch, buf, Cd, W, H)
             row[k] = {"psnr": round(dn_train.psnr(img, ref), 3), "ssim": round(dn_train.ssim(img, ref), 4)}
         row["filter_ms"] = round(dn_train.event_ms(torch, lambda: px.filter_lighting(Cd, fb)), 4)
+        # the lean network's input: the compact features the shading pass
+        # writes in place of Linear and Guide (its time beside shadePixels')
+        feat = torch.zeros(W * H * 8, dtype=torch.bfloat16, device="cuda")
+        px.shade_features(scene, Cd, fb, feat)
+        lin_keep, gui_keep = fb.Linear.clone(), fb.Guide.clone()
+        row["shade_ms"] = round(dn_train.event_ms(torch, lambda: px.shade(scene, Cd, fb), reps=5), 4)
+        row["shade_features_ms"] = round(dn_train.event_ms(torch, lambda: px.shade_features(scene, Cd, fb, feat),
+                                                           reps=5), 4)
+        fb.Linear.copy_(lin_keep)
+        fb.Guide.copy_(gui_keep)
         row["networks"] = []
         for path, inf in zip(paths, nets):
             def net():
-                inf.denoise(fb.Linear, fb.Guide, normal, fb.Hit, W, H, out)
+                if inf.shape()[0] == 0:
+                    inf.denoise_features(feat, W, H, out)
+                else:
+                    inf.denoise(fb.Linear, fb.Guide, normal, fb.Hit, W, H, out)
 
             def layered():
                 inf.set_input(0, fb.Linear, fb.Guide, normal, fb.Hit, W, H)
@@ -274,10 +301,12 @@ def bench_denoiser(args):
         den = float(np.mean([r["networks"][i]["denoiser_ms"] for r in rows]))
         lay = float(np.mean([r["networks"][i]["layered_ms"] for r in rows]))
         fwd = float(np.mean([r["networks"][i]["forward_ms"] for r in rows]))
-        one_pass = levels == 1 and widths == [16]
-        flops = denoiser_flops(1088, 1920, widths)
-        # the one-pass kernel reads the four frame buffers and writes Filtered
-        nbytes = 80.0 * W * H if one_pass else denoiser_bytes(1088, 1920, H, W, widths)
+        lean = levels == 0
+        one_pass = lean or (levels == 1 and widths == [16])
+        flops = lean_flops(H, W, widths[0]) if lean else denoiser_flops(1088, 1920, widths)
+        # one pass: the lean kernel reads the 16-byte compact features and
+        # writes Filtered; the 1-level kernel reads the four frame buffers
+        nbytes = (32.0 if lean else 80.0) * W * H if one_pass else denoiser_bytes(1088, 1920, H, W, widths)
         tflops = flops / ((den if one_pass else fwd) * 1e-3) / 1e12
         gbs = nbytes / (den * 1e-3) / 1e9
         beats = all(r["networks"][i]["psnr"] >= r["filterLighting"]["psnr"] for r in rows)
@@ -297,9 +326,12 @@ def bench_denoiser(args):
         inf.close()
     first = summary[0]
     if first["bound"] == "hbm":
-        roof = {"bound": "hbm", "kernel": "denoiser pass (features, %d convolutions, output%s), %d-level %s"
-                % (len(denoiser_layers(first["widths"])), ": one kernel" if first["one_pass"] else "",
-                   first["levels"], first["widths"]),
+        kernel = ("lean denoiser pass (compact features, 3x3 and 1x1 convolutions, output: one kernel), %d channels"
+                  % first["widths"][0] if first["levels"] == 0 else
+                  "denoiser pass (features, %d convolutions, output%s), %d-level %s"
+                  % (len(denoiser_layers(first["widths"])), ": one kernel" if first["one_pass"] else "",
+                     first["levels"], first["widths"]))
+        roof = {"bound": "hbm", "kernel": kernel,
                 "achieved": first["pass_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": first["hbm_frac"],
                 "algorithmic_bytes": first["algorithmic_bytes_per_frame"], "traffic": None}
     else:
@@ -314,11 +346,13 @@ def bench_denoiser(args):
         "n_gpus": 1, "steps": len(rows), "warmup": 1, "ms_per_step": first["denoiser_ms"],
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
         "data": "synthetic (tracer-rendered held-out frames of the seed-7 forest, sun visibility on)",
-        "config": {"workload": "learned denoisers (BF16 MFMA U-Nets) vs filterLighting on 1-sample 1920x1080 "
+        "config": {"workload": "learned denoisers (BF16 MFMA networks) vs filterLighting on 1-sample 1920x1080 "
                                "frames, against 64-sample targets",
                    "checkpoints": [os.path.basename(p) for p in paths]},
         "filter_ms": round(flt, 4),
         "learned_vs_filter_time": first["vs_filter_time"],
+        "shade_ms": round(float(np.mean([r["shade_ms"] for r in rows])), 4),
+        "shade_features_ms": round(float(np.mean([r["shade_features_ms"] for r in rows])), 4),
         "networks": summary,
         "quality": rows,
         "roofline": roof,

@@ -29,6 +29,10 @@ extern "C" {
  * relu != 0; bias may be NULL. */
 int arbor_dn_conv3x3(const uint16_t* X, int n, int h, int w, int cin, const uint16_t* Wt, const float* bias, int relu,
                      uint16_t* Y, int cout, hipStream_t stream);
+/* Y = act(X W1^T + bias): the 1x1 convolution of `pixels` pixels of cin = 16
+ * or 32 channels with W1 [cout][cin] bf16 (cout a multiple of 16). */
+int arbor_dn_conv1x1(const uint16_t* X, long long pixels, int cin, const uint16_t* W1, const float* bias, int relu,
+                     uint16_t* Y, int cout, hipStream_t stream);
 /* The same convolution of the channel concatenation [up2(Xc), X]: Xc is
  * n x (h/2) x (w/2) x cc (nearest 2x upsampling), X is n x h x w x cs; Wt has
  * cin = cc + cs input channels. Equals conv3x3 of the materialised tensor. */
@@ -60,7 +64,18 @@ int arbor_dn_relu_bwd(uint16_t* G, const uint16_t* Yact, long long count, hipStr
 int arbor_dn_add(uint16_t* A, const uint16_t* B, long long count, hipStream_t stream);
 
 /* ---- the network ----
- * An L-level U-Net (L = 1..5), 3x3 convolutions (ReLU), 2x2 average pooling,
+ * Either the lean network (levels = 0) or a U-Net (levels 1..5).
+ *
+ * The lean network: one 3x3 convolution of widths[0] = 16 or 32 channels
+ * with ReLU, then a linear 1x1 output layer. Its input is the compact
+ * feature set, 8 bf16 per pixel (16 bytes): log(1 + radiance) rgb of one
+ * sample, albedo rgb (Guide), depth log(1 + min(t, 1000))/7 (1 for sky) and
+ * a zero channel. arbor_shadePixelsFeatures writes it directly, and the
+ * network runs as one kernel from it (arbor_denoiser_denoise_features).
+ * Its 1x1 layer is stored as a 3x3 layer whose taps other than the centre
+ * are zero.
+ *
+ * A U-Net: L levels, 3x3 convolutions (ReLU), 2x2 average pooling,
  * nearest upsampling and skip concatenation, a linear 3x3 output layer. The
  * default is 5 levels of 32/48/64/96/128 channels: 1,021,904 parameters stored
  * (1,016,419 excluding the zero padding of the input and output channels).
@@ -74,19 +89,22 @@ typedef struct arbor_denoiser arbor_denoiser;
  * He-initialised from seed. */
 int arbor_denoiser_create(arbor_denoiser** out, int max_batch, int height, int width, int train, int seed);
 /* The same with `levels` levels of widths[0..levels-1] channels (multiples of
- * 16; a width plus the next one at most ARBOR_DN_MAX_CHANNELS). */
+ * 16; a width plus the next one at most ARBOR_DN_MAX_CHANNELS); levels = 0:
+ * the lean network of widths[0] (16 or 32) channels. */
 int arbor_denoiser_create_net(arbor_denoiser** out, int max_batch, int height, int width, int train, int seed,
                               int levels, const int* widths);
 /* The network's levels and widths (widths: 5 ints, unused levels 0). */
 int arbor_denoiser_shape(const arbor_denoiser* d, int* levels, int* widths);
-/* The levels and widths a checkpoint file was saved from (widths: 5 ints). */
+/* The levels and widths a checkpoint file was saved from (widths: 5 ints;
+ * levels 0 and widths[0]: the lean network). */
 int arbor_denoiser_checkpoint_shape(const char* path, int* levels, int* widths);
 /* The runner's slot count and slot size (height x width). */
 int arbor_denoiser_slot(const arbor_denoiser* d, int* max_batch, int* height, int* width);
 int arbor_denoiser_destroy(arbor_denoiser* d);
 long long arbor_denoiser_parameters(const arbor_denoiser* d);
 /* Input features of slot `index` from one frame's buffers (float4 per pixel,
- * w x h): Linear, Guide, Normal, Hit. */
+ * w x h): Linear, Guide, Normal, Hit (the lean network does not read Normal,
+ * which may then be NULL). */
 int arbor_denoiser_set_input(arbor_denoiser* d, int index, const float* Linear, const float* Guide,
                              const float* Normal, const float* Hit, int w, int h, hipStream_t stream);
 /* Training target of slot `index`: radiance float4 per pixel (w x h). */
@@ -103,16 +121,30 @@ int arbor_denoiser_get_output(arbor_denoiser* d, int index, float* Out, int w, i
  * three calls. */
 int arbor_denoiser_denoise(arbor_denoiser* d, const float* Linear, const float* Guide, const float* Normal,
                            const float* Hit, int w, int h, float* Out, hipStream_t stream);
+/* shadePixels (arbor_shadePixels, Arbor.cu:277-289) writing the compact
+ * features of each pixel (8 bf16, 16 bytes per pixel, w x h) in place of
+ * Linear and Guide: the bits arbor_denoiser_set_input computes for the lean
+ * network from arbor_shadePixels' Linear and Guide and the Hit. */
+int arbor_shadePixelsFeatures(const float* P, const float* Nodes, const unsigned* Order, const float* C,
+                              const float* Hit, void* Feat, int width, int height, hipStream_t stream);
+/* The lean network on one frame's compact features (w x h, within the slot)
+ * into Out (float4, alpha 1; a drop-in for Filtered), in one kernel: the
+ * bits of arbor_denoiser_denoise on the frame the features came from.
+ * ARBOR_ERR_STATE for a U-Net. */
+int arbor_denoiser_denoise_features(arbor_denoiser* d, const void* Feat, int w, int h, float* Out,
+                                    hipStream_t stream);
 /* Forward, L1 loss on log(1 + radiance) against the targets, backward, one
  * Adam step (beta 0.9 / 0.999) over slots 0..n-1. */
 int arbor_denoiser_train_step(arbor_denoiser* d, int n, float lr, hipStream_t stream);
 /* Mean L1 loss of the last train step (synchronises). */
 int arbor_denoiser_loss(arbor_denoiser* d, float* host_loss);
-/* Weights as "ARBORDN1", layer count, then per layer cin, cout (int32) and
- * fp32 weights [9][cout][cin] and biases [cout]. */
+/* Weights as "ARBORDN1" (a U-Net) or "ARBORDN2" and the int32 kind 1 (the
+ * lean network), layer count, then per layer cin, cout (int32) and fp32
+ * weights [9][cout][cin] and biases [cout]. */
 int arbor_denoiser_save(const arbor_denoiser* d, const char* path);
 int arbor_denoiser_load(arbor_denoiser* d, const char* path);
-/* Device pointer of "input" (bf16 NHWC, 16 channels), "output" (bf16 NHWC,
+/* Device pointer of "input" (bf16 NHWC, 16 channels; 8 for the lean
+ * network), "output" (bf16 NHWC,
  * 16 channels, 3 used), "target" (float4: log(1 + radiance), mask), or of
  * layer i's fp32 "weight:i" / "bias:i" and, after a train step, their
  * gradients "wgrad:i" / "bgrad:i". */

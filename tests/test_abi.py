@@ -242,7 +242,7 @@ def test_committed_denoiser_checkpoints_and_costs(L):
     """The committed checkpoints' shapes as read from their files (host code
     only), the null checks of the slot query and the engine's denoiser
     option, and the bench's FLOP count of the 5-level network (379.0 GFLOP
-    at 1088x1920) and of the 1-level one."""
+    at 1088x1920), of the 1-level one and of the lean one."""
     import sys
 
     sys.path.insert(0, REPO)
@@ -251,11 +251,14 @@ def test_committed_denoiser_checkpoints_and_costs(L):
     weights = os.path.join(REPO, "arbor-cuda_amd", "weights")
     assert pyarbor.checkpoint_shape(os.path.join(weights, "denoiser_forest.bin")) == (5, [32, 48, 64, 96, 128])
     assert pyarbor.checkpoint_shape(os.path.join(weights, "denoiser_forest_small.bin")) == (1, [16])
+    assert pyarbor.checkpoint_shape(os.path.join(weights, "denoiser_forest_lean.bin")) == (0, [16])
     n = ct.c_int()
     assert L.arbor_denoiser_slot(NULL, ct.byref(n), ct.byref(n), ct.byref(n)) == -1
     assert L.arbor_engine_set_denoiser(NULL, NULL) == -1
     assert round(bench.denoiser_flops(1088, 1920) / 1e9, 4) == 379.0209
     # 1 level: 10 -> 16, 16 -> 16, 16 -> 3 used channels, 18 FLOP per tap pair
     assert bench.denoiser_flops(1088, 1920, [16]) == 18.0 * (10 * 16 + 16 * 16 + 16 * 3) * 1088 * 1920
+    # lean: 7 used features -> 16 (3x3), 16 -> 3 used outputs (1x1)
+    assert bench.lean_flops(1080, 1920, 16) == 2.0 * (9 * 7 * 16 + 16 * 3) * 1080 * 1920
     assert [l[:3] for l in bench.denoiser_layers([16, 32])] == [(10, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1),
                                                                  (48, 16, 0), (16, 16, 0), (16, 3, 0)]

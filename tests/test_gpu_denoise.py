@@ -4,7 +4,9 @@ same bf16-rounded operands:
 
 * conv3x3 (MFMA implicit GEMM) within bf16 output rounding of the fp32
   convolution (|err| <= 2^-8 |ref| + 2^-12 max|ref|), over ragged sizes and
-  every supported output width;
+  every supported output width, 8 input channels (four taps per
+  instruction) included;
+* conv1x1 within the same bound, 16 and 32 input channels;
 * weight / bias gradients and the flipped-weight input gradient against
   autograd (fp32 sums of exact bf16 products: rtol 1e-4 of the largest);
 * pooling, upsampling, concat/split, ReLU gradient, add: exact.
@@ -41,7 +43,7 @@ def _nhwc(torch, x):
 @pytest.mark.parametrize("n,h,w,cin,cout,relu", [
     (2, 10, 12, 16, 16, 1), (1, 7, 9, 48, 32, 0), (2, 16, 16, 32, 48, 1), (1, 5, 33, 224, 96, 1),
     (1, 8, 8, 160, 64, 0), (1, 6, 10, 80, 128, 1), (1, 3, 3, 8, 80, 0), (2, 9, 11, 112, 112, 1),
-    (1, 4, 6, 96, 224, 1)])
+    (1, 4, 6, 96, 224, 1), (2, 17, 21, 8, 16, 1), (1, 9, 40, 8, 32, 1)])
 def test_conv3x3_forward(env, n, h, w, cin, cout, relu):
     torch, px = env
     print("case: conv3x3 n%d %dx%d %d->%d relu%d" % (n, h, w, cin, cout, relu), flush=True)
@@ -58,6 +60,28 @@ def test_conv3x3_forward(env, n, h, w, cin, cout, relu):
     Y = torch.zeros(n, h, w, cout, dtype=torch.bfloat16, device="cuda")
     px.call("arbor_dn_conv3x3", X, n, h, w, cin, W, B, relu, Y, cout)
     got = Y.float().cpu().permute(0, 3, 1, 2)
+    err = (got - ref).abs()
+    bound = ref.abs() * 2.0 ** -8 + ref.abs().max() * 2.0 ** -12
+    print("max err %.3g (bound at max %.3g)" % (err.max(), bound.max()))
+    assert (err <= bound).all()
+
+
+@pytest.mark.parametrize("pixels,cin,cout,relu", [(1000, 16, 16, 0), (77, 32, 16, 1), (4099, 16, 48, 1),
+                                                 (513, 32, 32, 0)])
+def test_conv1x1_forward(env, pixels, cin, cout, relu):
+    torch, px = env
+    print("case: conv1x1 %d pixels %d->%d relu%d" % (pixels, cin, cout, relu), flush=True)
+    g = torch.Generator().manual_seed(pixels + cin)
+    x = _bf(torch, torch.randn(pixels, cin, generator=g)).float()
+    wt = _bf(torch, torch.randn(cout, cin, generator=g) * (1.0 / cin ** 0.5)).float()
+    b = torch.randn(cout, generator=g) * 0.1
+    ref = x @ wt.t() + b
+    if relu:
+        ref = torch.relu(ref)
+    Y = torch.zeros(pixels, cout, dtype=torch.bfloat16, device="cuda")
+    px.call("arbor_dn_conv1x1", x.to(torch.bfloat16).cuda(), pixels, cin, wt.to(torch.bfloat16).cuda(), b.cuda(),
+            relu, Y, cout)
+    got = Y.float().cpu()
     err = (got - ref).abs()
     bound = ref.abs() * 2.0 ** -8 + ref.abs().max() * 2.0 ** -12
     print("max err %.3g (bound at max %.3g)" % (err.max(), bound.max()))
