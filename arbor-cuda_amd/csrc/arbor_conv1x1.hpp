@@ -23,6 +23,28 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+// Four values rounded to bf16 (round to nearest even), converted in pairs
+// (v_cvt_pk_bf16_f32) and joined as two 32-bit words, so no element
+// shuffles follow.
+__device__ __forceinline__ bf16x4 toBf16(f32x4 v) {
+  const bf16x2 lo = __builtin_convertvector(f32x2{v[0], v[1]}, bf16x2);
+  const bf16x2 hi = __builtin_convertvector(f32x2{v[2], v[3]}, bf16x2);
+  return __builtin_bit_cast(bf16x4, u32x2{__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi)});
+}
+// ReLU of four finite values, then rounded to bf16: the sign bit decides (an
+// integer maximum of the bit pattern with 0, so -0 becomes +0), one
+// instruction per value.
+__device__ __forceinline__ bf16x4 reluToBf16(f32x4 v) {
+  f32x4 r;
+#pragma unroll
+  for (int i = 0; i < 4; i++) r[i] = __int_as_float(max(__float_as_int(v[i]), 0));
+  return toBf16(r);
+}
+
 template <int J>
 struct Op1x1;
 

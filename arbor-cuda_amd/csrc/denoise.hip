@@ -72,7 +72,9 @@ constexpr int convWaves(int nsub) { return nsub == 2 ? 5 : (nsub == 3 ? 4 : (nsu
 //    instructions per output subtile instead of nine half-empty ones.
 //  * 4 (8 input channels): K slots 8g..8g+7 hold tap 4k+g's 8 channels; the
 //    third instruction holds tap 8 and zero weights. Three instructions
-//    instead of nine quarter-full ones.
+//    instead of nine quarter-full ones. The bias is the accumulator's
+//    initial value (no addition after the products), as the lean network's
+//    one-pass kernel computes it.
 template <int NSUB, int MSUB, int kPack = 1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(convWaves(NSUB)))) void k_conv3x3(
     const __bf16* __restrict__ X, const __bf16* __restrict__ Xc, int Cc, int N, int H, int W, int Cin,
@@ -96,9 +98,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(convWaves(N
   // four consecutive channels of one pixel, stored as one 8-byte write.
   f32x4 acc[MSUB][NSUB];
 #pragma unroll
-  for (int m = 0; m < MSUB; m++)
+  for (int j = 0; j < NSUB; j++) {
+    f32x4 init = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (kPack == 4 && bias) {
 #pragma unroll
-    for (int j = 0; j < NSUB; j++) acc[m][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int i = 0; i < 4; i++) init[i] = bias[co0 + 16 * j + 4 * g + i];
+    }
+#pragma unroll
+    for (int m = 0; m < MSUB; m++) acc[m][j] = init;
+  }
   for (int c0 = 0; c0 < Cin; c0 += kChunk) {
     // Stage (kHaloH x 18 pixels) x 32 channels as 16-byte groups. Every load
     // is issued before the first store, from an address clamped into the
@@ -246,7 +254,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(convWaves(N
 #pragma unroll
   for (int j = 0; j < NSUB; j++)
 #pragma unroll
-    for (int i = 0; i < 4; i++) bv[j][i] = bias ? bias[co0 + 16 * j + 4 * g + i] : 0.0f;
+    for (int i = 0; i < 4; i++) bv[j][i] = bias && kPack != 4 ? bias[co0 + 16 * j + 4 * g + i] : 0.0f;
 #pragma unroll
   for (int m = 0; m < MSUB; m++) {
     const int y = y0 + MSUB * wave + m;
@@ -254,10 +262,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(convWaves(N
     const size_t p = (size_t)(n * H + y) * W + x;
 #pragma unroll
     for (int j = 0; j < NSUB; j++) {
+      if constexpr (kPack == 4) {  // the lean network's input layer: the one-pass kernel's ReLU and rounding
+        const f32x4 a = acc[m][j];
+        *reinterpret_cast<bf16x4*>(Y + p * CoutTotal + co0 + 16 * j + 4 * g) =
+            relu ? dn1::reluToBf16(a) : dn1::toBf16(a);
+        continue;
+      }
       bf16x4 o;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        float v = acc[m][j][i] + bv[j][i];
+        float v = kPack == 4 ? acc[m][j][i] : acc[m][j][i] + bv[j][i];
         if (relu) v = fmaxf(v, 0.0f);
         o[i] = f2bf(v);
       }
@@ -270,7 +284,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(convWaves(N
 // Y[p, co] = act(sum_ci X[p, ci] * W1[co, ci] + bias[co]) over P pixels of
 // 16*J channels, W1 [Cout][16*J]: each wave takes 16-pixel groups, one
 // matrix instruction per 16 output channels (operands as arbor_conv1x1.hpp
-// lays them out).
+// lays them out) whose accumulator starts at the bias.
 template <int J>
 __global__ __launch_bounds__(256) void k_conv1x1(const __bf16* __restrict__ X, long long P,
                                                  const __bf16* __restrict__ W1, const float* __restrict__ bias,
@@ -284,15 +298,13 @@ __global__ __launch_bounds__(256) void k_conv1x1(const __bf16* __restrict__ X, l
     const bool ok = p < P;
     const typename Op::T b = Op::load(X + (ok ? p : P - 1) * Cin, g);
     for (int co0 = 0; co0 < Cout; co0 += 16) {
-      const f32x4 acc = Op::mma(Op::load(W1 + (size_t)(co0 + row) * Cin, g), b, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-      bf16x4 o;
+      f32x4 init = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      if (bias) {
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        float v = acc[i] + (bias ? bias[co0 + 4 * g + i] : 0.0f);
-        if (relu) v = fmaxf(v, 0.0f);
-        o[i] = f2bf(v);
+        for (int i = 0; i < 4; i++) init[i] = bias[co0 + 4 * g + i];
       }
-      if (ok) *reinterpret_cast<bf16x4*>(Y + p * Cout + co0 + 4 * g) = o;
+      const f32x4 acc = Op::mma(Op::load(W1 + (size_t)(co0 + row) * Cin, g), b, init);
+      if (ok) *reinterpret_cast<bf16x4*>(Y + p * Cout + co0 + 4 * g) = relu ? dn1::reluToBf16(acc) : dn1::toBf16(acc);
     }
   }
 }

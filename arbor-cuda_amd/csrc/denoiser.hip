@@ -345,9 +345,10 @@ __global__ __launch_bounds__(fused::kThreads) void k_denoise1(
 // v_mfma_f32_16x16x32_bf16 per 16 output channels (four taps per
 // instruction: K slots 8g..8g+7 hold tap 4k+g's 8 channels; the third holds
 // tap 8 and zero weights), as arbor_dn_conv3x3 issues them for 8 input
-// channels; bias, ReLU and the rounding to bf16 leave the activations where
-// the 1x1 output layer takes its B operand (arbor_conv1x1.hpp), so that
-// layer is one more instruction on registers. The output, rounded to bf16
+// channels, the bias the accumulator's initial value; ReLU and the rounding
+// to bf16 leave the activations where the 1x1 output layer takes its B
+// operand (arbor_conv1x1.hpp), so that layer is one more instruction on
+// registers. The output, rounded to bf16
 // as the layered network's is, goes through LDS so that every thread then
 // finishes two pixels (expm1, Out, and with kResolve resolvePixels). Feat
 // holds the features of a w x h frame with row stride `stride` (zero outside
@@ -382,7 +383,7 @@ __global__ __launch_bounds__(lean::kThreads) void k_denoiseLean(
   for (int s = 0; s < kIter; s++) {
     const int p = threadIdx.x + s * kThreads, fy = p / kRW, y = y0 - 1 + fy, x = x0 - 1 + (p - fy * kRW);
     in[s] = p < kRPix && y >= 0 && y < h && x >= 0 && x < w;
-    v[s] = Feat[in[s] ? (size_t)y * stride + x : 0];
+    v[s] = Feat[in[s] ? y * stride + x : 0];  // frames stay below 2^31 pixels
   }
   // weight fragments: instruction k, output group j (lane row = channel
   // 16j + row, K slots 8g.. = tap 4k+g), the 1x1 layer's, and the biases
@@ -415,35 +416,37 @@ __global__ __launch_bounds__(lean::kThreads) void k_denoiseLean(
     }
   }
   __syncthreads();
-  for (int s = 0; s < kGroupsPerWave; s++) {
-    const int grp = wave * kGroupsPerWave + s, ty = grp >> 1, tx = (grp & 1) * 16 + row;
-    f32x4 acc[J];
+  // Group s of the wave covers tile row 4 * wave + s / 2, columns
+  // 16 * (s % 2) + row: each lane's three tap positions are fixed offsets
+  // from its first group's, so the loop's LDS addresses differ by constants.
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
+  int tap0[3];
 #pragma unroll
-    for (int j = 0; j < J; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k = 0; k < 3; k++) {
+    const int tl = min(4 * k + g, 8), dy = tl / 3, dx = tl - 3 * (tl / 3);
+    tap0[k] = (4 * wv + dy) * kRW + row + dx;
+  }
+  const int out0 = 4 * wv * kTW + row;
+#pragma unroll
+  for (int s = 0; s < kGroupsPerWave; s++) {
+    const int step = (s >> 1) * kRW + (s & 1) * 16;
+    f32x4 acc[J];  // the biases are the accumulators' initial values
+#pragma unroll
+    for (int j = 0; j < J; j++) acc[j] = f32x4{b0[j][0], b0[j][1], b0[j][2], b0[j][3]};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      const int tl = min(4 * k + g, 8), dy = tl / 3, dx = tl - 3 * (tl / 3);
-      const bf16x8 a = feat[(ty + dy) * kRW + tx + dx];
+      const bf16x8 a = feat[tap0[k] + step];
 #pragma unroll
       for (int j = 0; j < J; j++) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0[k][j], a, acc[j], 0, 0, 0);
     }
     bf16x4 hid[J];
 #pragma unroll
-    for (int j = 0; j < J; j++) {
-      f32x4 r;
-#pragma unroll
-      for (int i = 0; i < 4; i++) r[i] = fmaxf(acc[j][i] + b0[j][i], 0.0f);
-      hid[j] = __builtin_convertvector(r, bf16x4);
-    }
-    const f32x4 o = Op::mma(w1, Op::fromGroups(hid), f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-    if (g == 0) {
-      f32x4 r;
-#pragma unroll
-      for (int i = 0; i < 4; i++) r[i] = o[i] + b1[i];
-      rgb[ty * kTW + tx] = __builtin_convertvector(r, bf16x4);
-    }
+    for (int j = 0; j < J; j++) hid[j] = arbor::dn1::reluToBf16(acc[j]);
+    const f32x4 o = Op::mma(w1, Op::fromGroups(hid), f32x4{b1[0], b1[1], b1[2], b1[3]});
+    if (g == 0) rgb[out0 + (s >> 1) * kTW + (s & 1) * 16] = arbor::dn1::toBf16(o);
   }
   __syncthreads();
+#pragma unroll
   for (int q = threadIdx.x; q < kTW * kTH; q += kThreads) {
     const int ty = q / kTW, y = y0 + ty, x = x0 + (q - ty * kTW);
     if (y >= h || x >= w) continue;
