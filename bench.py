@@ -263,13 +263,18 @@ def bench_denoiser(args):
             row[k] = {"psnr": round(dn_train.psnr(img, ref), 3), "ssim": round(dn_train.ssim(img, ref), 4)}
         row["filter_ms"] = round(dn_train.event_ms(torch, lambda: px.filter_lighting(Cd, fb)), 4)
         # the lean network's input: the compact features the shading pass
-        # writes in place of Linear and Guide (its time beside shadePixels')
+        # writes in place of Linear and Guide; both shading forms are timed
+        # with sun visibility off (no shadow rays), so that the difference is
+        # what they write
         feat = torch.zeros(W * H * 8, dtype=torch.bfloat16, device="cuda")
         px.shade_features(scene, Cd, fb, feat)
         lin_keep, gui_keep = fb.Linear.clone(), fb.Guide.clone()
-        row["shade_ms"] = round(dn_train.event_ms(torch, lambda: px.shade(scene, Cd, fb), reps=5), 4)
-        row["shade_features_ms"] = round(dn_train.event_ms(torch, lambda: px.shade_features(scene, Cd, fb, feat),
-                                                           reps=5), 4)
+        dark = Cd.clone()
+        dark[22] = 0.0
+        scratch = torch.zeros_like(feat)
+        row["shade_ms"] = round(dn_train.event_ms(torch, lambda: px.shade(scene, dark, fb), reps=10), 4)
+        row["shade_features_ms"] = round(dn_train.event_ms(torch, lambda: px.shade_features(scene, dark, fb, scratch),
+                                                           reps=10), 4)
         fb.Linear.copy_(lin_keep)
         fb.Guide.copy_(gui_keep)
         row["networks"] = []
@@ -351,8 +356,8 @@ def bench_denoiser(args):
                    "checkpoints": [os.path.basename(p) for p in paths]},
         "filter_ms": round(flt, 4),
         "learned_vs_filter_time": first["vs_filter_time"],
-        "shade_ms": round(float(np.mean([r["shade_ms"] for r in rows])), 4),
-        "shade_features_ms": round(float(np.mean([r["shade_features_ms"] for r in rows])), 4),
+        "shading_sun_off_ms": {"Linear_Guide": round(float(np.mean([r["shade_ms"] for r in rows])), 4),
+                               "compact_features": round(float(np.mean([r["shade_features_ms"] for r in rows])), 4)},
         "networks": summary,
         "quality": rows,
         "roofline": roof,
