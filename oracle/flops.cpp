@@ -131,26 +131,33 @@ int orc_trace_ops_per_pixel(const float* P, const float* Nodes, const unsigned* 
 // >= the caller's best (the GPU's woodT/leafT), over whole frames (trace, and
 // shade with sun visibility on): the number of pixels whose Hit or Linear
 // differs in any bit (0 expected).
+// Three runs: the full tests; with the exits; with the exits and the
+// bounding-sphere rejections (whose entry-at-or-beyond-best exits need both).
 int orc_skip_mismatches(const float* P, const float* Nodes, const unsigned* Order, const float* C, int width,
                         int height) {
   using A = Arbor<float>;
   size_t n = (size_t)width * height * 4;
-  std::vector<float> h[2] = {std::vector<float>(n), std::vector<float>(n)};
-  std::vector<float> l[2] = {std::vector<float>(n), std::vector<float>(n)}, g(n);
-  for (int skip = 0; skip < 2; skip++) {
-    A::skipBeyondBest = skip != 0;
+  std::vector<float> h[3] = {std::vector<float>(n), std::vector<float>(n), std::vector<float>(n)};
+  std::vector<float> l[3] = {std::vector<float>(n), std::vector<float>(n), std::vector<float>(n)}, g(n);
+  for (int run = 0; run < 3; run++) {
+    A::skipBeyondBest = run > 0;
+    A::cullSpheres = run > 1;
 #pragma omp parallel for schedule(dynamic, 2)
     for (int y = 0; y < height; y++)
-      for (int x = 0; x < width; x++) A::tracePixel(P, Nodes, Order, C, h[skip].data(), width, height, x, y);
+      for (int x = 0; x < width; x++) A::tracePixel(P, Nodes, Order, C, h[run].data(), width, height, x, y);
 #pragma omp parallel for schedule(dynamic, 2)
     for (int y = 0; y < height; y++)
       for (int x = 0; x < width; x++)
-        A::shadePixel(P, Nodes, Order, C, h[skip].data(), l[skip].data(), g.data(), width, height, x, y);
+        A::shadePixel(P, Nodes, Order, C, h[run].data(), l[run].data(), g.data(), width, height, x, y);
   }
-  A::skipBeyondBest = false;
+  A::skipBeyondBest = A::cullSpheres = false;
   int bad = 0;
   for (size_t i = 0; i < n; i += 4)
-    if (std::memcmp(&h[0][i], &h[1][i], 16) != 0 || std::memcmp(&l[0][i], &l[1][i], 16) != 0) bad++;
+    for (int run = 1; run < 3; run++)
+      if (std::memcmp(&h[0][i], &h[run][i], 16) != 0 || std::memcmp(&l[0][i], &l[run][i], 16) != 0) {
+        bad++;
+        break;
+      }
   return bad;
 }
 
@@ -328,11 +335,14 @@ int orc_cull_mismatches(const float* P, const float* Nodes, const unsigned* Orde
 // in any bit (0 expected), out[3] = the nearest distance (m) of a ray whose
 // hit a rejection removed (0 if none).
 void orc_set_cull_pad_scale(float s) { Arbor<float>::cullPadScale = s; }
-int orc_cull_stress(const float* P, int raysPerObject, int seed, float time, float wind, long long* out) {
+namespace {
+struct StressRay { V3<float> ro, rd; float fp, dist; int id; };
+// The grazing rays of the stress tests below.
+std::vector<StressRay> stressRays(const float* P, int raysPerObject, int seed, float time, float wind) {
   using A = Arbor<float>;
+  using Ray = StressRay;
   const int objects = WOOD_COUNT + LEAF_COUNT;
   const size_t n = (size_t)objects * raysPerObject;
-  struct Ray { V3<float> ro, rd; float fp, dist; int id; };
   std::vector<Ray> rays(n);
 #pragma omp parallel for schedule(static)
   for (int id = 0; id < objects; id++) {
@@ -379,12 +389,19 @@ int orc_cull_stress(const float* P, int raysPerObject, int seed, float time, flo
       ray.id = id;
     }
   }
+  return rays;
+}
+}  // namespace
+int orc_cull_stress(const float* P, int raysPerObject, int seed, float time, float wind, long long* out) {
+  using A = Arbor<float>;
+  const std::vector<StressRay> rays = stressRays(P, raysPerObject, seed, time, wind);
+  const size_t n = rays.size();
   std::vector<float> res[2] = {std::vector<float>(2 * n), std::vector<float>(2 * n)};
   for (int on = 0; on < 2; on++) {
     A::cullSpheres = on != 0;
 #pragma omp parallel for schedule(dynamic, 256)
     for (long long i = 0; i < (long long)n; i++) {
-      const Ray& ray = rays[i];
+      const StressRay& ray = rays[i];
       res[on][2 * i] = A::objectT(P, ray.id, ray.ro, ray.rd, ray.fp, time, wind, true);
       res[on][2 * i + 1] = A::objectT(P, ray.id, ray.ro, ray.rd, ray.fp, time, wind, false);
     }
@@ -403,6 +420,75 @@ int orc_cull_stress(const float* P, int raysPerObject, int seed, float time, flo
   out[1] = hits;
   out[2] = bad;
   out[3] = (long long)worst;
+  return 0;
+}
+
+// The early exits past the caller's best under stress: the grazing rays of
+// orc_cull_stress, each object test (refined and not) run in full, then with
+// the rejections and the exits against five bests: one ulp above the full
+// result (the SAH walk's tie form), 1e-4 relative below it, and 0.6, 1.0
+// and 1.4 times the ray's distance to its aim point. Contract: a full result
+// below the best comes back bit for bit, any other result stays at or above
+// the best. out[0] = evaluations, out[1] = those with the full result below
+// the best, out[2] = violations (0 expected), out[3] = evaluations whose
+// answer an exit or rejection changed (results at or above the best), out[4]
+// = the nearest distance (m) of a violation (0 if none).
+int orc_exit_stress(const float* P, int raysPerObject, int seed, float time, float wind, long long* out) {
+  using A = Arbor<float>;
+  const std::vector<StressRay> rays = stressRays(P, raysPerObject, seed, time, wind);
+  const long long n = (long long)rays.size();
+  constexpr int kBests = 5;
+  std::vector<float> full(2 * n), bests(2 * n * kBests), got(2 * n * kBests);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long long i = 0; i < n; i++) {
+    const StressRay& ray = rays[i];
+    for (int refine = 0; refine < 2; refine++) {
+      const float f = A::objectT(P, ray.id, ray.ro, ray.rd, ray.fp, time, wind, refine != 0);
+      full[2 * i + refine] = f;
+      float* b = &bests[(2 * i + refine) * kBests];
+      b[0] = f < A::FAR() ? std::nextafter(f, 2.0f * A::FAR()) : A::FAR();
+      b[1] = f < A::FAR() ? f * (1.0f - 1e-4f) : A::FAR();
+      b[2] = 0.6f * ray.dist;
+      b[3] = 1.0f * ray.dist;
+      b[4] = 1.4f * ray.dist;
+    }
+  }
+  A::cullSpheres = A::skipBeyondBest = true;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long long i = 0; i < n; i++) {
+    const StressRay& ray = rays[i];
+    for (int refine = 0; refine < 2; refine++)
+      for (int k = 0; k < kBests; k++) {
+        const size_t e = (2 * i + refine) * kBests + k;
+        got[e] = A::objectT(P, ray.id, ray.ro, ray.rd, ray.fp, time, wind, refine != 0, bests[e]);
+      }
+  }
+  A::cullSpheres = A::skipBeyondBest = false;
+  long long kept = 0, bad = 0, changed = 0;
+  float worst = 0.0f;
+  for (long long i = 0; i < n; i++)
+    for (int refine = 0; refine < 2; refine++)
+      for (int k = 0; k < kBests; k++) {
+        const size_t e = (2 * i + refine) * kBests + k;
+        const float f = full[2 * i + refine], b = bests[e], g = got[e];
+        bool wrong;
+        if (f < b) {
+          kept++;
+          wrong = std::memcmp(&g, &f, sizeof(float)) != 0;
+        } else {
+          wrong = g < b;
+          if (!wrong && std::memcmp(&g, &f, sizeof(float)) != 0) changed++;
+        }
+        if (wrong) {
+          bad++;
+          worst = worst == 0.0f ? rays[i].dist : std::fmin(worst, rays[i].dist);
+        }
+      }
+  out[0] = 2 * n * kBests;
+  out[1] = kept;
+  out[2] = bad;
+  out[3] = changed;
+  out[4] = (long long)worst;
   return 0;
 }
 

@@ -200,6 +200,8 @@ class Oracle(_Common):
         self._cull_stress = _bind(L, "orc_cull_stress", _i,
                                   [_f32p, _i, _i, _f, _f, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")])
         self._cull_pad_scale = _bind(L, "orc_set_cull_pad_scale", None, [_f])
+        self._exit_stress = _bind(L, "orc_exit_stress", _i,
+                                  [_f32p, _i, _i, _f, _f, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")])
 
     def cull_stress(self, P, rays_per_object, seed, time, wind):
         """Grazing-ray stress of the rejection spheres (flops.cpp orc_cull_stress):
@@ -207,6 +209,15 @@ class Oracle(_Common):
         removed hit)."""
         out = np.zeros(4, np.int64)
         assert self._cull_stress(P, rays_per_object, seed, time, wind, out) == 0
+        return tuple(int(v) for v in out)
+
+    def exit_stress(self, P, rays_per_object, seed, time, wind):
+        """The early exits past the caller's best under the grazing rays, at
+        five bests per test (flops.cpp orc_exit_stress): (evaluations, those
+        whose full result is below the best, violations, answers an exit
+        changed at or above the best, nearest distance of a violation)."""
+        out = np.zeros(5, np.int64)
+        assert self._exit_stress(P, rays_per_object, seed, time, wind, out) == 0
         return tuple(int(v) for v in out)
 
     def set_cull_pad_scale(self, scale):

@@ -458,12 +458,35 @@ def test_sphere_culls_exact_on_grazing_rays(oracle, scenes, time, wind):
     assert bad_tight > 20
 
 
+@pytest.mark.parametrize("time,wind", [(0.0, 0.0), (1.3, 0.6)])
+def test_early_exits_exact_on_grazing_rays(oracle, scenes, time, wind):
+    """The early exits past the caller's best, with the rejection spheres on,
+    under the grazing rays of the rejection stress test: each wood and leaf
+    test (refined and not) at five bests (one ulp above the full result, as
+    the SAH walk's tie form passes it; 1e-4 below it; 0.6, 1.0 and 1.4 times
+    the ray's distance). A full result below the best must come back bit for
+    bit, any other stay at or above the best. With the pads cut to a tenth (a
+    planted mistake in the oracle) violations appear."""
+    P = scenes["7"].P
+    evals, kept, bad, changed, _ = oracle.exit_stress(P, 8, 1, time, wind)
+    print("evaluations %d, full result below the best %d, violations %d, answers an exit changed %d"
+          % (evals, kept, bad, changed))
+    assert evals == 2 * 5 * 8 * (5124 + 24192) and kept > 100000 and changed > 10000 and bad == 0
+    try:
+        oracle.set_cull_pad_scale(0.1)
+        _, _, bad_tight, _, nearest = oracle.exit_stress(P, 8, 1, time, wind)
+    finally:
+        oracle.set_cull_pad_scale(1.0)
+    print("pads x0.1: %d violations (nearest at %d m)" % (bad_tight, nearest))
+    assert bad_tight > 20
+
+
 @pytest.mark.parametrize("frame,view", [(0, 0), (40, 0), (0, 2), (0, 4)])
 def test_early_exits_change_nothing(oracle, scenes, frame, view):
     """woodT/leafT stopping once their answer is known to be >= the caller's
     best (march from a tube hit at or behind best; Newton slab entered at or
     behind best) leave every Hit and Linear bit unchanged: trace, and shade
-    with sun visibility on."""
+    with sun visibility on; with the rejection spheres off and on."""
     if view:
         C = oracle.init_camera(7)
         I = np.zeros(32, np.float32)
