@@ -10,7 +10,19 @@
 
 namespace arbor {
 
-AD float aces(float x) { return sat((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f)); }
+// The display mapping of resolvePixels (Arbor.cu:304) on the vector ALU's
+// transcendental units: the ACES fit's quotient through the hardware
+// reciprocal (1 ulp), the 1/2.2 power as exp2(log2(x) / 2.2) (v_log_f32,
+// v_exp_f32: under 1e-6 relative for x in (0, 1]), and the vignette's
+// coordinates from reciprocals of the view size. Each value is within a few
+// ulp of the correctly rounded operations, far inside the 8-bit output step
+// (the reference's glibc powf and divisions are matched within one step);
+// the accumulation into History is exact. IEEE powf and divisions cost
+// about 575 vector instructions per wave of pixels, this mapping about 100.
+AD float aces(float x) {
+  return sat(x * (2.51f * x + 0.03f) * __builtin_amdgcn_rcpf(x * (2.43f * x + 0.59f) + 0.14f));
+}
+AD float gamma22(float x) { return x > 0.0f ? __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 2.2f)) : 0.0f; }
 
 // One pixel of resolvePixels (Arbor.cu:304) given its Filtered colour.
 __device__ __forceinline__ void resolveOne(v3 current, float4* History, uint32_t* Pixels, const float* C, int x,
@@ -24,9 +36,9 @@ __device__ __forceinline__ void resolveOne(v3 current, float4* History, uint32_t
   History[i] = make_float4(mean.x, mean.y, mean.z, count + 1.0f);
   v3 col = mean * C[20];
   col = mk(aces(col.x), aces(col.y), aces(col.z));
-  const float g = 1.0f / 2.2f;
-  col = mk(powf(col.x, g), powf(col.y, g), powf(col.z, g));
-  float u = ((float)x / (float)width - 0.5f) * 2.0f, v = ((float)y / (float)height - 0.5f) * 2.0f;
+  col = mk(gamma22(col.x), gamma22(col.y), gamma22(col.z));
+  float u = ((float)x * __builtin_amdgcn_rcpf((float)width) - 0.5f) * 2.0f;
+  float v = ((float)y * __builtin_amdgcn_rcpf((float)height) - 0.5f) * 2.0f;
   col = col * (1.0f - 0.09f * (u * u + v * v));
   Pixels[i] = packRGB(col);
 }
