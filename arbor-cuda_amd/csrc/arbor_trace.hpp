@@ -396,10 +396,13 @@ AD int heapDepth(uint32_t node) { return 31 - __builtin_clz(node); }
 //              tests of the waiting lanes then run together. Each lane
 //              performs the same steps in the same order as with g = 0; only
 //              their interleaving within the wave changes.
-template <bool kCull, bool kPrefetch, bool kLdsTop, bool kSah = false, int kLdsStack = 0, int kGather = 0>
+//   kSteps:    with kGather, the interior steps a stepping lane takes per
+//              round of the gathered loop (each round ends in two ballots).
+template <bool kCull, bool kPrefetch, bool kLdsTop, bool kSah = false, int kLdsStack = 0, int kGather = 0,
+          int kSteps = 1>
 struct Walk {
   static constexpr bool cull = kCull, prefetch = kPrefetch, ldsTop = kLdsTop, sah = kSah;
-  static constexpr int ldsStack = kLdsStack, gather = kGather;
+  static constexpr int ldsStack = kLdsStack, gather = kGather, steps = kSteps;
 };
 // Dynamic LDS of a kernel launch (the SAH walk's stack entries, kLdsStack).
 extern __shared__ uint64_t sahLdsStack[];
@@ -641,7 +644,13 @@ __device__ __forceinline__ v2 traceTreeSah(const float* P, const float4* Nodes4,
         if (stepping == 0) break;
         const uint64_t waiting = __ballot(live && ref < 0);
         if (8 * __popcll(waiting) >= W::gather * __popcll(waiting | stepping)) break;
-        if (step && !stepNode()) live = popNext();
+        if (step) {
+#pragma unroll
+          for (int k = 0; k < W::steps; k++) {
+            if (k > 0 && !(live && ref >= 0)) break;
+            if (!stepNode()) live = popNext();
+          }
+        }
       }
       if (live && ref < 0) {
         testLeaf();

@@ -298,6 +298,9 @@ const BatchVariant kBatchVariants[] = {
      "split as 6 (SAH walk: 1 wave/block, record trees, 20-entry LDS stack)"},
     {k_traceViews<1, Walk<true, false, false>>, true, 1,
      "split as 5 (SAH walk: 2 waves/block, record trees, 16-entry LDS stack)"},
+    {k_traceViews<2, Walk<true, false, false>>, true, 2,
+     "split as 6 (SAH walk: 1 wave/block, record trees, 20-entry LDS stack, three interior steps per gathered "
+     "round)"},
 };
 constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 int batchVariantCount() { return kBatchVariantCount; }
@@ -310,10 +313,11 @@ using WalkSahB = Walk<true, false, false, true, 0, 6>;
 using WalkSahBVis = Walk<true, false, false, true, 0, 5>;
 // The record-tree split passes: the trace with 20 (1-wave blocks) or 16
 // (2-wave blocks) LDS stack entries per lane, the visibility pass with 14
-// (1-wave blocks, 5 waves per SIMD).
+// (1-wave blocks, 5 waves per SIMD, three interior steps per gathered round).
 using WalkSahBRec1 = Walk<true, false, false, true, 20, 6>;
+using WalkSahBRec3 = Walk<true, false, false, true, 20, 5, 3>;  // three interior steps per gathered round
 using WalkSahBRec2 = Walk<true, false, false, true, 16, 6>;
-using WalkSahBVisRec = Walk<true, false, false, true, 14, 5>;
+using WalkSahBVisRec = Walk<true, false, false, true, 14, 5, 3>;
 constexpr int kBatchLds(int k) { return k * 64 * 8; }
 const BatchVariant kBatchVariantsSah[] = {
     {k_traceShadeViews<1, WalkSahB>, false, 1, ""}, {k_traceShadeViews<2, WalkSahB>, false, 2, ""},
@@ -322,6 +326,7 @@ const BatchVariant kBatchVariantsSah[] = {
     {k_traceViews<2, WalkSahB>, true, 2, ""},
     {k_traceViewsRec<1, WalkSahBRec1>, true, 1, "", kBatchLds(20), true},
     {k_traceViewsRec<2, WalkSahBRec2>, true, 2, "", kBatchLds(16), true},
+    {k_traceViewsRec<1, WalkSahBRec3>, true, 1, "", kBatchLds(20), true},
 };
 static_assert(sizeof(kBatchVariantsSah) == sizeof(kBatchVariants), "one SAH entry per batch variant");
 

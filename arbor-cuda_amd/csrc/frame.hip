@@ -360,6 +360,8 @@ template <int K>
 using WalkSahL = Walk<true, false, false, true, K>;  // stack entries [0, K) in LDS
 template <int K, int G>
 using WalkSahG = Walk<true, false, false, true, K, G>;  // the same, object tests gathered per wave
+template <int K, int G, int S>
+using WalkSahGS = Walk<true, false, false, true, K, G, S>;  // the same, S interior steps per gathered round
 constexpr int sahLdsPerWave(int k) { return k * 64 * 8; }
 
 // shadePixels (Arbor.cu:277-289).
@@ -503,7 +505,8 @@ int launchTrace(bool forest, const float* P, const float* Nodes, const unsigned*
 // depth <= 21, keeps every stack entry out of scratch (PMC WRITE_SIZE about
 // Hit's bytes). 19 and 20 (4 waves per SIMD, registers spilled to scratch)
 // are as fast at 1080p and 12% faster at 4K (7.5-7.7 ms against 8.8), at
-// four times the write traffic.
+// four times the write traffic. 24 (the default) is 16 with three interior
+// steps per gathered round and gathering at 5/8: 3.05 ms against 3.23.
 const TraceVariant kAccelTraceVariants[] = {
     {k_tracePixels<1, WalkSah>, k_traceTreePixels<1, WalkSah>, 1},
     {k_tracePixels<2, WalkSah>, k_traceTreePixels<2, WalkSah>, 2},
@@ -539,6 +542,20 @@ const TraceVariant kAccelTraceVariants[] = {
      nullptr, true},
     {k_tracePixels<1, WalkSahG<24, 6>, 1, true>, k_traceTreePixels<1, WalkSahG<24, 6>>, 1, sahLdsPerWave(24),
      nullptr, true},
+    // 22-27: 16 with several interior steps per gathered round (steps S,
+    // gathering at G/8): S2 G6, S3 G6, S3 G5, S3 G4, S4 G5, S2 G5
+    {k_tracePixels<1, WalkSahGS<20, 6, 2>, 1, true>, k_traceTreePixels<1, WalkSahGS<20, 6, 2>>, 1,
+     sahLdsPerWave(20), nullptr, true},
+    {k_tracePixels<1, WalkSahGS<20, 6, 3>, 1, true>, k_traceTreePixels<1, WalkSahGS<20, 6, 3>>, 1,
+     sahLdsPerWave(20), nullptr, true},
+    {k_tracePixels<1, WalkSahGS<20, 5, 3>, 1, true>, k_traceTreePixels<1, WalkSahGS<20, 5, 3>>, 1,
+     sahLdsPerWave(20), nullptr, true},
+    {k_tracePixels<1, WalkSahGS<20, 4, 3>, 1, true>, k_traceTreePixels<1, WalkSahGS<20, 4, 3>>, 1,
+     sahLdsPerWave(20), nullptr, true},
+    {k_tracePixels<1, WalkSahGS<20, 5, 4>, 1, true>, k_traceTreePixels<1, WalkSahGS<20, 5, 4>>, 1,
+     sahLdsPerWave(20), nullptr, true},
+    {k_tracePixels<1, WalkSahGS<20, 5, 2>, 1, true>, k_traceTreePixels<1, WalkSahGS<20, 5, 2>>, 1,
+     sahLdsPerWave(20), nullptr, true},
 };
 // the shape an unscheduled call runs in place of a record-tree one
 constexpr int kAccelUnscheduled = 8;
@@ -746,7 +763,9 @@ int launchVisibility(const float* P, const float* Nodes, const unsigned* Order, 
 // shadows against 8.8 ms for 0, the same kernel without it. 15 (the
 // default) reads the tree records from the schedule and keeps 14 stack
 // entries per lane in LDS, one-wave blocks at 5 waves per SIMD: 118-120 ms
-// for a 4-sample 4K frame with sun visibility against 131 ms with 8.
+// for a 4-sample 4K frame with sun visibility against 131 ms with 8. 23
+// (the default) is 15 with three interior steps per gathered round: 114.2
+// against 118.0 ms.
 const VisVariant kAccelVisVariants[] = {
     {k_sunVisibility<2, WalkSah, 5>, 2},
     {k_sunVisibility<1, WalkSah>, 1},
@@ -775,6 +794,12 @@ const VisVariant kAccelVisVariants[] = {
     {k_sunVisibility<1, WalkSahG<12, 5>, 6, true>, 1, sahLdsPerWave(12), true},
     {k_sunVisibility<1, WalkSahG<14, 6>, 5, true>, 1, sahLdsPerWave(14), true},
     {k_sunVisibility<1, WalkSahG<20, 5>, 4, true>, 1, sahLdsPerWave(20), true},
+    // 22-25: 15 with several interior steps per gathered round: S2 G5,
+    // S3 G5, S3 G4, S3 G6
+    {k_sunVisibility<1, WalkSahGS<14, 5, 2>, 5, true>, 1, sahLdsPerWave(14), true},
+    {k_sunVisibility<1, WalkSahGS<14, 5, 3>, 5, true>, 1, sahLdsPerWave(14), true},
+    {k_sunVisibility<1, WalkSahGS<14, 4, 3>, 5, true>, 1, sahLdsPerWave(14), true},
+    {k_sunVisibility<1, WalkSahGS<14, 6, 3>, 5, true>, 1, sahLdsPerWave(14), true},
 };
 // the shape an unscheduled call runs in place of a record-tree one
 constexpr int kAccelVisUnscheduled = 8;

@@ -23,9 +23,9 @@ import pytest
 
 # accel_variant 0-9 and 13-21: tile-per-wave SAH kernels (13, 14 and 16-21 read
 # the tree records from the schedule); 10-12: the persistent form
-ACCEL_VARIANTS = 22
+ACCEL_VARIANTS = 28
 # accel_vis_variant 0-21 (10-21 read the tree records from the schedule)
-ACCEL_VIS_VARIANTS = 22
+ACCEL_VIS_VARIANTS = 26
 
 from accel_check import LeafSet, check_accel_structure, skewed_leaf_set
 from gpu_util import require_gpu, to_dev, to_host
@@ -247,7 +247,7 @@ def test_batch_accel_equals_batch_heap(px, scene):
     """The batched views over the SAH hierarchy (every batch_variant, shadows
     on) give the heap batch's Hit, Linear and Guide, up to the
     walk-order-dependent pixels."""
-    print("case: batch of 6 walk views, accel vs heap, batch_variant 0..8, shadows on", flush=True)
+    print("case: batch of 6 walk views, accel vs heap, batch_variant 0..9, shadows on", flush=True)
     dev, acc = scene
     cams, sizes = px.batch_walk_views(3, 7, 6)
     sizes = [(min(w, 640), min(h, 360)) for w, h in sizes]
@@ -256,7 +256,7 @@ def test_batch_accel_equals_batch_heap(px, scene):
     lib = px.lib()
     old = lib.arbor_tune_get(b"batch_variant")
     try:
-        for v in range(9):
+        for v in range(10):
             assert lib.arbor_tune_set(b"batch_variant", v) == 0
             out = []
             for use in (None, acc):

@@ -10,9 +10,10 @@ from gpu_util import require_gpu, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-# batch_variant 0-4: fused trace + shade; 5-8: split (7 and 8 read the tree
-# records from device memory over the SAH hierarchy)
-BATCH_VARIANTS = 9
+# batch_variant 0-4: fused trace + shade; 5-9: split (7, 8 and 9 read the
+# tree records from device memory over the SAH hierarchy; 9 takes three
+# interior steps per gathered round)
+BATCH_VARIANTS = 10
 
 SIZES = [(256, 256), (320, 180), (200, 120), (64, 64), (333, 97), (128, 72), (96, 200), (40, 24)]
 
