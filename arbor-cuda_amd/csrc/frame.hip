@@ -556,6 +556,11 @@ const TraceVariant kAccelTraceVariants[] = {
      sahLdsPerWave(20), nullptr, true},
     {k_tracePixels<1, WalkSahGS<20, 5, 2>, 1, true>, k_traceTreePixels<1, WalkSahGS<20, 5, 2>>, 1,
      sahLdsPerWave(20), nullptr, true},
+    // 28-29: 19 and 20 (4 waves per SIMD) with three interior steps per round
+    {k_tracePixels<1, WalkSahGS<16, 5, 3>, 4, true>, k_traceTreePixels<1, WalkSahGS<16, 5, 3>>, 1,
+     sahLdsPerWave(16), nullptr, true},
+    {k_tracePixels<1, WalkSahGS<20, 5, 3>, 4, true>, k_traceTreePixels<1, WalkSahGS<20, 5, 3>>, 1,
+     sahLdsPerWave(20), nullptr, true},
 };
 // the shape an unscheduled call runs in place of a record-tree one
 constexpr int kAccelUnscheduled = 8;
