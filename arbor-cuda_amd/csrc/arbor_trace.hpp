@@ -595,10 +595,15 @@ __device__ __forceinline__ v2 traceTreeSah(const float* P, const float4* Nodes4,
     }
     return false;
   };
-  // one interior step: false when neither child is entered
+  // one interior step: false when neither child is entered. The node's
+  // loads are issued at a raised wave priority (s_setprio), so a wave about
+  // to wait on memory gets its loads out ahead of the SIMD's other waves'
+  // arithmetic; priorities change only the issue order, never a result.
   auto stepNode = [&]() -> bool {
+    __builtin_amdgcn_s_setprio(2);
     const float4* nd = accel + kAccelHeader + 4 * ref;
     const float4 k0 = nd[0], k1 = nd[1], k2 = nd[2], k3 = nd[3];
+    __builtin_amdgcn_s_setprio(0);
     const v2 ls = slabSpan(roi, inv, mk(k0.x, k0.y, k0.z), mk(k1.x, k1.y, k1.z));
     const v2 rs = slabSpan(roi, inv, mk(k2.x, k2.y, k2.z), mk(k3.x, k3.y, k3.z));
     const bool lh = spanHit(ls, 1.0f, best), rhit = spanHit(rs, 1.0f, best);
@@ -652,8 +657,12 @@ __device__ __forceinline__ v2 traceTreeSah(const float* P, const float4* Nodes4,
           }
         }
       }
+      // the gathered object tests at the highest priority: a wave holding
+      // most of its lanes there finishes them before the stepping waves
       if (live && ref < 0) {
+        __builtin_amdgcn_s_setprio(3);
         testLeaf();
+        __builtin_amdgcn_s_setprio(0);
         live = popNext();
       }
       if (!live) break;
