@@ -447,10 +447,14 @@ __device__ __forceinline__ v2 traceTree(const float* P, const float4* Nodes4, co
     if (live) {
       if (node < (uint32_t)kCapacity) {
         if (!haveKids) {
+          // node loads at a raised wave priority, the object test at the
+          // highest (as in traceTreeSah; issue order only)
+          __builtin_amdgcn_s_setprio(2);
           k0 = nodeQ<W>(Nodes4, top, node * 4);
           k1 = nodeQ<W>(Nodes4, top, node * 4 + 1);
           k2 = nodeQ<W>(Nodes4, top, node * 4 + 2);
           k3 = nodeQ<W>(Nodes4, top, node * 4 + 3);
+          __builtin_amdgcn_s_setprio(0);
         }
         // grandchildren exist below depth 14 (node < CAPACITY/2)
         const bool pf = W::prefetch && node < (uint32_t)(kCapacity / 2);
@@ -480,8 +484,10 @@ __device__ __forceinline__ v2 traceTree(const float* P, const float4* Nodes4, co
         int id = (int)Order[node - kCapacity];
         if (id != ignore) {
           float footprint = fmaxf(0.000005f, cone * fmaxf(0.05f, span.x));
+          __builtin_amdgcn_s_setprio(3);
           Prim pr = loadPrim(P, id);
           float t = objectT(pr, ro, rd, footprint, time, wind, !shadows, W::cull, best);
+          __builtin_amdgcn_s_setprio(0);
           if (t < best) { best = t; found = id; }
         }
       }
@@ -491,7 +497,9 @@ __device__ __forceinline__ v2 traceTree(const float* P, const float4* Nodes4, co
     int d = 31 - __builtin_clz(pending);
     pending &= ~(1u << d);
     node = (node >> (heapDepth(node) - d)) ^ 1u;
+    __builtin_amdgcn_s_setprio(2);
     float4 b0 = nodeQ<W>(Nodes4, top, node * 2), b1 = nodeQ<W>(Nodes4, top, node * 2 + 1);
+    __builtin_amdgcn_s_setprio(0);
     span = slabSpan(roi, inv, mk(b0.x, b0.y, b0.z), mk(b1.x, b1.y, b1.z));
     live = spanHit(span, b0.w, best);
   }
