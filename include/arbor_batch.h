@@ -43,7 +43,7 @@ int arbor_batch_launch(arbor_batch* b, hipStream_t stream);
 /* Trace and sun visibility over a SAH hierarchy of arbor_buildTraceAccel
  * (arbor.h) for the scene passed to render/capture, or over the heap BVH
  * with accel NULL (the default). batch_variant keeps selecting the launch
- * shape; over the hierarchy the default (7) and 8 read the views' tree
+ * shape; over the hierarchy 7, 8 and the default (9) read the views' tree
  * records from a copy set_views writes and keep the walk's stack in LDS.
  * Invalidates a captured graph. No counterpart in the reference. */
 int arbor_batch_set_accel(arbor_batch* b, const void* accel);
