@@ -499,14 +499,15 @@ int launchTrace(bool forest, const float* P, const float* Nodes, const unsigned*
 // a scheduled 1080p walk frame against 3.37 ms with the if-if order of 3; a
 // threshold of 2/8 took 3.80 ms, 8/8 3.46 ms. The record-tree shapes (13,
 // 14, 16-21) read the 324 tree records from the tile schedule instead of
-// building a copy in LDS per block, so LDS holds the stack. 16 (the
-// default: one-wave blocks, 20 LDS entries, 3 waves per SIMD) traces a
+// building a copy in LDS per block, so LDS holds the stack. 16 (one-wave
+// blocks, 20 LDS entries, 3 waves per SIMD) traces a
 // 1080p walk frame in 3.22 ms against 3.31 for 8 and, for a hierarchy of
 // depth <= 21, keeps every stack entry out of scratch (PMC WRITE_SIZE about
 // Hit's bytes). 19 and 20 (4 waves per SIMD, registers spilled to scratch)
 // are as fast at 1080p and 12% faster at 4K (7.5-7.7 ms against 8.8), at
 // four times the write traffic. 24 (the default) is 16 with three interior
-// steps per gathered round and gathering at 5/8: 3.05 ms against 3.23.
+// steps per gathered round and gathering at 5/8: 3.05 ms against 3.23; 28
+// and 29 are 19 and 20 with the same, the autotuner's choice at 4K.
 const TraceVariant kAccelTraceVariants[] = {
     {k_tracePixels<1, WalkSah>, k_traceTreePixels<1, WalkSah>, 1},
     {k_tracePixels<2, WalkSah>, k_traceTreePixels<2, WalkSah>, 2},
@@ -765,8 +766,8 @@ int launchVisibility(const float* P, const float* Nodes, const unsigned* Order, 
 
 // accel_vis_variant index -> sun-visibility kernel over the SAH hierarchy. 8
 // gathers object tests at 5/8 of the live lanes: 7.8 ms for 1080p with
-// shadows against 8.8 ms for 0, the same kernel without it. 15 (the
-// default) reads the tree records from the schedule and keeps 14 stack
+// shadows against 8.8 ms for 0, the same kernel without it. 15 reads the
+// tree records from the schedule and keeps 14 stack
 // entries per lane in LDS, one-wave blocks at 5 waves per SIMD: 118-120 ms
 // for a 4-sample 4K frame with sun visibility against 131 ms with 8. 23
 // (the default) is 15 with three interior steps per gathered round: 114.2
