@@ -505,9 +505,11 @@ int launchTrace(bool forest, const float* P, const float* Nodes, const unsigned*
 // depth <= 21, keeps every stack entry out of scratch (PMC WRITE_SIZE about
 // Hit's bytes). 19 and 20 (4 waves per SIMD, registers spilled to scratch)
 // are as fast at 1080p and 12% faster at 4K (7.5-7.7 ms against 8.8), at
-// four times the write traffic. 24 (the default) is 16 with three interior
-// steps per gathered round and gathering at 5/8: 3.05 ms against 3.23; 28
-// and 29 are 19 and 20 with the same, the autotuner's choice at 4K.
+// four times the write traffic. 24 is 16 with three interior steps per
+// gathered round and gathering at 5/8: 3.05 ms against 3.23; 28 and 29 are
+// 19 and 20 with the same, the autotuner's choice at 4K. 26 (the default)
+// takes up to four steps: with the walk's wave priorities 2.92-2.93 ms
+// against 2.95-2.97 for 24.
 const TraceVariant kAccelTraceVariants[] = {
     {k_tracePixels<1, WalkSah>, k_traceTreePixels<1, WalkSah>, 1},
     {k_tracePixels<2, WalkSah>, k_traceTreePixels<2, WalkSah>, 2},
@@ -561,6 +563,11 @@ const TraceVariant kAccelTraceVariants[] = {
     {k_tracePixels<1, WalkSahGS<16, 5, 3>, 4, true>, k_traceTreePixels<1, WalkSahGS<16, 5, 3>>, 1,
      sahLdsPerWave(16), nullptr, true},
     {k_tracePixels<1, WalkSahGS<20, 5, 3>, 4, true>, k_traceTreePixels<1, WalkSahGS<20, 5, 3>>, 1,
+     sahLdsPerWave(20), nullptr, true},
+    // 30-31: 24 with five and six interior steps per round
+    {k_tracePixels<1, WalkSahGS<20, 5, 5>, 1, true>, k_traceTreePixels<1, WalkSahGS<20, 5, 5>>, 1,
+     sahLdsPerWave(20), nullptr, true},
+    {k_tracePixels<1, WalkSahGS<20, 5, 6>, 1, true>, k_traceTreePixels<1, WalkSahGS<20, 5, 6>>, 1,
      sahLdsPerWave(20), nullptr, true},
 };
 // the shape an unscheduled call runs in place of a record-tree one
@@ -806,6 +813,9 @@ const VisVariant kAccelVisVariants[] = {
     {k_sunVisibility<1, WalkSahGS<14, 5, 3>, 5, true>, 1, sahLdsPerWave(14), true},
     {k_sunVisibility<1, WalkSahGS<14, 4, 3>, 5, true>, 1, sahLdsPerWave(14), true},
     {k_sunVisibility<1, WalkSahGS<14, 6, 3>, 5, true>, 1, sahLdsPerWave(14), true},
+    // 26-27: 23 with four and five interior steps per round
+    {k_sunVisibility<1, WalkSahGS<14, 5, 4>, 5, true>, 1, sahLdsPerWave(14), true},
+    {k_sunVisibility<1, WalkSahGS<14, 5, 5>, 5, true>, 1, sahLdsPerWave(14), true},
 };
 // the shape an unscheduled call runs in place of a record-tree one
 constexpr int kAccelVisUnscheduled = 8;

@@ -15,7 +15,7 @@ struct Tuning {
   int sched_newton_min = 24;  // lanes that must be ready before a NEWTON phase runs
   int batch_variant = 9;      // index into the batched trace+shade kernel table (batch.hip)
   int vis_variant = 6;        // index into the sun-visibility kernel table (frame.hip)
-  int accel_variant = 24;      // index into the SAH-walk trace kernel table (frame.hip)
+  int accel_variant = 26;      // index into the SAH-walk trace kernel table (frame.hip)
   int accel_vis_variant = 23;  // index into the SAH-walk sun-visibility kernel table (frame.hip)
 };
 int batchVariantCount();

@@ -23,9 +23,9 @@ import pytest
 
 # accel_variant 0-9 and 13-21: tile-per-wave SAH kernels (13, 14 and 16-21 read
 # the tree records from the schedule); 10-12: the persistent form
-ACCEL_VARIANTS = 30
+ACCEL_VARIANTS = 32
 # accel_vis_variant 0-21 (10-21 read the tree records from the schedule)
-ACCEL_VIS_VARIANTS = 26
+ACCEL_VIS_VARIANTS = 28
 
 from accel_check import LeafSet, check_accel_structure, skewed_leaf_set
 from gpu_util import require_gpu, to_dev, to_host
