@@ -22,6 +22,7 @@
 #include "arbor_filter.hpp"
 #include "arbor_persist.hpp"
 #include "arbor_shade.hpp"
+#include "arbor_split.hpp"
 #include "arbor_wavefront.hpp"
 #include "engine_passes.hpp"
 #include "launch.hpp"
@@ -364,6 +365,20 @@ template <int K, int G, int S>
 using WalkSahGS = Walk<true, false, false, true, K, G, S>;  // the same, S interior steps per gathered round
 constexpr int sahLdsPerWave(int k) { return k * 64 * 8; }
 
+// The stage-split trace's launches (arbor_split.hpp): one-wave blocks, the
+// tree records from the tile schedule, the walk's stack entries in LDS.
+__global__ __launch_bounds__(64) void k_splitStart(const float* C, float4* Hit, int width, int height, int tilesX,
+                                                   int tiles, const float4* treeRecs, SplitBuffers b) {
+  splitStart(TreeRecs{treeRecs}, C, Hit, width, height, tilesX, tiles, b);
+}
+template <class W, bool kFinish>
+__global__ __launch_bounds__(64) void k_splitRound(const float* P, const float4* Nodes4, const uint32_t* Order,
+                                                   const float* C, float4* Hit, int width, int height,
+                                                   const float4* accel, const float4* treeRecs, SplitBuffers b,
+                                                   int round) {
+  splitRound<W, kFinish>(P, Nodes4, Order, TreeRecs{treeRecs}, C, Hit, width, height, accel, b, round);
+}
+
 // shadePixels (Arbor.cu:277-289).
 // kTraceVis = false: Vis is given whenever sun visibility is on (the shadow
 // traversals are compiled out, and with them the call's register cost).
@@ -457,6 +472,7 @@ struct TraceVariant {
   int ldsPerWave = 0;              // dynamic LDS bytes per wave (the SAH walk's stack entries)
   PersistKernel persist = nullptr;  // the persistent form (forest and tree unused)
   bool recTrees = false;            // reads the schedule's tree records (scheduled calls only)
+  int splitRounds = 0;              // the stage-split form with this many queue rounds (forest and tree unused)
 };
 // trace_variant index -> kernels. Identical results for every entry.
 const TraceVariant kTraceVariants[] = {
@@ -569,7 +585,13 @@ const TraceVariant kAccelTraceVariants[] = {
      sahLdsPerWave(20), nullptr, true},
     {k_tracePixels<1, WalkSahGS<20, 5, 6>, 1, true>, k_traceTreePixels<1, WalkSahGS<20, 5, 6>>, 1,
      sahLdsPerWave(20), nullptr, true},
+    // 32-34: the stage-split form (start, 3/6/12 queue rounds, finish) with
+    // 26's walk
+    {nullptr, nullptr, 1, sahLdsPerWave(20), nullptr, true, 3},
+    {nullptr, nullptr, 1, sahLdsPerWave(20), nullptr, true, 6},
+    {nullptr, nullptr, 1, sahLdsPerWave(20), nullptr, true, 12},
 };
+using WalkSplit = WalkSahGS<20, 5, 4>;
 // the shape an unscheduled call runs in place of a record-tree one
 constexpr int kAccelUnscheduled = 8;
 constexpr int kAccelTraceVariantCount = sizeof(kAccelTraceVariants) / sizeof(kAccelTraceVariants[0]);
@@ -603,6 +625,76 @@ int residentBlocks(PersistKernel k, int waves, int ldsBytes) {
 
 void scheduleArrays(void* schedule, int width, int height, const uint32_t** order, uint32_t** cost);
 
+// The stage-split trace's ray state and queues: 32 bytes per ray plus the
+// round counts, allocated on a variant's first use on a device and grown to
+// the largest view traced (the device is synchronised before a smaller
+// buffer is freed). Not for callers that cannot allocate: the default shapes
+// allocate nothing.
+int splitBuffers(size_t rays, SplitBuffers* b) {
+  static std::mutex lock;
+  static void* base[64] = {};
+  static size_t have[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return ARBOR_ERR_DEVICE;
+  std::lock_guard<std::mutex> g(lock);
+  if (have[dev] < rays) {
+    if (base[dev]) {
+      if (hipDeviceSynchronize() != hipSuccess) return ARBOR_ERR_DEVICE;
+      (void)hipFree(base[dev]);
+      base[dev] = nullptr;
+      have[dev] = 0;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, rays * 8 * sizeof(uint32_t) + kSplitWords * sizeof(uint32_t)) != hipSuccess)
+      return ARBOR_ERR_DEVICE;
+    base[dev] = p;
+    have[dev] = rays;
+  }
+  uint32_t* w = static_cast<uint32_t*>(base[dev]);
+  b->cell = w;
+  b->tx = reinterpret_cast<float*>(w + rays);
+  b->tz = reinterpret_cast<float*>(w + 2 * rays);
+  b->best = reinterpret_cast<float*>(w + 3 * rays);
+  b->found = reinterpret_cast<float*>(w + 4 * rays);
+  b->tree = reinterpret_cast<int*>(w + 5 * rays);
+  b->queue[0] = w + 6 * rays;
+  b->queue[1] = w + 7 * rays;
+  b->counts = w + 8 * rays;
+  return ARBOR_OK;
+}
+// The stage-split trace: clear, start, `rounds` queue rounds and the finish.
+int launchSplit(const float* P, const float* Nodes, const unsigned* Order, const float4* accel, const float* C,
+                float* Hit, int width, int height, hipStream_t stream, const float4* treeRecs, int rounds) {
+  if (rounds < 1 || rounds > kSplitMaxRounds) return ARBOR_ERR_ARG;
+  SplitBuffers b;
+  int rc = splitBuffers((size_t)width * (size_t)height, &b);
+  if (rc != ARBOR_OK) return rc;
+  const int tx = (width + 7) / 8, tiles = tx * ((height + 7) / 8);
+  float4* hit = reinterpret_cast<float4*>(Hit);
+  const float4* nodes = reinterpret_cast<const float4*>(Nodes);
+  const int lds = sahLdsPerWave(20);
+  hipLaunchKernelGGL(k_splitClear, dim3(1), dim3(kSplitWords), 0, stream, b.counts, kSplitWords);
+  hipLaunchKernelGGL(k_splitStart, dim3(tileGrid(tiles, 1)), dim3(64), 0, stream, C, hit, width, height, tx, tiles,
+                     treeRecs, b);
+  static std::atomic<int> resident{0};
+  int blocks = resident.load(std::memory_order_relaxed);
+  if (blocks <= 0) {
+    int perCU = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &perCU, reinterpret_cast<const void*>(k_splitRound<WalkSplit, false>), 64, lds) != hipSuccess ||
+        perCU < 1)
+      perCU = 1;
+    blocks = perCU * deviceCUs();
+    resident.store(blocks, std::memory_order_relaxed);
+  }
+  for (int r = 0; r < rounds; r++)
+    hipLaunchKernelGGL((k_splitRound<WalkSplit, false>), dim3(blocks), dim3(64), lds, stream, P, nodes, Order, C,
+                       hit, width, height, accel, treeRecs, b, r);
+  hipLaunchKernelGGL((k_splitRound<WalkSplit, true>), dim3(blocks), dim3(64), lds, stream, P, nodes, Order, C, hit,
+                     width, height, accel, treeRecs, b, rounds);
+  return launch_status();
+}
+
 // The SAH-walk trace. schedule: an ordered tile schedule (its order and cost
 // arrays, and the persistent kernels' counters), or null.
 int launchTraceAccel(const float* P, const float* Nodes, const unsigned* Order, const void* accel, const float* C,
@@ -614,6 +706,9 @@ int launchTraceAccel(const float* P, const float* Nodes, const unsigned* Order, 
   const int tuned = tuning().accel_variant;
   // an unscheduled call has no tree records: it runs the default shape instead
   const TraceVariant& v = kAccelTraceVariants[kAccelTraceVariants[tuned].recTrees && !schedule ? kAccelUnscheduled : tuned];
+  if (v.splitRounds)
+    return launchSplit(P, Nodes, Order, static_cast<const float4*>(accel), C, Hit, width, height, stream,
+                       static_cast<const TileSchedule*>(schedule)->trees, v.splitRounds);
   if (v.persist) {
     uint32_t* counters = schedule ? static_cast<TileSchedule*>(schedule)->persist : unscheduledCounters();
     if (!counters) return ARBOR_ERR_DEVICE;

@@ -221,12 +221,14 @@ int arbor_tune_set(const char* key, int value);
  * "sched_newton_min" (1..64) for the wave-scheduled trace kernel,
  * "batch_variant" (arbor_batch.h), "vis_variant" (0..7, the
  * sun-visibility kernel's launch shape and register budget), and for the
- * SAH walk "accel_variant" (0..31) and "accel_vis_variant" (0..27): block
+ * SAH walk "accel_variant" (0..34) and "accel_vis_variant" (0..27): block
  * size, register budget, how many of each lane's stack entries live in LDS,
  * whether a wave holds its object tests until a share of its lanes reach a
  * leaf, how many interior steps a lane takes between those checks, whether
- * the tree records come from the tile schedule, and the persistent form
- * (accel_variant 10-12); the defaults are 26 and 23. */
+ * the tree records come from the tile schedule, the persistent form
+ * (accel_variant 10-12) and the stage-split form over ray queues
+ * (accel_variant 32-34, which allocates its queues on first use per device);
+ * the defaults are 26 and 23. */
 int arbor_tune_get(const char* key);
 /* Persisted tuning cache: writes / reads "key value" lines. A file with an
  * unknown key or out-of-range value is rejected whole (tuning unchanged). */

@@ -170,7 +170,7 @@ def test_trace_accel_size_and_checks(L):
     assert L.arbor_engine_set_accel(NULL, 1) == E
     assert L.arbor_autotune_trace_accel(DUMMY, DUMMY, DUMMY, NULL, DUMMY, DUMMY, 64, 64, 1, NULL, NULL, 0) == E
     assert L.arbor_autotune_trace_accel(DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, 64, 64, 0, NULL, NULL, 0) == -2
-    for key, n in ((b"accel_variant", 32), (b"accel_vis_variant", 28)):
+    for key, n in ((b"accel_variant", 35), (b"accel_vis_variant", 28)):
         old = L.arbor_tune_get(key)
         assert L.arbor_tune_set(key, n) == -2 and L.arbor_tune_set(key, -1) == -2
         assert L.arbor_tune_set(key, n - 1) == 0 and L.arbor_tune_get(key) == n - 1

@@ -21,9 +21,10 @@ the trace and sun-visibility passes over it.
 import numpy as np
 import pytest
 
-# accel_variant 0-9 and 13-21: tile-per-wave SAH kernels (13, 14 and 16-21 read
-# the tree records from the schedule); 10-12: the persistent form
-ACCEL_VARIANTS = 32
+# accel_variant 0-9 and 13-31: tile-per-wave SAH kernels (13, 14 and 16-31 read
+# the tree records from the schedule); 10-12: the persistent form; 32-34: the
+# stage-split form (start, queue rounds, finish)
+ACCEL_VARIANTS = 35
 # accel_vis_variant 0-21 (10-21 read the tree records from the schedule)
 ACCEL_VIS_VARIANTS = 28
 
